@@ -1,0 +1,154 @@
+"""Pin the CPU oracle against the reference's own outputs (golden vectors).
+
+The fixtures were written by tests/golden/make_golden.py from the reference
+implementation (FlexDTW.py / Parflex.py) on the seeds and shapes of the
+reference test file tests/test_gpu_flexdtw.py plus multi-tile stage-2 cases.
+Everything here is bit-exact: integers and float64 values must be identical.
+"""
+import numpy as np
+import pytest
+
+from conftest import STAGE2_CASES, load_golden
+
+
+def _same_f64(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return a.shape == b.shape and np.array_equal(a, b, equal_nan=True)
+
+
+def test_flexdtw_reference_seeds(oracle_mod):
+    """tests/test_gpu_flexdtw.py:20-39 inputs; D, P, B, wp, best_cost and the
+    endpoint debug list identical to FlexDTW.flexdtw."""
+    g = load_golden("flexdtw_seed1")
+    steps = [(1, 1), (1, 2), (2, 1)]
+    for k in range(3):
+        bc, wp, D, P, B, debug = oracle_mod.flexdtw(g[f"C{k}"], steps, [2, 3, 3], buffer=1)
+        assert _same_f64(D, g[f"D{k}"])
+        assert np.array_equal(P, g[f"P{k}"])
+        assert np.array_equal(B, g[f"B{k}"])
+        assert np.array_equal(wp, g[f"wp{k}"])
+        assert bc == g[f"best{k}"]
+        dbg = np.array([[d[0], d[1], d[2], d[3]] for d in debug])
+        assert _same_f64(dbg, g[f"debug{k}"])
+
+
+def test_flexdtw_other_step_sets(oracle_mod):
+    g = load_golden("flexdtw_seed1")
+    for k in range(3):
+        bc, wp, D, P, B, _ = oracle_mod.flexdtw(g[f"xC{k}"], g[f"xsteps{k}"],
+                                                g[f"xweights{k}"], buffer=1)
+        assert _same_f64(D, g[f"xD{k}"])
+        assert np.array_equal(P, g[f"xP{k}"])
+        assert np.array_equal(B, g[f"xB{k}"])
+        assert np.array_equal(wp, g[f"xwp{k}"])
+        assert bc == g[f"xbest{k}"]
+
+
+def test_stage1_tiles_match_reference(oracle_mod):
+    """Per-tile D/S/B of run_flexdtw_on_tiles (tests/test_gpu_flexdtw.py:42-62)."""
+    g = load_golden("tiles_seed2")
+    C, L = g["C"], int(g["L"])
+    n1, n2 = oracle_mod.tile_grid(C.shape[0], C.shape[1], L)
+    assert (n1, n2) == (int(g["n1"]), int(g["n2"]))
+    for i in range(n1):
+        for j in range(n2):
+            s1, e1, s2, e2 = oracle_mod.tile_bounds(i, j, C.shape[0], C.shape[1], L)
+            bc, wp, D, P, B, _ = oracle_mod.flexdtw(C[s1:e1, s2:e2], g["steps"],
+                                                    g["weights"], buffer=1)
+            assert _same_f64(D, g[f"tile_{i}_{j}_D"])
+            assert np.array_equal(P, g[f"tile_{i}_{j}_S"])
+            assert np.array_equal(B, g[f"tile_{i}_{j}_B"])
+            assert bc == g[f"tile_{i}_{j}_best"]
+            assert np.array_equal(wp, g[f"tile_{i}_{j}_wp"])
+
+
+def test_edge_data_and_mask(oracle_mod):
+    """_build_edge_data / _build_valid_mask inputs of the reference test
+    tests/test_gpu_flexdtw.py:65-89: edge D and start decoding agree."""
+    g = load_golden("edges_seed4")
+    C, L = g["C"], int(g["L"])
+    s1 = oracle_mod.run_stage1(C, L, [(1, 1), (1, 2), (2, 1)], [2, 3, 3])
+    assert _same_f64(s1["edge_D"], g["edge_Df"])
+    P = s1["edge_P"].astype(np.int64)
+    sr = np.where(P < 0, -P, 0)
+    sc = np.where(P > 0, P, 0)
+    el = g["edge_lens"]
+    for i in range(el.shape[0]):
+        for j in range(el.shape[1]):
+            for e in range(2):
+                n = el[i, j, e]
+                assert np.array_equal(sr[i, j, e, :n], g["edge_start_r"][i, j, e, :n])
+                assert np.array_equal(sc[i, j, e, :n], g["edge_start_c"][i, j, e, :n])
+
+
+@pytest.mark.parametrize("name", STAGE2_CASES)
+def test_stage2_matches_reference(oracle_mod, name):
+    g = load_golden(name)
+    C, L, beta = g["C"], int(g["L"]), float(g["beta"])
+    s1 = oracle_mod.run_stage1(C, L, g["steps"], g["weights"])
+    r = oracle_mod.run_stage2(C, s1, beta)
+    assert r["found"] == bool(g["found"])
+    if not r["found"]:
+        with pytest.raises(ValueError, match="No valid endpoint"):
+            oracle_mod.parflex(C, g["steps"], g["weights"], beta, L)
+        return
+    assert _same_f64(r["D_arr"], g["s2_D_arr"])
+    assert _same_f64(r["L_arr"], g["s2_L_arr"])
+    assert np.array_equal(r["edge_lens"], g["s2_edge_lens"])
+    assert r["best_cost"] == g["s2_best_cost"]
+    assert np.array_equal(r["best_out"][:6], g["s2_best_end"])
+    assert np.array_equal(r["stitched_wp"], g["s2_stitched_wp"])
+    assert _same_f64(r["top_D"], g["s2_top_D"])
+    assert _same_f64(r["top_L"], g["s2_top_L"])
+    assert _same_f64(r["right_D"], g["s2_right_D"])
+    assert _same_f64(r["right_L"], g["s2_right_L"])
+    # per-segment winners and their stitched paths
+    segs = g["s2_segments"]
+    si = r["seg_i64"]
+    got = []
+    norms = []
+    paths = []
+    # the fixture is ordered like sorted(best_per_segment): 'right' before 'top'
+    for side in (1, 0):
+        for s in range(si.shape[2]):
+            if si[0, side, s] < 0:
+                continue
+            got.append([side, s, si[0, side, s], si[1, side, s], si[2, side, s],
+                        si[3, side, s], si[4, side, s], si[5, side, s]])
+            norms.append(r["seg_norm"][side, s])
+            paths.append(oracle_mod.backtrace_and_stitch(s1, si[0, side, s], si[1, side, s],
+                                                         si[2, side, s], si[3, side, s]))
+    assert np.array_equal(np.array(got, np.int64).reshape(-1, 8), segs)
+    assert _same_f64(norms, g["s2_segment_norm"])
+    lens = g["s2_segment_path_lens"]
+    assert [len(p) for p in paths] == list(lens)
+    if len(paths):
+        assert np.array_equal(np.concatenate(paths), g["s2_segment_paths"])
+    bc, wp = oracle_mod.parflex(C, g["steps"], g["weights"], beta, L)
+    assert bc == g["parflex_best_cost"]
+    assert np.array_equal(wp, g["parflex_wp"])
+
+
+def test_parflex_threads_do_not_change_result(oracle_mod):
+    g = load_golden("stage2_chroma600_L100")
+    a = oracle_mod.parflex(g["C"], g["steps"], g["weights"], 0.1, 100, nthreads=1)
+    b = oracle_mod.parflex(g["C"], g["steps"], g["weights"], 0.1, 100, nthreads=7)
+    assert a[0] == b[0] and np.array_equal(a[1], b[1])
+
+
+def test_cfg0_workload(oracle_mod):
+    """The 2000 x 2400, L=500 chroma workload end to end."""
+    import os
+    from conftest import GOLDEN
+    if not os.path.exists(os.path.join(GOLDEN, "cfg0_chroma2000_L500.npz")):
+        pytest.skip("cfg0 fixture not generated")
+    g = load_golden("cfg0_chroma2000_L500")
+    from parflexdtw_amd.synth import cost_matrix_det
+    C = cost_matrix_det(g["F1"], g["F2"])
+    Co = oracle_mod.cost_matrix(g["F1"], g["F2"])
+    assert np.array_equal(C, Co)
+    bc, wp = oracle_mod.parflex(C, g["steps"], g["weights"], float(g["beta"]), int(g["L"]),
+                                nthreads=8)
+    assert bc == g["parflex_best_cost"]
+    assert np.array_equal(wp, g["parflex_wp"])
