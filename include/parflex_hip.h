@@ -1,0 +1,163 @@
+/*
+ * parflex_hip.h -- C ABI of the MI355X (gfx950) ParFlex kernels.
+ *
+ * libparflex_hip.so exports exactly the functions below.  All pointers named
+ * d_* are device pointers (e.g. torch.Tensor.data_ptr() of a ROCm tensor);
+ * the library allocates no device memory in any of them: the caller owns and
+ * sizes every buffer (the Python layer parflexdtw_amd/_plan.py computes the
+ * sizes).  `stream` is a hipStream_t (torch.cuda.current_stream().cuda_stream)
+ * and every call is asynchronous on it.  Every function returns 0 on success
+ * or a negative pf_status; pf_last_error() returns a thread-local message.
+ *
+ * The reference (ishitajain21/ParFlexDTW) is Python: its GPU interface is the
+ * set of Python functions in gpu_flexdtw.py / Parflex_gpu.py, which reach one
+ * CUDA kernel through CuPy.  Each entry point below names the reference
+ * function (file:line) whose work it replaces; the ctypes binding a
+ * maintainer would add to the reference is shown in INTEGRATION.md.
+ */
+#ifndef PARFLEX_HIP_H
+#define PARFLEX_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum pf_status {
+  PF_OK = 0,
+  PF_EINVAL = -1,   /* bad argument (shape, step set, size) */
+  PF_EHIP = -2,     /* a HIP runtime call failed */
+  PF_ENOENDPOINT = -3 /* stage 2 found no endpoint (Parflex.py:1285-1286) */
+};
+
+/* Precision of C and of the DP accumulator. */
+enum pf_dtype { PF_F64 = 0, PF_F32 = 1, PF_BF16 = 2 };
+
+/* Thread-local description of the last error. */
+const char *pf_last_error(void);
+
+/* ABI version (major*100 + minor). */
+int pf_version(void);
+
+/* Step set: K <= 4 steps (dr, dc), 1 <= dr <= 2, 0 <= dc <= 2, with at least
+ * one step of dr == 1 and dc <= 1 (every interior cell then has a valid
+ * predecessor); weights as doubles. Validation only, no device work.
+ * Replaces the step handling of gpu_flexdtw.py:326-328. */
+int pf_check_steps(const int32_t *steps, const double *weights, int K);
+
+/*
+ * Tile table: int64[8] per tile, row-major, on the device:
+ *   [0] c_off  element offset of the tile's (0,0) cell in the C buffer
+ *   [1] ldc    row stride of C in elements
+ *   [2] R      tile rows       [3] W  tile columns
+ *   [4] b_off  offset (uint32 words) of the tile's packed back-pointers
+ *   [5] e_off  offset (elements) of the tile's edge slots (2*Lmax per tile)
+ *   [6] pair   owning pair     [7] (i << 32) | j tile coordinates
+ * Packed back-pointers: word (rb, c) = b_off + rb*W + c holds the 2-bit step
+ * indices of rows 16*rb .. 16*rb+15 of column c, row r at bit 2*(r%16).
+ * Edge slots: edge_D/edge_P[e_off + e*Lmax + pos], e = 0 last row (pos =
+ * column), e = 1 last column (pos = row).
+ */
+
+/* Stage 1, all tiles of a tile table in one launch: per-tile FlexDTW DP
+ * writing packed back-pointers and D/P on the tile edges only.
+ * Replaces the tile loop of Parflex_gpu.run_flexdtw_on_tiles
+ * (Parflex_gpu.py:461-563) with its per-tile flexdtw_chunk_from_global_C
+ * (gpu_flexdtw.py:288-402) and _edge_starts_from_S (Parflex.py:347-373).
+ * dtype: PF_F64 (C f64, DP f64) or PF_F32 (C f32, DP f32, edges f64) or
+ * PF_BF16 (C bf16, DP f32, edges f64). */
+int pf_stage1(int dtype, const void *d_C, const int64_t *d_tiles, int64_t ntiles,
+              int64_t Lmax, const int32_t *steps, const double *weights, int K,
+              uint32_t *d_Bpacked, double *d_edge_D, int32_t *d_edge_P,
+              void *stream);
+
+/* Per-tile DP with full D/P/B outputs for one tile C[r0:r1, c0:c1] of a
+ * row-major FP64 matrix with leading dimension ldc (FlexDTW.py:1320-1364;
+ * gpu_flexdtw.py:146-263,356-393).  D is f64 R*W, P int32, B int8. */
+int pf_tile_full(const double *d_C, int64_t ldc, int64_t r0, int64_t r1,
+                 int64_t c0, int64_t c1, const int32_t *steps,
+                 const double *weights, int K, double *d_D, int32_t *d_P,
+                 int8_t *d_B, void *stream);
+
+/* find_best_endpoint (FlexDTW.py:1098-1148) on full D/P of one tile:
+ * d_avg receives the (W-buffer)+(R-1-buffer) candidate averages in the
+ * reference's order; d_best receives {best_cost, (double)best_r,
+ * (double)best_c}. */
+int pf_tile_endpoint(const double *d_D, const int32_t *d_P, int64_t R, int64_t W,
+                     int64_t buffer, double *d_avg, double *d_best, void *stream);
+
+/* backtrace_flexdtw + reverse (FlexDTW.py:1155-1180,1366-1368) on full int8
+ * B from the endpoint in d_best (as written by pf_tile_endpoint).
+ * d_path receives (row, col) int64 pairs, capacity R+W; *d_len the count. */
+int pf_tile_backtrace(const int8_t *d_B, int64_t R, int64_t W,
+                      const int32_t *steps, int K, const double *d_best,
+                      int64_t *d_path, int64_t *d_len, void *stream);
+
+/*
+ * Stage 2 for a batch of pairs that share the tile length L (= Lmax).
+ * Pair table: int64[16] per pair on the device:
+ *   [0] L1 [1] L2 [2] n1 [3] n2 [4] tile_base (index of tile (0,0))
+ *   [5] c_off (element offset of the pair's C) [6] ldc
+ *   [7] buf (int(buffer_global), Parflex.py:1248)
+ *   [8] lk_off (offset of the pair's L2+L1 candidate slots)
+ *   [9] seg_off (offset of the pair's 2*max_segs segment slots) [10] max_segs
+ *   [11] path_off (offset, in points, of the pair's path buffer; capacity L1+L2+8)
+ *   [12..15] reserved (0)
+ * Workspace (device, caller-allocated, sizes from parflexdtw_amd/_plan.py):
+ *   d_D_arr, d_L_arr, d_olap  f64 [ntiles*2*L]      (Parflex.py:509-564,631-855)
+ *   d_bits                    uint32 [ntiles*2*bitw], bitw = (L+32)/32
+ *   d_cand_D, d_cand_L        f64 [sum(L2+L1)]      (edge_summary top/right D, L)
+ *   d_seg_f                   f64 [sum(2*max_segs)*4] (norm, D, L, order)
+ *   d_seg_i                   int64 [sum(2*max_segs)*8] (ci,cj,et,idx,grow,gcol,found,order)
+ *   d_best_f                  f64 [npairs]           best normalised cost
+ *   d_best_i                  int64 [npairs*8]       (g_row,g_col,ci,cj,et,idx,found,order)
+ *   d_chain                   int64 [npairs*101*8]   stitched segment list
+ *   d_pts                     int64 [npairs*101*(2*L+8)*2] per-segment points
+ *   d_path                    int64 [sum(L1+L2+8)*2]  stitched path (row, col)
+ *   d_path_len                int64 [npairs]
+ * Batch dimensions (host values): max_diag = max(n1+n2-1), max_tile_diag =
+ * max(min(n1,n2)), max_tiles_pair = max(n1*n2), max_segs = max(max_segs).
+ * Replaces Parflex_gpu.propagate_tile_edge_costs (Parflex_gpu.py:247-339),
+ * Parflex.sync_tile_overlap_edges (Parflex.py:919-969) and
+ * Parflex.stage2_scan_and_stitch (Parflex.py:1229-1356).  Only the path (and,
+ * on request, the edge arrays) need to be copied back.
+ */
+int pf_stage2(int c_dtype, const void *d_C, const int64_t *d_pairs, int64_t npairs,
+              const int64_t *d_tiles, int64_t ntiles, int64_t L, int64_t max_diag,
+              int64_t max_tile_diag, int64_t max_tiles_pair, int64_t max_segs,
+              const int32_t *steps, int K, const uint32_t *d_Bpacked,
+              const double *d_edge_D, const int32_t *d_edge_P, double *d_D_arr,
+              double *d_L_arr, double *d_olap, uint32_t *d_bits, double *d_cand_D,
+              double *d_cand_L, double *d_seg_f, int64_t *d_seg_i, double *d_best_f,
+              int64_t *d_best_i, int64_t *d_chain, int64_t *d_pts, int64_t *d_path,
+              int64_t *d_path_len, void *stream);
+
+/* backtrace_and_stitch (Parflex.py:1032-1103) for n requests, each
+ * int64[5] = (pair, ci, cj, et, idx) in d_req; stage-1 outputs as for
+ * pf_stage2.  Request q writes its path at d_path + q*path_cap*2 and its
+ * length to d_path_len[q].  d_chain: int64[n*101*8]; d_pts: int64
+ * [n*101*(2L+8)*2]. */
+int pf_stitch(const int64_t *d_req, int64_t nreq, const int64_t *d_pairs,
+              const int64_t *d_tiles, int64_t L, const int32_t *steps, int K,
+              const uint32_t *d_Bpacked, const int32_t *d_edge_P, int64_t *d_chain,
+              int64_t *d_pts, int64_t path_cap, int64_t *d_path, int64_t *d_path_len,
+              void *stream);
+
+/* Cost matrix C = 1 - L2norm(F1)^T L2norm(F2) (FlexDTW.py:1207-1210,
+ * Parflex.py:1470; gpu_flexdtw.py:95-109).  F1 (K x N), F2 (K x M) f64
+ * row-major.  out_dtype PF_F64: fixed k order, no FMA contraction
+ * (bit-stable); PF_F32 / PF_BF16: MFMA-tiled BF16 product with FP32
+ * accumulation, written as f32 or bf16.  d_C row stride ldc.
+ * d_work: f64 scratch of K*(N+M) elements (normalised features). */
+int pf_cost(int out_dtype, const double *d_F1, const double *d_F2, int64_t K,
+            int64_t N, int64_t M, void *d_C, int64_t ldc, double *d_work,
+            void *stream);
+
+/* Fill helpers used instead of hipMemsetAsync (safe under graph capture). */
+int pf_fill_f64(double *d_p, double v, int64_t n, void *stream);
+int pf_fill_i64(int64_t *d_p, int64_t v, int64_t n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

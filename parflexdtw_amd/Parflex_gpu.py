@@ -1,0 +1,421 @@
+"""MI355X drop-in for the reference's ``Parflex_gpu`` module.
+
+Import it in place of the reference module:
+
+    from parflexdtw_amd import Parflex_gpu as Parflex_gpu
+    best_cost, wp = Parflex_gpu.parflex(C, steps, weights, beta, L)
+
+Entry points (same names, arguments, defaults and return values as
+Parflex_gpu.py / Parflex.py):
+
+    parflex(C, steps, weights, beta, L=None, profile_dir='Profiling_results',
+            backtrace_segments=False, warn_slow_chunks=False,
+            slow_chunk_seconds=1.0, use_gpu=None) -> (best_cost, wp (2, n))
+                                                   Parflex_gpu.py:656-854
+    tiled_stage1_from_features(F1, F2, steps=None, weights=None, beta=0.1,
+            L=None, use_gpu=None) -> (C or None, tiled_result)
+                                                   Parflex_gpu.py:610-653
+    run_stage2_from_tiled(tiled_result, C=None, beta=0.1, show_fig=False,
+            top_k=1, profile_dir=None, backtrace_segments=False) -> dict
+                                                   Parflex_gpu.py:857-910
+    run_flexdtw_on_tiles(C, L, steps=None, weights=None, buffer=1, ...,
+            use_gpu=None, C_dev=None) -> (chunks_dict, L, n1, n2)
+                                                   Parflex_gpu.py:342-607
+
+Everything runs on the GPU: stage 1 is one launch over all tiles, stage 2
+(propagation, sync, scan, stitch, backtrace) runs on the device, and
+parflex() copies back only the best cost and the path.  There is no CPU
+implementation in this package: use_gpu=False, or PARFLEX_DISABLE_GPU set with
+use_gpu=None, raises RuntimeError (the reference would fall back to its Numba
+code, which this package does not ship).
+"""
+from __future__ import annotations
+
+import csv
+import math
+import os
+import time
+
+import numpy as np
+
+from . import engine
+from ._plan import BatchPlan, buffer_global, tile_bounds, tile_grid
+from .engine import NO_ENDPOINT_MSG, StepWeights
+from .gpu_flexdtw import (cost_matrix_from_features_gpu, cost_matrix_to_gpu,
+                          cost_matrix_to_gpu_f32, flexdtw_chunk_from_global_C,
+                          gather_c_at_global_indices, is_gpu_available)
+
+__all__ = ["parflex", "tiled_stage1_from_features", "run_stage2_from_tiled",
+           "run_flexdtw_on_tiles", "build_tiled_metadata", "edge_slot_to_local",
+           "local_to_global_cell", "global_cell_to_prev_tile_edge", "DEFAULT_CHUNK_LENGTH",
+           "cost_matrix_from_features_gpu", "cost_matrix_to_gpu_f32",
+           "flexdtw_chunk_from_global_C", "gather_c_at_global_indices", "is_gpu_available"]
+
+DEFAULT_CHUNK_LENGTH = 4000  # Parflex.py:126 (the effective value)
+_DEFAULT_STEPS = [(1, 1), (1, 2), (2, 1)]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _resolve_use_gpu(use_gpu):
+    """Parflex_gpu.py:391-392 / :617 / :691 semantics, minus the CPU fallback."""
+    if use_gpu is None:
+        use_gpu = os.environ.get("PARFLEX_DISABLE_GPU", "").strip() == ""
+    if not use_gpu:
+        raise RuntimeError(
+            "parflexdtw_amd runs ParFlex on the GPU only; use_gpu=False (or "
+            "PARFLEX_DISABLE_GPU) asks for the reference's CPU implementation")
+    if not is_gpu_available():
+        raise RuntimeError("no ROCm GPU is available (or libparflex_hip.so failed to load)")
+    return True
+
+
+def _sync():
+    _torch().cuda.synchronize()
+
+
+# --------------------------------------------------------------------------
+# small helpers of Parflex.py kept for API compatibility
+# --------------------------------------------------------------------------
+
+def edge_slot_to_local(edge_type, position, chunk_shape):
+    """Parflex.py:139-147."""
+    if edge_type == 0:
+        return chunk_shape[0] - 1, position
+    return position, chunk_shape[1] - 1
+
+
+def local_to_global_cell(chunk_i, chunk_j, local_row, local_col, chunks_dict):
+    """Parflex.py:482-489."""
+    s1, _, s2, _ = chunks_dict[(chunk_i, chunk_j)]["bounds"]
+    return s1 + local_row, s2 + local_col
+
+
+def global_cell_to_prev_tile_edge(global_row, global_col, prev_i, prev_j, chunks_dict, L):
+    """Parflex.py:492-506."""
+    s1, _, s2, _ = chunks_dict[(prev_i, prev_j)]["bounds"]
+    lr, lc = global_row - s1, global_col - s2
+    shape = chunks_dict[(prev_i, prev_j)]["shape"]
+    if lr == shape[0] - 1:
+        return 0, lc
+    if lc == shape[1] - 1:
+        return 1, lr
+    raise ValueError(f"({lr},{lc}) not on edge of prev chunk")
+
+
+def _edge_starts_from_S(S):
+    """Parflex.py:347-373: start sets from a tile's last row and column."""
+    S = np.asarray(S)
+    vals = np.concatenate([S[-1, :], S[:, -1]]).astype(np.int64)
+    bot = set(int(v) for v in vals[vals > 0])
+    left = set(int(-v) for v in vals[vals <= 0])
+    return bot, left
+
+
+def build_tiled_metadata(chunks_dict, L, n_chunks_1, n_chunks_2, C, stage1_params=None):
+    """Parflex.py:1360-1416."""
+    if C is None:
+        L1 = max(ch["bounds"][1] for ch in chunks_dict.values())
+        L2 = max(ch["bounds"][3] for ch in chunks_dict.values())
+    else:
+        L1, L2 = C.shape
+    blocks = []
+    for (i, j), ch in chunks_dict.items():
+        s1, e1, s2, e2 = ch["bounds"]
+        if np.asarray(ch["wp"]).size == 0:
+            continue
+        blocks.append({"bi": i, "bj": j, "rows": (int(s1), int(e1)), "cols": (int(s2), int(e2)),
+                       "Ck_shape": tuple(ch["shape"]), "S_single": ch["S"]})
+    if stage1_params is None:
+        stage1_params = {"steps": np.array([[1, 1], [1, 2], [2, 1]], dtype=int),
+                         "weights": np.array([1.5, 3.0, 3.0], dtype=float), "buffer": 1.0}
+    return {"C_shape": (L1, L2), "L_block": L, "hop": L - 1, "n_row": n_chunks_1,
+            "n_col": n_chunks_2, "blocks": blocks, "C": C, "stage1_params": stage1_params}
+
+
+def _write_rows(profile_dir, name, header, rows):
+    if profile_dir is None:
+        return
+    os.makedirs(profile_dir, exist_ok=True)
+    with open(os.path.join(profile_dir, name), "w", newline="") as fp:
+        w = csv.writer(fp)
+        w.writerow(header)
+        w.writerows(rows)
+
+
+# --------------------------------------------------------------------------
+# stage 1 with full per-tile outputs (API parity path)
+# --------------------------------------------------------------------------
+
+def run_flexdtw_on_tiles(C, L, steps=None, weights=None, buffer=1, profile_dir="Profiling_results",
+                         warn_slow_chunks=False, slow_chunk_seconds=1.0, use_gpu=None, C_dev=None):
+    """Stage 1 with the reference's per-tile dictionary (Parflex_gpu.py:342-607).
+
+    Each tile's full D, S (=P), B come back to the host, as in the reference;
+    parflex() does not use this path (it keeps tile interiors on the device)."""
+    steps = _DEFAULT_STEPS if steps is None else steps
+    weights = [2, 3, 3] if weights is None else weights
+    _resolve_use_gpu(use_gpu)
+    if C is None:
+        if C_dev is None:
+            raise ValueError("run_flexdtw_on_tiles: provide NumPy C and/or C_dev (both None)")
+        L1, L2 = int(C_dev.shape[0]), int(C_dev.shape[1])
+    else:
+        L1, L2 = C.shape
+        if C_dev is not None and tuple(C.shape) != tuple(C_dev.shape):
+            raise ValueError("C_dev shape must match C")
+    dev_buf = C_dev if C_dev is not None else cost_matrix_to_gpu(C)
+    hop = L - 1
+    n1, n2 = tile_grid(L1, L2, L)
+    chunks = {}
+    rows = []
+    for i in range(n1):
+        for j in range(n2):
+            t0 = time.perf_counter()
+            s1, e1, s2, e2 = tile_bounds(i, j, L1, L2, L)
+            bc, wp, D, P, B, debug = flexdtw_chunk_from_global_C(
+                dev_buf, int(L2), s1, e1, s2, e2, steps, weights, buffer=buffer)
+            bot, left = _edge_starts_from_S(P)
+            chunks[(i, j)] = {
+                "C": None if C is None else C[s1:e1, s2:e2], "D": D, "S": P, "B": B,
+                "debug": debug, "best_cost": bc, "wp": wp, "bounds": (s1, e1, s2, e2),
+                "hop": (hop if e1 < L1 else L1 - s1, hop if e2 < L2 else L2 - s2),
+                "shape": (e1 - s1, e2 - s2), "starts_bot_edge": bot, "starts_left_edge": left,
+            }
+            el = time.perf_counter() - t0
+            rows.append([i, j, t0, t0 + el, el])
+            if warn_slow_chunks and el > slow_chunk_seconds:
+                print(f"  Warning, chunk ({i},{j}) took {el} seconds")
+    _write_rows(profile_dir, "chunk_flexdtw.csv",
+                ["chunk_i", "chunk_j", "start_time", "end_time", "elapsed_seconds"], rows)
+    return chunks, L, n1, n2
+
+
+# --------------------------------------------------------------------------
+# device pipeline
+# --------------------------------------------------------------------------
+
+class _DeviceTiles:
+    """Stage-1 state kept on the GPU between tiled_stage1_from_features and
+    run_stage2_from_tiled: the plan, workspace, step set and C."""
+
+    def __init__(self, C_dev, plan, sw, ws):
+        self.C_dev, self.plan, self.sw, self.ws = C_dev, plan, sw, ws
+
+
+def _stage1_device(C_dev, L, steps, weights, beta):
+    L1, L2 = int(C_dev.shape[0]), int(C_dev.shape[1])
+    plan = BatchPlan([(L1, L2)], int(L), beta=float(beta))
+    sw = StepWeights(steps, weights)
+    ws = plan.allocate(C_dev.device)
+    engine.stage1(C_dev, plan, sw, ws)
+    return _DeviceTiles(C_dev, plan, sw, ws)
+
+
+def _set_beta(dt, beta):
+    L1, L2 = int(dt.plan.pairs[0, 0]), int(dt.plan.pairs[0, 1])
+    buf = int(buffer_global(L1, L2, beta))
+    if dt.plan.pairs[0, 7] != buf:
+        dt.plan.pairs[0, 7] = buf
+        dt.ws["pairs"].copy_(_torch().from_numpy(dt.plan.pairs))
+
+
+def parflex(C, steps, weights, beta, L=None, profile_dir="Profiling_results",
+            backtrace_segments=False, warn_slow_chunks=False, slow_chunk_seconds=1.0,
+            use_gpu=None):
+    """End-to-end ParFlex on a cost matrix (Parflex_gpu.py:656-854).
+
+    C: host array (L1, L2) or a ROCm tensor.  Returns (best_cost, wp) with wp
+    int64 of shape (2, n); raises ValueError when no endpoint exists."""
+    _resolve_use_gpu(use_gpu)
+    L = DEFAULT_CHUNK_LENGTH if L is None else int(L)
+    t_all = time.perf_counter()
+    t0 = time.perf_counter()
+    C_dev = cost_matrix_to_gpu(C)
+    if profile_dir is not None:
+        _sync()
+    t_h2d = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dt = _stage1_device(C_dev, L, steps, weights, beta)
+    if profile_dir is not None:
+        _sync()
+    t_s1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    engine.stage2(C_dev, dt.plan, dt.sw, dt.ws)
+    res = engine.collect(dt.plan, dt.ws)
+    t_s2 = time.perf_counter() - t0
+    if profile_dir is not None:
+        total = time.perf_counter() - t_all
+        _write_rows(profile_dir, "parflex_end_to_end_phases.csv",
+                    ["phase", "elapsed_seconds", "had_device_C"],
+                    [["gpu_host_to_device_C", t_h2d, 1],
+                     ["stage1_chunk_flexdtw_all_tiles", t_s1, 1],
+                     ["stage2_propagate_sync_scan_stitch", t_s2, 1],
+                     ["parflex_total_runtime", total, 1]])
+    if not res.found[0]:
+        raise ValueError(NO_ENDPOINT_MSG)
+    wp = res.paths[0]
+    if wp.size == 0:
+        wp = np.array([[], []], dtype=np.int64)
+    return float(res.best_cost[0]), wp
+
+
+def tiled_stage1_from_features(F1, F2, steps=None, weights=None, beta=0.1, L=None, use_gpu=None):
+    """Features -> cost matrix on the GPU -> stage 1 (Parflex_gpu.py:610-653).
+
+    Returns (None, tiled_result); tiled_result['C_dev'] holds the device cost
+    matrix and tiled_result['chunks_dict'] materialises a tile's full D/S/B on
+    first access (they are not kept on the device by stage 1)."""
+    _resolve_use_gpu(use_gpu)
+    L = DEFAULT_CHUNK_LENGTH if L is None else int(L)
+    steps = np.array([[1, 1], [1, 2], [2, 1]]) if steps is None else steps
+    weights = np.array([1.25, 3.0, 3.0]) if weights is None else weights
+    C_dev = cost_matrix_from_features_gpu(F1, F2)
+    dt = _stage1_device(C_dev, L, steps, weights, beta)
+    n1, n2 = dt.plan.grids[0]
+    L1, L2 = int(C_dev.shape[0]), int(C_dev.shape[1])
+    stage1_params = {"steps": np.array(steps).reshape((-1, 2)), "weights": np.array(weights),
+                     "buffer": 1.0}
+    chunks = _LazyChunks(C_dev, L1, L2, L, n1, n2, steps, weights)
+    tiled = {"C_shape": (L1, L2), "L_block": L, "hop": L - 1, "n_row": n1, "n_col": n2,
+             "blocks": [], "C": None, "stage1_params": stage1_params, "chunks_dict": chunks,
+             "C_dev": C_dev, "_device": dt}
+    return None, tiled
+
+
+class _LazyChunks(dict):
+    """chunks_dict whose per-tile entries are computed on first access with
+    the full-output tile kernel (the reference keeps them all on the host)."""
+
+    def __init__(self, C_dev, L1, L2, L, n1, n2, steps, weights):
+        super().__init__()
+        self._args = (C_dev, L1, L2, L, steps, weights)
+        self._keys = [(i, j) for i in range(n1) for j in range(n2)]
+
+    def _fill(self, key):
+        C_dev, L1, L2, L, steps, weights = self._args
+        i, j = key
+        s1, e1, s2, e2 = tile_bounds(i, j, L1, L2, L)
+        bc, wp, D, P, B, debug = flexdtw_chunk_from_global_C(C_dev, L2, s1, e1, s2, e2, steps,
+                                                             weights, buffer=1)
+        bot, left = _edge_starts_from_S(P)
+        hop = L - 1
+        dict.__setitem__(self, key, {
+            "C": None, "D": D, "S": P, "B": B, "debug": debug, "best_cost": bc, "wp": wp,
+            "bounds": (s1, e1, s2, e2),
+            "hop": (hop if e1 < L1 else L1 - s1, hop if e2 < L2 else L2 - s2),
+            "shape": (e1 - s1, e2 - s2), "starts_bot_edge": bot, "starts_left_edge": left})
+
+    def __getitem__(self, key):
+        if not dict.__contains__(self, key):
+            if key not in self._keys:
+                raise KeyError(key)
+            self._fill(key)
+        return dict.__getitem__(self, key)
+
+    def __contains__(self, key):
+        return key in self._keys
+
+    def __iter__(self):
+        return iter(self._keys)
+
+    def __len__(self):
+        return len(self._keys)
+
+    def keys(self):
+        return list(self._keys)
+
+    def items(self):
+        return [(k, self[k]) for k in self._keys]
+
+    def values(self):
+        return [self[k] for k in self._keys]
+
+
+def _stage2_dict(dt, backtrace_segments):
+    """Assemble the stage-2 result dict of Parflex.py:1342-1356 from device
+    buffers."""
+    plan, ws, sw = dt.plan, dt.ws, dt.sw
+    L = plan.L
+    n1, n2 = plan.grids[0]
+    L1, L2 = int(plan.pairs[0, 0]), int(plan.pairs[0, 1])
+    res = engine.collect(plan, ws)
+    if not res.found[0]:
+        raise ValueError(NO_ENDPOINT_MSG)
+    D_arr = ws["D_arr"].cpu().numpy().reshape(n1, n2, 2, L)
+    L_arr = ws["L_arr"].cpu().numpy().reshape(n1, n2, 2, L)
+    edge_lens = plan.tiles[:, [3, 2]].reshape(n1, n2, 2).copy()
+    candD = ws["cand_D"].cpu().numpy()
+    candL = ws["cand_L"].cpu().numpy()
+    top_D, top_L = candD[:L2].copy(), candL[:L2].copy()
+    right_D, right_L = candD[L2:L2 + L1].copy(), candL[L2:L2 + L1].copy()
+    top_norm = np.full(L2, np.nan)
+    right_norm = np.full(L1, np.nan)
+    m = (top_L > 0) & np.isfinite(top_D)
+    top_norm[m] = top_D[m] / top_L[m]
+    m = (right_L > 0) & np.isfinite(right_D)
+    right_norm[m] = right_D[m] / right_L[m]
+    segf = ws["seg_f"].cpu().numpy().reshape(-1, 4)
+    segi = ws["seg_i"].cpu().numpy().reshape(-1, 8)
+    max_segs = int(plan.pairs[0, 10])
+    n_segs = (-(-L2 // L), -(-L1 // L))
+    names = ("top", "right")
+    best_per_segment = {}
+    for side in range(2):
+        for s in range(n_segs[side]):
+            k = side * max_segs + s
+            if segi[k, 6] != 1:
+                continue
+            key = (names[side], s)
+            best_per_segment[key] = {
+                "chunk_i": int(segi[k, 0]), "chunk_j": int(segi[k, 1]), "edge": int(segi[k, 2]),
+                "idx": int(segi[k, 3]), "norm_cost": float(segf[k, 0]),
+                "global_coord": (int(segi[k, 4]), int(segi[k, 5])), "segment": key}
+    paths_per_segment = {}
+    if backtrace_segments and best_per_segment:
+        keys = list(best_per_segment)
+        req = [[0, best_per_segment[k]["chunk_i"], best_per_segment[k]["chunk_j"],
+                best_per_segment[k]["edge"], best_per_segment[k]["idx"]] for k in keys]
+        paths = engine.stitch_requests(plan, sw, ws, req)
+        for k, p in zip(keys, paths):
+            paths_per_segment[k] = {"endpoint": best_per_segment[k],
+                                    "path": [(int(a), int(b)) for a, b in p]}
+    stitched = res.paths[0].T.copy() if res.paths[0].size else np.zeros((0, 2), dtype=int)
+    be = res.best_end[0]
+    return {
+        "D_arr": D_arr, "L_arr": L_arr, "edge_lens": edge_lens,
+        "best_cost": float(res.best_cost[0]), "best_end": tuple(int(v) for v in be),
+        "stitched_wp": stitched, "n_row": n1, "n_col": n2,
+        "edge_summary": {"top": {"D": top_D, "L": top_L, "norm": top_norm},
+                         "right": {"D": right_D, "L": right_L, "norm": right_norm}},
+        "best_per_segment": best_per_segment, "paths_per_segment": paths_per_segment,
+    }
+
+
+def run_stage2_from_tiled(tiled_result, C=None, beta=0.1, show_fig=False, top_k=1,
+                          profile_dir=None, backtrace_segments=False):
+    """Stage 2 on the device (Parflex_gpu.py:857-910); returns the dict of
+    Parflex.stage2_scan_and_stitch (Parflex.py:1342-1356)."""
+    dt = tiled_result.get("_device")
+    if dt is None:
+        C_dev = tiled_result.get("C_dev")
+        if C_dev is None:
+            if C is None:
+                C = tiled_result.get("C")
+            if C is None:
+                raise ValueError("run_stage2_from_tiled needs C, C_dev or a device stage 1")
+            C_dev = cost_matrix_to_gpu(C)
+        p = tiled_result["stage1_params"]
+        dt = _stage1_device(C_dev, tiled_result["L_block"], p["steps"], p["weights"], beta)
+        tiled_result["_device"] = dt
+    _set_beta(dt, beta)
+    t0 = time.perf_counter()
+    engine.stage2(dt.C_dev, dt.plan, dt.sw, dt.ws)
+    out = _stage2_dict(dt, backtrace_segments)
+    if profile_dir is not None:
+        _write_rows(profile_dir, "stage_2_backtrace_compatible.csv",
+                    ["phase", "elapsed_seconds"], [["stage2_total", time.perf_counter() - t0]])
+    return out

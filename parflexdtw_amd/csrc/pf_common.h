@@ -1,0 +1,65 @@
+// Shared definitions of the gfx950 ParFlex kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <string>
+
+#include "parflex_hip.h"
+
+namespace pf {
+
+// Thread-local last-error text behind pf_last_error().
+void set_error(const std::string &msg);
+
+#define PF_CHECK_HIP(expr)                                                        \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess) {                                                       \
+      ::pf::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));         \
+      return PF_EHIP;                                                             \
+    }                                                                             \
+  } while (0)
+
+#define PF_REQUIRE(cond, msg)                                                     \
+  do {                                                                            \
+    if (!(cond)) {                                                                \
+      ::pf::set_error(msg);                                                       \
+      return PF_EINVAL;                                                           \
+    }                                                                             \
+  } while (0)
+
+// Step set passed by value in kernel arguments (uniform, lives in SGPRs).
+struct StepSet {
+  int K;
+  int dr[4];
+  int dc[4];
+  double w[4];
+};
+
+int make_stepset(const int32_t *steps, const double *weights, int K, StepSet *out);
+
+// Tile table row (int64[8]), see parflex_hip.h.
+struct TileDesc {
+  int64_t c_off, ldc, R, W, b_off, e_off, pair, ij;
+};
+static_assert(sizeof(TileDesc) == 64, "tile table rows are int64[8]");
+
+// Pair table row (int64[16]), see parflex_hip.h.
+struct PairDesc {
+  int64_t L1, L2, n1, n2, tile_base, c_off, ldc, buf, lk_off, seg_off, max_segs,
+      path_off, r12, r13, r14, r15;
+};
+static_assert(sizeof(PairDesc) == 128, "pair table rows are int64[16]");
+
+__device__ __forceinline__ double to_f64(double v) { return v; }
+__device__ __forceinline__ double to_f64(float v) { return (double)v; }
+__device__ __forceinline__ double to_f64(__hip_bfloat16 v) { return (double)__bfloat162float(v); }
+__device__ __forceinline__ float to_f32(float v) { return v; }
+__device__ __forceinline__ float to_f32(__hip_bfloat16 v) { return __bfloat162float(v); }
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace pf

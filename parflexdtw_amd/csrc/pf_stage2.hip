@@ -1,0 +1,817 @@
+// Stage 2 of ParFlex on gfx950: edge-cost propagation across tiles, overlap
+// sync, global boundary scan, stitch and backtrace -- all on the device, for
+// a batch of pairs sharing one tile length L.
+//
+// Semantics follow Parflex.py exactly (file:line at each kernel).  Layout:
+// every per-tile quantity lives in slot t*2L + e*L + pos of a tile-indexed
+// array (e = 0: tile's last row indexed by column, e = 1: last column indexed
+// by row), so a pair's (n1, n2, 2, L) arrays of the reference are contiguous
+// slices starting at tile_base*2L.
+//
+// Parallel structure:
+//  * edge data, start sets and overlap costs: one thread per edge slot;
+//  * init/fill (Parflex.py:631-855): tile (i,j) reads only tiles (i-1,j) and
+//    (i,j-1), so the i-major sweep of the reference equals a sweep over tile
+//    anti-diagonals; one launch per anti-diagonal, one thread per slot;
+//  * sync (Parflex.py:919-969): each of the three passes touches disjoint
+//    slot pairs (edges are >= 2 long), so each pass is one parallel launch;
+//  * scan (Parflex.py:1153-1222): one workgroup per (pair, side, segment of L
+//    positions) reduces (norm, scan order) lexicographically -- the strict '<'
+//    of the sequential scan keeps the first minimum -- then one workgroup per
+//    pair reduces the segment winners (global threshold 1e9);
+//  * stitch (Parflex.py:1032-1103): one thread per request walks the <= 101
+//    tiles (only edge P is needed), then one wave per segment walks the
+//    packed back-pointers through an LDS window, and one workgroup per request
+//    concatenates, removes the duplicated overlap cells and reverses.
+#include "pf_common.h"
+
+namespace pf {
+
+struct Geo {
+  int64_t s1, s2, rows, cols;
+};
+
+__device__ __forceinline__ Geo tile_geo(const PairDesc &pd, int64_t L, int64_t i, int64_t j) {
+  const int64_t hop = L - 1;
+  Geo g;
+  g.s1 = i * hop;
+  g.s2 = j * hop;
+  const int64_t e1 = g.s1 + L < pd.L1 ? g.s1 + L : pd.L1;
+  const int64_t e2 = g.s2 + L < pd.L2 ? g.s2 + L : pd.L2;
+  g.rows = e1 - g.s1;
+  g.cols = e2 - g.s2;
+  return g;
+}
+
+__device__ __forceinline__ double c_at(int c_dtype, const void *C, int64_t idx) {
+  if (c_dtype == PF_F64) return static_cast<const double *>(C)[idx];
+  if (c_dtype == PF_F32) return (double)static_cast<const float *>(C)[idx];
+  return (double)__bfloat162float(static_cast<const __hip_bfloat16 *>(C)[idx]);
+}
+
+__device__ __forceinline__ int64_t iabs64(int64_t v) { return v < 0 ? -v : v; }
+
+__global__ void k_fill_f64(double *p, double v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_i64(int64_t *p, int64_t v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// _build_edge_data (Parflex.py:509-564) + _edge_starts_from_S
+// (Parflex.py:347-373): overlap cost C[start] per slot, start-set bitmaps of
+// every tile (bit s of map 0: some edge cell starts at (0, s), s > 0; bit a of
+// map 1: some edge cell starts at (a, 0), a >= 0), and D_arr = L_arr = inf.
+__global__ void k_s2_prep(int c_dtype, const void *__restrict__ C, const PairDesc *__restrict__ pairs,
+                          const TileDesc *__restrict__ tiles, int64_t ntiles, int64_t L,
+                          const int32_t *__restrict__ eP, double *__restrict__ Darr,
+                          double *__restrict__ Larr, double *__restrict__ olap,
+                          uint32_t *__restrict__ bits, int64_t bitw) {
+  const int64_t n = ntiles * 2 * L;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = q / (2 * L);
+    const int64_t rem = q - t * 2 * L;
+    const int e = (int)(rem / L);
+    const int64_t pos = rem - (int64_t)e * L;
+    Darr[q] = __builtin_huge_val();
+    Larr[q] = __builtin_huge_val();
+    const TileDesc td = tiles[t];
+    const int64_t elen = e == 0 ? td.W : td.R;
+    if (pos >= elen) {
+      olap[q] = 0.0;
+      continue;
+    }
+    const PairDesc pd = pairs[td.pair];
+    const int64_t s = eP[q];
+    const int64_t sr = s < 0 ? -s : 0, sc = s > 0 ? s : 0;
+    const int64_t ti = td.ij >> 32, tj = td.ij & 0xffffffffll;
+    const Geo g = tile_geo(pd, L, ti, tj);
+    olap[q] = c_at(c_dtype, C, pd.c_off + (g.s1 + sr) * pd.ldc + (g.s2 + sc));
+    if (s > 0) {
+      atomicOr(bits + (t * 2 + 0) * bitw + (s >> 5), 1u << (s & 31));
+    } else {
+      const int64_t a = -s;
+      atomicOr(bits + (t * 2 + 1) * bitw + (a >> 5), 1u << (a & 31));
+    }
+  }
+}
+
+__device__ __forceinline__ bool bit_at(const uint32_t *bits, int64_t bitw, int64_t t, int map,
+                                       int64_t pos) {
+  return (bits[(t * 2 + map) * bitw + (pos >> 5)] >> (pos & 31)) & 1u;
+}
+
+// One tile anti-diagonal d of _nb_initialize_chunks_sparse (Parflex.py:631-760)
+// and _nb_dp_fill_chunks_sparse (Parflex.py:763-855), with the valid mask of
+// _build_valid_mask (Parflex.py:582-624) evaluated per slot.
+__global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs, int64_t L,
+                          const double *__restrict__ eD, const int32_t *__restrict__ eP,
+                          const double *__restrict__ olap, const uint32_t *__restrict__ bits,
+                          int64_t bitw, double *__restrict__ Darr, double *__restrict__ Larr) {
+  const PairDesc pd = pairs[blockIdx.y];
+  const int64_t n1 = pd.n1, n2 = pd.n2;
+  if (d > n1 + n2 - 2) return;
+  const int64_t i0 = d - (n2 - 1) > 0 ? d - (n2 - 1) : 0;
+  const int64_t i1 = d < n1 - 1 ? d : n1 - 1;
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t tq = q / (2 * L);
+  if (i0 + tq > i1) return;
+  const int64_t rem = q - tq * 2 * L;
+  const int e = (int)(rem / L);
+  const int64_t pos = rem - (int64_t)e * L;
+  const int64_t i = i0 + tq, j = d - i;
+  const Geo g = tile_geo(pd, L, i, j);
+  const int64_t elen = e == 0 ? g.cols : g.rows;
+  if (pos >= elen) return;
+  const int64_t t = pd.tile_base + i * n2 + j;
+  const int64_t slot = (t * 2 + e) * L + pos;
+
+  bool valid;
+  if (e == 0) {
+    valid = (i == n1 - 1) || pos == 0 || pos == g.cols - 1 ||
+            bit_at(bits, bitw, t + n2, 0, pos);
+  } else {
+    valid = (j == n2 - 1) || pos == 0 || pos == g.rows - 1 ||
+            bit_at(bits, bitw, t + 1, 1, pos);
+  }
+
+  const int64_t lr = e == 0 ? g.rows - 1 : pos, lc = e == 0 ? pos : g.cols - 1;
+  const int64_t s = eP[slot];
+  const int64_t sr = s < 0 ? -s : 0, sc = s > 0 ? s : 0;
+  const double seg_len = (double)(iabs64(lr - sr) + iabs64(lc - sc));
+
+  // Chain from predecessor tile (pi, pj) as in Parflex.py:683-707/734-758/824-853.
+  auto chain = [&](int64_t pi, int64_t pj) {
+    const Geo pg = tile_geo(pd, L, pi, pj);
+    const int64_t plr = (g.s1 + sr) - pg.s1, plc = (g.s2 + sc) - pg.s2;
+    int pedge;
+    int64_t ppos, pelen;
+    if (plr == pg.rows - 1) {
+      pedge = 0;
+      ppos = plc;
+      pelen = pg.cols;
+    } else if (plc == pg.cols - 1) {
+      pedge = 1;
+      ppos = plr;
+      pelen = pg.rows;
+    } else {
+      return;
+    }
+    if (ppos < 0 || ppos >= pelen) return;
+    const int64_t pslot = ((pd.tile_base + pi * n2 + pj) * 2 + pedge) * L + ppos;
+    const double prev = Darr[pslot];
+    if (!isfinite(prev)) return;
+    Darr[slot] = (prev + eD[slot]) - olap[slot];
+    Larr[slot] = Larr[pslot] + seg_len;
+  };
+
+  if (i == 0 && j == 0) {
+    if (valid && (sr == 0 || sc == 0)) {
+      Darr[slot] = eD[slot];
+      Larr[slot] = seg_len;
+    }
+    return;
+  }
+  if (i == 0) {
+    if (pos == 0) {
+      if (e == 0) {
+        const Geo pg = tile_geo(pd, L, 0, j - 1);
+        const int64_t pslot = ((pd.tile_base + (j - 1)) * 2 + 0) * L + pg.cols - 1;
+        Darr[slot] = Darr[pslot];
+        Larr[slot] = Larr[pslot];
+      }
+      return;
+    }
+    if (!valid) return;
+    if (sr == 0) {
+      Darr[slot] = eD[slot];
+      Larr[slot] = seg_len;
+    } else if (sc == 0) {
+      chain(0, j - 1);
+    }
+    return;
+  }
+  if (j == 0) {
+    if (pos == 0) {
+      if (e == 1) {
+        const Geo pg = tile_geo(pd, L, i - 1, 0);
+        const int64_t pslot = ((pd.tile_base + (i - 1) * n2) * 2 + 1) * L + pg.rows - 1;
+        Darr[slot] = Darr[pslot];
+        Larr[slot] = Larr[pslot];
+      }
+      return;
+    }
+    if (!valid) return;
+    if (sc == 0) {
+      Darr[slot] = eD[slot];
+      Larr[slot] = seg_len;
+    } else if (sr == 0) {
+      chain(i - 1, 0);
+    }
+    return;
+  }
+  if (!valid) return;
+  if (pos == 0) {
+    int64_t pslot;
+    if (e == 0) {
+      const Geo pg = tile_geo(pd, L, i, j - 1);
+      pslot = ((t - 1) * 2 + 0) * L + pg.cols - 1;
+    } else {
+      const Geo pg = tile_geo(pd, L, i - 1, j);
+      pslot = ((t - n2) * 2 + 1) * L + pg.rows - 1;
+    }
+    const double v = Darr[pslot];
+    if (isfinite(v)) {
+      Darr[slot] = v;
+      Larr[slot] = Larr[pslot];
+    }
+    return;
+  }
+  if (sr == 0) {
+    chain(i - 1, j);
+  } else if (sc == 0) {
+    chain(i, j - 1);
+  } else {
+    Darr[slot] = eD[slot];
+    Larr[slot] = seg_len;
+  }
+}
+
+// sync_tile_overlap_edges (Parflex.py:919-969), pass `pass` (0, 1, 2).
+__global__ void k_s2_sync(int pass, const PairDesc *__restrict__ pairs, int64_t L,
+                          double *__restrict__ Darr, double *__restrict__ Larr) {
+  const PairDesc pd = pairs[blockIdx.y];
+  const int64_t n1 = pd.n1, n2 = pd.n2;
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= n1 * n2) return;
+  const int64_t i = q / n2, j = q - i * n2;
+  const int64_t t = pd.tile_base + q;
+  const Geo g = tile_geo(pd, L, i, j);
+  int64_t a, b;  // slot a is the tile's own, b the other one
+  if (pass == 0) {
+    if (j + 1 >= n2) return;
+    a = (t * 2 + 0) * L + g.cols - 1;
+    b = ((t + 1) * 2 + 0) * L;
+  } else if (pass == 1) {
+    if (i + 1 >= n1) return;
+    a = (t * 2 + 1) * L + g.rows - 1;
+    b = ((t + n2) * 2 + 1) * L;
+  } else {
+    a = (t * 2 + 0) * L + g.cols - 1;  // corner seen by the top edge
+    b = (t * 2 + 1) * L + g.rows - 1;  // corner seen by the right edge
+  }
+  const double da = Darr[a], db = Darr[b];
+  const bool fa = isfinite(da), fb = isfinite(db);
+  if (fa && !fb) {
+    Darr[b] = da;
+    Larr[b] = Larr[a];
+  } else if (!fa && fb) {
+    Darr[a] = db;
+    Larr[a] = Larr[b];
+  } else if (pass == 2 && fa && fb && fabs(da - db) > 1e-10) {
+    Darr[b] = da;
+    Larr[b] = Larr[a];
+  }
+}
+
+// Lexicographic (value, order) minimum; order < 0 means "no candidate".
+__device__ __forceinline__ bool better(double v, int64_t o, double bv, int64_t bo) {
+  if (o < 0) return false;
+  if (bo < 0) return true;
+  return v < bv || (v == bv && o < bo);
+}
+
+// _nb_scan_edges (Parflex.py:1153-1222) for one (pair, side, segment): the
+// owning slot of global top cell (L1-1, g) is tile (n1-1, (g-1)//hop) edge 0
+// (tile 0 for g = 0), of right cell (g, L2-1) tile ((g-1)//hop, n2-1) edge 1:
+// the closed form of _build_edge_lookups (Parflex.py:1110-1150).
+__global__ __launch_bounds__(256) void k_s2_scan(const PairDesc *__restrict__ pairs, int64_t L,
+                                                 const double *__restrict__ Darr,
+                                                 const double *__restrict__ Larr,
+                                                 double *__restrict__ candD,
+                                                 double *__restrict__ candL,
+                                                 double *__restrict__ segf,
+                                                 int64_t *__restrict__ segi) {
+  __shared__ double sv[256];
+  __shared__ int64_t so[256];
+  const PairDesc pd = pairs[blockIdx.z];
+  const int side = blockIdx.y;
+  const int64_t seg = blockIdx.x;
+  if (seg >= pd.max_segs) return;
+  const int64_t n = side == 0 ? pd.L2 : pd.L1;
+  const int64_t hop = L - 1;
+  const int64_t g0 = seg * L;
+  const int64_t g1 = g0 + L < n ? g0 + L : n;
+  double bv = __builtin_huge_val();
+  int64_t bo = -1;
+  for (int64_t g = g0 + threadIdx.x; g < g1; g += blockDim.x) {
+    const int64_t cslot = pd.lk_off + (side == 0 ? g : pd.L2 + g);
+    candD[cslot] = __builtin_nan("");
+    candL[cslot] = 0.0;
+    if (pd.buf > 0 && g < pd.buf) continue;
+    int64_t ci, cj, idx;
+    if (side == 0) {
+      ci = pd.n1 - 1;
+      cj = g == 0 ? 0 : (g - 1) / hop;
+      idx = g - cj * hop;
+    } else {
+      ci = g == 0 ? 0 : (g - 1) / hop;
+      cj = pd.n2 - 1;
+      idx = g - ci * hop;
+    }
+    const int64_t slot = ((pd.tile_base + ci * pd.n2 + cj) * 2 + side) * L + idx;
+    const double D = Darr[slot], Lv = Larr[slot];
+    if (!isfinite(D) || Lv <= 0.0) continue;
+    if (side == 0 && D >= 1e9) continue;
+    const double norm = D / Lv;
+    candD[cslot] = D;
+    candL[cslot] = Lv;
+    if (better(norm, g, bv, bo)) {
+      bv = norm;
+      bo = g;
+    }
+  }
+  sv[threadIdx.x] = bv;
+  so[threadIdx.x] = bo;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s && better(sv[threadIdx.x + s], so[threadIdx.x + s], sv[threadIdx.x],
+                                       so[threadIdx.x])) {
+      sv[threadIdx.x] = sv[threadIdx.x + s];
+      so[threadIdx.x] = so[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int64_t k = pd.seg_off + side * pd.max_segs + seg;
+    const int64_t g = so[0];
+    double *f = segf + k * 4;
+    int64_t *ii = segi + k * 8;
+    if (g < 0) {
+      f[0] = __builtin_huge_val();
+      f[1] = __builtin_huge_val();
+      f[2] = 0.0;
+      f[3] = -1.0;
+      for (int z = 0; z < 8; ++z) ii[z] = -1;
+      ii[6] = 0;
+      return;
+    }
+    int64_t ci, cj, idx;
+    if (side == 0) {
+      ci = pd.n1 - 1;
+      cj = g == 0 ? 0 : (g - 1) / hop;
+      idx = g - cj * hop;
+    } else {
+      ci = g == 0 ? 0 : (g - 1) / hop;
+      cj = pd.n2 - 1;
+      idx = g - ci * hop;
+    }
+    const int64_t slot = ((pd.tile_base + ci * pd.n2 + cj) * 2 + side) * L + idx;
+    const int64_t order = side == 0 ? g : pd.L2 + g;
+    f[0] = sv[0];
+    f[1] = Darr[slot];
+    f[2] = Larr[slot];
+    f[3] = (double)order;
+    ii[0] = ci;
+    ii[1] = cj;
+    ii[2] = side;
+    ii[3] = idx;
+    ii[4] = side == 0 ? pd.L1 - 1 : g;
+    ii[5] = side == 0 ? g : pd.L2 - 1;
+    ii[6] = 1;
+    ii[7] = order;
+  }
+}
+
+// Global winner per pair: the first (in scan order) minimum among segment
+// winners below LARGE = 1e9 (Parflex.py:1167-1168).
+__global__ __launch_bounds__(256) void k_s2_best(const PairDesc *__restrict__ pairs,
+                                                 const double *__restrict__ segf,
+                                                 const int64_t *__restrict__ segi,
+                                                 double *__restrict__ bestf,
+                                                 int64_t *__restrict__ besti) {
+  __shared__ double sv[256];
+  __shared__ int64_t so[256];
+  const PairDesc pd = pairs[blockIdx.x];
+  double bv = 1e9;
+  int64_t bo = -1;
+  for (int64_t k = threadIdx.x; k < 2 * pd.max_segs; k += blockDim.x) {
+    const int64_t kk = pd.seg_off + k;
+    if (segi[kk * 8 + 6] != 1) continue;
+    const double v = segf[kk * 4];
+    if (!(v < 1e9)) continue;
+    const int64_t o = segi[kk * 8 + 7];
+    if (better(v, o, bv, bo)) {
+      bv = v;
+      bo = kk;  // remember the segment slot; orders are distinct across slots
+    }
+  }
+  // Reduce on (value, scan order).
+  sv[threadIdx.x] = bv;
+  so[threadIdx.x] = bo;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const int64_t oa = so[threadIdx.x], ob = so[threadIdx.x + s];
+      const int64_t ka = oa < 0 ? -1 : segi[oa * 8 + 7];
+      const int64_t kb = ob < 0 ? -1 : segi[ob * 8 + 7];
+      if (better(sv[threadIdx.x + s], kb, sv[threadIdx.x], ka)) {
+        sv[threadIdx.x] = sv[threadIdx.x + s];
+        so[threadIdx.x] = ob;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int64_t kk = so[0];
+    int64_t *o = besti + blockIdx.x * 8;
+    if (kk < 0) {
+      bestf[blockIdx.x] = 1e9;
+      for (int z = 0; z < 8; ++z) o[z] = -1;
+      o[6] = 0;
+      return;
+    }
+    bestf[blockIdx.x] = sv[0];
+    o[0] = segi[kk * 8 + 4];
+    o[1] = segi[kk * 8 + 5];
+    o[2] = segi[kk * 8 + 0];
+    o[3] = segi[kk * 8 + 1];
+    o[4] = segi[kk * 8 + 2];
+    o[5] = segi[kk * 8 + 3];
+    o[6] = 1;
+    o[7] = segi[kk * 8 + 7];
+  }
+}
+
+// Tile chain of backtrace_and_stitch (Parflex.py:1038-1101).  Request q is
+// (pair, ci, cj, et, idx) -- or, with from_best, pair q's winner.  Chain
+// entry s: (tile, end_r, end_c, start_r, start_c, s1, s2, npts).  The count
+// goes to cnt[q].
+__global__ void k_s2_chain(int from_best, const int64_t *__restrict__ req, int64_t nreq,
+                           const int64_t *__restrict__ besti, const PairDesc *__restrict__ pairs,
+                           int64_t L, const int32_t *__restrict__ eP, int64_t *__restrict__ chain,
+                           int64_t *__restrict__ cnt) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= nreq) return;
+  int64_t pair, ci, cj, et, idx;
+  if (from_best) {
+    pair = q;
+    if (besti[q * 8 + 6] != 1) {
+      cnt[q] = 0;
+      return;
+    }
+    ci = besti[q * 8 + 2];
+    cj = besti[q * 8 + 3];
+    et = besti[q * 8 + 4];
+    idx = besti[q * 8 + 5];
+  } else {
+    pair = req[q * 5];
+    ci = req[q * 5 + 1];
+    cj = req[q * 5 + 2];
+    et = req[q * 5 + 3];
+    idx = req[q * 5 + 4];
+  }
+  const PairDesc pd = pairs[pair];
+  int64_t *ch = chain + q * 101 * 8;
+  int64_t ns = 0;
+  for (int it = 0; it < 101; ++it) {
+    if (ci < 0 || cj < 0 || ci >= pd.n1 || cj >= pd.n2) break;
+    const Geo g = tile_geo(pd, L, ci, cj);
+    const int64_t t = pd.tile_base + ci * pd.n2 + cj;
+    int64_t er, ec;
+    if (et == 0) {
+      er = g.rows - 1;
+      ec = idx;
+    } else {
+      er = idx;
+      ec = g.cols - 1;
+    }
+    if (er < 0 || ec < 0 || er >= g.rows || ec >= g.cols) break;
+    const int64_t S = (er == g.rows - 1) ? eP[(t * 2 + 0) * L + ec] : eP[(t * 2 + 1) * L + er];
+    const int64_t sr = S >= 0 ? 0 : -S, sc = S >= 0 ? S : 0;
+    int64_t *en = ch + ns * 8;
+    en[0] = t;
+    en[1] = er;
+    en[2] = ec;
+    en[3] = sr;
+    en[4] = sc;
+    en[5] = g.s1;
+    en[6] = g.s2;
+    en[7] = 0;
+    ++ns;
+    const int64_t gsr = g.s1 + sr, gsc = g.s2 + sc;
+    if (gsr == 0 || gsc == 0) break;
+    const bool corner = (sr == 0 && sc == 0);
+    int64_t pi, pj, pe;
+    if (corner) {
+      pi = ci - 1;
+      pj = cj - 1;
+      pe = 0;
+    } else if (S >= 0) {
+      pi = ci - 1;
+      pj = cj;
+      pe = 0;
+    } else {
+      pi = ci;
+      pj = cj - 1;
+      pe = 1;
+    }
+    if (pi < 0 || pj < 0) break;
+    const Geo pg = tile_geo(pd, L, pi, pj);
+    int64_t pidx, pmax;
+    if (corner) {
+      pidx = pg.cols - 1;
+      pmax = pg.cols;
+    } else {
+      pidx = pe == 0 ? gsc - pg.s2 : gsr - pg.s1;
+      pmax = pe == 0 ? pg.cols : pg.rows;
+    }
+    if (pidx < 0 || pidx >= pmax) break;
+    ci = pi;
+    cj = pj;
+    et = pe;
+    idx = pidx;
+  }
+  cnt[q] = ns;
+}
+
+// _nb_backtrace_within_chunk (Parflex.py:976-1012) for chain segment
+// blockIdx.x of request blockIdx.y, one wave: all lanes stage a 64-row x
+// 64-column window of packed back-pointers in LDS, lane 0 walks inside it.
+__global__ __launch_bounds__(64) void k_s2_walk(const TileDesc *__restrict__ tiles, int64_t L,
+                                                StepSet ss, const uint32_t *__restrict__ Bp,
+                                                int64_t *__restrict__ chain,
+                                                const int64_t *__restrict__ cnt,
+                                                int64_t *__restrict__ pts) {
+  __shared__ uint32_t win[4][64];
+  __shared__ int64_t st[4];  // r, c, k, done
+  const int64_t q = blockIdx.y, s = blockIdx.x;
+  if (s >= cnt[q]) return;
+  int64_t *en = chain + (q * 101 + s) * 8;
+  const int64_t t = en[0];
+  const TileDesc td = tiles[t];
+  const int64_t rows = td.R, cols = td.W;
+  const int64_t sr = en[3], sc = en[4], gro = en[5], gco = en[6];
+  int64_t *out = pts + (q * 101 + s) * (2 * L + 8) * 2;
+  const int lane = threadIdx.x;
+  const int64_t max_iters = rows + cols;
+  if (lane == 0) {
+    st[0] = en[1];
+    st[1] = en[2];
+    st[2] = 0;
+    st[3] = 0;
+  }
+  __syncthreads();
+  int64_t it = 0;  // lane 0's iteration counter
+  while (true) {
+    const int64_t r = st[0], c = st[1];
+    if (st[3]) break;
+    const int64_t rb_hi = r >> 4;
+    const int64_t rb_lo = rb_hi - 3 > 0 ? rb_hi - 3 : 0;
+    const int64_t clo = c - 63 > 0 ? c - 63 : 0;
+    for (int qq = 0; qq < 4; ++qq) {
+      const int64_t rb = rb_lo + qq;
+      const int64_t cc = clo + lane;
+      if (rb <= rb_hi && cc <= c) win[qq][lane] = Bp[td.b_off + rb * cols + cc];
+    }
+    __syncthreads();
+    if (lane == 0) {
+      int64_t rr = r, cc = c, k = st[2];
+      bool done = false;
+      while (true) {
+        if (it >= max_iters) {
+          if (rr != sr || cc != sc) {
+            out[2 * k] = sr + gro;
+            out[2 * k + 1] = sc + gco;
+            ++k;
+          }
+          done = true;
+          break;
+        }
+        if ((rr >> 4) < rb_lo || cc < clo) break;  // left the window: reload
+        out[2 * k] = rr + gro;
+        out[2 * k + 1] = cc + gco;
+        ++k;
+        ++it;
+        if (rr == sr && cc == sc) {
+          done = true;
+          break;
+        }
+        const int code = (win[(rr >> 4) - rb_lo][cc - clo] >> (2 * (rr & 15))) & 3u;
+        if (code >= ss.K) {
+          out[2 * k] = sr + gro;
+          out[2 * k + 1] = sc + gco;
+          ++k;
+          done = true;
+          break;
+        }
+        const int64_t pr = rr - ss.dr[code], pc = cc - ss.dc[code];
+        if (pr < 0 || pc < 0 || pr >= rows || pc >= cols) {
+          out[2 * k] = sr + gro;
+          out[2 * k + 1] = sc + gco;
+          ++k;
+          done = true;
+          break;
+        }
+        rr = pr;
+        cc = pc;
+      }
+      st[0] = rr;
+      st[1] = cc;
+      st[2] = k;
+      st[3] = done ? 1 : 0;
+    }
+    __syncthreads();
+  }
+  if (lane == 0) en[7] = st[2];
+}
+
+// Concatenate the segments of request q, drop the repeated overlap cell at
+// each join, reverse (Parflex.py:1060-1062, :1103).
+__global__ __launch_bounds__(256) void k_s2_compact(const int64_t *__restrict__ chain,
+                                                    int64_t *__restrict__ cnt,
+                                                    const int64_t *__restrict__ pts, int64_t L,
+                                                    const PairDesc *__restrict__ pairs,
+                                                    int use_pair_off, int64_t path_cap,
+                                                    int64_t *__restrict__ path) {
+  __shared__ int64_t off[102];
+  __shared__ int64_t skip[101];
+  __shared__ int64_t total;
+  const int64_t q = blockIdx.x;
+  const int64_t ns = cnt[q];
+  const int64_t *ch = chain + q * 101 * 8;
+  const int64_t stride = (2 * L + 8) * 2;
+  int64_t cap;
+  int64_t *dst;
+  if (use_pair_off) {
+    const PairDesc pd = pairs[q];
+    cap = pd.L1 + pd.L2 + 8;
+    dst = path + pd.path_off * 2;
+  } else {
+    cap = path_cap;
+    dst = path + q * path_cap * 2;
+  }
+  if (threadIdx.x == 0) {
+    int64_t o = 0;
+    for (int64_t s = 0; s < ns; ++s) {
+      const int64_t n = ch[s * 8 + 7];
+      int64_t sk = 0;
+      if (s > 0 && n > 0 && o > 0) {
+        const int64_t *prev = pts + (q * 101 + s - 1) * stride;
+        const int64_t pn = ch[(s - 1) * 8 + 7];
+        const int64_t *cur = pts + (q * 101 + s) * stride;
+        if (pn > 0 && prev[2 * (pn - 1)] == cur[0] && prev[2 * (pn - 1) + 1] == cur[1]) sk = 1;
+      }
+      off[s] = o;
+      skip[s] = sk;
+      o += n - sk;
+    }
+    total = o < cap ? o : cap;
+  }
+  __syncthreads();
+  const int64_t n_out = total;
+  for (int64_t s = 0; s < ns; ++s) {
+    const int64_t n = ch[s * 8 + 7];
+    const int64_t *src = pts + (q * 101 + s) * stride;
+    for (int64_t k = skip[s] + threadIdx.x; k < n; k += blockDim.x) {
+      const int64_t pos = off[s] + (k - skip[s]);
+      if (pos < n_out) {
+        const int64_t o = n_out - 1 - pos;
+        dst[2 * o] = src[2 * k];
+        dst[2 * o + 1] = src[2 * k + 1];
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) cnt[q] = n_out;
+}
+
+static inline unsigned nblk(int64_t n, int bs) {
+  int64_t b = (n + bs - 1) / bs;
+  if (b < 1) b = 1;
+  if (b > (1 << 20)) b = 1 << 20;
+  return (unsigned)b;
+}
+
+}  // namespace pf
+
+using namespace pf;
+
+extern "C" int pf_fill_f64(double *d_p, double v, int64_t n, void *stream) {
+  if (n <= 0) return PF_OK;
+  hipLaunchKernelGGL(k_fill_f64, dim3(nblk(n, 256)), dim3(256), 0, as_stream(stream), d_p, v, n);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_fill_i64(int64_t *d_p, int64_t v, int64_t n, void *stream) {
+  if (n <= 0) return PF_OK;
+  hipLaunchKernelGGL(k_fill_i64, dim3(nblk(n, 256)), dim3(256), 0, as_stream(stream), d_p, v, n);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+static int stitch_common(hipStream_t st, int from_best, const int64_t *d_req, int64_t nreq,
+                         const int64_t *d_besti, const PairDesc *pairs, const TileDesc *tiles,
+                         int64_t L, const StepSet &ss, const uint32_t *Bp, const int32_t *eP,
+                         int64_t *chain, int64_t *pts, int use_pair_off, int64_t path_cap,
+                         int64_t *path, int64_t *cnt) {
+  hipLaunchKernelGGL(k_s2_chain, dim3(nblk(nreq, 64)), dim3(64), 0, st, from_best, d_req, nreq,
+                     d_besti, pairs, L, eP, chain, cnt);
+  PF_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_s2_walk, dim3(101, (unsigned)nreq), dim3(64), 0, st, tiles, L, ss, Bp,
+                     chain, cnt, pts);
+  PF_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_s2_compact, dim3((unsigned)nreq), dim3(256), 0, st, chain, cnt, pts, L,
+                     pairs, use_pair_off, path_cap, path);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_stage2(int c_dtype, const void *d_C, const int64_t *d_pairs, int64_t npairs,
+                         const int64_t *d_tiles, int64_t ntiles, int64_t L, int64_t max_diag,
+                         int64_t max_tile_diag, int64_t max_tiles_pair, int64_t max_segs,
+                         const int32_t *steps, int K, const uint32_t *d_Bpacked,
+                         const double *d_edge_D, const int32_t *d_edge_P, double *d_D_arr,
+                         double *d_L_arr, double *d_olap, uint32_t *d_bits, double *d_cand_D,
+                         double *d_cand_L, double *d_seg_f, int64_t *d_seg_i, double *d_best_f,
+                         int64_t *d_best_i, int64_t *d_chain, int64_t *d_pts, int64_t *d_path,
+                         int64_t *d_path_len, void *stream) {
+  StepSet ss;
+  double w[4] = {1, 1, 1, 1};
+  int rc = make_stepset(steps, w, K, &ss);
+  if (rc) return rc;
+  PF_REQUIRE(npairs > 0 && ntiles > 0 && L >= 2, "pf_stage2: empty batch or bad tile length");
+  PF_REQUIRE(max_diag >= 1 && max_tile_diag >= 1 && max_tiles_pair >= 1 && max_segs >= 1,
+             "pf_stage2: bad batch dimensions");
+  PF_REQUIRE(c_dtype == PF_F64 || c_dtype == PF_F32 || c_dtype == PF_BF16,
+             "pf_stage2: unknown C dtype");
+  PF_REQUIRE(d_C && d_pairs && d_tiles && d_Bpacked && d_edge_D && d_edge_P && d_D_arr &&
+                 d_L_arr && d_olap && d_bits && d_cand_D && d_cand_L && d_seg_f && d_seg_i &&
+                 d_best_f && d_best_i && d_chain && d_pts && d_path && d_path_len,
+             "pf_stage2: null buffer");
+  PF_REQUIRE(npairs <= 65535 && max_segs <= (1 << 20), "pf_stage2: batch too large");
+  hipStream_t st = as_stream(stream);
+  const PairDesc *pairs = reinterpret_cast<const PairDesc *>(d_pairs);
+  const TileDesc *tiles = reinterpret_cast<const TileDesc *>(d_tiles);
+  const int64_t bitw = (L + 32) / 32;
+  const int64_t nbits = ntiles * 2 * bitw;
+  hipLaunchKernelGGL(k_fill_u32, dim3(nblk(nbits, 256)), dim3(256), 0, st, d_bits, 0u, nbits);
+  PF_CHECK_HIP(hipGetLastError());
+  const int64_t nslots = ntiles * 2 * L;
+  hipLaunchKernelGGL(k_s2_prep, dim3(nblk(nslots, 256)), dim3(256), 0, st, c_dtype, d_C, pairs,
+                     tiles, ntiles, L, d_edge_P, d_D_arr, d_L_arr, d_olap, d_bits, bitw);
+  PF_CHECK_HIP(hipGetLastError());
+  for (int64_t d = 0; d < max_diag; ++d) {
+    const int64_t ntd = (d + 1) < max_tile_diag ? (d + 1) : max_tile_diag;
+    const int64_t nth = ntd * 2 * L;
+    PF_REQUIRE((nth + 255) / 256 <= (1ll << 31) - 1, "pf_stage2: diagonal too large");
+    hipLaunchKernelGGL(k_s2_diag, dim3((unsigned)((nth + 255) / 256), (unsigned)npairs),
+                       dim3(256), 0, st, d, pairs, L, d_edge_D, d_edge_P, d_olap, d_bits, bitw,
+                       d_D_arr, d_L_arr);
+    PF_CHECK_HIP(hipGetLastError());
+  }
+  for (int pass = 0; pass < 3; ++pass) {
+    hipLaunchKernelGGL(k_s2_sync, dim3((unsigned)((max_tiles_pair + 255) / 256), (unsigned)npairs),
+                       dim3(256), 0, st, pass, pairs, L, d_D_arr, d_L_arr);
+    PF_CHECK_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_s2_scan, dim3((unsigned)max_segs, 2, (unsigned)npairs), dim3(256), 0, st,
+                     pairs, L, d_D_arr, d_L_arr, d_cand_D, d_cand_L, d_seg_f, d_seg_i);
+  PF_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_s2_best, dim3((unsigned)npairs), dim3(256), 0, st, pairs, d_seg_f,
+                     d_seg_i, d_best_f, d_best_i);
+  PF_CHECK_HIP(hipGetLastError());
+  return stitch_common(st, 1, nullptr, npairs, d_best_i, pairs, tiles, L, ss, d_Bpacked,
+                       d_edge_P, d_chain, d_pts, 1, 0, d_path, d_path_len);
+}
+
+extern "C" int pf_stitch(const int64_t *d_req, int64_t nreq, const int64_t *d_pairs,
+                         const int64_t *d_tiles, int64_t L, const int32_t *steps, int K,
+                         const uint32_t *d_Bpacked, const int32_t *d_edge_P, int64_t *d_chain,
+                         int64_t *d_pts, int64_t path_cap, int64_t *d_path, int64_t *d_path_len,
+                         void *stream) {
+  StepSet ss;
+  double w[4] = {1, 1, 1, 1};
+  int rc = make_stepset(steps, w, K, &ss);
+  if (rc) return rc;
+  if (nreq == 0) return PF_OK;
+  PF_REQUIRE(nreq > 0 && nreq <= 65535 && L >= 2 && path_cap > 0, "pf_stitch: bad arguments");
+  PF_REQUIRE(d_req && d_pairs && d_tiles && d_Bpacked && d_edge_P && d_chain && d_pts &&
+                 d_path && d_path_len,
+             "pf_stitch: null buffer");
+  return stitch_common(as_stream(stream), 0, d_req, nreq, nullptr,
+                       reinterpret_cast<const PairDesc *>(d_pairs),
+                       reinterpret_cast<const TileDesc *>(d_tiles), L, ss, d_Bpacked, d_edge_P,
+                       d_chain, d_pts, 0, path_cap, d_path, d_path_len);
+}

@@ -1,0 +1,154 @@
+"""Device pipeline: stage 1 (all tiles, one launch) and stage 2 (on device).
+
+This is the layer the drop-in modules (gpu_flexdtw, Parflex_gpu) and the
+benchmark call.  Inputs are torch tensors resident on the GPU; only the
+stitched paths and best costs are copied back.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import PF_BF16, PF_F32, PF_F64, check, lib
+from ._plan import MAX_STITCH_TILES, BatchPlan
+
+__all__ = ["StepWeights", "stage1", "stage2", "run_batch", "BatchResult", "dtype_code",
+           "NO_ENDPOINT_MSG"]
+
+NO_ENDPOINT_MSG = "Stage 2: No valid endpoint found on global top/right edges."
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def dtype_code(t):
+    torch = _torch()
+    if t.dtype == torch.float64:
+        return PF_F64
+    if t.dtype == torch.float32:
+        return PF_F32
+    if t.dtype == torch.bfloat16:
+        return PF_BF16
+    raise ValueError(f"unsupported cost-matrix dtype {t.dtype}")
+
+
+class StepWeights:
+    """Host copies of the step set in the C-ABI layout (int32 pairs, f64)."""
+
+    def __init__(self, steps, weights):
+        s = np.asarray(steps, dtype=np.int64).reshape(-1, 2)
+        w = np.asarray(weights, dtype=np.float64).reshape(-1)
+        if s.shape[0] != w.shape[0]:
+            raise ValueError("steps and weights must have the same length")
+        self.steps = np.ascontiguousarray(s.astype(np.int32))
+        self.weights = np.ascontiguousarray(w)
+        self.K = int(s.shape[0])
+        check(lib().pf_check_steps(self.steps.ctypes.data, self.weights.ctypes.data, self.K),
+              "step set")
+        self.steps_i64 = s
+
+    @property
+    def sp(self):
+        return self.steps.ctypes.data
+
+    @property
+    def wp(self):
+        return self.weights.ctypes.data
+
+
+def _stream(stream):
+    torch = _torch()
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return stream.cuda_stream
+
+
+def stage1(C_dev, plan, sw, ws, stream=None):
+    """Per-tile DP for every tile of the plan: packed back-pointers plus edge
+    D/P (replaces the tile loop at Parflex_gpu.py:461-563)."""
+    rc = lib().pf_stage1(dtype_code(C_dev), C_dev.data_ptr(), ws["tiles"].data_ptr(),
+                         plan.ntiles, plan.L, sw.sp, sw.wp, sw.K, ws["Bp"].data_ptr(),
+                         ws["edge_D"].data_ptr(), ws["edge_P"].data_ptr(), _stream(stream))
+    check(rc, "pf_stage1")
+
+
+def stage2(C_dev, plan, sw, ws, stream=None):
+    """Propagation, sync, scan and stitch on the device (Parflex.py:862-1356)."""
+    rc = lib().pf_stage2(
+        dtype_code(C_dev), C_dev.data_ptr(), ws["pairs"].data_ptr(), plan.npairs,
+        ws["tiles"].data_ptr(), plan.ntiles, plan.L, plan.max_diag, plan.max_tile_diag,
+        plan.max_tiles_pair, plan.max_segs, sw.sp, sw.K, ws["Bp"].data_ptr(),
+        ws["edge_D"].data_ptr(), ws["edge_P"].data_ptr(), ws["D_arr"].data_ptr(),
+        ws["L_arr"].data_ptr(), ws["olap"].data_ptr(), ws["bits"].data_ptr(),
+        ws["cand_D"].data_ptr(), ws["cand_L"].data_ptr(), ws["seg_f"].data_ptr(),
+        ws["seg_i"].data_ptr(), ws["best_f"].data_ptr(), ws["best_i"].data_ptr(),
+        ws["chain"].data_ptr(), ws["pts"].data_ptr(), ws["path"].data_ptr(),
+        ws["path_len"].data_ptr(), _stream(stream))
+    check(rc, "pf_stage2")
+
+
+def stitch_requests(plan, sw, ws, requests, stream=None):
+    """backtrace_and_stitch (Parflex.py:1032-1103) from arbitrary edge slots.
+    requests: int64 (n, 5) = (pair, ci, cj, et, idx).  Returns list of (n_i, 2)."""
+    torch = _torch()
+    req = np.ascontiguousarray(np.asarray(requests, dtype=np.int64).reshape(-1, 5))
+    n = req.shape[0]
+    if n == 0:
+        return []
+    dev = ws["pairs"].device
+    d_req = torch.from_numpy(req).to(dev)
+    cap = int(max(plan.pairs[req[:, 0], 0] + plan.pairs[req[:, 0], 1])) + 8
+    chain = torch.empty(n * MAX_STITCH_TILES * 8, dtype=torch.int64, device=dev)
+    pts = torch.empty(n * MAX_STITCH_TILES * (2 * plan.L + 8) * 2, dtype=torch.int64, device=dev)
+    path = torch.empty(n * cap * 2, dtype=torch.int64, device=dev)
+    plen = torch.empty(n, dtype=torch.int64, device=dev)
+    rc = lib().pf_stitch(d_req.data_ptr(), n, ws["pairs"].data_ptr(), ws["tiles"].data_ptr(),
+                         plan.L, sw.sp, sw.K, ws["Bp"].data_ptr(), ws["edge_P"].data_ptr(),
+                         chain.data_ptr(), pts.data_ptr(), cap, path.data_ptr(),
+                         plen.data_ptr(), _stream(stream))
+    check(rc, "pf_stitch")
+    path_h = path.cpu().numpy().reshape(n, cap, 2)
+    plen_h = plen.cpu().numpy()
+    return [path_h[q, :plen_h[q]].copy() for q in range(n)]
+
+
+@dataclass
+class BatchResult:
+    best_cost: np.ndarray     # (P,) float64; NaN where no endpoint was found
+    found: np.ndarray         # (P,) bool
+    paths: list               # P arrays (2, n) int64 (wp layout of Parflex.parflex)
+    best_end: np.ndarray      # (P, 6) (g_row, g_col, ci, cj, et, idx)
+
+
+def collect(plan, ws):
+    """Copy only best costs, endpoints and stitched paths back to the host."""
+    best_f = ws["best_f"].cpu().numpy()
+    best_i = ws["best_i"].cpu().numpy().reshape(plan.npairs, 8)
+    plen = ws["path_len"].cpu().numpy()
+    path = ws["path"].cpu().numpy().reshape(-1, 2)
+    found = best_i[:, 6] == 1
+    paths = []
+    for p in range(plan.npairs):
+        o = int(plan.pairs[p, 11])
+        pts = path[o:o + int(plen[p])]
+        paths.append(np.ascontiguousarray(pts.T))
+    bc = np.where(found, best_f, np.nan)
+    return BatchResult(bc, found, paths, best_i[:, :6].copy())
+
+
+def run_batch(C_dev, plan, steps, weights, stream=None, ws=None):
+    """Stage 1 + stage 2 for every pair of the plan; returns (BatchResult, ws)."""
+    sw = steps if isinstance(steps, StepWeights) else StepWeights(steps, weights)
+    if ws is None:
+        ws = plan.allocate(C_dev.device)
+    stage1(C_dev, plan, sw, ws, stream)
+    stage2(C_dev, plan, sw, ws, stream)
+    return collect(plan, ws), ws
+
+
+def single_pair_plan(L1, L2, L, beta):
+    return BatchPlan([(int(L1), int(L2))], int(L), beta=float(beta))
