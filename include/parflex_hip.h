@@ -65,11 +65,23 @@ int pf_check_steps(const int32_t *steps, const double *weights, int K);
  * (Parflex_gpu.py:461-563) with its per-tile flexdtw_chunk_from_global_C
  * (gpu_flexdtw.py:288-402) and _edge_starts_from_S (Parflex.py:347-373).
  * dtype: PF_F64 (C f64, DP f64) or PF_F32 (C f32, DP f32, edges f64) or
- * PF_BF16 (C bf16, DP f32, edges f64). */
+ * PF_BF16 (C bf16, DP f32, edges f64).
+ * Launch shape: each tile is cut into `strips` column strips, one workgroup
+ * of `waves` waves per strip (strips*waves*64*8 >= Lmax); strips of a tile
+ * hand their boundary columns to the right every `rows_per_block` rows
+ * (1..64) through d_xch (ntiles*strips*Lmax entries of 32 bytes for PF_F64,
+ * 16 bytes otherwise) and d_flags (int32, ntiles*strips + 1).  With
+ * strips == 1 both may be null. */
 int pf_stage1(int dtype, const void *d_C, const int64_t *d_tiles, int64_t ntiles,
               int64_t Lmax, const int32_t *steps, const double *weights, int K,
-              uint32_t *d_Bpacked, double *d_edge_D, int32_t *d_edge_P,
+              int strips, int waves, int rows_per_block, int32_t *d_flags,
+              void *d_xch, uint32_t *d_Bpacked, double *d_edge_D, int32_t *d_edge_P,
               void *stream);
+
+/* Sizes of the strip-exchange workspace of pf_stage1: int32 count of
+ * d_flags and byte size of d_xch (both 0 when strips == 1). */
+int pf_stage1_workspace(int dtype, int64_t ntiles, int64_t Lmax, int strips,
+                        int64_t *flags_count, int64_t *xch_bytes);
 
 /* Per-tile DP with full D/P/B outputs for one tile C[r0:r1, c0:c1] of a
  * row-major FP64 matrix with leading dimension ldc (FlexDTW.py:1320-1364;

@@ -30,7 +30,9 @@ _SIGNATURES = {
     "pf_last_error": (ctypes.c_char_p, []),
     "pf_version": (_int, []),
     "pf_check_steps": (_int, [_vp, _vp, _int]),
-    "pf_stage1": (_int, [_int, _vp, _vp, _i64, _i64, _vp, _vp, _int, _vp, _vp, _vp, _vp]),
+    "pf_stage1": (_int, [_int, _vp, _vp, _i64, _i64, _vp, _vp, _int, _int, _int, _int, _vp, _vp,
+                         _vp, _vp, _vp, _vp]),
+    "pf_stage1_workspace": (_int, [_int, _i64, _i64, _int, _vp, _vp]),
     "pf_tile_full": (_int, [_vp, _i64, _i64, _i64, _i64, _i64, _vp, _vp, _int, _vp, _vp, _vp,
                             _vp]),
     "pf_tile_endpoint": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp]),

@@ -1,26 +1,40 @@
 // Stage 1 of ParFlex on gfx950: per-tile FlexDTW dynamic programme.
 //
-// Semantics follow FlexDTW.flexdtw (FlexDTW.py:1320-1364) exactly, in IEEE
-// arithmetic with every multiply and add rounded on its own (the library is
-// compiled with -ffp-contract=off):
+// Semantics follow FlexDTW.flexdtw (FlexDTW.py:1320-1364) exactly:
 //   boundary: D[0,:] = C[0,:], D[:,0] = C[:,0], P = B = 0 there;
-//   interior: for step k in order, pathcost = D[pred] + C*w_k,
-//             mdist = row + col - |P[pred]|, cpm = pathcost / mdist,
-//             strict '<' keeps the first minimum;
+//   interior: for step k in order, pathcost = D[pred] + C*w_k (a multiply and
+//             an add, each rounded: the library is built with
+//             -ffp-contract=off), mdist = row + col - |P[pred]|,
+//             cpm = pathcost / mdist, strict '<' keeps the first minimum;
 //             D = pathcost of the winner, B = k, P = start column (pred on
 //             row 0), -start row (pred on column 0) or P[pred].
 //
-// Mapping: one workgroup per tile; the tile's columns are split over all
-// lanes (CPL consecutive columns per lane) and the workgroup sweeps the rows
-// top to bottom.  A cell depends only on rows r-1 and r-2 (every step has
-// dr >= 1), so a whole row is computed in parallel; each lane keeps its
-// columns of rows r-1 and r-2 in registers, takes the two columns to its left
-// from lane-1 by a lane shuffle, and lane 0 of each wave reads them from the
-// previous wave through a 3-slot LDS ring.  One barrier per row.
+// Comparisons without division.  Only the ORDER of the rounded quotients
+// matters (cpm itself is never stored), so RN(a1/m1) < RN(a2/m2) is decided
+// from the cross products a1*m2 and a2*m1: when they differ by more than
+// 2^-49 of their magnitude the rounded quotients are more than two ulps apart
+// and the sign of the difference is the answer; otherwise (a near tie) both
+// quotients are computed with IEEE division and compared, exactly as the
+// reference does.  The result is bit-identical to dividing every time.
+//
+// Mapping.  A cell depends only on rows r-1 and r-2 (every step has
+// dr >= 1), so a row is computed in parallel across its columns.  A tile's
+// columns are cut into S strips of SW = NW*64*CPL columns; each (tile, strip)
+// is one workgroup of NW waves that sweeps the rows top to bottom, CPL
+// consecutive columns per lane, rows r-1 and r-2 in registers.  A lane takes
+// its two left neighbour columns from lane-1 (shuffle), lane 0 of each wave
+// from the previous wave through a 3-slot LDS ring (one barrier per row), and
+// lane 0 of wave 0 from the previous strip through a global exchange buffer:
+// the strip to the left publishes its last two columns (D, P) row by row and
+// bumps a per-strip row counter (release, agent scope) every G rows; the
+// strip to the right waits on it (acquire) before each block of G rows.
+// Work items are taken from a ticket counter in strip order, so a strip only
+// ever waits for an item that is already running: no deadlock whatever the
+// grid size or residency.
 //
 // Outputs (packed mode): 2-bit back-pointers packed 16 rows per uint32 per
 // column, and D/P on the tile's last row and last column only -- everything
-// stage 2 needs (Parflex.py:509-564, :1063-1064).  Full mode writes D/P/B for
+// stage 2 needs (Parflex.py:509-564, :1063-1064).  Full mode writes D/P/B of
 // every cell (the 6-tuple of gpu_flexdtw.flexdtw_chunk_from_global_C).
 #include "pf_common.h"
 
@@ -42,41 +56,158 @@ __device__ __forceinline__ DT load_c(const CT *p) {
   }
 }
 
-// One tile.  FULL selects full D/P/B outputs instead of packed B + edges.
-template <int CPL, typename CT, typename DT, bool FULL>
-__device__ __forceinline__ void tile_dp(const TileDesc td, const CT *__restrict__ Cb,
-                                        const StepSet &ss, int64_t Lmax,
-                                        uint32_t *__restrict__ Bp, double *__restrict__ eD,
-                                        int32_t *__restrict__ eP, double *__restrict__ Df,
-                                        int32_t *__restrict__ Pf, int8_t *__restrict__ Bf) {
+// RN(a1/m1) < RN(a2/m2) for positive integer-valued m (see the header).
+__device__ __forceinline__ bool quot_less(double a1, double m1, double a2, double m2) {
+  const double x = a1 * m2, y = a2 * m1;
+  const double d = x - y;
+  const double mag = fmax(fabs(x), fabs(y));
+  if (fabs(d) > mag * 0x1p-49 && mag > 0x1p-900) return d < 0.0;
+  return (a1 / m1) < (a2 / m2);
+}
+__device__ __forceinline__ bool quot_less(float a1, float m1, float a2, float m2) {
+  const float x = a1 * m2, y = a2 * m1;
+  const float d = x - y;
+  const float mag = fmaxf(fabsf(x), fabsf(y));
+  if (fabsf(d) > mag * 0x1p-20f && mag > 0x1p-100f) return d < 0.0f;
+  return (a1 / m1) < (a2 / m2);
+}
+
+// Branch-free form of quot_less: returns the fast decision and sets ok when
+// it is certain; callers redo the whole cell with quot_less when !ok.
+__device__ __forceinline__ bool fast_less(double a1, double m1, double a2, double m2, bool &ok) {
+  const double x = a1 * m2, y = a2 * m1;
+  const double d = x - y;
+  const double mag = fmax(fabs(x), fabs(y));
+  ok = (fabs(d) > mag * 0x1p-49) && (mag > 0x1p-900);
+  return d < 0.0;
+}
+__device__ __forceinline__ bool fast_less(float a1, float m1, float a2, float m2, bool &ok) {
+  const float x = a1 * m2, y = a2 * m1;
+  const float d = x - y;
+  const float mag = fmaxf(fabsf(x), fabsf(y));
+  ok = (fabsf(d) > mag * 0x1p-20f) && (mag > 0x1p-100f);
+  return d < 0.0f;
+}
+
+// Boundary values a strip hands to the strip on its right, per row: D and
+// the signed path length X of its last two columns.
+template <typename DT>
+struct Xch {
+  DT d0, d1, x0, x1;
+};
+
+static_assert(sizeof(Xch<double>) == 32 && sizeof(Xch<float>) == 16,
+              "pf_stage1_workspace reports these entry sizes");
+
+constexpr int kMaxG = 64;   // rows per exchange block, upper bound
+constexpr int kPF = 4;      // rows of C prefetched ahead
+
+// Lane i receives lane i-1's value (DPP wave_shr:1); lane 0 keeps `old`.
+__device__ __forceinline__ int shr1(int v, int old) {
+  return __builtin_amdgcn_update_dpp(old, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float shr1(float v, float old) {
+  return __int_as_float(shr1(__float_as_int(v), __float_as_int(old)));
+}
+__device__ __forceinline__ double shr1(double v, double old) {
+  const long long vi = __double_as_longlong(v), oi = __double_as_longlong(old);
+  const int lo = shr1((int)(vi & 0xffffffffll), (int)(oi & 0xffffffffll));
+  const int hi = shr1((int)(vi >> 32), (int)(oi >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double dabs(double v) { return fabs(v); }
+__device__ __forceinline__ float dabs(float v) { return fabsf(v); }
+__device__ __forceinline__ double copy_sign(double mag, double sgn) { return copysign(mag, sgn); }
+__device__ __forceinline__ float copy_sign(float mag, float sgn) { return copysignf(mag, sgn); }
+
+// Start encoding P of a cell from its signed path length X (see below).
+template <typename DT>
+__device__ __forceinline__ int p_of(DT X, int rc) {
+  return (X >= (DT)0) ? rc - (int)X : -(rc + (int)X);
+}
+
+// State of a cell: D and X = +-M, where M = row + col - |P| is the
+// Manhattan length of the path from its start (all small integers, exact in
+// floating point) and the sign is that of P (+ for a start on row 0, - for a
+// start on column 0).  The reference's mdist of a step from a predecessor is
+// then |X_pred| + dr + dc, and the new cell inherits X = sign(X_pred) * mdist
+// except when the predecessor is a boundary cell, where P is re-derived from
+// the predecessor's coordinates (FlexDTW.py:1359-1364).
+// Threads per workgroup the packed kernels are compiled for: 2 columns per
+// lane up to 16 waves, 4 columns up to 8, 8 columns up to 4 (larger register
+// budgets per lane).  The full-output kernels (a single tile, API path) take
+// up to 16 waves for every width.
+template <int CPL, bool FULL>
+struct MaxThreads {
+  static constexpr int value = (FULL || CPL <= 2) ? 1024 : (CPL == 4 ? 512 : 256);
+};
+
+template <int CPL, typename CT, typename DT, bool FULL, bool STD>
+__global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
+    const CT *__restrict__ Cb, const TileDesc *__restrict__ tiles, TileDesc td0, StepSet ss,
+    int64_t Lmax, int S, int G, int *__restrict__ flags, Xch<DT> *__restrict__ xch,
+    uint32_t *__restrict__ Bp, double *__restrict__ eD, int32_t *__restrict__ eP,
+    double *__restrict__ Df, int32_t *__restrict__ Pf, int8_t *__restrict__ Bf) {
   static_assert(CPL >= 2, "a lane needs two columns of left context");
   __shared__ DT shD[3][16][2];
-  __shared__ int shP[3][16][2];
+  __shared__ DT shX[3][16][2];
+  __shared__ DT lxD[kMaxG][2];
+  __shared__ DT lxX[kMaxG][2];
+  __shared__ int sh_item;
 
+  // Work item from the ticket counter in strip order.
+  int64_t item;
+  if (S > 1) {
+    // flags[-1] is the ticket counter (see pf_stage1).
+    if (threadIdx.x == 0) sh_item = atomicAdd(&flags[-1], 1);
+    __syncthreads();
+    item = sh_item;
+  } else {
+    item = blockIdx.x;
+  }
+  const int64_t t = item / S;
+  const int s = (int)(item - t * S);
+  const TileDesc td = tiles ? tiles[t] : td0;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
+  const int SW = (int)blockDim.x * CPL;
   const int R = (int)td.R, W = (int)td.W;
+  const int cbeg = s * SW;
+  if (cbeg >= W) return;  // idle strip of a narrow tile (nobody waits on it)
+  const bool producer = (S > 1) && (cbeg + SW < W) && (tid == (int)blockDim.x - 1);
+  const bool consumer = (s > 0);
+  int *fin = consumer ? flags + t * S + s - 1 : nullptr;
+  int *fout = producer ? flags + t * S + s : nullptr;
+  const Xch<DT> *xin = consumer ? xch + (t * S + s - 1) * Lmax : nullptr;
+  Xch<DT> *xout = producer ? xch + (t * S + s) * Lmax : nullptr;
+
   const CT *__restrict__ C = Cb + td.c_off;
   const int64_t ldc = td.ldc;
-  const int c0 = tid * CPL;
+  const int c0 = cbeg + tid * CPL;
+  const int jlast = W - 1 - c0;  // this lane's index of the tile's last column
+  const bool own_last = (jlast >= 0 && jlast < CPL);
   const DT INF = dp_inf<DT>();
 
   DT wk[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) wk[k] = (DT)ss.w[k];
 
-  DT D1[CPL], D2[CPL], Cn[CPL];
-  int P1[CPL], P2[CPL];
+  DT D1[CPL], X1[CPL], D2[CPL], X2[CPL];
   uint32_t bacc[CPL];
+  DT Cq[kPF][CPL];
+  // Left context: columns c0-2 (a), c0-1 (b) of rows r-1 (1) and r-2 (2).
+  DT LD2a = INF, LD2b = INF, LX2a = 0, LX2b = 0;
 
+  // Row 0: boundary cells, P = 0, X = +col.
 #pragma unroll
   for (int j = 0; j < CPL; ++j) {
     const int c = c0 + j;
     D1[j] = (c < W) ? load_c<DT>(C + c) : INF;
-    P1[j] = 0;
+    X1[j] = (DT)c;
     D2[j] = INF;
-    P2[j] = 0;
+    X2[j] = 0;
     bacc[j] = 0u;
     if (c < W) {
       if constexpr (FULL) {
@@ -96,324 +227,373 @@ __device__ __forceinline__ void tile_dp(const TileDesc td, const CT *__restrict_
       }
     }
   }
-  if (R > 1) {
+#pragma unroll
+  for (int u = 0; u < kPF; ++u) {
+    const int row = 1 + u;
 #pragma unroll
     for (int j = 0; j < CPL; ++j) {
       const int c = c0 + j;
-      Cn[j] = (c < W) ? load_c<DT>(C + ldc + c) : (DT)0;
+      Cq[u][j] = (row < R && c < W) ? load_c<DT>(C + (int64_t)row * ldc + c) : (DT)0;
     }
+  }
+  if (producer) {
+    xout[0] = Xch<DT>{D1[CPL - 2], D1[CPL - 1], X1[CPL - 2], X1[CPL - 1]};
+    if (R == 1) __hip_atomic_store(fout, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (lane == 63) {
     shD[0][wave][0] = D1[CPL - 2];
     shD[0][wave][1] = D1[CPL - 1];
-    shP[0][wave][0] = P1[CPL - 2];
-    shP[0][wave][1] = P1[CPL - 1];
+    shX[0][wave][0] = X1[CPL - 2];
+    shX[0][wave][1] = X1[CPL - 1];
+  }
+  int blk0 = 1;  // first row of the current exchange block
+  // Rows 0 .. G-1 of the strip on the left feed rows 1 .. G.
+  if (consumer && wave == 0 && R > 1) {
+    const int need = (G < R) ? G : R;
+    while (__hip_atomic_load(fin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need)
+      __builtin_amdgcn_s_sleep(2);
+    for (int q0 = lane; q0 < G; q0 += 64) {
+      if (q0 < R) {
+        const Xch<DT> v = xin[q0];
+        lxD[q0][0] = v.d0;
+        lxD[q0][1] = v.d1;
+        lxX[q0][0] = v.x0;
+        lxX[q0][1] = v.x1;
+      }
+    }
   }
   __syncthreads();
 
-  for (int r = 1; r < R; ++r) {
-    DT Cc[CPL];
+  for (int rb = 1; rb < R; rb += kPF) {
 #pragma unroll
-    for (int j = 0; j < CPL; ++j) Cc[j] = Cn[j];
-    if (r + 1 < R) {
-      const CT *row = C + (int64_t)(r + 1) * ldc;
+    for (int u = 0; u < kPF; ++u) {
+      const int r = rb + u;
+      if (r >= R) break;
+      DT Cc[CPL];
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) Cc[j] = Cq[u][j];
+      if (r + kPF < R) {
+        const CT *row = C + (int64_t)(r + kPF) * ldc;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+          const int c = c0 + j;
+          Cq[u][j] = (c < W) ? load_c<DT>(row + c) : (DT)0;
+        }
+      }
+
+      // Row r-1 of columns c0-2, c0-1: from lane-1 by DPP; lane 0 from the
+      // previous wave (LDS ring), the previous strip (exchange rows) or the
+      // missing columns -2, -1 of the tile.
+      DT oD1a = INF, oD1b = INF, oX1a = 0, oX1b = 0;
+      if (lane == 0) {
+        if (wave > 0) {
+          const int s1 = (r + 2) % 3;  // slot of row r-1
+          oD1a = shD[s1][wave - 1][0];
+          oD1b = shD[s1][wave - 1][1];
+          oX1a = shX[s1][wave - 1][0];
+          oX1b = shX[s1][wave - 1][1];
+        } else if (consumer) {
+          const int i1 = r - blk0;
+          oD1a = lxD[i1][0];
+          oD1b = lxD[i1][1];
+          oX1a = lxX[i1][0];
+          oX1b = lxX[i1][1];
+        }
+      }
+      const DT LD1a = shr1(D1[CPL - 2], oD1a), LD1b = shr1(D1[CPL - 1], oD1b);
+      const DT LX1a = shr1(X1[CPL - 2], oX1a), LX1b = shr1(X1[CPL - 1], oX1b);
+
+      DT ND[CPL], NX[CPL];
+      int code[CPL];
+      bool amb = false;  // some cell of this lane needs the exact comparison
+      bool border = false;
 #pragma unroll
       for (int j = 0; j < CPL; ++j) {
         const int c = c0 + j;
-        Cn[j] = (c < W) ? load_c<DT>(row + c) : (DT)0;
+        const DT cval = Cc[j];
+        DT ba, bm, bx;
+        int bk;
+        if constexpr (STD) {
+          // Steps (1,1) w0, (1,2) w1, (2,1) w2, decided without division.
+          const DT pD0 = j >= 1 ? D1[j - 1] : LD1b, pX0 = j >= 1 ? X1[j - 1] : LX1b;
+          const DT pD1 = j >= 2 ? D1[j - 2] : (j == 1 ? LD1b : LD1a);
+          const DT pX1 = j >= 2 ? X1[j - 2] : (j == 1 ? LX1b : LX1a);
+          const DT pD2 = j >= 1 ? D2[j - 1] : LD2b, pX2 = j >= 1 ? X2[j - 1] : LX2b;
+          const DT a0 = pD0 + cval * wk[0], m0 = dabs(pX0) + (DT)2;
+          const DT a1 = pD1 + cval * wk[1], m1 = dabs(pX1) + (DT)3;
+          const DT a2 = pD2 + cval * wk[2], m2 = dabs(pX2) + (DT)3;
+          bool ok1, ok2;
+          const bool v1 = c >= 2, v2 = r >= 2;
+          const bool t1 = fast_less(a1, m1, a0, m0, ok1) && v1;
+          ba = t1 ? a1 : a0;
+          bm = t1 ? m1 : m0;
+          bx = t1 ? pX1 : pX0;
+          bk = t1 ? 1 : 0;
+          const bool t2 = fast_less(a2, m2, ba, bm, ok2) && v2;
+          ba = t2 ? a2 : ba;
+          bm = t2 ? m2 : bm;
+          bx = t2 ? pX2 : bx;
+          bk = t2 ? 2 : bk;
+          // Column 0 and columns past the tile are not computed here.
+          amb = amb || ((c >= 1) && (c < W) && ((v1 && !ok1) || (v2 && !ok2)));
+        } else {
+          // Generic step set (runtime, K <= 4): exact comparisons.
+          DT E1[CPL + 2], F1[CPL + 2], E2[CPL + 2], F2[CPL + 2];
+          E1[0] = LD1a;
+          E1[1] = LD1b;
+          F1[0] = LX1a;
+          F1[1] = LX1b;
+          E2[0] = LD2a;
+          E2[1] = LD2b;
+          F2[0] = LX2a;
+          F2[1] = LX2b;
+#pragma unroll
+          for (int q = 0; q < CPL; ++q) {
+            E1[q + 2] = D1[q];
+            F1[q + 2] = X1[q];
+            E2[q + 2] = D2[q];
+            F2[q + 2] = X2[q];
+          }
+          ba = INF;
+          bm = 1;
+          bx = 0;
+          bk = 0;
+          bool have = false;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            if (k < ss.K) {
+              const int dr = ss.dr[k], dc = ss.dc[k];
+              const DT v1 = (dc == 0) ? E1[j + 2] : ((dc == 1) ? E1[j + 1] : E1[j]);
+              const DT v2 = (dc == 0) ? E2[j + 2] : ((dc == 1) ? E2[j + 1] : E2[j]);
+              const DT y1 = (dc == 0) ? F1[j + 2] : ((dc == 1) ? F1[j + 1] : F1[j]);
+              const DT y2 = (dc == 0) ? F2[j + 2] : ((dc == 1) ? F2[j + 1] : F2[j]);
+              const DT px = (dr == 1) ? y1 : y2;
+              const DT a = ((dr == 1) ? v1 : v2) + cval * wk[k];
+              const DT m = dabs(px) + (DT)(dr + dc);
+              if (r - dr >= 0 && c - dc >= 0 && (!have || quot_less(a, m, ba, bm))) {
+                ba = a;
+                bm = m;
+                bx = px;
+                bk = k;
+                have = true;
+              }
+            }
+          }
+        }
+        ND[j] = ba;
+        NX[j] = copy_sign(bm, bx);
+        code[j] = bk;
+        border = border || (c <= 2) || (r <= 2);
       }
-    }
+      if constexpr (STD) {
+        if (amb) {
+          // Near tie (rare): redo this lane's cells with the reference's
+          // comparisons, IEEE division included.
+#pragma unroll
+          for (int j = 0; j < CPL; ++j) {
+            const int c = c0 + j;
+            const DT cval = Cc[j];
+            const DT pD0 = j >= 1 ? D1[j - 1] : LD1b, pX0 = j >= 1 ? X1[j - 1] : LX1b;
+            const DT pD1 = j >= 2 ? D1[j - 2] : (j == 1 ? LD1b : LD1a);
+            const DT pX1 = j >= 2 ? X1[j - 2] : (j == 1 ? LX1b : LX1a);
+            const DT pD2 = j >= 1 ? D2[j - 1] : LD2b, pX2 = j >= 1 ? X2[j - 1] : LX2b;
+            const DT a0 = pD0 + cval * wk[0], m0 = dabs(pX0) + (DT)2;
+            const DT a1 = pD1 + cval * wk[1], m1 = dabs(pX1) + (DT)3;
+            const DT a2 = pD2 + cval * wk[2], m2 = dabs(pX2) + (DT)3;
+            DT ba = a0, bm = m0, bx = pX0;
+            int bk = 0;
+            if (c >= 2 && quot_less(a1, m1, ba, bm)) {
+              ba = a1;
+              bm = m1;
+              bx = pX1;
+              bk = 1;
+            }
+            if (r >= 2 && quot_less(a2, m2, ba, bm)) {
+              ba = a2;
+              bm = m2;
+              bx = pX2;
+              bk = 2;
+            }
+            ND[j] = ba;
+            NX[j] = copy_sign(bm, bx);
+            code[j] = bk;
+          }
+        }
+      }
+      if (border) {
+        // Cells whose predecessor is on row 0 or column 0, and column 0
+        // itself (FlexDTW.py:1326-1327, :1359-1364).
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+          const int c = c0 + j;
+          int bdr, bdc;
+          if constexpr (STD) {
+            bdr = (code[j] == 2) ? 2 : 1;
+            bdc = (code[j] == 1) ? 2 : 1;
+          } else {
+            bdr = ss.dr[code[j]];
+            bdc = ss.dc[code[j]];
+          }
+          const int bpr = r - bdr, bpc = c - bdc;
+          if (c == 0) {
+            ND[j] = Cc[j];
+            NX[j] = (DT)r;
+            code[j] = 0;
+          } else if (bpr == 0) {
+            NX[j] = (DT)(r + c - bpc);
+          } else if (bpc == 0) {
+            NX[j] = -(DT)(r + c - bpr);
+          }
+        }
+      }
 
-    // Left context: columns c0-2, c0-1 of rows r-1 (E1) and r-2 (E2).
-    DT a1 = __shfl_up(D1[CPL - 2], 1), b1 = __shfl_up(D1[CPL - 1], 1);
-    DT a2 = __shfl_up(D2[CPL - 2], 1), b2 = __shfl_up(D2[CPL - 1], 1);
-    int pa1 = __shfl_up(P1[CPL - 2], 1), pb1 = __shfl_up(P1[CPL - 1], 1);
-    int pa2 = __shfl_up(P2[CPL - 2], 1), pb2 = __shfl_up(P2[CPL - 1], 1);
-    if (lane == 0) {
-      if (wave > 0) {
-        const int s1 = (r + 2) % 3, s2 = (r + 1) % 3;  // rows r-1, r-2
-        a1 = shD[s1][wave - 1][0];
-        b1 = shD[s1][wave - 1][1];
-        pa1 = shP[s1][wave - 1][0];
-        pb1 = shP[s1][wave - 1][1];
-        a2 = shD[s2][wave - 1][0];
-        b2 = shD[s2][wave - 1][1];
-        pa2 = shP[s2][wave - 1][0];
-        pb2 = shP[s2][wave - 1][1];
+      LD2a = LD1a;
+      LD2b = LD1b;
+      LX2a = LX1a;
+      LX2b = LX1b;
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) {
+        D2[j] = D1[j];
+        X2[j] = X1[j];
+        D1[j] = ND[j];
+        X1[j] = NX[j];
+      }
+
+      if constexpr (FULL) {
+        const int64_t base = (int64_t)r * W;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+          const int c = c0 + j;
+          if (c < W) {
+            Df[base + c] = (double)ND[j];
+            Pf[base + c] = p_of(NX[j], r + c);
+            Bf[base + c] = (int8_t)code[j];
+          }
+        }
       } else {
-        // Columns -2, -1 do not exist; every step reaching them is invalid.
-        a1 = b1 = a2 = b2 = INF;
-        pa1 = pb1 = pa2 = pb2 = 0;
-      }
-    }
-    DT E1[CPL + 2], E2[CPL + 2];
-    int Q1[CPL + 2], Q2[CPL + 2];
-    E1[0] = a1;
-    E1[1] = b1;
-    E2[0] = a2;
-    E2[1] = b2;
-    Q1[0] = pa1;
-    Q1[1] = pb1;
-    Q2[0] = pa2;
-    Q2[1] = pb2;
+        const int sh = 2 * (r & 15);
 #pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      E1[j + 2] = D1[j];
-      E2[j + 2] = D2[j];
-      Q1[j + 2] = P1[j];
-      Q2[j + 2] = P2[j];
-    }
+        for (int j = 0; j < CPL; ++j) bacc[j] |= (uint32_t)code[j] << sh;
+        if ((r & 15) == 15 || r == R - 1) {
+          uint32_t *dst = Bp + td.b_off + (int64_t)(r >> 4) * W;
+#pragma unroll
+          for (int j = 0; j < CPL; ++j) {
+            const int c = c0 + j;
+            if (c < W) dst[c] = bacc[j];
+            bacc[j] = 0u;
+          }
+        }
+        if (own_last) {
+#pragma unroll
+          for (int j = 0; j < CPL; ++j)
+            if (j == jlast) {
+              eD[td.e_off + Lmax + r] = (double)ND[j];
+              eP[td.e_off + Lmax + r] = p_of(NX[j], r + W - 1);
+            }
+        }
+      }
 
-    DT ND[CPL];
-    int NP[CPL];
-    int code[CPL];
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      const int c = c0 + j;
-      const DT cval = Cc[j];
-      if (c == 0) {
-        ND[j] = cval;
-        NP[j] = 0;
-        code[j] = 0;
-        continue;
+      if (producer) {
+        xout[r] = Xch<DT>{D1[CPL - 2], D1[CPL - 1], X1[CPL - 2], X1[CPL - 1]};
+        if ((r + 1) % G == 0 || r == R - 1)
+          __hip_atomic_store(fout, r + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       }
-      DT best = INF, bestD = INF;
-      int kb = 0, bestPp = 0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        if (k < ss.K) {
-          const int dr = ss.dr[k], dc = ss.dc[k];
-          if (r - dr >= 0 && c - dc >= 0) {
-            const DT v1 = (dc == 0) ? E1[j + 2] : ((dc == 1) ? E1[j + 1] : E1[j]);
-            const DT v2 = (dc == 0) ? E2[j + 2] : ((dc == 1) ? E2[j + 1] : E2[j]);
-            const int q1 = (dc == 0) ? Q1[j + 2] : ((dc == 1) ? Q1[j + 1] : Q1[j]);
-            const int q2 = (dc == 0) ? Q2[j + 2] : ((dc == 1) ? Q2[j + 1] : Q2[j]);
-            const DT Dp = (dr == 1) ? v1 : v2;
-            const int Pp = (dr == 1) ? q1 : q2;
-            const DT prod = cval * wk[k];
-            const DT pathcost = Dp + prod;
-            const int mdist = r + c - (Pp >= 0 ? Pp : -Pp);
-            const DT cpm = pathcost / (DT)mdist;
-            if (cpm < best) {
-              best = cpm;
-              bestD = pathcost;
-              kb = k;
-              bestPp = Pp;
+      if (lane == 63) {
+        const int s0 = r % 3;
+        shD[s0][wave][0] = D1[CPL - 2];
+        shD[s0][wave][1] = D1[CPL - 1];
+        shX[s0][wave][0] = X1[CPL - 2];
+        shX[s0][wave][1] = X1[CPL - 1];
+      }
+      // Next exchange block: wait for the strip on the left and stage its
+      // rows blk0-1 .. blk0+G-2 (row r-1 of every row r of the block).
+      if (consumer && r + 1 < R && (r + 1 - blk0) == G) {
+        blk0 = r + 1;
+        if (wave == 0) {
+          const int last = blk0 + G - 1;
+          const int need = (last < R) ? last : R;
+          while (__hip_atomic_load(fin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need)
+            __builtin_amdgcn_s_sleep(2);
+          for (int q0 = lane; q0 < G; q0 += 64) {
+            const int q = blk0 - 1 + q0;
+            if (q < R) {
+              const Xch<DT> v = xin[q];
+              lxD[q0][0] = v.d0;
+              lxD[q0][1] = v.d1;
+              lxX[q0][0] = v.x0;
+              lxX[q0][1] = v.x1;
             }
           }
         }
       }
-      const int bpr = r - ss.dr[kb], bpc = c - ss.dc[kb];
-      ND[j] = bestD;
-      NP[j] = (bpr == 0) ? bpc : ((bpc == 0) ? -bpr : bestPp);
-      code[j] = kb;
+      __syncthreads();
     }
-
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      D2[j] = D1[j];
-      P2[j] = P1[j];
-      D1[j] = ND[j];
-      P1[j] = NP[j];
-    }
-
-    if constexpr (FULL) {
-      const int64_t base = (int64_t)r * W;
+  }
+  if constexpr (!FULL) {
+    // Last row of the tile (edge 0); R == 1 was written with row 0.
+    if (R > 1) {
 #pragma unroll
       for (int j = 0; j < CPL; ++j) {
         const int c = c0 + j;
         if (c < W) {
-          Df[base + c] = (double)ND[j];
-          Pf[base + c] = NP[j];
-          Bf[base + c] = (int8_t)code[j];
-        }
-      }
-    } else {
-      const int sh = 2 * (r & 15);
-#pragma unroll
-      for (int j = 0; j < CPL; ++j) bacc[j] |= (uint32_t)code[j] << sh;
-      if ((r & 15) == 15 || r == R - 1) {
-        uint32_t *dst = Bp + td.b_off + (int64_t)(r >> 4) * W;
-#pragma unroll
-        for (int j = 0; j < CPL; ++j) {
-          const int c = c0 + j;
-          if (c < W) dst[c] = bacc[j];
-          bacc[j] = 0u;
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < CPL; ++j) {
-        const int c = c0 + j;
-        if (c == W - 1) {
-          eD[td.e_off + Lmax + r] = (double)ND[j];
-          eP[td.e_off + Lmax + r] = NP[j];
-        }
-        if (r == R - 1 && c < W) {
-          eD[td.e_off + c] = (double)ND[j];
-          eP[td.e_off + c] = NP[j];
+          eD[td.e_off + c] = (double)D1[j];
+          eP[td.e_off + c] = p_of(X1[j], R - 1 + c);
         }
       }
     }
-
-    if (lane == 63) {
-      const int s0 = r % 3;
-      shD[s0][wave][0] = D1[CPL - 2];
-      shD[s0][wave][1] = D1[CPL - 1];
-      shP[s0][wave][0] = P1[CPL - 2];
-      shP[s0][wave][1] = P1[CPL - 1];
-    }
-    __syncthreads();
   }
 }
 
-template <int CPL, typename CT, typename DT>
-__global__ __launch_bounds__(1024) void k_stage1_packed(const CT *__restrict__ Cb,
-                                                        const TileDesc *__restrict__ tiles,
-                                                        StepSet ss, int64_t Lmax,
-                                                        uint32_t *__restrict__ Bp,
-                                                        double *__restrict__ eD,
-                                                        int32_t *__restrict__ eP) {
-  const TileDesc td = tiles[blockIdx.x];
-  tile_dp<CPL, CT, DT, false>(td, Cb, ss, Lmax, Bp, eD, eP, nullptr, nullptr, nullptr);
-}
-
-template <int CPL>
-__global__ __launch_bounds__(1024) void k_tile_full(const double *__restrict__ Cb, TileDesc td,
-                                                    StepSet ss, double *__restrict__ Df,
-                                                    int32_t *__restrict__ Pf,
-                                                    int8_t *__restrict__ Bf) {
-  tile_dp<CPL, double, double, true>(td, Cb, ss, 0, nullptr, nullptr, nullptr, Df, Pf, Bf);
-}
-
-// find_best_endpoint (FlexDTW.py:1098-1148): candidate i < n_row is
-// (R-1, buffer+i); later ones walk the last column upward from row R-2.
-__global__ __launch_bounds__(256) void k_tile_endpoint(const double *__restrict__ D,
-                                                       const int32_t *__restrict__ P,
-                                                       int64_t R, int64_t W, int64_t buffer,
-                                                       double *__restrict__ avg,
-                                                       double *__restrict__ best) {
-  __shared__ double sv[256];
-  __shared__ int64_t si[256];
-  const int64_t n_row = W > buffer ? W - buffer : 0;
-  const int64_t n_col = R - 1 > buffer ? R - 1 - buffer : 0;
-  const int64_t n = n_row + n_col;
-  double bv = __builtin_huge_val();
-  int64_t bi = -1;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-    int64_t r, c;
-    if (i < n_row) {
-      r = R - 1;
-      c = buffer + i;
-    } else {
-      r = R - 2 - (i - n_row);
-      c = W - 1;
-    }
-    const int64_t p = P[r * W + c];
-    const int64_t rs = p >= 0 ? 0 : -p, cs = p >= 0 ? p : 0;
-    const int64_t md = (r - rs) + (c - cs);
-    const double a = D[r * W + c] / (double)md;
-    avg[i] = a;
-    if (a < bv) {  // i increases within a thread: first minimum kept
-      bv = a;
-      bi = i;
-    }
-  }
-  sv[threadIdx.x] = bv;
-  si[threadIdx.x] = bi;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      const double ov = sv[threadIdx.x + s];
-      const int64_t oi = si[threadIdx.x + s];
-      if (oi >= 0 && (si[threadIdx.x] < 0 || ov < sv[threadIdx.x] ||
-                      (ov == sv[threadIdx.x] && oi < si[threadIdx.x]))) {
-        sv[threadIdx.x] = ov;
-        si[threadIdx.x] = oi;
-      }
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const int64_t i = si[0];
-    int64_t r = -1, c = -1;
-    if (i >= 0) {
-      if (i < n_row) {
-        r = R - 1;
-        c = buffer + i;
-      } else {
-        r = R - 2 - (i - n_row);
-        c = W - 1;
-      }
-    }
-    best[0] = i >= 0 ? sv[0] : __builtin_huge_val();
-    best[1] = (double)r;
-    best[2] = (double)c;
-  }
-}
-
-// backtrace_flexdtw + reverse (FlexDTW.py:1155-1180, :1366-1368).
-__global__ void k_tile_backtrace(const int8_t *__restrict__ B, int64_t R, int64_t W,
-                                 StepSet ss, const double *__restrict__ best,
-                                 int64_t *__restrict__ path, int64_t *__restrict__ len) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int64_t r = (int64_t)best[1], c = (int64_t)best[2];
-  if (r < 0 || c < 0) {
-    *len = 0;
-    return;
-  }
-  int64_t n = 0;
-  path[0] = r;
-  path[1] = c;
-  n = 1;
-  while (r != 0 && c != 0 && n < R + W) {
-    if (r < 0 || c < 0 || r >= R || c >= W) break;  // B always points inside the tile
-    const int k = B[r * W + c];
-    if (k < 0 || k >= ss.K) break;
-    r -= ss.dr[k];
-    c -= ss.dc[k];
-    path[2 * n] = r;
-    path[2 * n + 1] = c;
-    ++n;
-  }
-  for (int64_t a = 0, b = n - 1; a < b; ++a, --b) {
-    const int64_t t0 = path[2 * a], t1 = path[2 * a + 1];
-    path[2 * a] = path[2 * b];
-    path[2 * a + 1] = path[2 * b + 1];
-    path[2 * b] = t0;
-    path[2 * b + 1] = t1;
-  }
-  *len = n;
-}
-
-// Columns per lane and waves per workgroup for tiles up to Wmax columns.
-static int pick_cpl(int64_t Wmax) {
-  if (Wmax <= 2048) return 2;
-  if (Wmax <= 4096) return 4;
-  if (Wmax <= 8192) return 8;
-  return 0;
+template <typename CT, typename DT, bool STD, int CPL>
+static void launch_cpl(dim3 grid, dim3 block, hipStream_t st, const CT *C, const TileDesc *tiles,
+                       const TileDesc &td0, const StepSet &ss, int64_t Lmax, int S, int G,
+                       int *flags, void *xch, uint32_t *Bp, double *eD, int32_t *eP, double *Df,
+                       int32_t *Pf, int8_t *Bf, bool full) {
+  Xch<DT> *x = static_cast<Xch<DT> *>(xch);
+  if (full)
+    hipLaunchKernelGGL((k_stage1<CPL, CT, DT, true, STD>), grid, block, 0, st, C, tiles, td0, ss,
+                       Lmax, S, G, flags, x, Bp, eD, eP, Df, Pf, Bf);
+  else
+    hipLaunchKernelGGL((k_stage1<CPL, CT, DT, false, STD>), grid, block, 0, st, C, tiles, td0, ss,
+                       Lmax, S, G, flags, x, Bp, eD, eP, Df, Pf, Bf);
 }
 
 template <typename CT, typename DT>
-static int launch_stage1(const void *C, const int64_t *tiles, int64_t ntiles, int64_t Lmax,
-                         const StepSet &ss, uint32_t *Bp, double *eD, int32_t *eP,
-                         hipStream_t st) {
-  const int cpl = pick_cpl(Lmax);
-  PF_REQUIRE(cpl > 0, "tile length above 8192 is not supported");
-  const int nthreads = (int)(((Lmax + cpl - 1) / cpl + 63) / 64) * 64;
-  const dim3 grid((unsigned)ntiles), block((unsigned)nthreads);
-  const CT *Cp = static_cast<const CT *>(C);
-  const TileDesc *td = reinterpret_cast<const TileDesc *>(tiles);
-  if (cpl == 2)
-    hipLaunchKernelGGL((k_stage1_packed<2, CT, DT>), grid, block, 0, st, Cp, td, ss, Lmax, Bp,
-                       eD, eP);
-  else if (cpl == 4)
-    hipLaunchKernelGGL((k_stage1_packed<4, CT, DT>), grid, block, 0, st, Cp, td, ss, Lmax, Bp,
-                       eD, eP);
-  else
-    hipLaunchKernelGGL((k_stage1_packed<8, CT, DT>), grid, block, 0, st, Cp, td, ss, Lmax, Bp,
-                       eD, eP);
+static int launch_dp(int cpl, int nw, int S, int G, int64_t n_items, hipStream_t st, const CT *C,
+                     const TileDesc *tiles, const TileDesc &td0, const StepSet &ss, int64_t Lmax,
+                     int *flags, void *xch, uint32_t *Bp, double *eD, int32_t *eP, double *Df,
+                     int32_t *Pf, int8_t *Bf, bool full) {
+  const bool std3 = ss.K == 3 && ss.dr[0] == 1 && ss.dc[0] == 1 && ss.dr[1] == 1 &&
+                    ss.dc[1] == 2 && ss.dr[2] == 2 && ss.dc[2] == 1;
+  const dim3 grid((unsigned)n_items), block((unsigned)(nw * 64));
+#define PF_LAUNCH(STDV, CPLV)                                                                  \
+  launch_cpl<CT, DT, STDV, CPLV>(grid, block, st, C, tiles, td0, ss, Lmax, S, G, flags, xch,  \
+                                 Bp, eD, eP, Df, Pf, Bf, full)
+  if (std3) {
+    if (cpl == 2) PF_LAUNCH(true, 2);
+    else if (cpl == 4) PF_LAUNCH(true, 4);
+    else PF_LAUNCH(true, 8);
+  } else {
+    if (cpl == 2) PF_LAUNCH(false, 2);
+    else if (cpl == 4) PF_LAUNCH(false, 4);
+    else PF_LAUNCH(false, 8);
+  }
+#undef PF_LAUNCH
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
+}
+
+// Strip geometry: smallest columns-per-lane in {2,4,8} with S*NW*64*CPL >= W.
+static int pick_cpl(int64_t W, int S, int nw) {
+  for (int cpl = 2; cpl <= 8; cpl *= 2)
+    if ((int64_t)S * nw * 64 * cpl >= W) return cpl;
+  return 0;
+}
+
+__global__ void k_fill_i32(int *p, int v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
 }  // namespace pf
@@ -422,28 +602,67 @@ using namespace pf;
 
 extern "C" int pf_stage1(int dtype, const void *d_C, const int64_t *d_tiles, int64_t ntiles,
                          int64_t Lmax, const int32_t *steps, const double *weights, int K,
-                         uint32_t *d_Bpacked, double *d_edge_D, int32_t *d_edge_P,
+                         int strips, int waves, int rows_per_block, int32_t *d_flags,
+                         void *d_xch, uint32_t *d_Bpacked, double *d_edge_D, int32_t *d_edge_P,
                          void *stream) {
   StepSet ss;
   int rc = make_stepset(steps, weights, K, &ss);
   if (rc) return rc;
   PF_REQUIRE(ntiles >= 0 && Lmax >= 2, "pf_stage1: bad tile count or tile length");
+  PF_REQUIRE(strips >= 1 && waves >= 1 && waves <= 16, "pf_stage1: strips >= 1, 1 <= waves <= 16");
+  PF_REQUIRE(rows_per_block >= 1 && rows_per_block <= kMaxG, "pf_stage1: 1 <= G <= 64");
+  const int cpl = pick_cpl(Lmax, strips, waves);
+  PF_REQUIRE(cpl > 0, "pf_stage1: strips*waves*64*8 must cover the tile length");
+  PF_REQUIRE(waves * 64 <= (cpl == 2 ? 1024 : (cpl == 4 ? 512 : 256)),
+             "pf_stage1: at most 16/8/4 waves for 2/4/8 columns per lane");
+  PF_REQUIRE(Lmax <= 8192, "tile length above 8192 is not supported");
   if (ntiles == 0) return PF_OK;
   PF_REQUIRE(d_C && d_tiles && d_Bpacked && d_edge_D && d_edge_P, "pf_stage1: null buffer");
+  PF_REQUIRE(strips == 1 || (d_flags && d_xch), "pf_stage1: strips > 1 need flags and xch");
+  const int64_t n_items = ntiles * strips;
+  PF_REQUIRE(n_items < (1ll << 31), "pf_stage1: too many work items");
   hipStream_t st = as_stream(stream);
+  const TileDesc *tiles = reinterpret_cast<const TileDesc *>(d_tiles);
+  const TileDesc td0{};
+  int *flags = nullptr;
+  if (strips > 1) {
+    // d_flags: [0] ticket counter, [1 .. n_items] per-strip published rows.
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n_items + 1 + 255) / 256)), dim3(256), 0,
+                       st, d_flags, 0, n_items + 1);
+    PF_CHECK_HIP(hipGetLastError());
+    flags = d_flags + 1;
+  }
   switch (dtype) {
     case PF_F64:
-      return launch_stage1<double, double>(d_C, d_tiles, ntiles, Lmax, ss, d_Bpacked, d_edge_D,
-                                           d_edge_P, st);
+      return launch_dp<double, double>(cpl, waves, strips, rows_per_block, n_items, st,
+                                       static_cast<const double *>(d_C), tiles, td0, ss, Lmax,
+                                       flags, d_xch, d_Bpacked, d_edge_D, d_edge_P, nullptr,
+                                       nullptr, nullptr, false);
     case PF_F32:
-      return launch_stage1<float, float>(d_C, d_tiles, ntiles, Lmax, ss, d_Bpacked, d_edge_D,
-                                         d_edge_P, st);
+      return launch_dp<float, float>(cpl, waves, strips, rows_per_block, n_items, st,
+                                     static_cast<const float *>(d_C), tiles, td0, ss, Lmax, flags,
+                                     d_xch, d_Bpacked, d_edge_D, d_edge_P, nullptr, nullptr,
+                                     nullptr, false);
     case PF_BF16:
-      return launch_stage1<__hip_bfloat16, float>(d_C, d_tiles, ntiles, Lmax, ss, d_Bpacked,
-                                                  d_edge_D, d_edge_P, st);
+      return launch_dp<__hip_bfloat16, float>(cpl, waves, strips, rows_per_block, n_items, st,
+                                              static_cast<const __hip_bfloat16 *>(d_C), tiles,
+                                              td0, ss, Lmax, flags, d_xch, d_Bpacked, d_edge_D,
+                                              d_edge_P, nullptr, nullptr, nullptr, false);
     default:
       PF_REQUIRE(false, "pf_stage1: unknown dtype");
   }
+}
+
+extern "C" int pf_stage1_workspace(int dtype, int64_t ntiles, int64_t Lmax, int strips,
+                                   int64_t *flags_count, int64_t *xch_bytes) {
+  PF_REQUIRE(flags_count && xch_bytes, "pf_stage1_workspace: null output");
+  PF_REQUIRE(ntiles >= 0 && Lmax >= 2 && strips >= 1, "pf_stage1_workspace: bad arguments");
+  const int64_t esz = dtype == PF_F64 ? (int64_t)sizeof(Xch<double>) : (int64_t)sizeof(Xch<float>);
+  PF_REQUIRE(dtype == PF_F64 || dtype == PF_F32 || dtype == PF_BF16,
+             "pf_stage1_workspace: unknown dtype");
+  *flags_count = strips > 1 ? ntiles * strips + 1 : 0;
+  *xch_bytes = strips > 1 ? ntiles * strips * Lmax * esz : 0;
+  return PF_OK;
 }
 
 extern "C" int pf_tile_full(const double *d_C, int64_t ldc, int64_t r0, int64_t r1, int64_t c0,
@@ -455,43 +674,12 @@ extern "C" int pf_tile_full(const double *d_C, int64_t ldc, int64_t r0, int64_t 
   const int64_t R = r1 - r0, W = c1 - c0;
   PF_REQUIRE(R > 0 && W > 0, "Empty chunk");
   PF_REQUIRE(r0 >= 0 && c0 >= 0 && c1 <= ldc, "pf_tile_full: chunk outside C");
+  PF_REQUIRE(W <= 8192, "tile width above 8192 is not supported");
   PF_REQUIRE(d_C && d_D && d_P && d_B, "pf_tile_full: null buffer");
-  const int cpl = pick_cpl(W);
-  PF_REQUIRE(cpl > 0, "tile width above 8192 is not supported");
-  TileDesc td{r0 * ldc + c0, ldc, R, W, 0, 0, 0, 0};
-  const int nthreads = (int)(((W + cpl - 1) / cpl + 63) / 64) * 64;
-  hipStream_t st = as_stream(stream);
-  if (cpl == 2)
-    hipLaunchKernelGGL(k_tile_full<2>, dim3(1), dim3(nthreads), 0, st, d_C, td, ss, d_D, d_P, d_B);
-  else if (cpl == 4)
-    hipLaunchKernelGGL(k_tile_full<4>, dim3(1), dim3(nthreads), 0, st, d_C, td, ss, d_D, d_P, d_B);
-  else
-    hipLaunchKernelGGL(k_tile_full<8>, dim3(1), dim3(nthreads), 0, st, d_C, td, ss, d_D, d_P, d_B);
-  PF_CHECK_HIP(hipGetLastError());
-  return PF_OK;
-}
-
-extern "C" int pf_tile_endpoint(const double *d_D, const int32_t *d_P, int64_t R, int64_t W,
-                                int64_t buffer, double *d_avg, double *d_best, void *stream) {
-  PF_REQUIRE(R > 0 && W > 0 && buffer >= 0, "pf_tile_endpoint: bad shape or buffer");
-  PF_REQUIRE(d_D && d_P && d_avg && d_best, "pf_tile_endpoint: null buffer");
-  hipLaunchKernelGGL(k_tile_endpoint, dim3(1), dim3(256), 0, as_stream(stream), d_D, d_P, R, W,
-                     buffer, d_avg, d_best);
-  PF_CHECK_HIP(hipGetLastError());
-  return PF_OK;
-}
-
-extern "C" int pf_tile_backtrace(const int8_t *d_B, int64_t R, int64_t W, const int32_t *steps,
-                                 int K, const double *d_best, int64_t *d_path, int64_t *d_len,
-                                 void *stream) {
-  StepSet ss;
-  double w[4] = {1, 1, 1, 1};
-  int rc = make_stepset(steps, w, K, &ss);
-  if (rc) return rc;
-  PF_REQUIRE(R > 0 && W > 0, "pf_tile_backtrace: bad shape");
-  PF_REQUIRE(d_B && d_best && d_path && d_len, "pf_tile_backtrace: null buffer");
-  hipLaunchKernelGGL(k_tile_backtrace, dim3(1), dim3(64), 0, as_stream(stream), d_B, R, W, ss,
-                     d_best, d_path, d_len);
-  PF_CHECK_HIP(hipGetLastError());
-  return PF_OK;
+  int cpl = W <= 2048 ? 2 : (W <= 4096 ? 4 : 8);
+  const int nw = (int)(((W + cpl - 1) / cpl + 63) / 64);
+  const TileDesc td{r0 * ldc + c0, ldc, R, W, 0, 0, 0, 0};
+  return launch_dp<double, double>(cpl, nw, 1, 16, 1, as_stream(stream), d_C, nullptr, td, ss,
+                                   W, nullptr, nullptr, nullptr, nullptr, nullptr, d_D, d_P, d_B,
+                                   true);
 }

@@ -6,6 +6,8 @@ stitched paths and best costs are copied back.
 """
 from __future__ import annotations
 
+import ctypes
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -67,12 +69,57 @@ def _stream(stream):
     return stream.cuda_stream
 
 
-def stage1(C_dev, plan, sw, ws, stream=None):
+def _cpl_for(width, strips, waves):
+    for cpl in (2, 4, 8):
+        if strips * waves * 64 * cpl >= width:
+            return cpl
+    return 0
+
+
+def stage1_config(plan, n_cu=256):
+    """Launch shape of the stage-1 DP: (strips per tile, waves per workgroup,
+    rows per exchange block).  Few tiles are cut into column strips so that
+    the grid covers the machine; many tiles run one workgroup each.
+    PARFLEX_S1_STRIPS / PARFLEX_S1_WAVES / PARFLEX_S1_G override the choice."""
+    L = plan.L
+    env = os.environ
+    if "PARFLEX_S1_STRIPS" in env or "PARFLEX_S1_WAVES" in env:
+        S = int(env.get("PARFLEX_S1_STRIPS", "1"))
+        nw = int(env.get("PARFLEX_S1_WAVES", "0"))
+        if nw <= 0:
+            nw = max(1, -(-L // (S * 128)))
+        G = int(env.get("PARFLEX_S1_G", "16"))
+        return S, nw, G
+    # Workgroups of up to 4 waves, 2 columns per lane (512-column strips):
+    # the best measured shape for both one 16000 x 18209 pair at L = 4000
+    # (8 strips per tile) and 1195 tiles at L = 500 (one workgroup per tile).
+    nw = max(1, min(4, -(-L // 128)))
+    S = -(-L // (nw * 128))
+    return S, nw, int(env.get("PARFLEX_S1_G", "32"))
+
+
+def stage1(C_dev, plan, sw, ws, stream=None, config=None):
     """Per-tile DP for every tile of the plan: packed back-pointers plus edge
     D/P (replaces the tile loop at Parflex_gpu.py:461-563)."""
+    S, nw, G = config or ws.get("s1_config") or stage1_config(plan)
+    torch = _torch()
+    if S > 1:
+        nf, nx = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().pf_stage1_workspace(dtype_code(C_dev), plan.ntiles, plan.L, S,
+                                        ctypes.byref(nf), ctypes.byref(nx)), "pf_stage1_workspace")
+        need_f = nf.value
+        need_x = -(-nx.value // 8)
+        if ws.get("s1_flags") is None or ws["s1_flags"].numel() < need_f:
+            ws["s1_flags"] = torch.empty(need_f, dtype=torch.int32, device=C_dev.device)
+        if ws.get("s1_xch") is None or ws["s1_xch"].numel() < need_x:
+            ws["s1_xch"] = torch.empty(need_x, dtype=torch.float64, device=C_dev.device)
+        fl, xc = ws["s1_flags"].data_ptr(), ws["s1_xch"].data_ptr()
+    else:
+        fl = xc = None
     rc = lib().pf_stage1(dtype_code(C_dev), C_dev.data_ptr(), ws["tiles"].data_ptr(),
-                         plan.ntiles, plan.L, sw.sp, sw.wp, sw.K, ws["Bp"].data_ptr(),
-                         ws["edge_D"].data_ptr(), ws["edge_P"].data_ptr(), _stream(stream))
+                         plan.ntiles, plan.L, sw.sp, sw.wp, sw.K, S, nw, G, fl, xc,
+                         ws["Bp"].data_ptr(), ws["edge_D"].data_ptr(), ws["edge_P"].data_ptr(),
+                         _stream(stream))
     check(rc, "pf_stage1")
 
 

@@ -171,20 +171,37 @@ def test_cfg0_workload_matches_reference(gpu):
     assert _same(r["stitched_wp"].T, g["parflex_wp"])
 
 
-def test_stage1_packed_outputs_vs_oracle(gpu, oracle_mod):
-    """Edge D/P and every packed back-pointer of a multi-tile stage 1."""
+S1_CONFIGS = [  # (shape, L, steps, weights, (strips, waves, rows per exchange block))
+    ((333, 410), 70, STEPS, [2, 3, 3], None),
+    ((333, 410), 70, STEPS, [2, 3, 3], (1, 1, 16)),
+    ((333, 410), 300, STEPS, [2, 3, 3], (3, 1, 7)),
+    ((333, 410), 300, STEPS, [2, 3, 3], (2, 2, 1)),
+    ((700, 1500), 700, STEPS, [1.25, 3, 3], (6, 1, 64)),
+    ((700, 1500), 700, STEPS, [2, 3, 3], (4, 2, 16)),
+    ((130, 3000), 2999, STEPS, [2, 3, 3], (5, 5, 16)),
+    ((500, 520), 260, [(1, 1), (1, 2), (2, 1), (2, 2)], [2, 3, 3, 4], (3, 1, 5)),
+    ((500, 520), 260, [(1, 0), (1, 1), (2, 1)], [1, 2, 3], (2, 2, 3)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(S1_CONFIGS)))
+def test_stage1_packed_outputs_vs_oracle(gpu, oracle_mod, case):
+    """Edge D/P and every packed back-pointer of a multi-tile stage 1, for
+    one-workgroup tiles and for tiles cut into pipelined column strips
+    (ragged last strips, idle strips, exchange blocks of 1..64 rows, generic
+    and specialised step sets)."""
     import torch
     from parflexdtw_amd import engine
     from parflexdtw_amd._plan import BatchPlan
-    rng = np.random.RandomState(21)
-    C = rng.rand(333, 410)
-    L = 70
+    shape, L, steps, weights, cfg = S1_CONFIGS[case]
+    rng = np.random.RandomState(21 + case)
+    C = rng.rand(*shape)
     plan = BatchPlan([C.shape], L)
-    sw = engine.StepWeights(STEPS, [2, 3, 3])
+    sw = engine.StepWeights(steps, weights)
     C_dev = torch.from_numpy(C).cuda()
     ws = plan.allocate(C_dev.device)
-    engine.stage1(C_dev, plan, sw, ws)
-    s1 = oracle_mod.run_stage1(C, L, STEPS, [2, 3, 3])
+    engine.stage1(C_dev, plan, sw, ws, config=cfg)
+    s1 = oracle_mod.run_stage1(C, L, steps, weights)
     n1, n2 = plan.grids[0]
     eD = ws["edge_D"].cpu().numpy().reshape(n1, n2, 2, L)
     eP = ws["edge_P"].cpu().numpy().reshape(n1, n2, 2, L)

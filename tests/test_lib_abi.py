@@ -49,6 +49,21 @@ def test_step_set_validation():
     assert _check([(1, 1)], [float("nan")]) == _lib.PF_EINVAL
 
 
+def test_stage1_workspace_sizes():
+    from parflexdtw_amd import _lib
+    h = _lib.lib()
+    nf, nx = ctypes.c_int64(), ctypes.c_int64()
+    assert h.pf_stage1_workspace(_lib.PF_F64, 25, 4000, 16, ctypes.byref(nf),
+                                 ctypes.byref(nx)) == 0
+    assert nf.value == 25 * 16 + 1 and nx.value == 25 * 16 * 4000 * 32
+    assert h.pf_stage1_workspace(_lib.PF_BF16, 3, 100, 2, ctypes.byref(nf),
+                                 ctypes.byref(nx)) == 0
+    assert nf.value == 7 and nx.value == 3 * 2 * 100 * 16
+    assert h.pf_stage1_workspace(_lib.PF_F32, 3, 100, 1, ctypes.byref(nf),
+                                 ctypes.byref(nx)) == 0
+    assert nf.value == 0 and nx.value == 0
+
+
 def test_errors_map_to_exceptions():
     from parflexdtw_amd import engine
     with pytest.raises(ValueError, match="step"):
@@ -63,15 +78,22 @@ def test_stage1_rejects_bad_arguments_without_device_work():
     s = np.array([1, 1], np.int32)
     w = np.array([1.0])
     # zero tiles: nothing to do
-    assert h.pf_stage1(0, None, None, 0, 10, s.ctypes.data, w.ctypes.data, 1, None, None, None,
-                       None) == 0
+    assert h.pf_stage1(0, None, None, 0, 10, s.ctypes.data, w.ctypes.data, 1, 1, 1, 16, None,
+                       None, None, None, None, None) == 0
     # unknown dtype
     fake = ctypes.c_void_p(16)
-    assert h.pf_stage1(7, fake, fake, 1, 10, s.ctypes.data, w.ctypes.data, 1, fake, fake, fake,
-                       None) == _lib.PF_EINVAL
+    assert h.pf_stage1(7, fake, fake, 1, 10, s.ctypes.data, w.ctypes.data, 1, 1, 1, 16, None,
+                       None, fake, fake, fake, None) == _lib.PF_EINVAL
     # tile length above the supported maximum
-    assert h.pf_stage1(0, fake, fake, 1, 9000, s.ctypes.data, w.ctypes.data, 1, fake, fake,
-                       fake, None) == _lib.PF_EINVAL
+    assert h.pf_stage1(0, fake, fake, 1, 9000, s.ctypes.data, w.ctypes.data, 1, 2, 16, 16, fake,
+                       fake, fake, fake, fake, None) == _lib.PF_EINVAL
+    # strips too narrow to cover the tile, and strips without exchange buffers
+    assert h.pf_stage1(0, fake, fake, 1, 4000, s.ctypes.data, w.ctypes.data, 1, 1, 1, 16, None,
+                       None, fake, fake, fake, None) == _lib.PF_EINVAL
+    assert h.pf_stage1(0, fake, fake, 1, 4000, s.ctypes.data, w.ctypes.data, 1, 8, 4, 16, None,
+                       None, fake, fake, fake, None) == _lib.PF_EINVAL
+    assert h.pf_stage1(0, fake, fake, 1, 400, s.ctypes.data, w.ctypes.data, 1, 1, 4, 65, None,
+                       None, fake, fake, fake, None) == _lib.PF_EINVAL
     assert h.pf_tile_full(fake, 10, 3, 3, 0, 5, s.ctypes.data, w.ctypes.data, 1, fake, fake,
                           fake, None) == _lib.PF_EINVAL
     assert "Empty chunk" in _lib.last_error()
