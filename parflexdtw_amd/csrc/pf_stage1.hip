@@ -22,12 +22,15 @@
 // columns are cut into S strips of SW = NW*64*CPL columns; each (tile, strip)
 // is one workgroup of NW waves that sweeps the rows top to bottom, CPL
 // consecutive columns per lane, rows r-1 and r-2 in registers.  A lane takes
-// its two left neighbour columns from lane-1 (shuffle), lane 0 of each wave
-// from the previous wave through a 3-slot LDS ring (one barrier per row), and
-// lane 0 of wave 0 from the previous strip through a global exchange buffer:
-// the strip to the left publishes its last two columns (D, P) row by row and
-// bumps a per-strip row counter (release, agent scope) every G rows; the
-// strip to the right waits on it (acquire) before each block of G rows.
+// its two left neighbour columns of row r-1 from lane-1 (DPP wave_shr), lane
+// 0 of each wave from the wave on its left through an 8-row LDS ring with a
+// per-wave progress counter (waves run decoupled, no barrier per row: a wave
+// waits only for its left neighbour's previous row and, before overwriting a
+// ring slot, for its right neighbour to have read it), and lane 0 of wave 0
+// from the previous strip through a global exchange buffer: the strip to the
+// left publishes its last two columns row by row and bumps a per-strip row
+// counter (release, agent scope) every G rows; the strip to the right waits
+// on it (acquire) before each block of G rows.
 // Work items are taken from a ticket counter in strip order, so a strip only
 // ever waits for an item that is already running: no deadlock whatever the
 // grid size or residency.
@@ -101,6 +104,7 @@ static_assert(sizeof(Xch<double>) == 32 && sizeof(Xch<float>) == 16,
 
 constexpr int kMaxG = 64;   // rows per exchange block, upper bound
 constexpr int kPF = 4;      // rows of C prefetched ahead
+constexpr int kRing = 8;    // rows of wave-to-wave hand-over kept in LDS
 
 // Lane i receives lane i-1's value (DPP wave_shr:1); lane 0 keeps `old`.
 __device__ __forceinline__ int shr1(int v, int old) {
@@ -150,8 +154,9 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
     uint32_t *__restrict__ Bp, double *__restrict__ eD, int32_t *__restrict__ eP,
     double *__restrict__ Df, int32_t *__restrict__ Pf, int8_t *__restrict__ Bf) {
   static_assert(CPL >= 2, "a lane needs two columns of left context");
-  __shared__ DT shD[3][16][2];
-  __shared__ DT shX[3][16][2];
+  __shared__ DT shD[kRing][16][2];
+  __shared__ DT shX[kRing][16][2];
+  __shared__ int prog[16];
   __shared__ DT lxD[kMaxG][2];
   __shared__ DT lxX[kMaxG][2];
   __shared__ int sh_item;
@@ -200,11 +205,18 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
   // Left context: columns c0-2 (a), c0-1 (b) of rows r-1 (1) and r-2 (2).
   DT LD2a = INF, LD2b = INF, LX2a = 0, LX2b = 0;
 
+  // Column of each lane slot clamped into the tile: loads are unconditional
+  // (no branch around them), values of columns past the tile are unused.
+  int cl[CPL];
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) cl[j] = (c0 + j < W) ? c0 + j : W - 1;
+
   // Row 0: boundary cells, P = 0, X = +col.
 #pragma unroll
   for (int j = 0; j < CPL; ++j) {
     const int c = c0 + j;
-    D1[j] = (c < W) ? load_c<DT>(C + c) : INF;
+    const DT v = load_c<DT>(C + cl[j]);
+    D1[j] = (c < W) ? v : INF;
     X1[j] = (DT)c;
     D2[j] = INF;
     X2[j] = 0;
@@ -229,12 +241,9 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
   }
 #pragma unroll
   for (int u = 0; u < kPF; ++u) {
-    const int row = 1 + u;
+    const int row = (1 + u < R) ? 1 + u : R - 1;
 #pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      const int c = c0 + j;
-      Cq[u][j] = (row < R && c < W) ? load_c<DT>(C + (int64_t)row * ldc + c) : (DT)0;
-    }
+    for (int j = 0; j < CPL; ++j) Cq[u][j] = load_c<DT>(C + (int64_t)row * ldc + cl[j]);
   }
   if (producer) {
     xout[0] = Xch<DT>{D1[CPL - 2], D1[CPL - 1], X1[CPL - 2], X1[CPL - 1]};
@@ -245,7 +254,10 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
     shD[0][wave][1] = D1[CPL - 1];
     shX[0][wave][0] = X1[CPL - 2];
     shX[0][wave][1] = X1[CPL - 1];
+    prog[wave] = 0;
   }
+  const int nwaves = (int)blockDim.x >> 6;
+  int seen_left = 0, seen_right = 0;  // last observed progress of the neighbour waves
   int blk0 = 1;  // first row of the current exchange block
   // Rows 0 .. G-1 of the strip on the left feed rows 1 .. G.
   if (consumer && wave == 0 && R > 1) {
@@ -264,41 +276,32 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
   }
   __syncthreads();
 
-  for (int rb = 1; rb < R; rb += kPF) {
-#pragma unroll
-    for (int u = 0; u < kPF; ++u) {
-      const int r = rb + u;
-      if (r >= R) break;
-      DT Cc[CPL];
-#pragma unroll
-      for (int j = 0; j < CPL; ++j) Cc[j] = Cq[u][j];
-      if (r + kPF < R) {
-        const CT *row = C + (int64_t)(r + kPF) * ldc;
-#pragma unroll
-        for (int j = 0; j < CPL; ++j) {
-          const int c = c0 + j;
-          Cq[u][j] = (c < W) ? load_c<DT>(row + c) : (DT)0;
-        }
-      }
-
+  // One row of the sweep; Cc holds C of row r.
+  auto row_step = [&](const int r, const DT (&Cc)[CPL]) {
       // Row r-1 of columns c0-2, c0-1: from lane-1 by DPP; lane 0 from the
       // previous wave (LDS ring), the previous strip (exchange rows) or the
       // missing columns -2, -1 of the tile.
-      DT oD1a = INF, oD1b = INF, oX1a = 0, oX1b = 0;
-      if (lane == 0) {
-        if (wave > 0) {
-          const int s1 = (r + 2) % 3;  // slot of row r-1
-          oD1a = shD[s1][wave - 1][0];
-          oD1b = shD[s1][wave - 1][1];
-          oX1a = shX[s1][wave - 1][0];
-          oX1b = shX[s1][wave - 1][1];
-        } else if (consumer) {
-          const int i1 = r - blk0;
-          oD1a = lxD[i1][0];
-          oD1b = lxD[i1][1];
-          oX1a = lxX[i1][0];
-          oX1b = lxX[i1][1];
-        }
+      DT oD1a = INF, oD1b = INF, oX1a = 0, oX1b = 0;  // only lane 0's copy is used
+      if (wave > 0 && seen_left < r - 1) {
+        // wait until the wave on the left has handed over row r-1
+        do {
+          seen_left = __hip_atomic_load(&prog[wave - 1], __ATOMIC_ACQUIRE,
+                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (seen_left < r - 1) __builtin_amdgcn_s_sleep(1);
+        } while (seen_left < r - 1);
+      }
+      if (wave > 0) {
+        const int s1 = (r - 1) & (kRing - 1);  // slot of row r-1
+        oD1a = shD[s1][wave - 1][0];
+        oD1b = shD[s1][wave - 1][1];
+        oX1a = shX[s1][wave - 1][0];
+        oX1b = shX[s1][wave - 1][1];
+      } else if (consumer) {
+        const int i1 = r - blk0;
+        oD1a = lxD[i1][0];
+        oD1b = lxD[i1][1];
+        oX1a = lxX[i1][0];
+        oX1b = lxX[i1][1];
       }
       const DT LD1a = shr1(D1[CPL - 2], oD1a), LD1b = shr1(D1[CPL - 1], oD1b);
       const DT LX1a = shr1(X1[CPL - 2], oX1a), LX1b = shr1(X1[CPL - 1], oX1b);
@@ -306,7 +309,8 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
       DT ND[CPL], NX[CPL];
       int code[CPL];
       bool amb = false;  // some cell of this lane needs the exact comparison
-      bool border = false;
+      // rows 1-2 and columns 0-2 of the tile: predecessors on the boundary
+      const bool border = (r <= 2) || (cbeg + wave * 64 * CPL <= 2);
 #pragma unroll
       for (int j = 0; j < CPL; ++j) {
         const int c = c0 + j;
@@ -383,7 +387,6 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
         ND[j] = ba;
         NX[j] = copy_sign(bm, bx);
         code[j] = bk;
-        border = border || (c <= 2) || (r <= 2);
       }
       if constexpr (STD) {
         if (amb) {
@@ -498,12 +501,21 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
         if ((r + 1) % G == 0 || r == R - 1)
           __hip_atomic_store(fout, r + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       }
+      // Hand row r to the wave on the right once it has read row r - kRing.
+      if (wave + 1 < nwaves && seen_right < r - kRing + 1) {
+        do {
+          seen_right = __hip_atomic_load(&prog[wave + 1], __ATOMIC_ACQUIRE,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (seen_right < r - kRing + 1) __builtin_amdgcn_s_sleep(1);
+        } while (seen_right < r - kRing + 1);
+      }
       if (lane == 63) {
-        const int s0 = r % 3;
+        const int s0 = r & (kRing - 1);
         shD[s0][wave][0] = D1[CPL - 2];
         shD[s0][wave][1] = D1[CPL - 1];
         shX[s0][wave][0] = X1[CPL - 2];
         shX[s0][wave][1] = X1[CPL - 1];
+        __hip_atomic_store(&prog[wave], r, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
       // Next exchange block: wait for the strip on the left and stage its
       // rows blk0-1 .. blk0+G-2 (row r-1 of every row r of the block).
@@ -524,10 +536,34 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
               lxX[q0][1] = v.x1;
             }
           }
+          // lane 0 of this wave reads the staged rows from the next row on
+          __threadfence_block();
         }
       }
-      __syncthreads();
+  };
+
+  // Main sweep in blocks of kPF rows with no early exit, so the C loads for
+  // kPF rows ahead stay in flight across rows (a branch or loop exit around
+  // them would make the compiler wait for all outstanding loads).
+  int rb = 1;
+  for (; rb + kPF <= R; rb += kPF) {
+#pragma unroll
+    for (int u = 0; u < kPF; ++u) {
+      const int r = rb + u;
+      row_step(r, Cq[u]);
+      // Refill the slot only after its row is done, so the load targets the
+      // same registers (no copy of an in-flight load at the loop edge).
+      const int rp = (r + kPF < R) ? r + kPF : R - 1;  // clamped: always a valid row
+      const CT *row = C + (int64_t)rp * ldc;
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) Cq[u][j] = load_c<DT>(row + cl[j]);
     }
+  }
+  // Remaining rows (fewer than kPF), already prefetched.
+#pragma unroll
+  for (int u = 0; u < kPF; ++u) {
+    const int r = rb + u;
+    if (r < R) row_step(r, Cq[u]);
   }
   if constexpr (!FULL) {
     // Last row of the tile (edge 0); R == 1 was written with row 0.

@@ -546,94 +546,106 @@ __global__ void k_s2_chain(int from_best, const int64_t *__restrict__ req, int64
 }
 
 // _nb_backtrace_within_chunk (Parflex.py:976-1012) for chain segment
-// blockIdx.x of request blockIdx.y, one wave: all lanes stage a 64-row x
-// 64-column window of packed back-pointers in LDS, lane 0 walks inside it.
+// blockIdx.x of request blockIdx.y, one wave.  The wave holds a 64-row x
+// 64-column window of packed back-pointers ending at the current cell in
+// registers (lane l: column wc0+l, one word per 16-row block) and walks it
+// with wave-uniform scalar state, reading each step's word with v_readlane
+// (no memory access per step).  Points are buffered one per lane and written
+// 64 at a time.
 __global__ __launch_bounds__(64) void k_s2_walk(const TileDesc *__restrict__ tiles, int64_t L,
                                                 StepSet ss, const uint32_t *__restrict__ Bp,
                                                 int64_t *__restrict__ chain,
                                                 const int64_t *__restrict__ cnt,
                                                 int64_t *__restrict__ pts) {
-  __shared__ uint32_t win[4][64];
-  __shared__ int64_t st[4];  // r, c, k, done
   const int64_t q = blockIdx.y, s = blockIdx.x;
   if (s >= cnt[q]) return;
   int64_t *en = chain + (q * 101 + s) * 8;
   const int64_t t = en[0];
   const TileDesc td = tiles[t];
-  const int64_t rows = td.R, cols = td.W;
-  const int64_t sr = en[3], sc = en[4], gro = en[5], gco = en[6];
+  const int rows = (int)td.R, cols = (int)td.W;
+  const int sr = (int)en[3], sc = (int)en[4];
+  const int64_t gro = en[5], gco = en[6];
   int64_t *out = pts + (q * 101 + s) * (2 * L + 8) * 2;
   const int lane = threadIdx.x;
-  const int64_t max_iters = rows + cols;
-  if (lane == 0) {
-    st[0] = en[1];
-    st[1] = en[2];
-    st[2] = 0;
-    st[3] = 0;
-  }
-  __syncthreads();
-  int64_t it = 0;  // lane 0's iteration counter
-  while (true) {
-    const int64_t r = st[0], c = st[1];
-    if (st[3]) break;
-    const int64_t rb_hi = r >> 4;
-    const int64_t rb_lo = rb_hi - 3 > 0 ? rb_hi - 3 : 0;
-    const int64_t clo = c - 63 > 0 ? c - 63 : 0;
-    for (int qq = 0; qq < 4; ++qq) {
-      const int64_t rb = rb_lo + qq;
-      const int64_t cc = clo + lane;
-      if (rb <= rb_hi && cc <= c) win[qq][lane] = Bp[td.b_off + rb * cols + cc];
+  const int max_iters = rows + cols;
+  // Walk state is wave-uniform; readfirstlane keeps it in scalar registers.
+  int r = __builtin_amdgcn_readfirstlane((int)en[1]);
+  int c = __builtin_amdgcn_readfirstlane((int)en[2]);
+  int k = 0, it = 0;
+  int br = 0, bc = 0;  // this lane's buffered point (tile-local)
+  auto record = [&](int pr, int pc) {
+    if (lane == (k & 63)) {
+      br = pr;
+      bc = pc;
     }
-    __syncthreads();
-    if (lane == 0) {
-      int64_t rr = r, cc = c, k = st[2];
-      bool done = false;
-      while (true) {
-        if (it >= max_iters) {
-          if (rr != sr || cc != sc) {
-            out[2 * k] = sr + gro;
-            out[2 * k + 1] = sc + gco;
-            ++k;
-          }
-          done = true;
-          break;
-        }
-        if ((rr >> 4) < rb_lo || cc < clo) break;  // left the window: reload
-        out[2 * k] = rr + gro;
-        out[2 * k + 1] = cc + gco;
-        ++k;
-        ++it;
-        if (rr == sr && cc == sc) {
-          done = true;
-          break;
-        }
-        const int code = (win[(rr >> 4) - rb_lo][cc - clo] >> (2 * (rr & 15))) & 3u;
-        if (code >= ss.K) {
-          out[2 * k] = sr + gro;
-          out[2 * k + 1] = sc + gco;
-          ++k;
-          done = true;
-          break;
-        }
-        const int64_t pr = rr - ss.dr[code], pc = cc - ss.dc[code];
-        if (pr < 0 || pc < 0 || pr >= rows || pc >= cols) {
-          out[2 * k] = sr + gro;
-          out[2 * k + 1] = sc + gco;
-          ++k;
-          done = true;
-          break;
-        }
-        rr = pr;
-        cc = pc;
+    if ((k & 63) == 63) {
+      const int64_t o = (int64_t)(k - 63 + lane) * 2;
+      out[o] = br + gro;
+      out[o + 1] = bc + gco;
+    }
+    ++k;
+  };
+  // step table packed 4 bits per step: dr in bits 0-1, dc in bits 2-3
+  int steptab = 0;
+  for (int z = 0; z < 4; ++z) steptab |= (ss.dr[z] | (ss.dc[z] << 2)) << (4 * z);
+  bool done = false;
+  while (!done) {
+    // Window of 4 row blocks (64 rows) x 64 columns ending at (r, c), one
+    // word per row block in each lane: lane l holds column wc0 + l.
+    const int rb0 = (r >> 4) - 3 > 0 ? (r >> 4) - 3 : 0;
+    const int wc0 = c - 63 > 0 ? c - 63 : 0;
+    const int cc = wc0 + lane <= c ? wc0 + lane : c;  // clamped, never read past c
+    uint32_t w0, w1, w2, w3;
+    {
+      const int rmax = r >> 4;
+      const uint32_t *base = Bp + td.b_off + cc;
+      w0 = base[(int64_t)(rb0 + 0 <= rmax ? rb0 + 0 : rmax) * cols];
+      w1 = base[(int64_t)(rb0 + 1 <= rmax ? rb0 + 1 : rmax) * cols];
+      w2 = base[(int64_t)(rb0 + 2 <= rmax ? rb0 + 2 : rmax) * cols];
+      w3 = base[(int64_t)(rb0 + 3 <= rmax ? rb0 + 3 : rmax) * cols];
+    }
+    while (true) {
+      if (it >= max_iters) {
+        if (r != sr || c != sc) record(sr, sc);
+        done = true;
+        break;
       }
-      st[0] = rr;
-      st[1] = cc;
-      st[2] = k;
-      st[3] = done ? 1 : 0;
+      if ((r >> 4) < rb0 || c < wc0) break;  // left the window: load the next one
+      record(r, c);
+      ++it;
+      if (r == sr && c == sc) {
+        done = true;
+        break;
+      }
+      const int qi = (r >> 4) - rb0;
+      const uint32_t wsel = qi == 0 ? w0 : (qi == 1 ? w1 : (qi == 2 ? w2 : w3));
+      const uint32_t word = __builtin_amdgcn_readlane(wsel, c - wc0);
+      const int code = (int)((word >> (2 * (r & 15))) & 3u);
+      if (code >= ss.K) {
+        record(sr, sc);
+        done = true;
+        break;
+      }
+      const int e = steptab >> (4 * code);
+      const int pr = r - (e & 3), pc = c - ((e >> 2) & 3);
+      // dr >= 1 and dc >= 0, so only the lower bounds can be crossed
+      if ((pr | pc) < 0) {
+        record(sr, sc);
+        done = true;
+        break;
+      }
+      r = pr;
+      c = pc;
     }
-    __syncthreads();
   }
-  if (lane == 0) en[7] = st[2];
+  // flush the partial block of buffered points
+  const int base = k & ~63;
+  if (lane < (k & 63)) {
+    const int64_t o = (int64_t)(base + lane) * 2;
+    out[o] = br + gro;
+    out[o + 1] = bc + gco;
+  }
+  if (lane == 0) en[7] = k;
 }
 
 // Concatenate the segments of request q, drop the repeated overlap cell at
@@ -661,20 +673,29 @@ __global__ __launch_bounds__(256) void k_s2_compact(const int64_t *__restrict__ 
     cap = path_cap;
     dst = path + q * path_cap * 2;
   }
+  // Per segment (in parallel): its length and whether its first point repeats
+  // the previous segment's last point; then a serial prefix over <= 101 values.
+  __shared__ int64_t len_s[101];
+  for (int64_t s = threadIdx.x; s < ns; s += blockDim.x) {
+    const int64_t n = ch[s * 8 + 7];
+    int64_t sk = 0;
+    if (s > 0 && n > 0) {
+      const int64_t *prev = pts + (q * 101 + s - 1) * stride;
+      const int64_t pn = ch[(s - 1) * 8 + 7];
+      const int64_t *cur = pts + (q * 101 + s) * stride;
+      if (pn > 0 && prev[2 * (pn - 1)] == cur[0] && prev[2 * (pn - 1) + 1] == cur[1]) sk = 1;
+    }
+    len_s[s] = n;
+    skip[s] = sk;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
     int64_t o = 0;
     for (int64_t s = 0; s < ns; ++s) {
-      const int64_t n = ch[s * 8 + 7];
-      int64_t sk = 0;
-      if (s > 0 && n > 0 && o > 0) {
-        const int64_t *prev = pts + (q * 101 + s - 1) * stride;
-        const int64_t pn = ch[(s - 1) * 8 + 7];
-        const int64_t *cur = pts + (q * 101 + s) * stride;
-        if (pn > 0 && prev[2 * (pn - 1)] == cur[0] && prev[2 * (pn - 1) + 1] == cur[1]) sk = 1;
-      }
+      // a join can only repeat a point once something has been emitted
+      if (o == 0) skip[s] = 0;
       off[s] = o;
-      skip[s] = sk;
-      o += n - sk;
+      o += len_s[s] - skip[s];
     }
     total = o < cap ? o : cap;
   }
