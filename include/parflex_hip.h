@@ -160,10 +160,14 @@ int pf_stitch(const int64_t *d_req, int64_t nreq, const int64_t *d_pairs,
  * row-major.  out_dtype PF_F64: fixed k order, no FMA contraction
  * (bit-stable); PF_F32 / PF_BF16: MFMA-tiled BF16 product with FP32
  * accumulation, written as f32 or bf16.  d_C row stride ldc.
- * d_work: f64 scratch of K*(N+M) elements (normalised features). */
+ * d_work: scratch for the normalised features, pf_cost_workspace bytes,
+ * 16-byte aligned. */
 int pf_cost(int out_dtype, const double *d_F1, const double *d_F2, int64_t K,
             int64_t N, int64_t M, void *d_C, int64_t ldc, double *d_work,
             void *stream);
+
+/* Bytes of d_work that pf_cost needs for out_dtype. */
+int pf_cost_workspace(int out_dtype, int64_t K, int64_t N, int64_t M, int64_t *bytes);
 
 /* Fill helpers used instead of hipMemsetAsync (safe under graph capture). */
 int pf_fill_f64(double *d_p, double v, int64_t n, void *stream);

@@ -228,13 +228,21 @@ def parflex(C, steps, weights, beta, L=None, profile_dir="Profiling_results",
             use_gpu=None):
     """End-to-end ParFlex on a cost matrix (Parflex_gpu.py:656-854).
 
-    C: host array (L1, L2) or a ROCm tensor.  Returns (best_cost, wp) with wp
-    int64 of shape (2, n); raises ValueError when no endpoint exists."""
+    C: host array (L1, L2) or a ROCm tensor.  A device tensor of dtype
+    bfloat16 or float32 selects the precision form (FP32 stage-1 DP, FP64
+    stage 2); anything else runs in FP64 and matches the reference exactly.
+    Returns (best_cost, wp) with wp int64 of shape (2, n); raises ValueError
+    when no endpoint exists."""
     _resolve_use_gpu(use_gpu)
     L = DEFAULT_CHUNK_LENGTH if L is None else int(L)
     t_all = time.perf_counter()
     t0 = time.perf_counter()
-    C_dev = cost_matrix_to_gpu(C)
+    torch = _torch()
+    if (isinstance(C, torch.Tensor) and C.is_cuda and C.dim() == 2
+            and C.dtype in (torch.bfloat16, torch.float32)):
+        C_dev = C.contiguous()
+    else:
+        C_dev = cost_matrix_to_gpu(C)
     if profile_dir is not None:
         _sync()
     t_h2d = time.perf_counter() - t0
@@ -263,17 +271,20 @@ def parflex(C, steps, weights, beta, L=None, profile_dir="Profiling_results",
     return float(res.best_cost[0]), wp
 
 
-def tiled_stage1_from_features(F1, F2, steps=None, weights=None, beta=0.1, L=None, use_gpu=None):
+def tiled_stage1_from_features(F1, F2, steps=None, weights=None, beta=0.1, L=None, use_gpu=None,
+                               cost_dtype=None):
     """Features -> cost matrix on the GPU -> stage 1 (Parflex_gpu.py:610-653).
 
     Returns (None, tiled_result); tiled_result['C_dev'] holds the device cost
     matrix and tiled_result['chunks_dict'] materialises a tile's full D/S/B on
-    first access (they are not kept on the device by stage 1)."""
+    first access (they are not kept on the device by stage 1).  cost_dtype
+    torch.bfloat16 or torch.float32 selects the precision form (MFMA cost
+    build, FP32 DP); the default FP64 matches the reference exactly."""
     _resolve_use_gpu(use_gpu)
     L = DEFAULT_CHUNK_LENGTH if L is None else int(L)
     steps = np.array([[1, 1], [1, 2], [2, 1]]) if steps is None else steps
     weights = np.array([1.25, 3.0, 3.0]) if weights is None else weights
-    C_dev = cost_matrix_from_features_gpu(F1, F2)
+    C_dev = cost_matrix_from_features_gpu(F1, F2, dtype=cost_dtype)
     dt = _stage1_device(C_dev, L, steps, weights, beta)
     n1, n2 = dt.plan.grids[0]
     L1, L2 = int(C_dev.shape[0]), int(C_dev.shape[1])

@@ -43,6 +43,7 @@ _SIGNATURES = {
     "pf_stitch": (_int, [_vp, _i64, _vp, _vp, _i64, _vp, _int, _vp, _vp, _vp, _vp, _i64, _vp,
                          _vp, _vp]),
     "pf_cost": (_int, [_int, _vp, _vp, _i64, _i64, _i64, _vp, _i64, _vp, _vp]),
+    "pf_cost_workspace": (_int, [_int, _i64, _i64, _i64, _vp]),
     "pf_fill_f64": (_int, [_vp, _dbl, _i64, _vp]),
     "pf_fill_i64": (_int, [_vp, _i64, _i64, _vp]),
 }

@@ -17,7 +17,7 @@ from ._lib import PF_BF16, PF_F32, PF_F64, check, lib
 from ._plan import MAX_STITCH_TILES, BatchPlan
 
 __all__ = ["StepWeights", "stage1", "stage2", "run_batch", "BatchResult", "dtype_code",
-           "NO_ENDPOINT_MSG"]
+           "NO_ENDPOINT_MSG", "BatchRunner", "autotune_stage1", "stage1_config"]
 
 NO_ENDPOINT_MSG = "Stage 2: No valid endpoint found on global top/right edges."
 
@@ -199,3 +199,121 @@ def run_batch(C_dev, plan, steps, weights, stream=None, ws=None):
 
 def single_pair_plan(L1, L2, L, beta):
     return BatchPlan([(int(L1), int(L2))], int(L), beta=float(beta))
+
+
+# --------------------------------------------------------------------------
+# batched pipeline: features -> cost -> stage 1 -> stage 2 for many pairs
+# --------------------------------------------------------------------------
+
+_TUNED = {}  # (L, dtype code, ntiles bucket) -> stage-1 launch shape
+
+
+def _config_candidates(L):
+    """(strips, waves) shapes with 2 columns per lane and no wave idle."""
+    out = []
+    for nw in (1, 2, 4, 8, 16):
+        if (nw - 1) * 128 < L:
+            out.append((-(-L // (nw * 128)), nw))
+    return out
+
+
+def autotune_stage1(C_dev, plan, sw, ws, Gs=(32,), reps=2, stream=None):
+    """Time stage 1 for the candidate launch shapes on this plan and keep the
+    fastest in ws['s1_config'] (and a process-wide cache keyed by tile length,
+    dtype and tile count).  Returns (config, {config: ms})."""
+    torch = _torch()
+    key = (plan.L, dtype_code(C_dev), plan.ntiles // 64)
+    if key in _TUNED:
+        ws["s1_config"] = _TUNED[key]
+        return _TUNED[key], {}
+    times = {}
+    for (S, nw) in _config_candidates(plan.L):
+        for G in Gs:
+            cfg = (S, nw, G)
+            stage1(C_dev, plan, sw, ws, stream, config=cfg)  # warm-up / allocation
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                stage1(C_dev, plan, sw, ws, stream, config=cfg)
+            e1.record()
+            torch.cuda.synchronize()
+            times[cfg] = e0.elapsed_time(e1) / reps
+    best = min(times, key=times.get)
+    _TUNED[key] = best
+    ws["s1_config"] = best
+    return best, times
+
+
+class BatchRunner:
+    """All pairs of a batch in one pipeline: cost matrices of every pair in one
+    device buffer (one pf_cost per pair), one stage-1 launch over every tile
+    of every pair, one stage-2 pass over the pair table.  The whole sequence
+    can be captured in a HIP graph (capture()) and replayed (replay()): every
+    buffer is allocated up front and every fill is a kernel."""
+
+    def __init__(self, pairs, L, steps, weights, beta=0.1, dtype=None, device=None):
+        torch = _torch()
+        self.dtype = torch.bfloat16 if dtype is None else dtype
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.sw = StepWeights(steps, weights)
+        shapes = [(int(F1.shape[1]), int(F2.shape[1])) for F1, F2 in pairs]
+        self.plan = BatchPlan(shapes, int(L), beta=float(beta))
+        total = sum(a * b for a, b in shapes)
+        self.C = torch.empty(total, dtype=self.dtype, device=self.device)
+        self.F = [(torch.from_numpy(np.ascontiguousarray(F1, dtype=np.float64)).to(self.device),
+                   torch.from_numpy(np.ascontiguousarray(F2, dtype=np.float64)).to(self.device))
+                  for F1, F2 in pairs]
+        code = dtype_code(self.C)
+        need = 0
+        for (F1, F2), (N, M) in zip(pairs, shapes):
+            nb = ctypes.c_int64()
+            check(lib().pf_cost_workspace(code, int(F1.shape[0]), N, M, ctypes.byref(nb)),
+                  "pf_cost_workspace")
+            need = max(need, nb.value)
+        self.work = torch.empty(-(-need // 8), dtype=torch.float64, device=self.device)
+        self.ws = self.plan.allocate(self.device)
+        self.graph = None
+        self.code = code
+
+    def cost(self, stream=None):
+        st = _stream(stream)
+        for p, (F1, F2) in enumerate(self.F):
+            N, M = int(self.plan.pairs[p, 0]), int(self.plan.pairs[p, 1])
+            off = int(self.plan.pairs[p, 5])
+            ptr = self.C.data_ptr() + off * self.C.element_size()
+            check(lib().pf_cost(self.code, F1.data_ptr(), F2.data_ptr(), int(F1.shape[0]), N, M,
+                                ptr, M, self.work.data_ptr(), st), "pf_cost")
+
+    def launch(self, stream=None):
+        """Cost build, stage 1 and stage 2 for every pair (asynchronous)."""
+        self.cost(stream)
+        stage1(self.C, self.plan, self.sw, self.ws, stream)
+        stage2(self.C, self.plan, self.sw, self.ws, stream)
+
+    def autotune(self):
+        self.cost()
+        return autotune_stage1(self.C, self.plan, self.sw, self.ws)
+
+    def capture(self):
+        """Record launch() in a HIP graph (after a warm-up run that sizes the
+        strip-exchange buffers)."""
+        torch = _torch()
+        self.launch()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self.launch(s)
+        torch.cuda.current_stream().wait_stream(s)
+        self.graph = g
+        return g
+
+    def replay(self):
+        self.graph.replay()
+
+    def collect(self):
+        return collect(self.plan, self.ws)

@@ -22,6 +22,8 @@ Differences from the reference, all deliberate:
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 
 from . import _lib
@@ -80,9 +82,11 @@ def cost_matrix_to_gpu_f32(C):
 def cost_matrix_from_features_gpu(F1, F2, dtype=None):
     """C = 1 - L2norm(F1)^T L2norm(F2) on the GPU (gpu_flexdtw.py:95-109).
 
-    F1, F2: (K, L1), (K, L2) host arrays or device tensors.  The FP64 form sums
-    the K products in increasing order with separately rounded operations
-    (bit-identical to parflexdtw_amd.synth.cost_matrix_det)."""
+    F1, F2: (K, L1), (K, L2) host arrays or device tensors.  dtype float64
+    (default): the K products are summed in increasing order with separately
+    rounded operations (bit-identical to parflexdtw_amd.synth.cost_matrix_det).
+    dtype bfloat16 or float32: the normalised features are rounded to BF16 and
+    multiplied on the MFMA units with FP32 accumulation (precision form)."""
     torch = _torch()
     dtype = torch.float64 if dtype is None else dtype
     dev = _device()
@@ -92,12 +96,17 @@ def cost_matrix_from_features_gpu(F1, F2, dtype=None):
         raise ValueError("F1 and F2 must be (K, L1) and (K, L2) with the same K")
     K, N = F1d.shape
     M = F2d.shape[1]
-    if dtype != torch.float64:
-        raise ValueError("this build computes the cost matrix in float64 only")
-    C = torch.empty((N, M), dtype=torch.float64, device=dev)
-    work = torch.empty(K * (N + M), dtype=torch.float64, device=dev)
+    codes = {torch.float64: _lib.PF_F64, torch.float32: _lib.PF_F32,
+             torch.bfloat16: _lib.PF_BF16}
+    if dtype not in codes:
+        raise ValueError(f"unsupported cost dtype {dtype}")
+    code = codes[dtype]
+    C = torch.empty((N, M), dtype=dtype, device=dev)
+    nbytes = ctypes.c_int64()
+    check(lib().pf_cost_workspace(code, K, N, M, ctypes.byref(nbytes)), "pf_cost_workspace")
+    work = torch.empty(-(-nbytes.value // 8), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
-    check(lib().pf_cost(_lib.PF_F64, F1d.data_ptr(), F2d.data_ptr(), K, N, M, C.data_ptr(), M,
+    check(lib().pf_cost(code, F1d.data_ptr(), F2d.data_ptr(), K, N, M, C.data_ptr(), M,
                         work.data_ptr(), stream), "pf_cost")
     return C
 

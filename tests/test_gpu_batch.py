@@ -1,0 +1,115 @@
+"""GPU tests of the batched pipeline (features -> cost -> stage 1 -> stage 2
+for many ragged pairs, HIP-graph replay, stage-1 autotuner) and of the
+precision form (BF16 MFMA cost, FP32 DP, FP64 stage 2).
+
+The FP64 batched form is checked bit for bit against the oracle pair by pair.
+The precision form has no exact counterpart: its best cost is compared with
+the FP64 path's (relative error bound in the test) and its path with the FP64
+path frame by frame (at least 99.5% of F1 frames within +-2 F2 frames)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STEPS = [(1, 1), (1, 2), (2, 1)]
+WEIGHTS = [2, 3, 3]
+
+
+@pytest.fixture(scope="module")
+def torch_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _pairs(shapes, seed0, K=12, kind="chroma"):
+    from parflexdtw_amd.synth import make_pair
+    out = []
+    for p, (n, m) in enumerate(shapes):
+        F1, F2, _ = make_pair(n, m, K=K, seed=seed0 + p, kind=kind)
+        out.append((F1, F2))
+    return out
+
+
+RAGGED = [(700, 800), (1100, 950), (433, 521), (1500, 1720), (260, 301)]
+
+
+def test_batch_runner_f64_matches_oracle(torch_gpu, oracle_mod):
+    from parflexdtw_amd.engine import BatchRunner
+    from parflexdtw_amd.synth import cost_matrix_det
+    torch = torch_gpu
+    pairs = _pairs(RAGGED, 500)
+    br = BatchRunner(pairs, 256, STEPS, WEIGHTS, beta=0.1, dtype=torch.float64)
+    br.launch()
+    res = br.collect()
+    for p, (F1, F2) in enumerate(pairs):
+        bc, wp = oracle_mod.parflex(cost_matrix_det(F1, F2), STEPS, WEIGHTS, 0.1, 256)
+        assert res.found[p]
+        assert res.best_cost[p] == bc, p
+        assert np.array_equal(res.paths[p], wp), p
+
+
+@pytest.mark.parametrize("dtype", ["float64", "bfloat16"])
+def test_batch_graph_replay_equals_eager(torch_gpu, dtype):
+    """Three replays of the captured cost + stage 1 + stage 2 graph, with the
+    outputs overwritten by eager work before each, all equal the eager run."""
+    from parflexdtw_amd.engine import BatchRunner
+    torch = torch_gpu
+    pairs = _pairs(RAGGED, 600)
+    br = BatchRunner(pairs, 300, STEPS, WEIGHTS, beta=0.1, dtype=getattr(torch, dtype))
+    br.launch()
+    ref = br.collect()
+    br.capture()
+    for r in range(3):
+        br.ws["path"].fill_(-5)
+        br.ws["best_f"].fill_(7.0)
+        br.ws["Bp"].fill_(0)
+        br.C.fill_(0)
+        torch.cuda.synchronize()
+        br.replay()
+        torch.cuda.synchronize()
+        got = br.collect()
+        assert np.array_equal(got.found, ref.found), r
+        assert np.array_equal(got.best_cost, ref.best_cost, equal_nan=True), r
+        for p in range(len(pairs)):
+            assert np.array_equal(got.paths[p], ref.paths[p]), (r, p)
+
+
+def test_autotune_keeps_results(torch_gpu):
+    """Every stage-1 launch shape the autotuner tries gives the same FP64 result."""
+    from parflexdtw_amd import engine
+    torch = torch_gpu
+    pairs = _pairs(RAGGED[:3], 700)
+    br = engine.BatchRunner(pairs, 600, STEPS, WEIGHTS, beta=0.1, dtype=torch.float64)
+    br.launch()
+    ref = br.collect()
+    engine._TUNED.clear()
+    best, times = br.autotune()
+    assert best in times and len(times) >= 3
+    for cfg in times:
+        engine.stage1(br.C, br.plan, br.sw, br.ws, config=cfg)
+        engine.stage2(br.C, br.plan, br.sw, br.ws)
+        got = br.collect()
+        assert np.array_equal(got.best_cost, ref.best_cost), cfg
+        for p in range(len(pairs)):
+            assert np.array_equal(got.paths[p], ref.paths[p]), (cfg, p)
+
+
+@pytest.mark.parametrize("kind,K,N,M,L", [("relu", 128, 3000, 3300, 1000),
+                                          ("chroma", 12, 2000, 2400, 500),
+                                          ("relu", 128, 1200, 1000, 257)])
+def test_precision_form_agreement(torch_gpu, kind, K, N, M, L):
+    """BF16 MFMA cost + FP32 DP + FP64 stage 2 against the FP64 plain path."""
+    from parflexdtw_amd import Parflex_gpu as PG
+    from parflexdtw_amd import gpu_flexdtw as G
+    from parflexdtw_amd.synth import make_pair, path_agreement
+    torch = torch_gpu
+    F1, F2, _ = make_pair(N, M, K=K, seed=N + K, kind=kind)
+    C64 = G.cost_matrix_from_features_gpu(F1, F2)
+    bc64, wp64 = PG.parflex(C64, STEPS, WEIGHTS, 0.1, L, profile_dir=None)
+    for dt in (torch.bfloat16, torch.float32):
+        Cp = G.cost_matrix_from_features_gpu(F1, F2, dtype=dt)
+        bcp, wpp = PG.parflex(Cp, STEPS, WEIGHTS, 0.1, L, profile_dir=None)
+        assert path_agreement(wp64, wpp, tol=2) >= 0.995, dt
+        assert abs(bcp - bc64) <= 2e-2 * abs(bc64) + 1e-3, (dt, bcp, bc64)

@@ -254,3 +254,34 @@ def test_full_size_single_pair_vs_oracle(gpu, oracle_mod, cfg):
     bo, wpo = oracle_mod.parflex(C, STEPS, [2, 3, 3], 0.1, 4000, nthreads=16)
     assert bc == bo
     assert _same(wp, wpo)
+
+
+def _bf16_round(x):
+    """Round float64 values to BF16 (nearest even) through float32."""
+    f = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    f = (f + 0x7FFF + ((f >> 16) & 1)) & 0xFFFF0000
+    return f.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("K,N,M", [(12, 300, 170), (128, 257, 129), (20, 64, 64), (3, 33, 200)])
+def test_cost_mfma_bf16_layout_and_error(gpu, K, N, M):
+    """MFMA cost build: FP32 output equals the product of the BF16-rounded
+    normalised features to FP32 accumulation accuracy (a wrong MFMA operand or
+    accumulator layout gives O(1) errors); BF16 output within BF16 rounding of
+    the FP64 cost."""
+    import torch
+    G, _ = gpu
+    from parflexdtw_amd.synth import cost_matrix_det, make_pair
+    F1, F2, _ = make_pair(N, M + 20, K=K, seed=K + N, kind="relu")
+    F2 = np.ascontiguousarray(F2[:, :M])
+    n1 = np.sqrt(np.sum(F1 * F1, axis=0)) + 1e-9
+    n2 = np.sqrt(np.sum(F2 * F2, axis=0)) + 1e-9
+    A = _bf16_round(F1 / n1)
+    B = _bf16_round(F2 / n2)
+    ref_bf = 1.0 - A.T @ B
+    C32 = G.cost_matrix_from_features_gpu(F1, F2, dtype=torch.float32).cpu().numpy()
+    assert np.max(np.abs(C32 - ref_bf)) < 2e-5
+    C16 = G.cost_matrix_from_features_gpu(F1, F2, dtype=torch.bfloat16).float().cpu().numpy()
+    ref64 = cost_matrix_det(F1, F2)
+    assert np.max(np.abs(C16 - ref64)) < 1.6e-2
+    assert np.max(np.abs(C32 - ref64)) < 1e-2
