@@ -1,17 +1,30 @@
 """ParFlex benchmark on one MI355X.
 
-Default workload: the plain FP64 drop-in on one chroma pair of 16000 x ~19000
-frames with tile length L = 4000 (25 tiles, ~3.0e8 DP cells), steps
-(1,1),(1,2),(2,1), weights (2,3,3), beta 0.1.  One step = stage 1 (all tiles,
-one launch) + stage 2 (device propagation, sync, scan, stitch) + copy of the
-best cost and path back to the host, with the cost matrix resident in HBM.
+Workloads (--workload):
 
-Printed JSON (one line): value = end-to-end GCUPS (DP cells / step time);
-also DP GCUPS (cells / stage-1 kernel time), pairs/s, the stage-1 roofline
-(algorithmic bytes: 8 B of C + 0.25 B of packed back-pointer per cell,
-timed with HIP events on the launch stream) and the CPU baseline (the
-project's C restatement of the reference run on the worker-pool schedule of
-cpu_parflex.py, same pair, on this host).
+  chroma_batch256_N3000-20000 (default): 256 seeded chroma pairs (K = 12),
+      N ~ U[3000, 20000], M = N * U[0.8, 1.25], trimmed ends, so every pair
+      has ragged edge tiles.  The batched form: all 256 cost matrices built
+      on the MFMA units in BF16, one stage-1 launch over every tile of every
+      pair (FP32 DP), one stage-2 pass (FP64 scan), captured as HIP graphs;
+      the tuner picks L in {500, 1000, 2000, 4000} and the stage-1 launch
+      shape.  One step = features (resident in HBM) -> cost -> stage 1 ->
+      stage 2 -> best costs and paths on the host for all 256 pairs.
+      Beside it: the plain FP64 drop-in (parflex per pair, cost build
+      included) on a sample of the same pairs at the same L, whose paths are
+      the judge of agreement (frames within +-2), and the CPU oracle on the
+      same sample.
+  chroma16000x19000_L4000: the plain FP64 drop-in on one pair, exact against
+      the CPU oracle (cost matrix resident in HBM).
+  relu128_40000x40000_L4000: the precision form on one dense pair (K = 128):
+      BF16 MFMA cost + FP32 DP + FP64 stage 2, with cost error and path
+      agreement against the FP64 plain form.
+  chroma2000x2400_L500: the small plain case.
+
+Printed JSON (one line) follows the contract of the repository; the roofline
+figure is for the stage-1 DP kernel (the dominant kernel), algorithmic bytes
+per cell = one read of C + 2 bits of packed back-pointer, timed with HIP
+events on the stream it runs on.
 """
 from __future__ import annotations
 
@@ -28,6 +41,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "DP GCUPS + pairs/s, batched/learned vs plain GPU at >=99.5% path agreement"
 HBM_PEAK_GBS = 8000.0
+STEPS, WEIGHTS, BETA = [(1, 1), (1, 2), (2, 1)], [2, 3, 3], 0.1
+WORKLOADS = ["chroma_batch256_N3000-20000", "chroma16000x19000_L4000",
+             "relu128_40000x40000_L4000", "chroma2000x2400_L500"]
 
 
 def _parse():
@@ -35,28 +51,166 @@ def _parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="chroma16000x19000_L4000",
-                    choices=["chroma16000x19000_L4000", "chroma2000x2400_L500"])
+    ap.add_argument("--workload", default=WORKLOADS[0], choices=WORKLOADS)
+    ap.add_argument("--pairs", type=int, default=256, help="batch size of the batched workload")
+    ap.add_argument("--L", type=int, nargs="*", default=[500, 1000, 2000, 4000],
+                    help="tile lengths the batched workload's tuner tries")
+    ap.add_argument("--plain-sample", type=int, default=16,
+                    help="pairs of the batch also run through the plain FP64 form")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-trials", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
 
 
-def main():
-    args = _parse()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); every GPU box here "
-              "has one GPU and the distributed path is not part of this build.", file=sys.stderr)
-        sys.exit(2)
-    import torch
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
+
+def _cpu_model():
+    try:
+        return [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
+                if l.startswith("model name")][0]
+    except Exception:
+        return "unknown"
+
+
+def _roofline(cells, s1_ms, bytes_per_cell, kernel):
+    achieved = cells * bytes_per_cell / (s1_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kernel,
+            "bytes_per_cell": bytes_per_cell}
+
+
+def _base(args, value, unit, ms_step, dtype, config):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": None,
+            "config": config}
+
+
+# ---------------------------------------------------------------------------
+def bench_batch(args, torch):
+    from parflexdtw_amd import Parflex_gpu as PG
+    from parflexdtw_amd import engine
+    from parflexdtw_amd.gpu_flexdtw import cost_matrix_from_features_gpu
+    from parflexdtw_amd.synth import batch_pairs, path_agreement
+
+    t0 = time.perf_counter()
+    pairs = [(F1, F2) for F1, F2, _ in batch_pairs(args.pairs)]
+    shapes = [(F1.shape[1], F2.shape[1]) for F1, F2 in pairs]
+    log(f"{len(pairs)} pairs generated in {time.perf_counter() - t0:.1f} s; "
+        f"{sum(a * b for a, b in shapes) / 1e9:.2f} Gcells")
+    t0 = time.perf_counter()
+    br, report = engine.tune_batch(pairs, args.L, STEPS, WEIGHTS, BETA, torch.bfloat16, log=log)
+    L = br.plan.L
+    log(f"tuned in {time.perf_counter() - t0:.1f} s: L={L} config={br.ws['s1_config']}")
+    br.capture()
+    for _ in range(args.warmup):
+        br.replay()
+        res = br.collect()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        br.replay(evs[k])
+        res = br.collect()
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    cost_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+    s1_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    s2_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in evs]))
+    cells = br.plan.cells
+    P = len(pairs)
+    pairs_s = P / (ms_step * 1e-3)
+    log(f"batched: {ms_step:.2f} ms/step ({pairs_s:.1f} pairs/s), cost {cost_ms:.2f} ms, "
+        f"stage1 {s1_ms:.2f} ms, stage2 {s2_ms:.2f} ms, found {int(res.found.sum())}/{P}")
+
+    # Plain FP64 drop-in on a sample of the same pairs, same L: the judge of
+    # agreement and the plain pairs/s.
+    ns = min(args.plain_sample, P)
+    plain, agree, rel_err = [], [], []
+    Cs_host = []
+    for p in range(ns):
+        F1, F2 = pairs[p]
+        C64 = cost_matrix_from_features_gpu(F1, F2)
+        PG.parflex(C64, STEPS, WEIGHTS, BETA, L, profile_dir=None)  # warm-up for this shape
+        del C64
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p in range(ns):
+        F1, F2 = pairs[p]
+        C64 = cost_matrix_from_features_gpu(F1, F2)
+        bc, wp = PG.parflex(C64, STEPS, WEIGHTS, BETA, L, profile_dir=None)
+        plain.append((bc, wp))
+        if p < 4 and not args.no_cpu:
+            Cs_host.append(C64.cpu().numpy())
+        del C64
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    plain_pairs_s = ns / plain_s
+    plain_cells = sum(shapes[p][0] * shapes[p][1] for p in range(ns))
+    for p in range(ns):
+        bc, wp = plain[p]
+        agree.append(path_agreement(wp, res.paths[p], tol=2))
+        rel_err.append(abs(res.best_cost[p] - bc) / max(abs(bc), 1e-300))
+    log(f"plain FP64 on {ns} pairs: {plain_s * 1e3:.1f} ms ({plain_pairs_s:.1f} pairs/s); "
+        f"agreement min {min(agree):.5f} mean {np.mean(agree):.5f}")
+
+    out = _base(args, round(pairs_s, 3), "pairs/s", ms_step, "bf16 cost, f32 DP, f64 stage 2",
+                {"workload": args.workload, "pairs": P, "N": "U[3000,20000]",
+                 "M": "N*U[0.8,1.25], 0-10% trimmed each end", "K": 12, "L": L,
+                 "L_tried": sorted(report), "stage1_shape": list(br.ws["s1_config"]),
+                 "tiles": br.plan.ntiles, "cells": cells, "steps": STEPS, "weights": WEIGHTS,
+                 "beta": BETA,
+                 "form": "batched: BF16 MFMA cost, FP32 DP, FP64 stage 2, HIP graphs"})
+    out["data"] = "synthetic (seeded chroma pairs with known time warps, features in HBM)"
+    out["dp_gcups"] = round(cells / (s1_ms * 1e-3) / 1e9, 3)
+    out["gcups_end_to_end"] = round(cells / (ms_step * 1e-3) / 1e9, 3)
+    out["cost_ms"] = round(cost_ms, 4)
+    out["stage1_ms"] = round(s1_ms, 4)
+    out["stage2_ms"] = round(s2_ms, 4)
+    out["tuner"] = {str(k): {"config": list(v[0]), "stage1_stage2_ms": round(v[1], 3)}
+                    for k, v in sorted(report.items())}
+    out["plain_gpu"] = {"pairs_per_s": round(plain_pairs_s, 3),
+                        "gcups": round(plain_cells / plain_s / 1e9, 3), "sample_pairs": ns,
+                        "form": "parflex drop-in per pair, FP64 cost + FP64 DP, same L"}
+    out["speedup_vs_plain"] = round(pairs_s / plain_pairs_s, 3)
+    out["agreement"] = {"frames_within_2_min": round(min(agree), 6),
+                        "frames_within_2_mean": round(float(np.mean(agree)), 6),
+                        "best_cost_rel_err_max": float(max(rel_err)),
+                        "pairs_compared": ns, "judge": "plain FP64 path, same pair and L"}
+    out["roofline"] = _roofline(cells, s1_ms, 2.25, "k_stage1<2,__hip_bfloat16,float,false,true>")
+
+    if not args.no_cpu:
+        from oracle import oracle
+        nthreads = args.cpu_threads
+        t_cpu, n_cpu, exact = 0.0, 0, True
+        oracle.parflex(Cs_host[0], STEPS, WEIGHTS, BETA, L, nthreads=nthreads)  # warm-up
+        for p, C in enumerate(Cs_host):
+            tc = time.perf_counter()
+            bo, wpo = oracle.parflex(C, STEPS, WEIGHTS, BETA, L, nthreads=nthreads)
+            t_cpu += time.perf_counter() - tc
+            n_cpu += 1
+            exact = exact and bo == plain[p][0] and np.array_equal(wpo, plain[p][1])
+        out["cpu_baseline"] = {
+            "value": round(n_cpu / t_cpu, 4), "unit": "pairs/s", "cores": nthreads, "kind": "port",
+            "sample": f"pairs 0-{n_cpu - 1} of the batch ("
+                      f"{sum(shapes[p][0] * shapes[p][1] for p in range(n_cpu)) / 1e9:.2f} Gcells), "
+                      f"FP64 parflex end to end at L={L}, {nthreads} worker threads "
+                      f"(cpu_parflex.py:392 schedule); {_cpu_model()}",
+            "seconds": round(t_cpu, 4)}
+        out["parity"] = {"plain_gpu_vs_cpu_oracle_exact": bool(exact), "pairs": n_cpu}
+    return out
+
+
+# ---------------------------------------------------------------------------
+def bench_plain(args, torch):
     from parflexdtw_amd import engine
     from parflexdtw_amd._plan import BatchPlan
     from parflexdtw_amd.gpu_flexdtw import cost_matrix_from_features_gpu
     from parflexdtw_amd.synth import cfg_pair, make_pair
 
-    steps, weights, beta = [(1, 1), (1, 2), (2, 1)], [2, 3, 3], 0.1
     if args.workload == "chroma16000x19000_L4000":
         F1, F2, _ = cfg_pair(1)
         L = 4000
@@ -64,15 +218,13 @@ def main():
         F1, F2, _ = make_pair(2000, 2400, seed=0)
         L = 500
     dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
     C_dev = cost_matrix_from_features_gpu(F1, F2)
     L1, L2 = int(C_dev.shape[0]), int(C_dev.shape[1])
-    plan = BatchPlan([(L1, L2)], L, beta=beta)
-    sw = engine.StepWeights(steps, weights)
+    plan = BatchPlan([(L1, L2)], L, beta=BETA)
+    sw = engine.StepWeights(STEPS, WEIGHTS)
     ws = plan.allocate(dev)
     stream = torch.cuda.current_stream()
-    print(f"[bench] workload {args.workload}: {L1}x{L2}, L={L}, tiles={plan.ntiles}, "
-          f"cells={plan.cells}", file=sys.stderr, flush=True)
+    log(f"workload {args.workload}: {L1}x{L2}, L={L}, tiles={plan.ntiles}, cells={plan.cells}")
 
     def step(ev=None):
         if ev is not None:
@@ -93,44 +245,21 @@ def main():
     for k in range(args.steps):
         res = step(events[k])
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    s1_ms = np.array([e[0].elapsed_time(e[1]) for e in events])
-    s2_ms = np.array([e[1].elapsed_time(e[2]) for e in events])
-    ms_step = (t1 - t0) * 1e3 / args.steps
+    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    s1_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in events]))
+    s2_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in events]))
     cells = plan.cells
-    gcups = cells / (ms_step * 1e-3) / 1e9
-    dp_gcups = cells / (s1_ms.mean() * 1e-3) / 1e9
-    bytes_per_cell = 8.0 + 0.25
-    achieved = cells * bytes_per_cell / (s1_ms.mean() * 1e-3) / 1e9
-    print(f"[bench] step {ms_step:.3f} ms, stage1 {s1_ms.mean():.3f} ms, stage2 "
-          f"{s2_ms.mean():.3f} ms", file=sys.stderr, flush=True)
-
-    out = {
-        "metric": METRIC,
-        "value": round(gcups, 3),
-        "unit": "GCUPS",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded chroma pair with known time warp)",
-        "config": {"workload": args.workload, "L1": L1, "L2": L2, "L": L, "tiles": plan.ntiles,
-                   "cells": cells, "steps": steps, "weights": weights, "beta": beta,
-                   "form": "plain FP64 drop-in, stage1+stage2 on device"},
-        "dp_gcups": round(dp_gcups, 3),
-        "pairs_per_s": round(1e3 / ms_step, 3),
-        "stage1_ms": round(float(s1_ms.mean()), 4),
-        "stage2_ms": round(float(s2_ms.mean()), 4),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "k_stage1_packed<4,double,double>",
-                     "bytes_per_cell": bytes_per_cell},
-    }
-
+    log(f"step {ms_step:.3f} ms, stage1 {s1_ms:.3f} ms, stage2 {s2_ms:.3f} ms")
+    out = _base(args, round(cells / (ms_step * 1e-3) / 1e9, 3), "GCUPS", ms_step, "f64",
+                {"workload": args.workload, "L1": L1, "L2": L2, "L": L, "tiles": plan.ntiles,
+                 "cells": cells, "steps": STEPS, "weights": WEIGHTS, "beta": BETA,
+                 "form": "plain FP64 drop-in, stage1+stage2 on device"})
+    out["data"] = "synthetic (seeded chroma pair with known time warp)"
+    out["dp_gcups"] = round(cells / (s1_ms * 1e-3) / 1e9, 3)
+    out["pairs_per_s"] = round(1e3 / ms_step, 3)
+    out["stage1_ms"] = round(s1_ms, 4)
+    out["stage2_ms"] = round(s2_ms, 4)
+    out["roofline"] = _roofline(cells, s1_ms, 8.25, "k_stage1<2,double,double,false,true>")
     if not args.no_cpu:
         from oracle import oracle
         C = C_dev.cpu().numpy()
@@ -139,26 +268,134 @@ def main():
         bo = wpo = None
         for _ in range(args.cpu_trials):
             tc = time.perf_counter()
-            bo, wpo = oracle.parflex(C, steps, weights, beta, L, nthreads=nthreads)
+            bo, wpo = oracle.parflex(C, STEPS, WEIGHTS, BETA, L, nthreads=nthreads)
             times.append(time.perf_counter() - tc)
         t_cpu = min(times[1:]) if len(times) > 1 else times[0]
         exact = (bo == float(res.best_cost[0])) and np.array_equal(wpo, res.paths[0])
-        try:
-            cpu_model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
-                         if l.startswith("model name")][0]
-        except Exception:
-            cpu_model = "unknown"
         out["cpu_baseline"] = {
             "value": round(cells / t_cpu / 1e9, 4), "unit": "GCUPS", "cores": nthreads,
             "kind": "port",
             "sample": f"whole {L1}x{L2} L={L} pair, parflex end to end, "
                       f"{nthreads} worker threads (cpu_parflex.py:392 schedule), best of "
                       f"{len(times) - 1 if len(times) > 1 else 1} timed trials after 1 warm-up; "
-                      f"{cpu_model}",
-            "seconds": round(t_cpu, 4),
-        }
+                      f"{_cpu_model()}",
+            "seconds": round(t_cpu, 4)}
         out["parity"] = {"path_exact": bool(exact), "best_cost_gpu": float(res.best_cost[0]),
                          "best_cost_cpu": float(bo), "path_len": int(res.paths[0].shape[1])}
+    return out
+
+
+# ---------------------------------------------------------------------------
+def bench_precision(args, torch):
+    from parflexdtw_amd import Parflex_gpu as PG
+    from parflexdtw_amd import engine
+    from parflexdtw_amd._plan import BatchPlan
+    from parflexdtw_amd.gpu_flexdtw import cost_matrix_from_features_gpu
+    from parflexdtw_amd.synth import cfg_pair, path_agreement
+
+    F1, F2, _ = cfg_pair(3)
+    L = 4000
+    dev = torch.device("cuda", 0)
+    F1d = torch.from_numpy(F1).to(dev)
+    F2d = torch.from_numpy(F2).to(dev)
+    L1, L2 = F1.shape[1], F2.shape[1]
+    plan = BatchPlan([(L1, L2)], L, beta=BETA)
+    sw = engine.StepWeights(STEPS, WEIGHTS)
+    ws = plan.allocate(dev)
+    stream = torch.cuda.current_stream()
+    log(f"workload {args.workload}: {L1}x{L2}, K={F1.shape[0]}, L={L}, tiles={plan.ntiles}")
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        C = cost_matrix_from_features_gpu(F1d, F2d, dtype=torch.bfloat16)
+        if ev is not None:
+            ev[1].record(stream)
+        engine.stage1(C, plan, sw, ws, stream)
+        if ev is not None:
+            ev[2].record(stream)
+        engine.stage2(C, plan, sw, ws, stream)
+        if ev is not None:
+            ev[3].record(stream)
+        return engine.collect(plan, ws), C
+
+    for _ in range(args.warmup):
+        res, Cb = step()
+    torch.cuda.synchronize()
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        res, Cb = step(events[k])
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    cost_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in events]))
+    s1_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in events]))
+    s2_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in events]))
+    cells = plan.cells
+
+    # FP64 plain form on the same pair: the judge of cost error and agreement.
+    C64 = cost_matrix_from_features_gpu(F1d, F2d)
+    cerr = 0.0
+    for r0 in range(0, L1, 4096):
+        cerr = max(cerr, float((Cb[r0:r0 + 4096].double() - C64[r0:r0 + 4096]).abs().max()))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bc64, wp64 = PG.parflex(C64, STEPS, WEIGHTS, BETA, L, profile_dir=None)
+    torch.cuda.synchronize()
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    bc = float(res.best_cost[0])
+    agree = path_agreement(wp64, res.paths[0], tol=2)
+    log(f"precision: {ms_step:.2f} ms/step (cost {cost_ms:.2f}, stage1 {s1_ms:.2f}, stage2 "
+        f"{s2_ms:.2f}); plain FP64 {plain_ms:.1f} ms; agreement {agree:.5f}; max |dC| {cerr:.3g}")
+    out = _base(args, round(cells / (ms_step * 1e-3) / 1e9, 3), "GCUPS", ms_step,
+                "bf16 cost, f32 DP, f64 stage 2",
+                {"workload": args.workload, "L1": L1, "L2": L2, "K": int(F1.shape[0]), "L": L,
+                 "tiles": plan.ntiles, "cells": cells, "steps": STEPS, "weights": WEIGHTS,
+                 "beta": BETA, "form": "precision: BF16 MFMA cost, FP32 DP, FP64 stage 2"})
+    out["data"] = "synthetic (seeded ReLU-Gaussian pair with known time warp, features in HBM)"
+    out["dp_gcups"] = round(cells / (s1_ms * 1e-3) / 1e9, 3)
+    out["pairs_per_s"] = round(1e3 / ms_step, 3)
+    out["cost_ms"] = round(cost_ms, 4)
+    out["cost_tflops"] = round(2.0 * F1.shape[0] * L1 * L2 / (cost_ms * 1e-3) / 1e12, 2)
+    out["stage1_ms"] = round(s1_ms, 4)
+    out["stage2_ms"] = round(s2_ms, 4)
+    out["plain_gpu"] = {"ms": round(plain_ms, 3), "pairs_per_s": round(1e3 / plain_ms, 3)}
+    out["accuracy"] = {"cost_max_abs_err": cerr, "best_cost": bc, "best_cost_fp64": bc64,
+                       "best_cost_abs_err": abs(bc - bc64),
+                       "best_cost_rel_err": abs(bc - bc64) / abs(bc64),
+                       "frames_within_2": round(agree, 6), "judge": "plain FP64 path"}
+    out["roofline"] = _roofline(cells, s1_ms, 2.25, "k_stage1<2,__hip_bfloat16,float,false,true>")
+    if not args.no_cpu:
+        from oracle import oracle
+        C = C64.cpu().numpy()
+        tc = time.perf_counter()
+        bo, wpo = oracle.parflex(C, STEPS, WEIGHTS, BETA, L, nthreads=args.cpu_threads)
+        t_cpu = time.perf_counter() - tc
+        out["cpu_baseline"] = {
+            "value": round(cells / t_cpu / 1e9, 4), "unit": "GCUPS", "cores": args.cpu_threads,
+            "kind": "port",
+            "sample": f"whole {L1}x{L2} L={L} pair, FP64 parflex end to end, "
+                      f"{args.cpu_threads} worker threads; {_cpu_model()}",
+            "seconds": round(t_cpu, 4)}
+        out["parity"] = {"plain_gpu_vs_cpu_oracle_exact":
+                         bool(bo == bc64 and np.array_equal(wpo, wp64))}
+    return out
+
+
+def main():
+    args = _parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); every GPU box here "
+              "has one GPU and the distributed path is not part of this build.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    torch.cuda.set_device(0)
+    if args.workload.startswith("chroma_batch"):
+        out = bench_batch(args, torch)
+    elif args.workload.startswith("relu"):
+        out = bench_precision(args, torch)
+    else:
+        out = bench_plain(args, torch)
     print(json.dumps(out))
 
 

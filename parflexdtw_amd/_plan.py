@@ -65,28 +65,41 @@ class BatchPlan:
         if self.ldcs is None:
             self.ldcs = [L2 for (_, L2) in self.shapes]
         hop = L - 1
-        tiles = []
+        blocks = []
         pairs = np.zeros((P, 16), np.int64)
         b_off = 0
         lk_off = seg_off = path_off = 0
+        ntile = 0
         self.grids = []
+        cells = 0
         for p, (L1, L2) in enumerate(self.shapes):
             L1, L2 = int(L1), int(L2)
             if L1 < 2 or L2 < 2:
                 raise ValueError(f"pair {p}: cost matrix must be at least 2x2, got {L1}x{L2}")
             n1, n2 = tile_grid(L1, L2, L)
             self.grids.append((n1, n2))
-            base = len(tiles)
+            base = ntile
             ldc = int(self.ldcs[p])
             co = int(self.c_offsets[p])
-            for i in range(n1):
-                for j in range(n2):
-                    s1, e1, s2, e2 = tile_bounds(i, j, L1, L2, L)
-                    R, W = e1 - s1, e2 - s2
-                    t = len(tiles)
-                    tiles.append([co + s1 * ldc + s2, ldc, R, W, b_off, t * 2 * L, p,
-                                  (i << 32) | j])
-                    b_off += ((R + 15) // 16) * W
+            # i-major tile order: tile (i, j) is number base + i*n2 + j
+            s1 = np.repeat(np.arange(n1, dtype=np.int64) * hop, n2)
+            s2 = np.tile(np.arange(n2, dtype=np.int64) * hop, n1)
+            R = np.minimum(s1 + L, L1) - s1
+            W = np.minimum(s2 + L, L2) - s2
+            words = ((R + 15) // 16) * W
+            t = np.empty((n1 * n2, 8), np.int64)
+            t[:, 0] = co + s1 * ldc + s2
+            t[:, 1] = ldc
+            t[:, 2] = R
+            t[:, 3] = W
+            t[:, 4] = b_off + np.concatenate([[0], np.cumsum(words)[:-1]])
+            t[:, 5] = (base + np.arange(n1 * n2, dtype=np.int64)) * 2 * L
+            t[:, 6] = p
+            t[:, 7] = (s1 // hop << 32) | (s2 // hop)
+            blocks.append(t)
+            b_off += int(words.sum())
+            cells += int((R * W).sum())
+            ntile += n1 * n2
             max_segs = max(-(-L2 // L), -(-L1 // L), 1)
             buf = int(buffer_global(L1, L2, self.beta))
             pairs[p] = [L1, L2, n1, n2, base, co, ldc, buf, lk_off, seg_off, max_segs,
@@ -94,9 +107,9 @@ class BatchPlan:
             lk_off += L1 + L2
             seg_off += 2 * max_segs
             path_off += L1 + L2 + 8
-        self.tiles = np.ascontiguousarray(np.array(tiles, dtype=np.int64).reshape(-1, 8))
+        self.tiles = np.ascontiguousarray(np.concatenate(blocks).reshape(-1, 8))
         self.pairs = pairs
-        self.ntiles = len(tiles)
+        self.ntiles = ntile
         self.npairs = P
         self.b_words = b_off
         self.lk_total = lk_off
@@ -106,7 +119,7 @@ class BatchPlan:
         self.max_tile_diag = max(min(n1, n2) for n1, n2 in self.grids)
         self.max_tiles_pair = max(n1 * n2 for n1, n2 in self.grids)
         self.max_segs = int(pairs[:, 10].max())
-        self.cells = int(sum(int(t[2]) * int(t[3]) for t in tiles))
+        self.cells = cells
 
     def tile_index(self, p, i, j):
         return int(self.pairs[p, 4]) + i * int(self.pairs[p, 3]) + j

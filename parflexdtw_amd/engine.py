@@ -17,7 +17,7 @@ from ._lib import PF_BF16, PF_F32, PF_F64, check, lib
 from ._plan import MAX_STITCH_TILES, BatchPlan
 
 __all__ = ["StepWeights", "stage1", "stage2", "run_batch", "BatchResult", "dtype_code",
-           "NO_ENDPOINT_MSG", "BatchRunner", "autotune_stage1", "stage1_config"]
+           "NO_ENDPOINT_MSG", "BatchRunner", "autotune_stage1", "stage1_config", "tune_batch"]
 
 NO_ENDPOINT_MSG = "Stage 2: No valid endpoint found on global top/right edges."
 
@@ -172,7 +172,8 @@ class BatchResult:
 
 
 def collect(plan, ws):
-    """Copy only best costs, endpoints and stitched paths back to the host."""
+    """Copy only best costs, endpoints and stitched paths back to the host
+    (ws entries may also be host tensors already holding them)."""
     best_f = ws["best_f"].cpu().numpy()
     best_i = ws["best_i"].cpu().numpy().reshape(plan.npairs, 8)
     plen = ws["path_len"].cpu().numpy()
@@ -253,29 +254,41 @@ class BatchRunner:
     can be captured in a HIP graph (capture()) and replayed (replay()): every
     buffer is allocated up front and every fill is a kernel."""
 
-    def __init__(self, pairs, L, steps, weights, beta=0.1, dtype=None, device=None):
+    def __init__(self, pairs, L, steps, weights, beta=0.1, dtype=None, device=None, share=None):
+        """pairs: list of (F1 (K, N), F2 (K, M)) host feature arrays.  dtype of
+        the cost matrices: torch.bfloat16 (default; FP32 DP) or float32 or
+        float64 (exact plain form).  share: another runner over the same
+        pairs and dtype whose features and cost buffer are reused (a sweep
+        over tile lengths builds the cost matrices once)."""
         torch = _torch()
-        self.dtype = torch.bfloat16 if dtype is None else dtype
-        self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.sw = StepWeights(steps, weights)
         shapes = [(int(F1.shape[1]), int(F2.shape[1])) for F1, F2 in pairs]
         self.plan = BatchPlan(shapes, int(L), beta=float(beta))
-        total = sum(a * b for a, b in shapes)
-        self.C = torch.empty(total, dtype=self.dtype, device=self.device)
-        self.F = [(torch.from_numpy(np.ascontiguousarray(F1, dtype=np.float64)).to(self.device),
-                   torch.from_numpy(np.ascontiguousarray(F2, dtype=np.float64)).to(self.device))
-                  for F1, F2 in pairs]
-        code = dtype_code(self.C)
-        need = 0
-        for (F1, F2), (N, M) in zip(pairs, shapes):
-            nb = ctypes.c_int64()
-            check(lib().pf_cost_workspace(code, int(F1.shape[0]), N, M, ctypes.byref(nb)),
-                  "pf_cost_workspace")
-            need = max(need, nb.value)
-        self.work = torch.empty(-(-need // 8), dtype=torch.float64, device=self.device)
+        if share is not None:
+            if [tuple(x) for x in share.plan.shapes] != shapes:
+                raise ValueError("share: runner over different pairs")
+            self.dtype, self.device = share.dtype, share.device
+            self.C, self.F, self.work, self.code = share.C, share.F, share.work, share.code
+            self.c_ready = share.c_ready
+        else:
+            self.dtype = torch.bfloat16 if dtype is None else dtype
+            self.device = device or torch.device("cuda", torch.cuda.current_device())
+            total = sum(a * b for a, b in shapes)
+            self.C = torch.empty(total, dtype=self.dtype, device=self.device)
+            self.F = [(torch.from_numpy(np.ascontiguousarray(F1, dtype=np.float64)).to(self.device),
+                       torch.from_numpy(np.ascontiguousarray(F2, dtype=np.float64)).to(self.device))
+                      for F1, F2 in pairs]
+            self.code = dtype_code(self.C)
+            need = 0
+            for (F1, F2), (N, M) in zip(pairs, shapes):
+                nb = ctypes.c_int64()
+                check(lib().pf_cost_workspace(self.code, int(F1.shape[0]), N, M, ctypes.byref(nb)),
+                      "pf_cost_workspace")
+                need = max(need, nb.value)
+            self.work = torch.empty(-(-need // 8), dtype=torch.float64, device=self.device)
+            self.c_ready = [False]
         self.ws = self.plan.allocate(self.device)
         self.graph = None
-        self.code = code
 
     def cost(self, stream=None):
         st = _stream(stream)
@@ -285,6 +298,7 @@ class BatchRunner:
             ptr = self.C.data_ptr() + off * self.C.element_size()
             check(lib().pf_cost(self.code, F1.data_ptr(), F2.data_ptr(), int(F1.shape[0]), N, M,
                                 ptr, M, self.work.data_ptr(), st), "pf_cost")
+        self.c_ready[0] = True
 
     def launch(self, stream=None):
         """Cost build, stage 1 and stage 2 for every pair (asynchronous)."""
@@ -292,28 +306,94 @@ class BatchRunner:
         stage1(self.C, self.plan, self.sw, self.ws, stream)
         stage2(self.C, self.plan, self.sw, self.ws, stream)
 
-    def autotune(self):
-        self.cost()
-        return autotune_stage1(self.C, self.plan, self.sw, self.ws)
+    def autotune(self, Gs=(32,)):
+        if not self.c_ready[0]:
+            self.cost()
+        return autotune_stage1(self.C, self.plan, self.sw, self.ws, Gs=Gs)
+
+    def time_launch(self, reps=2):
+        """Mean ms of stage 1 + stage 2 (the cost build does not depend on L)."""
+        torch = _torch()
+        if not self.c_ready[0]:
+            self.cost()
+        stage1(self.C, self.plan, self.sw, self.ws)
+        stage2(self.C, self.plan, self.sw, self.ws)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            stage1(self.C, self.plan, self.sw, self.ws)
+            stage2(self.C, self.plan, self.sw, self.ws)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
 
     def capture(self):
-        """Record launch() in a HIP graph (after a warm-up run that sizes the
-        strip-exchange buffers)."""
+        """Record the three phases (cost build, stage 1, stage 2) as three HIP
+        graphs, after a warm-up run that sizes the strip-exchange buffers."""
         torch = _torch()
         self.launch()
         torch.cuda.synchronize()
-        g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
+        graphs = []
         with torch.cuda.stream(s):
-            with torch.cuda.graph(g, stream=s):
-                self.launch(s)
+            for phase in (self.cost,
+                          lambda st: stage1(self.C, self.plan, self.sw, self.ws, st),
+                          lambda st: stage2(self.C, self.plan, self.sw, self.ws, st)):
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=s):
+                    phase(s)
+                graphs.append(g)
         torch.cuda.current_stream().wait_stream(s)
-        self.graph = g
-        return g
+        self.graph = graphs
+        return graphs
 
-    def replay(self):
-        self.graph.replay()
+    def replay(self, events=None):
+        """Replay the captured phases on the current stream; with events (a
+        list of 4 torch.cuda.Event) record one before and after each phase."""
+        torch = _torch()
+        st = torch.cuda.current_stream()
+        if events is not None:
+            events[0].record(st)
+        for k, g in enumerate(self.graph):
+            g.replay()
+            if events is not None:
+                events[k + 1].record(st)
 
     def collect(self):
-        return collect(self.plan, self.ws)
+        """Best costs, endpoints and stitched paths of every pair, copied
+        through pinned host buffers allocated on first use."""
+        torch = _torch()
+        if getattr(self, "_host", None) is None:
+            self._host = {k: torch.empty(self.ws[k].shape, dtype=self.ws[k].dtype,
+                                         pin_memory=True)
+                          for k in ("best_f", "best_i", "path_len", "path")}
+        for k, h in self._host.items():
+            h.copy_(self.ws[k], non_blocking=True)
+        torch.cuda.synchronize()
+        return collect(self.plan, self._host)
+
+
+def tune_batch(pairs, Ls, steps, weights, beta=0.1, dtype=None, Gs=(32,), reps=2, log=None):
+    """Autotuner over the tile length L and the stage-1 launch shape
+    (strips, waves, exchange block G) for one batch: for every L the fastest
+    launch shape is found (autotune_stage1), then the L whose stage 1 +
+    stage 2 is fastest is kept.  The cost matrices are built once and shared.
+    Returns (runner, {L: (config, ms)})."""
+    best, best_ms, report, base = None, None, {}, None
+    for L in Ls:
+        br = BatchRunner(pairs, L, steps, weights, beta, dtype, share=base)
+        if base is None:
+            base = br
+        cfg, _ = br.autotune(Gs=Gs)
+        ms = br.time_launch(reps)
+        report[int(L)] = (cfg, ms)
+        if log is not None:
+            log(f"L={L}: tiles={br.plan.ntiles} config={cfg} stage1+stage2 {ms:.2f} ms")
+        if best is None or ms < best_ms:
+            best, best_ms = br, ms
+        elif br is not base:
+            del br
+    return best, report

@@ -181,6 +181,9 @@ S1_CONFIGS = [  # (shape, L, steps, weights, (strips, waves, rows per exchange b
     ((130, 3000), 2999, STEPS, [2, 3, 3], (5, 5, 16)),
     ((500, 520), 260, [(1, 1), (1, 2), (2, 1), (2, 2)], [2, 3, 3, 4], (3, 1, 5)),
     ((500, 520), 260, [(1, 0), (1, 1), (2, 1)], [1, 2, 3], (2, 2, 3)),
+    ((300, 1100), 1100, STEPS, [2, 3, 3], (1, 8, 32)),
+    ((300, 2100), 2048, STEPS, [2, 3, 3], (1, 16, 16)),
+    ((200, 4100), 4000, STEPS, [2, 3, 3], (2, 16, 32)),
 ]
 
 
