@@ -143,11 +143,11 @@ def bench_batch(args, torch):
         C64 = cost_matrix_from_features_gpu(F1, F2)
         bc, wp = PG.parflex(C64, STEPS, WEIGHTS, BETA, L, profile_dir=None)
         plain.append((bc, wp))
-        if p < 4 and not args.no_cpu:
-            Cs_host.append(C64.cpu().numpy())
         del C64
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - t0
+    if not args.no_cpu:
+        Cs_host = [cost_matrix_from_features_gpu(*pairs[p]).cpu().numpy() for p in range(min(4, ns))]
     plain_pairs_s = ns / plain_s
     plain_cells = sum(shapes[p][0] * shapes[p][1] for p in range(ns))
     for p in range(ns):

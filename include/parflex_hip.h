@@ -56,7 +56,9 @@ int pf_check_steps(const int32_t *steps, const double *weights, int K);
  * Packed back-pointers: word (rb, c) = b_off + rb*W + c holds the 2-bit step
  * indices of rows 16*rb .. 16*rb+15 of column c, row r at bit 2*(r%16).
  * Edge slots: edge_D/edge_P[e_off + e*Lmax + pos], e = 0 last row (pos =
- * column), e = 1 last column (pos = row).
+ * column), e = 1 last column (pos = row); edge_C[e_off + e*Lmax + pos] holds
+ * C of the FIRST row (e = 0) and FIRST column (e = 1) of the tile, the start
+ * cells of the tile's paths.
  */
 
 /* Stage 1, all tiles of a tile table in one launch: per-tile FlexDTW DP
@@ -71,12 +73,13 @@ int pf_check_steps(const int32_t *steps, const double *weights, int K);
  * hand their boundary columns to the right every `rows_per_block` rows
  * (1..64) through d_xch (ntiles*strips*Lmax entries of 32 bytes for PF_F64,
  * 16 bytes otherwise) and d_flags (int32, ntiles*strips + 1).  With
- * strips == 1 both may be null. */
+ * strips == 1 both may be null.  d_edge_D, d_edge_P, d_edge_C: ntiles*2*Lmax
+ * entries each (f64, int32, f64). */
 int pf_stage1(int dtype, const void *d_C, const int64_t *d_tiles, int64_t ntiles,
               int64_t Lmax, const int32_t *steps, const double *weights, int K,
               int strips, int waves, int rows_per_block, int32_t *d_flags,
               void *d_xch, uint32_t *d_Bpacked, double *d_edge_D, int32_t *d_edge_P,
-              void *stream);
+              double *d_edge_C, void *stream);
 
 /* Sizes of the strip-exchange workspace of pf_stage1: int32 count of
  * d_flags and byte size of d_xch (both 0 when strips == 1). */
@@ -134,7 +137,7 @@ int pf_tile_backtrace(const int8_t *d_B, int64_t R, int64_t W,
  * Parflex.stage2_scan_and_stitch (Parflex.py:1229-1356).  Only the path (and,
  * on request, the edge arrays) need to be copied back.
  */
-int pf_stage2(int c_dtype, const void *d_C, const int64_t *d_pairs, int64_t npairs,
+int pf_stage2(const double *d_edge_C, const int64_t *d_pairs, int64_t npairs,
               const int64_t *d_tiles, int64_t ntiles, int64_t L, int64_t max_diag,
               int64_t max_tile_diag, int64_t max_tiles_pair, int64_t max_segs,
               const int32_t *steps, int K, const uint32_t *d_Bpacked,
@@ -166,8 +169,25 @@ int pf_cost(int out_dtype, const double *d_F1, const double *d_F2, int64_t K,
             int64_t N, int64_t M, void *d_C, int64_t ldc, double *d_work,
             void *stream);
 
-/* Bytes of d_work that pf_cost needs for out_dtype. */
+/* Bytes of d_work that pf_cost needs for out_dtype (for pf_cost_batch pass
+ * the total frame counts n1_total, n2_total as N, M). */
 int pf_cost_workspace(int out_dtype, int64_t K, int64_t N, int64_t M, int64_t *bytes);
+
+/* Rows x columns of C that one workgroup of the cost kernel for out_dtype
+ * covers (the unit of blk0 in pf_cost_batch job tables). */
+int pf_cost_tile(int out_dtype, int64_t *rows, int64_t *cols);
+
+/* Cost matrices of many pairs in one launch (the batched form of
+ * gpu_flexdtw.py:95-109 / Parflex.py:1470).  d_F1 (K, n1_total) and d_F2
+ * (K, n2_total): the feature arrays of every pair concatenated along frames.
+ * d_jobs: int64[8] per pair, (f1_off, f2_off, N, M, c_off, ldc, blk0, 0):
+ * the pair's frames [f1_off, f1_off+N) x [f2_off, f2_off+M), its C at
+ * element offset c_off with row stride ldc, and blk0 = sum over earlier
+ * pairs of ceil(N/rows)*ceil(M/cols) (pf_cost_tile); total_blocks = the
+ * same sum over all pairs.  Same arithmetic as pf_cost, bit for bit. */
+int pf_cost_batch(int out_dtype, const double *d_F1, int64_t n1_total, const double *d_F2,
+                  int64_t n2_total, int64_t K, const int64_t *d_jobs, int64_t njobs,
+                  int64_t total_blocks, void *d_C, double *d_work, void *stream);
 
 /* Fill helpers used instead of hipMemsetAsync (safe under graph capture). */
 int pf_fill_f64(double *d_p, double v, int64_t n, void *stream);

@@ -31,19 +31,21 @@ _SIGNATURES = {
     "pf_version": (_int, []),
     "pf_check_steps": (_int, [_vp, _vp, _int]),
     "pf_stage1": (_int, [_int, _vp, _vp, _i64, _i64, _vp, _vp, _int, _int, _int, _int, _vp, _vp,
-                         _vp, _vp, _vp, _vp]),
+                         _vp, _vp, _vp, _vp, _vp]),
     "pf_stage1_workspace": (_int, [_int, _i64, _i64, _int, _vp, _vp]),
     "pf_tile_full": (_int, [_vp, _i64, _i64, _i64, _i64, _i64, _vp, _vp, _int, _vp, _vp, _vp,
                             _vp]),
     "pf_tile_endpoint": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp]),
     "pf_tile_backtrace": (_int, [_vp, _i64, _i64, _vp, _int, _vp, _vp, _vp, _vp]),
-    "pf_stage2": (_int, [_int, _vp, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp,
+    "pf_stage2": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp,
                          _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                          _vp, _vp, _vp, _vp, _vp, _vp]),
     "pf_stitch": (_int, [_vp, _i64, _vp, _vp, _i64, _vp, _int, _vp, _vp, _vp, _vp, _i64, _vp,
                          _vp, _vp]),
     "pf_cost": (_int, [_int, _vp, _vp, _i64, _i64, _i64, _vp, _i64, _vp, _vp]),
     "pf_cost_workspace": (_int, [_int, _i64, _i64, _i64, _vp]),
+    "pf_cost_tile": (_int, [_int, _vp, _vp]),
+    "pf_cost_batch": (_int, [_int, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp]),
     "pf_fill_f64": (_int, [_vp, _dbl, _i64, _vp]),
     "pf_fill_i64": (_int, [_vp, _i64, _i64, _vp]),
 }

@@ -136,6 +136,7 @@ class BatchPlan:
             "Bp": torch.empty(max(self.b_words, 1), dtype=i32, device=device),
             "edge_D": torch.empty(T * 2 * L, dtype=f64, device=device),
             "edge_P": torch.empty(T * 2 * L, dtype=i32, device=device),
+            "edge_C": torch.empty(T * 2 * L, dtype=f64, device=device),
             "D_arr": torch.empty(T * 2 * L, dtype=f64, device=device),
             "L_arr": torch.empty(T * 2 * L, dtype=f64, device=device),
             "olap": torch.empty(T * 2 * L, dtype=f64, device=device),

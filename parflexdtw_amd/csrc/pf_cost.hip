@@ -25,45 +25,114 @@ __global__ void k_cost_normalise(const double *__restrict__ F, int64_t K, int64_
   for (int64_t k = 0; k < K; ++k) Fn[k * n + i] = F[k * n + i] / nrm;
 }
 
-// Each thread: ROWS consecutive rows of one column j; a block covers ROWS rows
-// x 256 columns, so F2n loads and C stores are coalesced along j and F1n
-// values are uniform across the block.
-template <int ROWS>
-__global__ __launch_bounds__(256) void k_cost_f64(const double *__restrict__ F1n,
-                                                  const double *__restrict__ F2n, int64_t K,
-                                                  int64_t N, int64_t M, double *__restrict__ C,
-                                                  int64_t ldc) {
-  const int64_t j = blockIdx.y * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t i0 = blockIdx.x * (int64_t)ROWS;
-  if (j >= M) return;
-  double acc[ROWS];
+// One cost job: a (N x M) block of C from frames [f1_off, f1_off + N) of the
+// first feature array and [f2_off, f2_off + M) of the second; job tables are
+// int64[8] rows (f1_off, f2_off, N, M, c_off, ldc, blk0, 0), blk0 = first
+// workgroup of the job in a batched launch.
+struct CostJob {
+  int64_t f1_off, f2_off, N, M, c_off, ldc, blk0, pad;
+};
+static_assert(sizeof(CostJob) == 64, "cost job rows are int64[8]");
+
+// Workgroup -> (job, local workgroup).  Batched: the job is the number of
+// jobs with blk0 <= blockIdx.x, minus one, counted 64 jobs per step with a
+// wave ballot (the loads of all steps are independent, so they overlap);
+// tables longer than 1024 jobs use a binary search.
+template <bool BATCH>
+__device__ __forceinline__ CostJob find_job(const CostJob &one, const CostJob *__restrict__ jobs,
+                                            int64_t njobs, int64_t &blk) {
+  blk = blockIdx.x;
+  if constexpr (!BATCH) {
+    return one;
+  } else {
+    int64_t p;
+    if (njobs <= 1024) {
+      const int lane = threadIdx.x & 63;
+      int cnt = 0;
+      for (int64_t base = 0; base < njobs; base += 64) {
+        const int64_t jj = base + lane;
+        const bool le = jj < njobs && jobs[jj].blk0 <= blk;
+        cnt += __popcll(__ballot(le));
+      }
+      p = cnt - 1;
+    } else {
+      int64_t lo = 0, hi = njobs - 1;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].blk0 <= blk) lo = mid; else hi = mid - 1;
+      }
+      p = lo;
+    }
+    const CostJob jb = jobs[p];
+    blk -= jb.blk0;
+    return jb;
+  }
+}
+
+__device__ __forceinline__ void st_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_nt(float *p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_nt(__hip_bfloat16 *p, __hip_bfloat16 v) {
+  __builtin_nontemporal_store(__builtin_bit_cast(unsigned short, v),
+                              reinterpret_cast<unsigned short *>(p));
+}
+
+// FP64: a workgroup of 256 threads covers 64 rows x 256 columns, one column
+// per thread, 16 rows at a time; F1n values are uniform across the
+// workgroup (scalar loads), F2n loads and C stores are coalesced along j.
+constexpr int kF64Rows = 64, kF64Cols = 256, kF64Chunk = 16;
+
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_cost_f64(const double *__restrict__ F1n, int64_t n1,
+                                                  const double *__restrict__ F2n, int64_t n2,
+                                                  int64_t K, CostJob one,
+                                                  const CostJob *__restrict__ jobs, int64_t njobs,
+                                                  double *__restrict__ C) {
+  int64_t blk;
+  const CostJob jb = find_job<BATCH>(one, jobs, njobs, blk);
+  const int64_t nbc = (jb.M + kF64Cols - 1) / kF64Cols;
+  const int64_t i0 = (blk / nbc) * kF64Rows;
+  const int64_t j = (blk % nbc) * kF64Cols + threadIdx.x;
+  const int64_t jc = j < jb.M ? j : jb.M - 1;
+  const double *a_base = F1n + jb.f1_off;
+  const double *b_base = F2n + jb.f2_off + jc;
+  double *c_base = C + jb.c_off + j;
+  for (int ch = 0; ch < kF64Rows; ch += kF64Chunk) {
+    if (i0 + ch >= jb.N) break;
+    double acc[kF64Chunk];
 #pragma unroll
-  for (int r = 0; r < ROWS; ++r) acc[r] = 0.0;
-  for (int64_t k = 0; k < K; ++k) {
-    const double b = F2n[k * M + j];
+    for (int r = 0; r < kF64Chunk; ++r) acc[r] = 0.0;
+    for (int64_t k = 0; k < K; ++k) {
+      const double b = b_base[k * n2];
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-      const int64_t i = i0 + r < N ? i0 + r : N - 1;
-      const double p = F1n[k * N + i] * b;
-      acc[r] = acc[r] + p;
+      for (int r = 0; r < kF64Chunk; ++r) {
+        const int64_t i = i0 + ch + r < jb.N ? i0 + ch + r : jb.N - 1;
+        const double p = a_base[k * n1 + i] * b;
+        acc[r] = acc[r] + p;
+      }
+    }
+    if (j < jb.M) {
+#pragma unroll
+      for (int r = 0; r < kF64Chunk; ++r)
+        if (i0 + ch + r < jb.N) st_nt(c_base + (i0 + ch + r) * jb.ldc, 1.0 - acc[r]);
     }
   }
-#pragma unroll
-  for (int r = 0; r < ROWS; ++r)
-    if (i0 + r < N) C[(i0 + r) * ldc + j] = 1.0 - acc[r];
 }
 
 // ---------------------------------------------------------------------------
 // BF16 MFMA form (precision form): features normalised in FP64, rounded to
 // BF16 and stored frame-major with K padded to a multiple of 16; the product
 // runs on v_mfma_f32_32x32x16_bf16 with FP32 accumulation.  A workgroup of
-// 4 waves computes a 128 x 128 block of C, each wave 2 x 2 MFMA blocks of
-// 32 x 32.  Operand layout (per lane l of a wave): A row l%32, k-offset
-// (l/32)*8, 8 consecutive k; B column l%32, same k; D column l%32, rows
-// (v/4)*8 + (l/32)*4 + v%4 for the 16 accumulators v.
+// 4 waves computes a 32 x 512 block of C (wide, so that each row of the
+// block is one long contiguous run in HBM), each wave 32 x 128 as four MFMA
+// blocks of 32 x 32 sharing the A operand.  Operand layout (per lane l of a
+// wave): A row l%32, k-offset (l/32)*8, 8 consecutive k; B column l%32,
+// same k; D column l%32, rows (v/4)*8 + (l/32)*4 + v%4 for the 16
+// accumulators v.  The block is staged through LDS and written row by row,
+// 16 bytes per lane where the row is 16-byte aligned in C.
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx16_t __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
 // Fb[i*Kp + k] = bf16(F[k*n + i] / (sqrt(sum_k F[k*n+i]^2) + 1e-9)), zero pad.
 __global__ void k_cost_prep_bf16(const double *__restrict__ F, int64_t K, int64_t n, int64_t Kp,
@@ -80,56 +149,84 @@ __global__ void k_cost_prep_bf16(const double *__restrict__ F, int64_t K, int64_
     Fb[i * Kp + k] = __float2bfloat16(k < K ? (float)(F[k * n + i] / nrm) : 0.0f);
 }
 
-__device__ __forceinline__ void store_c(float *p, float v) { *p = v; }
-__device__ __forceinline__ void store_c(__hip_bfloat16 *p, float v) { *p = __float2bfloat16(v); }
+__device__ __forceinline__ float cvt_out(float v, float *) { return v; }
+__device__ __forceinline__ __hip_bfloat16 cvt_out(float v, __hip_bfloat16 *) {
+  return __float2bfloat16(v);
+}
 
-template <typename OT>
-__global__ __launch_bounds__(256) void k_cost_mfma_bf16(const __hip_bfloat16 *__restrict__ A,
-                                                        const __hip_bfloat16 *__restrict__ B,
-                                                        int64_t Kp, int64_t N, int64_t M,
-                                                        OT *__restrict__ C, int64_t ldc) {
+constexpr int kMfmaRows = 32, kMfmaCols = 512;
+constexpr int kTilePad = 8;  // keeps staged rows 16-byte aligned, no bank conflicts
+
+template <typename OT, bool BATCH>
+__global__ __launch_bounds__(256) void k_cost_mfma_bf16(const __hip_bfloat16 *__restrict__ A0,
+                                                        const __hip_bfloat16 *__restrict__ B0,
+                                                        int64_t Kp, CostJob one,
+                                                        const CostJob *__restrict__ jobs,
+                                                        int64_t njobs, OT *__restrict__ C,
+                                                        int vec_ok) {
+  __shared__ __attribute__((aligned(16))) OT tile[kMfmaRows][kMfmaCols + kTilePad];
+  int64_t blk;
+  const CostJob jb = find_job<BATCH>(one, jobs, njobs, blk);
+  const int64_t N = jb.N, M = jb.M;
+  const int64_t nbc = (M + kMfmaCols - 1) / kMfmaCols;
+  const int64_t row0 = (blk / nbc) * kMfmaRows, col0 = (blk % nbc) * kMfmaCols;
+  const __hip_bfloat16 *A = A0 + jb.f1_off * Kp;
+  const __hip_bfloat16 *B = B0 + jb.f2_off * Kp;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int64_t row0 = blockIdx.x * 128ll + (wave >> 1) * 64;
-  const int64_t col0 = blockIdx.y * 128ll + (wave & 1) * 64;
+  const int wc = wave * 128;
   const int lr = lane & 31, kh = (lane >> 5) * 8;
   // rows/columns past the matrix read the last valid frame (results unused)
-  int64_t ra[2], cb[2];
+  const int64_t ra = row0 + lr < N ? row0 + lr : N - 1;
+  int64_t cb[4];
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    ra[i] = row0 + i * 32 + lr < N ? row0 + i * 32 + lr : N - 1;
-    cb[i] = col0 + i * 32 + lr < M ? col0 + i * 32 + lr : M - 1;
-  }
-  floatx16_t acc[2][2];
+  for (int j = 0; j < 4; ++j) cb[j] = col0 + wc + j * 32 + lr < M ? col0 + wc + j * 32 + lr : M - 1;
+  floatx16_t acc[4];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = floatx16_t{};
+  for (int j = 0; j < 4; ++j) acc[j] = floatx16_t{};
   for (int64_t kk = 0; kk < Kp; kk += 16) {
-    bf16x8_t a[2], b[2];
+    const bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(A + ra * Kp + kk + kh);
+    bf16x8_t b[4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      a[i] = *reinterpret_cast<const bf16x8_t *>(A + ra[i] * Kp + kk + kh);
-      b[i] = *reinterpret_cast<const bf16x8_t *>(B + cb[i] * Kp + kk + kh);
-    }
+    for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8_t *>(B + cb[j] * Kp + kk + kh);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    for (int j = 0; j < 4; ++j)
+      acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[j], acc[j], 0, 0, 0);
   }
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int j = 0; j < 4; ++j)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int64_t col = col0 + j * 32 + lr;
-      if (col >= M) continue;
+    for (int v = 0; v < 16; ++v)
+      tile[(v >> 2) * 8 + (lane >> 5) * 4 + (v & 3)][wc + j * 32 + lr] =
+          cvt_out(1.0f - acc[j][v], (OT *)nullptr);
+  __syncthreads();
+  // One row per wave per step: 16-byte chunks of E elements when the row's
+  // first element is 16-byte aligned in C, else one element per lane.
+  constexpr int E = 16 / (int)sizeof(OT);
+  const int64_t rows = N - row0 < kMfmaRows ? N - row0 : kMfmaRows;
+  const int64_t cols = M - col0 < kMfmaCols ? M - col0 : kMfmaCols;
+  for (int rr = wave; rr < rows; rr += 4) {
+    const int64_t e0 = jb.c_off + (row0 + rr) * jb.ldc + col0;
+    OT *dst = C + e0;
+    if (vec_ok && (e0 % E) == 0) {
 #pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int64_t row = row0 + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-        if (row < N) store_c(C + row * ldc + col, 1.0f - acc[i][j][v]);
+      for (int h = 0; h < kMfmaCols; h += 64 * E) {
+        const int c = h + lane * E;
+        if (c + E <= cols) {
+          const u32x4_t v = *reinterpret_cast<const u32x4_t *>(&tile[rr][c]);
+          __builtin_nontemporal_store(v, reinterpret_cast<u32x4_t *>(dst + c));
+        } else {
+#pragma unroll
+          for (int e = 0; e < E; ++e)
+            if (c + e < cols) st_nt(dst + c + e, tile[rr][c + e]);
+        }
       }
+    } else {
+#pragma unroll
+      for (int h = 0; h < kMfmaCols; h += 64)
+        if (h + lane < cols) st_nt(dst + h + lane, tile[rr][h + lane]);
     }
+  }
 }
 
 }  // namespace pf
@@ -148,49 +245,90 @@ extern "C" int pf_cost_workspace(int out_dtype, int64_t K, int64_t N, int64_t M,
   return PF_OK;
 }
 
+extern "C" int pf_cost_tile(int out_dtype, int64_t *rows, int64_t *cols) {
+  PF_REQUIRE(rows && cols, "pf_cost_tile: null argument");
+  if (out_dtype == PF_F64) {
+    *rows = kF64Rows;
+    *cols = kF64Cols;
+  } else {
+    PF_REQUIRE(out_dtype == PF_F32 || out_dtype == PF_BF16, "pf_cost_tile: unknown dtype");
+    *rows = kMfmaRows;
+    *cols = kMfmaCols;
+  }
+  return PF_OK;
+}
+
+namespace {
+
+// Normalise (FP64) or normalise-and-round (BF16) both feature arrays into
+// d_work, then launch the product over nblocks workgroups.
+template <bool BATCH>
+int cost_launch(int out_dtype, const double *d_F1, int64_t n1, const double *d_F2, int64_t n2,
+                int64_t K, const CostJob &one, const CostJob *d_jobs, int64_t njobs,
+                int64_t nblocks, void *d_C, double *d_work, hipStream_t st) {
+  PF_REQUIRE(nblocks >= 1 && nblocks < (1ll << 31), "pf_cost: grid size out of range");
+  if (out_dtype == PF_F32 || out_dtype == PF_BF16) {
+    const int64_t Kp = (K + 15) / 16 * 16;
+    PF_REQUIRE(reinterpret_cast<uintptr_t>(d_work) % 16 == 0,
+               "pf_cost: d_work must be 16-byte aligned");
+    __hip_bfloat16 *A = reinterpret_cast<__hip_bfloat16 *>(d_work);
+    __hip_bfloat16 *B = A + n1 * Kp;
+    hipLaunchKernelGGL(k_cost_prep_bf16, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, st,
+                       d_F1, K, n1, Kp, A);
+    PF_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_cost_prep_bf16, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st,
+                       d_F2, K, n2, Kp, B);
+    PF_CHECK_HIP(hipGetLastError());
+    const int vec_ok = reinterpret_cast<uintptr_t>(d_C) % 16 == 0;
+    if (out_dtype == PF_F32)
+      hipLaunchKernelGGL((k_cost_mfma_bf16<float, BATCH>), dim3((unsigned)nblocks), dim3(256), 0,
+                         st, A, B, Kp, one, d_jobs, njobs, static_cast<float *>(d_C), vec_ok);
+    else
+      hipLaunchKernelGGL((k_cost_mfma_bf16<__hip_bfloat16, BATCH>), dim3((unsigned)nblocks),
+                         dim3(256), 0, st, A, B, Kp, one, d_jobs, njobs,
+                         static_cast<__hip_bfloat16 *>(d_C), vec_ok);
+    PF_CHECK_HIP(hipGetLastError());
+    return PF_OK;
+  }
+  PF_REQUIRE(out_dtype == PF_F64, "pf_cost: unknown output dtype");
+  double *F1n = d_work, *F2n = d_work + K * n1;
+  hipLaunchKernelGGL(k_cost_normalise, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, st, d_F1,
+                     K, n1, F1n);
+  PF_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_cost_normalise, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, d_F2,
+                     K, n2, F2n);
+  PF_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL((k_cost_f64<BATCH>), dim3((unsigned)nblocks), dim3(256), 0, st, F1n, n1, F2n,
+                     n2, K, one, d_jobs, njobs, static_cast<double *>(d_C));
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+}  // namespace
+
 extern "C" int pf_cost(int out_dtype, const double *d_F1, const double *d_F2, int64_t K,
                        int64_t N, int64_t M, void *d_C, int64_t ldc, double *d_work,
                        void *stream) {
   PF_REQUIRE(K >= 1 && N >= 1 && M >= 1 && ldc >= M, "pf_cost: bad shape");
   PF_REQUIRE(d_F1 && d_F2 && d_C && d_work, "pf_cost: null buffer");
-  hipStream_t st = as_stream(stream);
-  if (out_dtype == PF_F32 || out_dtype == PF_BF16) {
-    const int64_t Kp = (K + 15) / 16 * 16;
-    __hip_bfloat16 *A = reinterpret_cast<__hip_bfloat16 *>(d_work);
-    __hip_bfloat16 *B = A + N * Kp;
-    PF_REQUIRE(reinterpret_cast<uintptr_t>(d_work) % 16 == 0, "pf_cost: d_work must be 16-byte aligned");
-    hipLaunchKernelGGL(k_cost_prep_bf16, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, d_F1,
-                       K, N, Kp, A);
-    PF_CHECK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_cost_prep_bf16, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st, d_F2,
-                       K, M, Kp, B);
-    PF_CHECK_HIP(hipGetLastError());
-    PF_REQUIRE((N + 127) / 128 < (1ll << 31) && (M + 127) / 128 < 65536,
-               "pf_cost: matrix too large for one launch");
-    const dim3 grid((unsigned)((N + 127) / 128), (unsigned)((M + 127) / 128));
-    if (out_dtype == PF_F32)
-      hipLaunchKernelGGL(k_cost_mfma_bf16<float>, grid, dim3(256), 0, st, A, B, Kp, N, M,
-                         static_cast<float *>(d_C), ldc);
-    else
-      hipLaunchKernelGGL(k_cost_mfma_bf16<__hip_bfloat16>, grid, dim3(256), 0, st, A, B, Kp, N, M,
-                         static_cast<__hip_bfloat16 *>(d_C), ldc);
-    PF_CHECK_HIP(hipGetLastError());
-    return PF_OK;
-  }
-  PF_REQUIRE(out_dtype == PF_F64, "pf_cost: unknown output dtype");
-  double *F1n = d_work, *F2n = d_work + K * N;
-  hipLaunchKernelGGL(k_cost_normalise, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, d_F1,
-                     K, N, F1n);
-  PF_CHECK_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_cost_normalise, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st, d_F2,
-                     K, M, F2n);
-  PF_CHECK_HIP(hipGetLastError());
-  constexpr int ROWS = 4;
-  PF_REQUIRE((N + ROWS - 1) / ROWS < (1ll << 31) && (M + 255) / 256 < 65536,
-             "pf_cost: matrix too large for one launch");
-  hipLaunchKernelGGL(k_cost_f64<ROWS>, dim3((unsigned)((N + ROWS - 1) / ROWS),
-                                            (unsigned)((M + 255) / 256)),
-                     dim3(256), 0, st, F1n, F2n, K, N, M, static_cast<double *>(d_C), ldc);
-  PF_CHECK_HIP(hipGetLastError());
-  return PF_OK;
+  int64_t tr, tc;
+  if (int rc = pf_cost_tile(out_dtype, &tr, &tc)) return rc;
+  const CostJob one{0, 0, N, M, 0, ldc, 0, 0};
+  const int64_t nblocks = ((N + tr - 1) / tr) * ((M + tc - 1) / tc);
+  return cost_launch<false>(out_dtype, d_F1, N, d_F2, M, K, one, nullptr, 1, nblocks, d_C,
+                            d_work, as_stream(stream));
+}
+
+extern "C" int pf_cost_batch(int out_dtype, const double *d_F1, int64_t n1_total,
+                             const double *d_F2, int64_t n2_total, int64_t K,
+                             const int64_t *d_jobs, int64_t njobs, int64_t total_blocks,
+                             void *d_C, double *d_work, void *stream) {
+  PF_REQUIRE(K >= 1 && n1_total >= 1 && n2_total >= 1 && njobs >= 1, "pf_cost_batch: bad shape");
+  PF_REQUIRE(d_F1 && d_F2 && d_jobs && d_C && d_work, "pf_cost_batch: null buffer");
+  int64_t tr, tc;
+  if (int rc = pf_cost_tile(out_dtype, &tr, &tc)) return rc;
+  const CostJob none{};
+  return cost_launch<true>(out_dtype, d_F1, n1_total, d_F2, n2_total, K, none,
+                           reinterpret_cast<const CostJob *>(d_jobs), njobs, total_blocks, d_C,
+                           d_work, as_stream(stream));
 }

@@ -16,6 +16,9 @@
 // and the sign of the difference is the answer; otherwise (a near tie) both
 // quotients are computed with IEEE division and compared, exactly as the
 // reference does.  The result is bit-identical to dividing every time.
+// The FP32 DP of the precision form (BF16 or FP32 cost) has no bit-exact
+// target: there path costs are fused multiply-adds and the order is taken
+// from the cross products alone.
 //
 // Mapping.  A cell depends only on rows r-1 and r-2 (every step has
 // dr >= 1), so a row is computed in parallel across its columns.  A tile's
@@ -152,7 +155,8 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
     const CT *__restrict__ Cb, const TileDesc *__restrict__ tiles, TileDesc td0, StepSet ss,
     int64_t Lmax, int S, int G, int *__restrict__ flags, Xch<DT> *__restrict__ xch,
     uint32_t *__restrict__ Bp, double *__restrict__ eD, int32_t *__restrict__ eP,
-    double *__restrict__ Df, int32_t *__restrict__ Pf, int8_t *__restrict__ Bf) {
+    double *__restrict__ eC, double *__restrict__ Df, int32_t *__restrict__ Pf,
+    int8_t *__restrict__ Bf) {
   static_assert(CPL >= 2, "a lane needs two columns of left context");
   __shared__ DT shD[kRing][16][2];
   __shared__ DT shX[kRing][16][2];
@@ -227,6 +231,9 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
         Pf[c] = 0;
         Bf[c] = 0;
       } else {
+        // first row and first column of C (the start cells stage 2 reads)
+        eC[td.e_off + c] = (double)v;
+        if (c == 0) eC[td.e_off + Lmax] = (double)v;
         if (c == W - 1) {
           eD[td.e_off + Lmax] = (double)D1[j];
           eP[td.e_off + Lmax] = 0;
@@ -323,23 +330,41 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
           const DT pD1 = j >= 2 ? D1[j - 2] : (j == 1 ? LD1b : LD1a);
           const DT pX1 = j >= 2 ? X1[j - 2] : (j == 1 ? LX1b : LX1a);
           const DT pD2 = j >= 1 ? D2[j - 1] : LD2b, pX2 = j >= 1 ? X2[j - 1] : LX2b;
-          const DT a0 = pD0 + cval * wk[0], m0 = dabs(pX0) + (DT)2;
-          const DT a1 = pD1 + cval * wk[1], m1 = dabs(pX1) + (DT)3;
-          const DT a2 = pD2 + cval * wk[2], m2 = dabs(pX2) + (DT)3;
-          bool ok1, ok2;
           const bool v1 = c >= 2, v2 = r >= 2;
-          const bool t1 = fast_less(a1, m1, a0, m0, ok1) && v1;
-          ba = t1 ? a1 : a0;
-          bm = t1 ? m1 : m0;
-          bx = t1 ? pX1 : pX0;
-          bk = t1 ? 1 : 0;
-          const bool t2 = fast_less(a2, m2, ba, bm, ok2) && v2;
-          ba = t2 ? a2 : ba;
-          bm = t2 ? m2 : bm;
-          bx = t2 ? pX2 : bx;
-          bk = t2 ? 2 : bk;
-          // Column 0 and columns past the tile are not computed here.
-          amb = amb || ((c >= 1) && (c < W) && ((v1 && !ok1) || (v2 && !ok2)));
+          if constexpr (sizeof(DT) == 4) {
+            // FP32 DP (precision form): fused multiply-add path costs and the
+            // plain cross-product order a1/m1 < a0/m0 <=> a1*m0 < a0*m1.
+            const DT a0 = __builtin_fmaf(cval, wk[0], pD0), m0 = dabs(pX0) + (DT)2;
+            const DT a1 = __builtin_fmaf(cval, wk[1], pD1), m1 = dabs(pX1) + (DT)3;
+            const DT a2 = __builtin_fmaf(cval, wk[2], pD2), m2 = dabs(pX2) + (DT)3;
+            const bool t1 = (a1 * m0 < a0 * m1) && v1;
+            ba = t1 ? a1 : a0;
+            bm = t1 ? m1 : m0;
+            bx = t1 ? pX1 : pX0;
+            bk = t1 ? 1 : 0;
+            const bool t2 = (a2 * bm < ba * m2) && v2;
+            ba = t2 ? a2 : ba;
+            bm = t2 ? m2 : bm;
+            bx = t2 ? pX2 : bx;
+            bk = t2 ? 2 : bk;
+          } else {
+            const DT a0 = pD0 + cval * wk[0], m0 = dabs(pX0) + (DT)2;
+            const DT a1 = pD1 + cval * wk[1], m1 = dabs(pX1) + (DT)3;
+            const DT a2 = pD2 + cval * wk[2], m2 = dabs(pX2) + (DT)3;
+            bool ok1, ok2;
+            const bool t1 = fast_less(a1, m1, a0, m0, ok1) && v1;
+            ba = t1 ? a1 : a0;
+            bm = t1 ? m1 : m0;
+            bx = t1 ? pX1 : pX0;
+            bk = t1 ? 1 : 0;
+            const bool t2 = fast_less(a2, m2, ba, bm, ok2) && v2;
+            ba = t2 ? a2 : ba;
+            bm = t2 ? m2 : bm;
+            bx = t2 ? pX2 : bx;
+            bk = t2 ? 2 : bk;
+            // Column 0 and columns past the tile are not computed here.
+            amb = amb || ((c >= 1) && (c < W) && ((v1 && !ok1) || (v2 && !ok2)));
+          }
         } else {
           // Generic step set (runtime, K <= 4): exact comparisons.
           DT E1[CPL + 2], F1[CPL + 2], E2[CPL + 2], F2[CPL + 2];
@@ -388,7 +413,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
         NX[j] = copy_sign(bm, bx);
         code[j] = bk;
       }
-      if constexpr (STD) {
+      if constexpr (STD && sizeof(DT) == 8) {
         if (amb) {
           // Near tie (rare): redo this lane's cells with the reference's
           // comparisons, IEEE division included.
@@ -422,6 +447,9 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
             code[j] = bk;
           }
         }
+      }
+      if constexpr (!FULL) {
+        if (c0 == 0) eC[td.e_off + Lmax + r] = (double)Cc[0];
       }
       if (border) {
         // Cells whose predecessor is on row 0 or column 0, and column 0
@@ -583,28 +611,28 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
 template <typename CT, typename DT, bool STD, int CPL>
 static void launch_cpl(dim3 grid, dim3 block, hipStream_t st, const CT *C, const TileDesc *tiles,
                        const TileDesc &td0, const StepSet &ss, int64_t Lmax, int S, int G,
-                       int *flags, void *xch, uint32_t *Bp, double *eD, int32_t *eP, double *Df,
-                       int32_t *Pf, int8_t *Bf, bool full) {
+                       int *flags, void *xch, uint32_t *Bp, double *eD, int32_t *eP, double *eC,
+                       double *Df, int32_t *Pf, int8_t *Bf, bool full) {
   Xch<DT> *x = static_cast<Xch<DT> *>(xch);
   if (full)
     hipLaunchKernelGGL((k_stage1<CPL, CT, DT, true, STD>), grid, block, 0, st, C, tiles, td0, ss,
-                       Lmax, S, G, flags, x, Bp, eD, eP, Df, Pf, Bf);
+                       Lmax, S, G, flags, x, Bp, eD, eP, eC, Df, Pf, Bf);
   else
     hipLaunchKernelGGL((k_stage1<CPL, CT, DT, false, STD>), grid, block, 0, st, C, tiles, td0, ss,
-                       Lmax, S, G, flags, x, Bp, eD, eP, Df, Pf, Bf);
+                       Lmax, S, G, flags, x, Bp, eD, eP, eC, Df, Pf, Bf);
 }
 
 template <typename CT, typename DT>
 static int launch_dp(int cpl, int nw, int S, int G, int64_t n_items, hipStream_t st, const CT *C,
                      const TileDesc *tiles, const TileDesc &td0, const StepSet &ss, int64_t Lmax,
-                     int *flags, void *xch, uint32_t *Bp, double *eD, int32_t *eP, double *Df,
-                     int32_t *Pf, int8_t *Bf, bool full) {
+                     int *flags, void *xch, uint32_t *Bp, double *eD, int32_t *eP, double *eC,
+                     double *Df, int32_t *Pf, int8_t *Bf, bool full) {
   const bool std3 = ss.K == 3 && ss.dr[0] == 1 && ss.dc[0] == 1 && ss.dr[1] == 1 &&
                     ss.dc[1] == 2 && ss.dr[2] == 2 && ss.dc[2] == 1;
   const dim3 grid((unsigned)n_items), block((unsigned)(nw * 64));
 #define PF_LAUNCH(STDV, CPLV)                                                                  \
   launch_cpl<CT, DT, STDV, CPLV>(grid, block, st, C, tiles, td0, ss, Lmax, S, G, flags, xch,  \
-                                 Bp, eD, eP, Df, Pf, Bf, full)
+                                 Bp, eD, eP, eC, Df, Pf, Bf, full)
   if (std3) {
     if (cpl == 2) PF_LAUNCH(true, 2);
     else if (cpl == 4) PF_LAUNCH(true, 4);
@@ -640,7 +668,7 @@ extern "C" int pf_stage1(int dtype, const void *d_C, const int64_t *d_tiles, int
                          int64_t Lmax, const int32_t *steps, const double *weights, int K,
                          int strips, int waves, int rows_per_block, int32_t *d_flags,
                          void *d_xch, uint32_t *d_Bpacked, double *d_edge_D, int32_t *d_edge_P,
-                         void *stream) {
+                         double *d_edge_C, void *stream) {
   StepSet ss;
   int rc = make_stepset(steps, weights, K, &ss);
   if (rc) return rc;
@@ -653,7 +681,8 @@ extern "C" int pf_stage1(int dtype, const void *d_C, const int64_t *d_tiles, int
              "pf_stage1: at most 16/8/4 waves for 2/4/8 columns per lane");
   PF_REQUIRE(Lmax <= 8192, "tile length above 8192 is not supported");
   if (ntiles == 0) return PF_OK;
-  PF_REQUIRE(d_C && d_tiles && d_Bpacked && d_edge_D && d_edge_P, "pf_stage1: null buffer");
+  PF_REQUIRE(d_C && d_tiles && d_Bpacked && d_edge_D && d_edge_P && d_edge_C,
+             "pf_stage1: null buffer");
   PF_REQUIRE(strips == 1 || (d_flags && d_xch), "pf_stage1: strips > 1 need flags and xch");
   const int64_t n_items = ntiles * strips;
   PF_REQUIRE(n_items < (1ll << 31), "pf_stage1: too many work items");
@@ -672,18 +701,18 @@ extern "C" int pf_stage1(int dtype, const void *d_C, const int64_t *d_tiles, int
     case PF_F64:
       return launch_dp<double, double>(cpl, waves, strips, rows_per_block, n_items, st,
                                        static_cast<const double *>(d_C), tiles, td0, ss, Lmax,
-                                       flags, d_xch, d_Bpacked, d_edge_D, d_edge_P, nullptr,
-                                       nullptr, nullptr, false);
+                                       flags, d_xch, d_Bpacked, d_edge_D, d_edge_P, d_edge_C,
+                                       nullptr, nullptr, nullptr, false);
     case PF_F32:
       return launch_dp<float, float>(cpl, waves, strips, rows_per_block, n_items, st,
                                      static_cast<const float *>(d_C), tiles, td0, ss, Lmax, flags,
-                                     d_xch, d_Bpacked, d_edge_D, d_edge_P, nullptr, nullptr,
-                                     nullptr, false);
+                                     d_xch, d_Bpacked, d_edge_D, d_edge_P, d_edge_C, nullptr,
+                                     nullptr, nullptr, false);
     case PF_BF16:
       return launch_dp<__hip_bfloat16, float>(cpl, waves, strips, rows_per_block, n_items, st,
                                               static_cast<const __hip_bfloat16 *>(d_C), tiles,
                                               td0, ss, Lmax, flags, d_xch, d_Bpacked, d_edge_D,
-                                              d_edge_P, nullptr, nullptr, nullptr, false);
+                                              d_edge_P, d_edge_C, nullptr, nullptr, nullptr, false);
     default:
       PF_REQUIRE(false, "pf_stage1: unknown dtype");
   }
@@ -716,6 +745,6 @@ extern "C" int pf_tile_full(const double *d_C, int64_t ldc, int64_t r0, int64_t 
   const int nw = (int)(((W + cpl - 1) / cpl + 63) / 64);
   const TileDesc td{r0 * ldc + c0, ldc, R, W, 0, 0, 0, 0};
   return launch_dp<double, double>(cpl, nw, 1, 16, 1, as_stream(stream), d_C, nullptr, td, ss,
-                                   W, nullptr, nullptr, nullptr, nullptr, nullptr, d_D, d_P, d_B,
+                                   W, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_D, d_P, d_B,
                                    true);
 }

@@ -43,12 +43,6 @@ __device__ __forceinline__ Geo tile_geo(const PairDesc &pd, int64_t L, int64_t i
   return g;
 }
 
-__device__ __forceinline__ double c_at(int c_dtype, const void *C, int64_t idx) {
-  if (c_dtype == PF_F64) return static_cast<const double *>(C)[idx];
-  if (c_dtype == PF_F32) return (double)static_cast<const float *>(C)[idx];
-  return (double)__bfloat162float(static_cast<const __hip_bfloat16 *>(C)[idx]);
-}
-
 __device__ __forceinline__ int64_t iabs64(int64_t v) { return v < 0 ? -v : v; }
 
 __global__ void k_fill_f64(double *p, double v, int64_t n) {
@@ -68,20 +62,29 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
 }
 
 // _build_edge_data (Parflex.py:509-564) + _edge_starts_from_S
-// (Parflex.py:347-373): overlap cost C[start] per slot, start-set bitmaps of
-// every tile (bit s of map 0: some edge cell starts at (0, s), s > 0; bit a of
-// map 1: some edge cell starts at (a, 0), a >= 0), and D_arr = L_arr = inf.
-__global__ void k_s2_prep(int c_dtype, const void *__restrict__ C, const PairDesc *__restrict__ pairs,
-                          const TileDesc *__restrict__ tiles, int64_t ntiles, int64_t L,
-                          const int32_t *__restrict__ eP, double *__restrict__ Darr,
-                          double *__restrict__ Larr, double *__restrict__ olap,
-                          uint32_t *__restrict__ bits, int64_t bitw) {
-  const int64_t n = ntiles * 2 * L;
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
-       q += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t t = q / (2 * L);
-    const int64_t rem = q - t * 2 * L;
-    const int e = (int)(rem / L);
+// (Parflex.py:347-373): overlap cost C[start] per slot (from the first row /
+// first column of C that stage 1 stored per tile), start-set bitmaps of
+// every tile (bit s of map 0: some edge cell starts at (0, s), s > 0; bit a
+// of map 1: some edge cell starts at (a, 0), a >= 0), and D_arr = L_arr =
+// inf.  Workgroup (t, y) covers slots y*256 .. y*256+255 of tile t; the
+// bitmaps are gathered in LDS and merged into global memory once per word.
+constexpr int kMaxBitw = 8192 / 32 + 1;
+
+__global__ __launch_bounds__(256) void k_s2_prep(const TileDesc *__restrict__ tiles, int64_t L,
+                                                 const int32_t *__restrict__ eP,
+                                                 const double *__restrict__ eC,
+                                                 double *__restrict__ Darr,
+                                                 double *__restrict__ Larr,
+                                                 double *__restrict__ olap,
+                                                 uint32_t *__restrict__ bits, int64_t bitw) {
+  __shared__ uint32_t lb[2][kMaxBitw];
+  const int64_t t = blockIdx.x;
+  for (int w = threadIdx.x; w < 2 * kMaxBitw; w += blockDim.x) lb[w / kMaxBitw][w % kMaxBitw] = 0u;
+  __syncthreads();
+  const int64_t rem = blockIdx.y * (int64_t)blockDim.x + threadIdx.x;
+  if (rem < 2 * L) {
+    const int64_t q = t * 2 * L + rem;
+    const int e = rem >= L ? 1 : 0;
     const int64_t pos = rem - (int64_t)e * L;
     Darr[q] = __builtin_huge_val();
     Larr[q] = __builtin_huge_val();
@@ -89,20 +92,19 @@ __global__ void k_s2_prep(int c_dtype, const void *__restrict__ C, const PairDes
     const int64_t elen = e == 0 ? td.W : td.R;
     if (pos >= elen) {
       olap[q] = 0.0;
-      continue;
-    }
-    const PairDesc pd = pairs[td.pair];
-    const int64_t s = eP[q];
-    const int64_t sr = s < 0 ? -s : 0, sc = s > 0 ? s : 0;
-    const int64_t ti = td.ij >> 32, tj = td.ij & 0xffffffffll;
-    const Geo g = tile_geo(pd, L, ti, tj);
-    olap[q] = c_at(c_dtype, C, pd.c_off + (g.s1 + sr) * pd.ldc + (g.s2 + sc));
-    if (s > 0) {
-      atomicOr(bits + (t * 2 + 0) * bitw + (s >> 5), 1u << (s & 31));
     } else {
-      const int64_t a = -s;
-      atomicOr(bits + (t * 2 + 1) * bitw + (a >> 5), 1u << (a & 31));
+      const int64_t s = eP[q];
+      olap[q] = s > 0 ? eC[td.e_off + s] : eC[td.e_off + L - s];
+      if (s > 0) atomicOr(&lb[0][s >> 5], 1u << (s & 31));
+      else atomicOr(&lb[1][(-s) >> 5], 1u << ((-s) & 31));
     }
+  }
+  __syncthreads();
+  for (int w = threadIdx.x; w < 2 * bitw; w += blockDim.x) {
+    const int m = w >= bitw ? 1 : 0;
+    const int64_t k = w - m * bitw;
+    const uint32_t v = lb[m][k];
+    if (v) atomicOr(bits + (t * 2 + m) * bitw + k, v);
   }
 }
 
@@ -759,7 +761,7 @@ static int stitch_common(hipStream_t st, int from_best, const int64_t *d_req, in
   return PF_OK;
 }
 
-extern "C" int pf_stage2(int c_dtype, const void *d_C, const int64_t *d_pairs, int64_t npairs,
+extern "C" int pf_stage2(const double *d_edge_C, const int64_t *d_pairs, int64_t npairs,
                          const int64_t *d_tiles, int64_t ntiles, int64_t L, int64_t max_diag,
                          int64_t max_tile_diag, int64_t max_tiles_pair, int64_t max_segs,
                          const int32_t *steps, int K, const uint32_t *d_Bpacked,
@@ -775,9 +777,8 @@ extern "C" int pf_stage2(int c_dtype, const void *d_C, const int64_t *d_pairs, i
   PF_REQUIRE(npairs > 0 && ntiles > 0 && L >= 2, "pf_stage2: empty batch or bad tile length");
   PF_REQUIRE(max_diag >= 1 && max_tile_diag >= 1 && max_tiles_pair >= 1 && max_segs >= 1,
              "pf_stage2: bad batch dimensions");
-  PF_REQUIRE(c_dtype == PF_F64 || c_dtype == PF_F32 || c_dtype == PF_BF16,
-             "pf_stage2: unknown C dtype");
-  PF_REQUIRE(d_C && d_pairs && d_tiles && d_Bpacked && d_edge_D && d_edge_P && d_D_arr &&
+  PF_REQUIRE(L <= 8192, "pf_stage2: tile length above 8192 is not supported");
+  PF_REQUIRE(d_edge_C && d_pairs && d_tiles && d_Bpacked && d_edge_D && d_edge_P && d_D_arr &&
                  d_L_arr && d_olap && d_bits && d_cand_D && d_cand_L && d_seg_f && d_seg_i &&
                  d_best_f && d_best_i && d_chain && d_pts && d_path && d_path_len,
              "pf_stage2: null buffer");
@@ -789,9 +790,10 @@ extern "C" int pf_stage2(int c_dtype, const void *d_C, const int64_t *d_pairs, i
   const int64_t nbits = ntiles * 2 * bitw;
   hipLaunchKernelGGL(k_fill_u32, dim3(nblk(nbits, 256)), dim3(256), 0, st, d_bits, 0u, nbits);
   PF_CHECK_HIP(hipGetLastError());
-  const int64_t nslots = ntiles * 2 * L;
-  hipLaunchKernelGGL(k_s2_prep, dim3(nblk(nslots, 256)), dim3(256), 0, st, c_dtype, d_C, pairs,
-                     tiles, ntiles, L, d_edge_P, d_D_arr, d_L_arr, d_olap, d_bits, bitw);
+  PF_REQUIRE(ntiles < (1ll << 31), "pf_stage2: too many tiles");
+  hipLaunchKernelGGL(k_s2_prep, dim3((unsigned)ntiles, (unsigned)((2 * L + 255) / 256)),
+                     dim3(256), 0, st, tiles, L, d_edge_P, d_edge_C, d_D_arr, d_L_arr, d_olap,
+                     d_bits, bitw);
   PF_CHECK_HIP(hipGetLastError());
   for (int64_t d = 0; d < max_diag; ++d) {
     const int64_t ntd = (d + 1) < max_tile_diag ? (d + 1) : max_tile_diag;

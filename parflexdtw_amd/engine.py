@@ -119,14 +119,14 @@ def stage1(C_dev, plan, sw, ws, stream=None, config=None):
     rc = lib().pf_stage1(dtype_code(C_dev), C_dev.data_ptr(), ws["tiles"].data_ptr(),
                          plan.ntiles, plan.L, sw.sp, sw.wp, sw.K, S, nw, G, fl, xc,
                          ws["Bp"].data_ptr(), ws["edge_D"].data_ptr(), ws["edge_P"].data_ptr(),
-                         _stream(stream))
+                         ws["edge_C"].data_ptr(), _stream(stream))
     check(rc, "pf_stage1")
 
 
 def stage2(C_dev, plan, sw, ws, stream=None):
     """Propagation, sync, scan and stitch on the device (Parflex.py:862-1356)."""
     rc = lib().pf_stage2(
-        dtype_code(C_dev), C_dev.data_ptr(), ws["pairs"].data_ptr(), plan.npairs,
+        ws["edge_C"].data_ptr(), ws["pairs"].data_ptr(), plan.npairs,
         ws["tiles"].data_ptr(), plan.ntiles, plan.L, plan.max_diag, plan.max_tile_diag,
         plan.max_tiles_pair, plan.max_segs, sw.sp, sw.K, ws["Bp"].data_ptr(),
         ws["edge_D"].data_ptr(), ws["edge_P"].data_ptr(), ws["D_arr"].data_ptr(),
@@ -263,41 +263,60 @@ class BatchRunner:
         torch = _torch()
         self.sw = StepWeights(steps, weights)
         shapes = [(int(F1.shape[1]), int(F2.shape[1])) for F1, F2 in pairs]
-        self.plan = BatchPlan(shapes, int(L), beta=float(beta))
+        # Rows of every cost matrix start on a 16-byte boundary (row stride a
+        # multiple of 8 elements) so the cost kernel writes whole 16-byte chunks.
+        ldcs = [-(-M // 8) * 8 for _, M in shapes]
+        offs = np.concatenate([[0], np.cumsum([N * ld for (N, _), ld in zip(shapes, ldcs)])])
+        self.plan = BatchPlan(shapes, int(L), beta=float(beta),
+                              c_offsets=[int(o) for o in offs[:-1]], ldcs=ldcs)
         if share is not None:
             if [tuple(x) for x in share.plan.shapes] != shapes:
                 raise ValueError("share: runner over different pairs")
             self.dtype, self.device = share.dtype, share.device
-            self.C, self.F, self.work, self.code = share.C, share.F, share.work, share.code
+            self.C, self.F1, self.F2, self.K = share.C, share.F1, share.F2, share.K
+            self.work, self.code, self.jobs = share.work, share.code, share.jobs
+            self.cost_blocks = share.cost_blocks
             self.c_ready = share.c_ready
         else:
             self.dtype = torch.bfloat16 if dtype is None else dtype
             self.device = device or torch.device("cuda", torch.cuda.current_device())
-            total = sum(a * b for a, b in shapes)
-            self.C = torch.empty(total, dtype=self.dtype, device=self.device)
-            self.F = [(torch.from_numpy(np.ascontiguousarray(F1, dtype=np.float64)).to(self.device),
-                       torch.from_numpy(np.ascontiguousarray(F2, dtype=np.float64)).to(self.device))
-                      for F1, F2 in pairs]
+            self.C = torch.empty(int(offs[-1]), dtype=self.dtype, device=self.device)
             self.code = dtype_code(self.C)
-            need = 0
-            for (F1, F2), (N, M) in zip(pairs, shapes):
-                nb = ctypes.c_int64()
-                check(lib().pf_cost_workspace(self.code, int(F1.shape[0]), N, M, ctypes.byref(nb)),
-                      "pf_cost_workspace")
-                need = max(need, nb.value)
-            self.work = torch.empty(-(-need // 8), dtype=torch.float64, device=self.device)
+            K = int(pairs[0][0].shape[0])
+            if any(int(F1.shape[0]) != K or int(F2.shape[0]) != K for F1, F2 in pairs):
+                raise ValueError("all feature arrays of a batch need the same feature count K")
+            # features of every pair concatenated along frames: (K, sum N), (K, sum M)
+            self.F1 = torch.from_numpy(np.ascontiguousarray(
+                np.concatenate([F1 for F1, _ in pairs], axis=1), dtype=np.float64)).to(self.device)
+            self.F2 = torch.from_numpy(np.ascontiguousarray(
+                np.concatenate([F2 for _, F2 in pairs], axis=1), dtype=np.float64)).to(self.device)
+            self.K = K
+            tr, tc = ctypes.c_int64(), ctypes.c_int64()
+            check(lib().pf_cost_tile(self.code, ctypes.byref(tr), ctypes.byref(tc)), "pf_cost_tile")
+            jobs = np.zeros((len(pairs), 8), np.int64)
+            f1 = f2 = blk = 0
+            for p, (N, M) in enumerate(shapes):
+                jobs[p] = [f1, f2, N, M, self.plan.pairs[p, 5], self.plan.pairs[p, 6], blk, 0]
+                f1 += N
+                f2 += M
+                blk += (-(-N // tr.value)) * (-(-M // tc.value))
+            self.cost_blocks = blk
+            self.jobs = torch.from_numpy(jobs).to(self.device)
+            nb = ctypes.c_int64()
+            check(lib().pf_cost_workspace(self.code, K, f1, f2, ctypes.byref(nb)),
+                  "pf_cost_workspace")
+            self.work = torch.empty(-(-nb.value // 8), dtype=torch.float64, device=self.device)
             self.c_ready = [False]
         self.ws = self.plan.allocate(self.device)
         self.graph = None
 
     def cost(self, stream=None):
-        st = _stream(stream)
-        for p, (F1, F2) in enumerate(self.F):
-            N, M = int(self.plan.pairs[p, 0]), int(self.plan.pairs[p, 1])
-            off = int(self.plan.pairs[p, 5])
-            ptr = self.C.data_ptr() + off * self.C.element_size()
-            check(lib().pf_cost(self.code, F1.data_ptr(), F2.data_ptr(), int(F1.shape[0]), N, M,
-                                ptr, M, self.work.data_ptr(), st), "pf_cost")
+        """Cost matrices of every pair: one batched launch (pf_cost_batch)."""
+        check(lib().pf_cost_batch(self.code, self.F1.data_ptr(), int(self.F1.shape[1]),
+                                  self.F2.data_ptr(), int(self.F2.shape[1]), self.K,
+                                  self.jobs.data_ptr(), int(self.jobs.shape[0]), self.cost_blocks,
+                                  self.C.data_ptr(), self.work.data_ptr(), _stream(stream)),
+              "pf_cost_batch")
         self.c_ready[0] = True
 
     def launch(self, stream=None):

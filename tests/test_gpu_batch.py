@@ -113,3 +113,29 @@ def test_precision_form_agreement(torch_gpu, kind, K, N, M, L):
         bcp, wpp = PG.parflex(Cp, STEPS, WEIGHTS, 0.1, L, profile_dir=None)
         assert path_agreement(wp64, wpp, tol=2) >= 0.995, dt
         assert abs(bcp - bc64) <= 2e-2 * abs(bc64) + 1e-3, (dt, bcp, bc64)
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32", "bfloat16"])
+def test_batched_cost_equals_single(torch_gpu, dtype):
+    """One pf_cost_batch launch over ragged pairs writes, for every pair,
+    exactly what pf_cost writes for that pair alone; FP64 also equals the
+    fixed-order host cost."""
+    from parflexdtw_amd.engine import BatchRunner
+    from parflexdtw_amd.gpu_flexdtw import cost_matrix_from_features_gpu
+    from parflexdtw_amd.synth import cost_matrix_det
+    torch = torch_gpu
+    dt = getattr(torch, dtype)
+    pairs = _pairs([(300, 257), (129, 1000), (64, 64), (2, 3), (700, 129)], 800, K=20,
+                   kind="relu")
+    br = BatchRunner(pairs, 50, STEPS, WEIGHTS, beta=0.1, dtype=dt)
+    br.cost()
+    torch.cuda.synchronize()
+    for p, (F1, F2) in enumerate(pairs):
+        N, M = F1.shape[1], F2.shape[1]
+        off, ldc = int(br.plan.pairs[p, 5]), int(br.plan.pairs[p, 6])
+        assert ldc % 8 == 0 and off % 8 == 0 and ldc >= M
+        got = br.C[off:off + N * ldc].view(N, ldc)[:, :M]
+        one = cost_matrix_from_features_gpu(F1, F2, dtype=dt)
+        assert torch.equal(got, one), p
+        if dtype == "float64":
+            assert np.array_equal(got.cpu().numpy(), cost_matrix_det(F1, F2)), p

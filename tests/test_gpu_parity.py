@@ -208,11 +208,16 @@ def test_stage1_packed_outputs_vs_oracle(gpu, oracle_mod, case):
     n1, n2 = plan.grids[0]
     eD = ws["edge_D"].cpu().numpy().reshape(n1, n2, 2, L)
     eP = ws["edge_P"].cpu().numpy().reshape(n1, n2, 2, L)
+    eC = ws["edge_C"].cpu().numpy().reshape(n1, n2, 2, L)
     Bp = ws["Bp"].cpu().numpy().view(np.uint32)
     for i in range(n1):
         for j in range(n2):
             t = plan.tiles[i * n2 + j]
             R, W, boff = int(t[2]), int(t[3]), int(t[4])
+            r0, c0 = i * (L - 1), j * (L - 1)
+            # first row and first column of the tile's C (start cells)
+            assert _same(eC[i, j, 0, :W], C[r0, c0:c0 + W])
+            assert _same(eC[i, j, 1, :R], C[r0:r0 + R, c0])
             assert _same(eD[i, j, 0, :W], s1["edge_D"][i, j, 0, :W])
             assert _same(eD[i, j, 1, :R], s1["edge_D"][i, j, 1, :R])
             assert _same(eP[i, j, 0, :W], s1["edge_P"][i, j, 0, :W])

@@ -79,21 +79,21 @@ def test_stage1_rejects_bad_arguments_without_device_work():
     w = np.array([1.0])
     # zero tiles: nothing to do
     assert h.pf_stage1(0, None, None, 0, 10, s.ctypes.data, w.ctypes.data, 1, 1, 1, 16, None,
-                       None, None, None, None, None) == 0
+                       None, None, None, None, None, None) == 0
     # unknown dtype
     fake = ctypes.c_void_p(16)
     assert h.pf_stage1(7, fake, fake, 1, 10, s.ctypes.data, w.ctypes.data, 1, 1, 1, 16, None,
-                       None, fake, fake, fake, None) == _lib.PF_EINVAL
+                       None, fake, fake, fake, fake, None) == _lib.PF_EINVAL
     # tile length above the supported maximum
     assert h.pf_stage1(0, fake, fake, 1, 9000, s.ctypes.data, w.ctypes.data, 1, 2, 16, 16, fake,
-                       fake, fake, fake, fake, None) == _lib.PF_EINVAL
+                       fake, fake, fake, fake, fake, None) == _lib.PF_EINVAL
     # strips too narrow to cover the tile, and strips without exchange buffers
     assert h.pf_stage1(0, fake, fake, 1, 4000, s.ctypes.data, w.ctypes.data, 1, 1, 1, 16, None,
-                       None, fake, fake, fake, None) == _lib.PF_EINVAL
+                       None, fake, fake, fake, fake, None) == _lib.PF_EINVAL
     assert h.pf_stage1(0, fake, fake, 1, 4000, s.ctypes.data, w.ctypes.data, 1, 8, 4, 16, None,
-                       None, fake, fake, fake, None) == _lib.PF_EINVAL
+                       None, fake, fake, fake, fake, None) == _lib.PF_EINVAL
     assert h.pf_stage1(0, fake, fake, 1, 400, s.ctypes.data, w.ctypes.data, 1, 1, 4, 65, None,
-                       None, fake, fake, fake, None) == _lib.PF_EINVAL
+                       None, fake, fake, fake, fake, None) == _lib.PF_EINVAL
     assert h.pf_tile_full(fake, 10, 3, 3, 0, 5, s.ctypes.data, w.ctypes.data, 1, fake, fake,
                           fake, None) == _lib.PF_EINVAL
     assert "Empty chunk" in _lib.last_error()
