@@ -40,6 +40,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "DP GCUPS + pairs/s, batched/learned vs plain GPU at >=99.5% path agreement"
+# HBM bytes per stage-1 launch measured with rocprofv3 PMC (FETCH_SIZE +
+# WRITE_SIZE, scaled by a 1 GiB copy); profiles/pmc_traffic_*.md hold the runs.
+TRAFFIC_BYTES = {("chroma_batch256_N3000-20000", 500): (83.97 + 12.84) * 1e9}
 HBM_PEAK_GBS = 8000.0
 STEPS, WEIGHTS, BETA = [(1, 1), (1, 2), (2, 1)], [2, 3, 3], 0.1
 WORKLOADS = ["chroma_batch256_N3000-20000", "chroma16000x19000_L4000",
@@ -60,6 +63,9 @@ def _parse():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-trials", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one 1 GiB device-to-device copy, the known-size reference "
+                         "for scaling HBM counters (FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -181,6 +187,8 @@ def bench_batch(args, torch):
                         "best_cost_rel_err_max": float(max(rel_err)),
                         "pairs_compared": ns, "judge": "plain FP64 path, same pair and L"}
     out["roofline"] = _roofline(cells, s1_ms, 2.25, "k_stage1<2,__hip_bfloat16,float,false,true>")
+    if args.pairs == 256 and (args.workload, L) in TRAFFIC_BYTES:
+        out["roofline"]["traffic"] = TRAFFIC_BYTES[(args.workload, L)]
 
     if not args.no_cpu:
         from oracle import oracle
@@ -390,6 +398,14 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: 1073741824 bytes read, 1073741824 bytes written")
+        del src, dst
     if args.workload.startswith("chroma_batch"):
         out = bench_batch(args, torch)
     elif args.workload.startswith("relu"):
