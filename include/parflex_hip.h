@@ -189,6 +189,51 @@ int pf_cost_batch(int out_dtype, const double *d_F1, int64_t n1_total, const dou
                   int64_t n2_total, int64_t K, const int64_t *d_jobs, int64_t njobs,
                   int64_t total_blocks, void *d_C, double *d_work, void *stream);
 
+/*
+ * Learned tile proposal (no reference counterpart: the reference computes
+ * every tile).  A 4-layer 3x3 convolution network over the 1/pool-pooled
+ * cost matrix scores each tile; only tiles above a threshold (plus a margin)
+ * run stage 1.  Activations are NHWC bf16 with 32 channels; weights of one
+ * layer are bf16 [32 cout][288], k = (dy*3 + dx)*32 + cin; master weights,
+ * gradients and Adam moments are f32 in the same layout followed by 32
+ * biases.
+ */
+/* Fp (K x ceil(n/pool)) f32: means of the L2-normalised frames of each
+ * pool-frame block (FlexDTW.py:1207-1210 normalisation). */
+int pf_pool_features(const double *d_F, int64_t K, int64_t n, int pool, float *d_Fp,
+                     void *stream);
+/* Pooled cost 1 - A^T B (H x W), written as network input d_X (bf16 NHWC,
+ * channel 0) and, if d_P is not null, as f32. */
+int pf_pooled_cost(const float *d_A, int64_t H, const float *d_B, int64_t W, int64_t K,
+                   void *d_X, float *d_P, void *stream);
+/* One 3x3 convolution layer on MFMA.  mode 0: relu(conv + bias) -> bf16;
+ * mode 1: conv + bias -> f32; mode 2: gradient w.r.t. the layer input,
+ * d_Wt = flipped weights (pf_pack_weights), masked by d_M > 0 -> bf16. */
+int pf_conv3x3(int mode, const void *d_X, int64_t H, int64_t W, const void *d_Wt,
+               const float *d_bias, const void *d_M, void *d_Y, void *stream);
+/* Weight and bias gradient of one layer, accumulated into d_grad (f32
+ * [32*288 + 32]); d_part: pf_conv3x3_wgrad_workspace floats. */
+int pf_conv3x3_wgrad_workspace(int64_t H, int64_t W, int64_t *floats);
+int pf_conv3x3_wgrad(const void *d_dY, const void *d_X, int64_t H, int64_t W, float *d_grad,
+                     float *d_part, void *stream);
+/* Weighted binary cross-entropy on the channel-0 logits of d_Z (f32 NHWC
+ * 32): writes the logit gradient d_dZ (bf16 NHWC 32, scaled by `scale`) and
+ * adds the scaled loss to *d_loss. */
+int pf_bce(const float *d_Z, const uint8_t *d_label, int64_t npix, float pos_weight,
+           float scale, void *d_dZ, float *d_loss, void *stream);
+/* Adam step (bias-corrected, step >= 1) on f32 master weights; clears d_g.
+ * d_frozen (uint8 per weight, may be null) keeps marked weights fixed. */
+int pf_adam(float *d_w, float *d_g, float *d_m, float *d_v, const uint8_t *d_frozen, int64_t n,
+            float lr, float b1, float b2, float eps, int64_t step, void *stream);
+/* bf16 weights of one layer and, if d_Wf is not null, the flipped copy used
+ * by mode 2 of pf_conv3x3. */
+int pf_pack_weights(const float *d_w, void *d_Wt, void *d_Wf, void *stream);
+/* Per-tile score = max channel-0 logit over the pooled cells that tile
+ * (i, j) of tile length L covers (tiling of Parflex.py:388-427); d_score
+ * has n1*n2 entries in i-major order. */
+int pf_tile_scores(const float *d_Z, int64_t H, int64_t W, int pool, int64_t L, int64_t L1,
+                   int64_t L2, float *d_score, void *stream);
+
 /* Fill helpers used instead of hipMemsetAsync (safe under graph capture). */
 int pf_fill_f64(double *d_p, double v, int64_t n, void *stream);
 int pf_fill_i64(int64_t *d_p, int64_t v, int64_t n, void *stream);

@@ -1,0 +1,394 @@
+// Learned tile proposal on gfx950: pooled cost, a 4-layer 3x3 convolution
+// network (32 channels) trained and run with BF16 MFMA operands and FP32
+// accumulation / master weights, BCE loss, Adam, and per-tile scores.
+//
+// Activations are NHWC with 32 channels, bf16: X[(y*W + x)*32 + c].  Every
+// layer is a 32 -> 32 channel 3x3 convolution (stride 1, zero padding 1); the
+// network input carries the pooled cost in channel 0 and zeros elsewhere, and
+// only channel 0 of the last layer is used (its logit).  A layer is an
+// implicit GEMM  Y (pixels x 32) = im2col(X) (pixels x 288) * W^T, with the
+// reduction index k = (dy*3 + dx)*32 + cin, so that 8 consecutive k are 8
+// consecutive channels of one neighbour pixel (one 16-byte load).  Weights are
+// stored [cout][288].  The gradient with respect to the layer input is the
+// same convolution of dY with the flipped weights Wf[cin][(8-dydx)*32 + cout]
+// = W[cout][dydx*32 + cin]; the weight gradient is the GEMM
+// dW (32 x 288) = dY^T (32 x pixels) * im2col(X) (pixels x 288).
+//
+// MFMA v_mfma_f32_32x32x16_bf16 lane layout (lane l of a wave): A row l%32,
+// k = (l/32)*8 .. +7; B column l%32, same k; D column l%32, rows
+// (v/4)*8 + (l/32)*4 + v%4 for accumulator v.
+#include "pf_common.h"
+
+namespace pf {
+
+typedef __bf16 lbf16x8_t __attribute__((ext_vector_type(8)));
+typedef float lfloatx16_t __attribute__((ext_vector_type(16)));
+
+constexpr int kCh = 32;           // channels of every layer
+constexpr int kKdim = 9 * kCh;    // im2col width (288)
+
+__device__ __forceinline__ float bf2f(__hip_bfloat16 v) { return __bfloat162float(v); }
+
+// Fp[k*nb + b] = mean over the frames i of block b (pool frames, the last
+// block ragged) of F[k*n + i] / (sqrt(sum_k F[k*n+i]^2) + 1e-9).
+__global__ void k_pool_features(const double *__restrict__ F, int64_t K, int64_t n, int pool,
+                                float *__restrict__ Fp, int64_t nb) {
+  const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int64_t i0 = b * pool, i1 = (i0 + pool < n) ? i0 + pool : n;
+  for (int64_t k = 0; k < K; ++k) Fp[k * nb + b] = 0.0f;
+  for (int64_t i = i0; i < i1; ++i) {
+    double s = 0.0;
+    for (int64_t k = 0; k < K; ++k) {
+      const double v = F[k * n + i];
+      s = s + v * v;
+    }
+    const double inv = 1.0 / ((sqrt(s) + 1e-9) * (double)(i1 - i0));
+    for (int64_t k = 0; k < K; ++k) Fp[k * nb + b] += (float)(F[k * n + i] * inv);
+  }
+}
+
+// Pooled cost P[y][x] = 1 - sum_k A[k*H + y] * B[k*W + x] (the mean of C over
+// the block, C being bilinear in the normalised features), written as the
+// network input (channel 0, other channels 0) and optionally as f32.
+__global__ void k_pooled_cost(const float *__restrict__ A, int64_t H, const float *__restrict__ B,
+                              int64_t W, int64_t K, __hip_bfloat16 *__restrict__ X,
+                              float *__restrict__ P) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= H * W) return;
+  const int64_t y = p / W, x = p - y * W;
+  float s = 0.0f;
+  for (int64_t k = 0; k < K; ++k) s += A[k * H + y] * B[k * W + x];
+  const float c = 1.0f - s;
+  if (P) P[p] = c;
+  __hip_bfloat16 *o = X + p * kCh;
+  o[0] = __float2bfloat16(c);
+  for (int q = 1; q < kCh; ++q) o[q] = __float2bfloat16(0.0f);
+}
+
+// 3x3 convolution, 32 -> 32 channels.  A workgroup of 4 waves covers 128
+// consecutive pixels (row-major order), one wave 32 pixels x 32 channels.
+// MODE 0: Y = relu(acc + bias) as bf16 (hidden layer).
+// MODE 1: Y = acc + bias as f32 (logits).
+// MODE 2: Y = (M > 0) ? acc : 0 as bf16 (gradient through the ReLU of the
+//         layer input M, which is that layer's output activation).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_conv3x3(const __hip_bfloat16 *__restrict__ X, int64_t H,
+                                                 int64_t W, const __hip_bfloat16 *__restrict__ Wt,
+                                                 const float *__restrict__ bias,
+                                                 const __hip_bfloat16 *__restrict__ Mk,
+                                                 void *__restrict__ Yv) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t npix = H * W;
+  const int64_t p0 = (blockIdx.x * 4ll + wave) * 32;
+  if (p0 >= npix) return;  // whole wave past the image (no barrier in this kernel)
+  const int64_t p = p0 + (lane & 31);
+  const bool pin = p < npix;
+  const int64_t y = pin ? p / W : 0, x = pin ? p - (p / W) * W : 0;
+  const int kh = (lane >> 5) * 8;
+  const lbf16x8_t zero = lbf16x8_t{};
+  lfloatx16_t acc = lfloatx16_t{};
+#pragma unroll
+  for (int s = 0; s < 18; ++s) {
+    const int dydx = s >> 1, dy = dydx / 3, dx = dydx - dy * 3;
+    const int64_t yy = y + dy - 1, xx = x + dx - 1;
+    const bool ok = pin && yy >= 0 && yy < H && xx >= 0 && xx < W;
+    const int cin0 = (s & 1) * 16 + kh;
+    const int64_t src = ok ? (yy * W + xx) * kCh + cin0 : 0;
+    const lbf16x8_t a0 = *reinterpret_cast<const lbf16x8_t *>(X + src);
+    const lbf16x8_t a = ok ? a0 : zero;
+    const lbf16x8_t b = *reinterpret_cast<const lbf16x8_t *>(Wt + (lane & 31) * kKdim + s * 16 + kh);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+  }
+  const int co = lane & 31;
+  const float bv = (MODE == 2) ? 0.0f : bias[co];
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const int64_t q = p0 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+    if (q >= npix) continue;
+    const float r = acc[v] + bv;
+    if constexpr (MODE == 0) {
+      static_cast<__hip_bfloat16 *>(Yv)[q * kCh + co] = __float2bfloat16(r > 0.0f ? r : 0.0f);
+    } else if constexpr (MODE == 1) {
+      static_cast<float *>(Yv)[q * kCh + co] = r;
+    } else {
+      const float m = bf2f(Mk[q * kCh + co]);
+      static_cast<__hip_bfloat16 *>(Yv)[q * kCh + co] = __float2bfloat16(m > 0.0f ? r : 0.0f);
+    }
+  }
+}
+
+// Weight gradient partials: one wave per chunk of kWgChunk pixels;
+// part[blk][o*288 + k] = sum over the chunk of dY[p][o] * im2col(X)[p][k],
+// part[blk][32*288 + o] = sum over the chunk of dY[p][o].
+constexpr int kWgChunk = 256;
+constexpr int kWgSize = kCh * kKdim + kCh;
+
+__global__ __launch_bounds__(64) void k_conv3x3_wgrad(const __hip_bfloat16 *__restrict__ dY,
+                                                      const __hip_bfloat16 *__restrict__ X,
+                                                      int64_t H, int64_t W,
+                                                      float *__restrict__ part) {
+  const int lane = threadIdx.x;
+  const int64_t npix = H * W;
+  const int64_t c0 = blockIdx.x * (int64_t)kWgChunk;
+  const int kh = (lane >> 5) * 8;
+  const int col = lane & 31;
+  lfloatx16_t acc[9];
+#pragma unroll
+  for (int nb = 0; nb < 9; ++nb) acc[nb] = lfloatx16_t{};
+  float bsum = 0.0f;
+  for (int step = 0; step < kWgChunk; step += 16) {
+    // A: dY^T, row o = lane%32, pixels c0+step+kh .. +7
+    lbf16x8_t a;
+    int64_t py[8], px[8];
+    bool pin[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int64_t q = c0 + step + kh + t;
+      pin[t] = q < npix;
+      const int64_t qq = pin[t] ? q : 0;
+      py[t] = qq / W;
+      px[t] = qq - py[t] * W;
+      const __hip_bfloat16 g = dY[qq * kCh + col];
+      const float gf = pin[t] ? bf2f(g) : 0.0f;
+      bsum += gf;
+      a[t] = (__bf16)gf;
+    }
+#pragma unroll
+    for (int nb = 0; nb < 9; ++nb) {
+      const int dy = nb / 3, dx = nb - dy * 3;
+      lbf16x8_t b;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int64_t yy = py[t] + dy - 1, xx = px[t] + dx - 1;
+        const bool ok = pin[t] && yy >= 0 && yy < H && xx >= 0 && xx < W;
+        const __hip_bfloat16 v = X[(ok ? yy * W + xx : 0) * kCh + col];
+        b[t] = ok ? (__bf16)bf2f(v) : (__bf16)0.0f;
+      }
+      acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[nb], 0, 0, 0);
+    }
+  }
+  float *dst = part + blockIdx.x * (int64_t)kWgSize;
+#pragma unroll
+  for (int nb = 0; nb < 9; ++nb)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int o = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+      dst[o * kKdim + nb * kCh + col] = acc[nb][v];
+    }
+  // lanes l and l+32 hold the same output channel
+  bsum += __shfl_xor(bsum, 32);
+  if (lane < 32) dst[kCh * kKdim + lane] = bsum;
+}
+
+// g[i] += sum over nblk partials of part[b*n + i]
+__global__ void k_reduce_add(const float *__restrict__ part, int64_t nblk, int64_t n,
+                             float *__restrict__ g) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.0f;
+  for (int64_t b = 0; b < nblk; ++b) s += part[b * n + i];
+  g[i] += s;
+}
+
+// Weighted BCE on the channel-0 logits of Z (f32, NHWC 32): dZ channel 0 =
+// scale * w * (sigmoid(z) - y), other channels 0; loss_acc += scale * sum w*bce.
+__global__ __launch_bounds__(256) void k_bce(const float *__restrict__ Z,
+                                             const uint8_t *__restrict__ label, int64_t npix,
+                                             float pos_weight, float scale,
+                                             __hip_bfloat16 *__restrict__ dZ,
+                                             float *__restrict__ loss_acc) {
+  __shared__ float red[256];
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  float l = 0.0f;
+  if (p < npix) {
+    const float z = Z[p * kCh];
+    const float y = label[p] ? 1.0f : 0.0f;
+    const float w = label[p] ? pos_weight : 1.0f;
+    l = w * (fmaxf(z, 0.0f) - z * y + log1pf(expf(-fabsf(z)))) * scale;
+    const float s = 1.0f / (1.0f + expf(-z));
+    __hip_bfloat16 *o = dZ + p * kCh;
+    o[0] = __float2bfloat16(w * (s - y) * scale);
+    for (int q = 1; q < kCh; ++q) o[q] = __float2bfloat16(0.0f);
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss_acc, red[0]);
+}
+
+// Adam on FP32 master weights; the gradient is cleared after use.
+__global__ void k_adam(float *__restrict__ w, float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, const uint8_t *__restrict__ frozen, int64_t n,
+                       float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  g[i] = 0.0f;
+  if (frozen && frozen[i]) return;
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// BF16 copies of one layer's master weights: Wt[o][k] and the flipped
+// Wf[cin][(8 - dydx)*32 + cout] = W[cout][dydx*32 + cin].
+__global__ void k_pack_weights(const float *__restrict__ w, __hip_bfloat16 *__restrict__ Wt,
+                               __hip_bfloat16 *__restrict__ Wf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kCh * kKdim) return;
+  const int o = i / kKdim, k = i - o * kKdim;
+  const int dydx = k / kCh, cin = k - dydx * kCh;
+  const __hip_bfloat16 b = __float2bfloat16(w[i]);
+  Wt[i] = b;
+  if (Wf) Wf[cin * kKdim + (8 - dydx) * kCh + o] = b;
+}
+
+// Per-tile score: the largest channel-0 logit over the pooled cells a tile
+// covers.  Tile (i, j) of tile length L covers rows [i*hop, min(i*hop+L, L1))
+// and columns [j*hop, min(j*hop+L, L2)), hop = L-1; pooled cell (y, x)
+// covers frames [y*pool, y*pool+pool) x [x*pool, x*pool+pool).
+__global__ __launch_bounds__(256) void k_tile_scores(const float *__restrict__ Z, int64_t H,
+                                                     int64_t W, int pool, int64_t L, int64_t L1,
+                                                     int64_t L2, int64_t n2,
+                                                     float *__restrict__ score) {
+  __shared__ float red[256];
+  const int64_t t = blockIdx.x;
+  const int64_t i = t / n2, j = t - i * n2;
+  const int64_t hop = L - 1;
+  const int64_t r0 = i * hop, r1 = (r0 + L < L1 ? r0 + L : L1) - 1;
+  const int64_t c0 = j * hop, c1 = (c0 + L < L2 ? c0 + L : L2) - 1;
+  const int64_t y0 = r0 / pool, y1 = r1 / pool < H - 1 ? r1 / pool : H - 1;
+  const int64_t x0 = c0 / pool, x1 = c1 / pool < W - 1 ? c1 / pool : W - 1;
+  const int64_t nx = x1 - x0 + 1, n = (y1 - y0 + 1) * nx;
+  float m = -__builtin_huge_valf();
+  for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
+    const int64_t y = y0 + q / nx, x = x0 + q % nx;
+    m = fmaxf(m, Z[(y * W + x) * kCh]);
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) score[t] = red[0];
+}
+
+inline unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace pf
+
+using namespace pf;
+
+extern "C" int pf_pool_features(const double *d_F, int64_t K, int64_t n, int pool, float *d_Fp,
+                                void *stream) {
+  PF_REQUIRE(d_F && d_Fp && K >= 1 && n >= 1 && pool >= 1, "pf_pool_features: bad arguments");
+  const int64_t nb = (n + pool - 1) / pool;
+  hipLaunchKernelGGL(k_pool_features, dim3(nb256(nb)), dim3(256), 0, as_stream(stream), d_F, K, n,
+                     pool, d_Fp, nb);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_pooled_cost(const float *d_A, int64_t H, const float *d_B, int64_t W, int64_t K,
+                              void *d_X, float *d_P, void *stream) {
+  PF_REQUIRE(d_A && d_B && d_X && H >= 1 && W >= 1 && K >= 1, "pf_pooled_cost: bad arguments");
+  hipLaunchKernelGGL(k_pooled_cost, dim3(nb256(H * W)), dim3(256), 0, as_stream(stream), d_A, H,
+                     d_B, W, K, static_cast<__hip_bfloat16 *>(d_X), d_P);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_conv3x3(int mode, const void *d_X, int64_t H, int64_t W, const void *d_Wt,
+                          const float *d_bias, const void *d_M, void *d_Y, void *stream) {
+  PF_REQUIRE(d_X && d_Wt && d_Y && H >= 1 && W >= 1, "pf_conv3x3: bad arguments");
+  PF_REQUIRE(mode == 2 ? d_M != nullptr : d_bias != nullptr, "pf_conv3x3: missing bias or mask");
+  PF_REQUIRE(reinterpret_cast<uintptr_t>(d_X) % 16 == 0 && reinterpret_cast<uintptr_t>(d_Wt) % 16 == 0,
+             "pf_conv3x3: X and weights must be 16-byte aligned");
+  const int64_t npix = H * W;
+  const dim3 grid((unsigned)((npix + 127) / 128));
+  const auto *X = static_cast<const __hip_bfloat16 *>(d_X);
+  const auto *Wt = static_cast<const __hip_bfloat16 *>(d_Wt);
+  const auto *M = static_cast<const __hip_bfloat16 *>(d_M);
+  hipStream_t st = as_stream(stream);
+  if (mode == 0)
+    hipLaunchKernelGGL(k_conv3x3<0>, grid, dim3(256), 0, st, X, H, W, Wt, d_bias, M, d_Y);
+  else if (mode == 1)
+    hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(256), 0, st, X, H, W, Wt, d_bias, M, d_Y);
+  else if (mode == 2)
+    hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, st, X, H, W, Wt, d_bias, M, d_Y);
+  else
+    PF_REQUIRE(false, "pf_conv3x3: mode must be 0, 1 or 2");
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_conv3x3_wgrad_workspace(int64_t H, int64_t W, int64_t *floats) {
+  PF_REQUIRE(floats && H >= 1 && W >= 1, "pf_conv3x3_wgrad_workspace: bad arguments");
+  *floats = ((H * W + kWgChunk - 1) / kWgChunk) * (int64_t)kWgSize;
+  return PF_OK;
+}
+
+extern "C" int pf_conv3x3_wgrad(const void *d_dY, const void *d_X, int64_t H, int64_t W,
+                                float *d_grad, float *d_part, void *stream) {
+  PF_REQUIRE(d_dY && d_X && d_grad && d_part && H >= 1 && W >= 1, "pf_conv3x3_wgrad: bad arguments");
+  const int64_t nblk = (H * W + kWgChunk - 1) / kWgChunk;
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_conv3x3_wgrad, dim3((unsigned)nblk), dim3(64), 0, st,
+                     static_cast<const __hip_bfloat16 *>(d_dY),
+                     static_cast<const __hip_bfloat16 *>(d_X), H, W, d_part);
+  PF_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_add, dim3(nb256(kWgSize)), dim3(256), 0, st, d_part, nblk,
+                     (int64_t)kWgSize, d_grad);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_bce(const float *d_Z, const uint8_t *d_label, int64_t npix, float pos_weight,
+                      float scale, void *d_dZ, float *d_loss, void *stream) {
+  PF_REQUIRE(d_Z && d_label && d_dZ && d_loss && npix >= 1, "pf_bce: bad arguments");
+  hipLaunchKernelGGL(k_bce, dim3(nb256(npix)), dim3(256), 0, as_stream(stream), d_Z, d_label, npix,
+                     pos_weight, scale, static_cast<__hip_bfloat16 *>(d_dZ), d_loss);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_adam(float *d_w, float *d_g, float *d_m, float *d_v, const uint8_t *d_frozen,
+                       int64_t n, float lr, float b1, float b2, float eps, int64_t step,
+                       void *stream) {
+  PF_REQUIRE(d_w && d_g && d_m && d_v && n >= 1 && step >= 1, "pf_adam: bad arguments");
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(nb256(n)), dim3(256), 0, as_stream(stream), d_w, d_g, d_m, d_v,
+                     d_frozen, n, lr, b1, b2, eps, c1, c2);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_pack_weights(const float *d_w, void *d_Wt, void *d_Wf, void *stream) {
+  PF_REQUIRE(d_w && d_Wt, "pf_pack_weights: bad arguments");
+  hipLaunchKernelGGL(k_pack_weights, dim3(nb256(kCh * kKdim)), dim3(256), 0, as_stream(stream), d_w,
+                     static_cast<__hip_bfloat16 *>(d_Wt), static_cast<__hip_bfloat16 *>(d_Wf));
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_tile_scores(const float *d_Z, int64_t H, int64_t W, int pool, int64_t L,
+                              int64_t L1, int64_t L2, float *d_score, void *stream) {
+  PF_REQUIRE(d_Z && d_score && H >= 1 && W >= 1 && pool >= 1 && L >= 2 && L1 >= 2 && L2 >= 2,
+             "pf_tile_scores: bad arguments");
+  PF_REQUIRE((L1 + pool - 1) / pool == H && (L2 + pool - 1) / pool == W,
+             "pf_tile_scores: pooled map does not match the pair");
+  const int64_t hop = L - 1;
+  const int64_t n1 = (L1 - 1 + hop - 1) / hop, n2 = (L2 - 1 + hop - 1) / hop;
+  hipLaunchKernelGGL(k_tile_scores, dim3((unsigned)(n1 * n2)), dim3(256), 0, as_stream(stream), d_Z,
+                     H, W, pool, L, L1, L2, n2, d_score);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}

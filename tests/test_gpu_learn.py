@@ -1,0 +1,186 @@
+"""GPU tests of the learned tile proposal kernels (csrc/pf_learn.hip) against
+the CPU oracle (oracle/tile_net_oracle.py), on identical BF16 operands.
+
+Tolerances: pooled features, pooled cost and tile scores are computed in the
+same order on both sides and must match exactly; a convolution or weight
+gradient differs only by FP32 accumulation inside the matrix cores
+(|diff| <= 1e-5 of the largest value); a BF16 output may differ by one BF16
+rounding step; the end-to-end gradient of the 4 layers, where those
+roundings feed each other, within 2% in norm."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from oracle import tile_net_oracle as tn
+    from parflexdtw_amd import tile_net
+    return torch, tn, tile_net
+
+
+def _bf(torch, a):
+    return torch.from_numpy(np.asarray(a, np.float32)).to(torch.bfloat16).cuda()
+
+
+def _lib():
+    from parflexdtw_amd._lib import check, lib
+    return lib(), check
+
+
+def test_pooled_input_matches_oracle(env):
+    torch, tn, tile_net = env
+    from parflexdtw_amd.synth import make_pair
+    F1, F2, _ = make_pair(333, 300, K=12, seed=3)
+    net = tile_net.TileNet(seed=0)
+    X, H, W, P = net.pooled_input(torch.from_numpy(F1).cuda(), torch.from_numpy(F2).cuda(),
+                                  want_cost=True)
+    Pref = tn.pooled_cost(tn.pool_features(F1, 32), tn.pool_features(F2, 32))
+    assert (H, W) == Pref.shape
+    assert np.array_equal(P.cpu().numpy().reshape(H, W), Pref)
+    Xh = X.float().cpu().numpy().reshape(H, W, 32)
+    assert np.array_equal(Xh[..., 0], tn.bf16(Pref).astype(np.float32))
+    assert not Xh[..., 1:].any()
+
+
+@pytest.mark.parametrize("H,W", [(13, 70), (1, 5), (40, 33)])
+def test_conv_modes_match_oracle(env, H, W):
+    torch, tn, _ = env
+    L, check = _lib()
+    rng = np.random.default_rng(H * 100 + W)
+    X = tn.bf16(rng.standard_normal((H, W, 32)))
+    M = tn.bf16(np.maximum(rng.standard_normal((H, W, 32)), 0))
+    Wt = tn.bf16(rng.standard_normal((32, 288)) * 0.1)
+    b = rng.standard_normal(32).astype(np.float32)
+    Xd, Md, Wd = _bf(torch, X), _bf(torch, M), _bf(torch, Wt)
+    Wfd = _bf(torch, tn.flip(Wt))
+    bd = torch.from_numpy(b).cuda()
+    ref = tn.conv(X, Wt, b.astype(np.float64))
+    tol = 1e-5 * np.abs(ref).max() + 1e-6
+    st = torch.cuda.current_stream().cuda_stream
+    Z = torch.empty(H * W * 32, dtype=torch.float32, device="cuda")
+    check(L.pf_conv3x3(1, Xd.data_ptr(), H, W, Wd.data_ptr(), bd.data_ptr(), None, Z.data_ptr(),
+                       st), "conv1")
+    Y = torch.empty(H * W * 32, dtype=torch.bfloat16, device="cuda")
+    check(L.pf_conv3x3(0, Xd.data_ptr(), H, W, Wd.data_ptr(), bd.data_ptr(), None, Y.data_ptr(),
+                       st), "conv0")
+    G = torch.empty(H * W * 32, dtype=torch.bfloat16, device="cuda")
+    check(L.pf_conv3x3(2, Xd.data_ptr(), H, W, Wfd.data_ptr(), None, Md.data_ptr(), G.data_ptr(),
+                       st), "conv2")
+    Zh = Z.cpu().numpy().reshape(H, W, 32)
+    assert np.max(np.abs(Zh - ref)) <= tol
+    Yh = Y.float().cpu().numpy().reshape(H, W, 32)
+    relu = np.maximum(ref, 0)
+    assert np.all(np.abs(Yh - relu) <= 2 ** -7 * np.abs(relu) + tol)
+    gref = np.where(M > 0, tn.conv(X, tn.flip(Wt)), 0.0)
+    Gh = G.float().cpu().numpy().reshape(H, W, 32)
+    assert np.all(np.abs(Gh - gref) <= 2 ** -7 * np.abs(gref) + 1e-5 * np.abs(gref).max() + 1e-6)
+
+
+@pytest.mark.parametrize("H,W", [(9, 31), (20, 70)])
+def test_wgrad_matches_oracle(env, H, W):
+    torch, tn, _ = env
+    L, check = _lib()
+    import ctypes
+    rng = np.random.default_rng(7 + H)
+    dY = tn.bf16(rng.standard_normal((H, W, 32)))
+    X = tn.bf16(rng.standard_normal((H, W, 32)))
+    dWr, dbr = tn.wgrad(dY, X)
+    n = ctypes.c_int64()
+    check(L.pf_conv3x3_wgrad_workspace(H, W, ctypes.byref(n)), "ws")
+    part = torch.empty(n.value, dtype=torch.float32, device="cuda")
+    g = torch.full((32 * 288 + 32,), 0.5, dtype=torch.float32, device="cuda")
+    dYd, Xd = _bf(torch, dY), _bf(torch, X)   # keep both alive during the call
+    check(L.pf_conv3x3_wgrad(dYd.data_ptr(), Xd.data_ptr(), H, W, g.data_ptr(), part.data_ptr(),
+                             torch.cuda.current_stream().cuda_stream), "wgrad")
+    torch.cuda.synchronize()
+    gh = g.cpu().numpy().astype(np.float64) - 0.5     # accumulates into the buffer
+    tol = 1e-5 * np.abs(dWr).max() + 1e-5
+    assert np.max(np.abs(gh[:32 * 288].reshape(32, 288) - dWr)) <= tol
+    assert np.max(np.abs(gh[32 * 288:] - dbr)) <= 1e-5 * np.abs(dbr).max() + 1e-5
+
+
+def test_bce_and_tile_scores_match_oracle(env):
+    torch, tn, _ = env
+    L, check = _lib()
+    rng = np.random.default_rng(11)
+    H, W = 37, 45
+    Z = rng.standard_normal((H, W, 32)).astype(np.float32) * 3
+    lab = (rng.random((H, W)) < 0.1).astype(np.uint8)
+    Zd = torch.from_numpy(Z).cuda()
+    labd = torch.from_numpy(lab).cuda()
+    dZ = torch.empty(H * W * 32, dtype=torch.bfloat16, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    check(L.pf_bce(Zd.data_ptr(), labd.data_ptr(), H * W, 5.0, 1.0 / (H * W), dZ.data_ptr(),
+                   loss.data_ptr(), st), "bce")
+    lr, gr = tn.bce(Z[..., 0].astype(np.float64), lab, 5.0, 1.0 / (H * W))
+    assert float(loss.item()) == pytest.approx(lr, rel=1e-5)
+    dh = dZ.float().cpu().numpy().reshape(H, W, 32)
+    assert np.all(np.abs(dh[..., 0] - gr) <= 2 ** -8 * np.abs(gr) + 1e-9)
+    assert not dh[..., 1:].any()
+    # (L1, L2) must pool to (H, W) = (37, 45)
+    for (L_, L1, L2) in [(100, 1180, 1440), (40, 1160, 1420), (2000, 1170, 1439)]:
+        s = torch.empty(-(-(L1 - 1) // (L_ - 1)) * -(-(L2 - 1) // (L_ - 1)), dtype=torch.float32,
+                        device="cuda")
+        check(L.pf_tile_scores(Zd.data_ptr(), H, W, 32, L_, L1, L2, s.data_ptr(), st), "scores")
+        assert np.array_equal(s.cpu().numpy(), tn.tile_scores(Z, 32, L_, L1, L2).astype(np.float32))
+
+
+def test_network_gradient_matches_oracle(env):
+    """Forward + backward of the 4 layers on the GPU against the oracle's
+    float64 backward with BF16 rounding at the same points."""
+    torch, tn, tile_net = env
+    rng = np.random.default_rng(5)
+    H, W = 23, 29
+    net = tile_net.TileNet(seed=3)
+    wh = net.weights_host().reshape(4, -1).astype(np.float64)
+    # a less trivial last layer than the initial prior
+    wh[3, 32 * 288] = 0.0
+    net.w.copy_(torch.from_numpy(wh.reshape(-1).astype(np.float32)))
+    net.pack()
+    Ws = [wh[i, :32 * 288].reshape(32, 288) for i in range(4)]
+    bs = [wh[i, 32 * 288:].astype(np.float32).astype(np.float64) for i in range(4)]
+    X0 = np.zeros((H, W, 32))
+    X0[..., 0] = tn.bf16(rng.uniform(0.1, 1.2, (H, W)))
+    lab = (rng.random((H, W)) < 0.15).astype(np.uint8)
+    Xd = _bf(torch, X0).reshape(-1)
+    acts, Z = net.forward(Xd, H, W)
+    net.backward(acts, Z, H, W, torch.from_numpy(lab).cuda(), 4.0, 1.0 / (H * W))
+    torch.cuda.synchronize()
+    racts, rZ = tn.forward(X0, Ws, bs)
+    Zh = Z.cpu().numpy().reshape(H, W, 32)[..., 0]
+    assert np.max(np.abs(Zh - rZ[..., 0])) <= 2e-2 * np.abs(rZ[..., 0]).max() + 1e-4
+    rloss, dWs, dbs = tn.backward(racts, rZ, Ws, lab, 4.0, 1.0 / (H * W))
+    assert float(net.loss.item()) == pytest.approx(rloss, rel=1e-2)
+    g = net.g.cpu().numpy().reshape(4, -1).astype(np.float64)
+    for i in range(4):
+        ref = np.concatenate([dWs[i].reshape(-1), dbs[i]])
+        err = np.linalg.norm(g[i] - ref) / max(np.linalg.norm(ref), 1e-30)
+        assert err < 2e-2, (i, err)
+
+
+def test_adam_matches_numpy(env):
+    torch, _, tile_net = env
+    net = tile_net.TileNet(seed=1)
+    w0 = net.weights_host().astype(np.float64)
+    rng = np.random.default_rng(2)
+    fz = tile_net.TileNet.frozen_mask().astype(bool)
+    m = np.zeros_like(w0)
+    v = np.zeros_like(w0)
+    w = w0.copy()
+    for step in (1, 2, 3):
+        gr = rng.standard_normal(w0.size).astype(np.float32)
+        net.g.copy_(torch.from_numpy(gr))
+        net.adam(lr=1e-2)
+        g = gr.astype(np.float64)
+        m = 0.9 * m + 0.1 * g
+        v = 0.999 * v + 0.001 * g * g
+        upd = 1e-2 * (m / (1 - 0.9 ** step)) / (np.sqrt(v / (1 - 0.999 ** step)) + 1e-8)
+        w = np.where(fz, w, w - upd)
+    assert np.allclose(net.weights_host(), w, rtol=1e-5, atol=1e-6)
+    assert not net.g.any()
