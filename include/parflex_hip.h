@@ -199,13 +199,17 @@ int pf_cost_batch(int out_dtype, const double *d_F1, int64_t n1_total, const dou
  * biases.
  */
 /* Fp (K x ceil(n/pool)) f32: means of the L2-normalised frames of each
- * pool-frame block (FlexDTW.py:1207-1210 normalisation). */
+ * pool-frame block (FlexDTW.py:1207-1210 normalisation); pool must be 32. */
 int pf_pool_features(const double *d_F, int64_t K, int64_t n, int pool, float *d_Fp,
                      void *stream);
-/* Pooled cost 1 - A^T B (H x W), written as network input d_X (bf16 NHWC,
- * channel 0) and, if d_P is not null, as f32. */
+/* Network input d_X (bf16 NHWC, channel 0) from the pooled features: the
+ * pooled cost P = 1 - A^T B double-centred (row and column means removed,
+ * global mean added back) and divided by its standard deviation; if d_P is
+ * not null, also P itself (H x W f32).  d_work: pf_pooled_cost_workspace
+ * floats. */
+int pf_pooled_cost_workspace(int64_t K, int64_t *floats);
 int pf_pooled_cost(const float *d_A, int64_t H, const float *d_B, int64_t W, int64_t K,
-                   void *d_X, float *d_P, void *stream);
+                   void *d_X, float *d_P, float *d_work, void *stream);
 /* One 3x3 convolution layer on MFMA.  mode 0: relu(conv + bias) -> bf16;
  * mode 1: conv + bias -> f32; mode 2: gradient w.r.t. the layer input,
  * d_Wt = flipped weights (pf_pack_weights), masked by d_M > 0 -> bf16. */
@@ -229,10 +233,11 @@ int pf_adam(float *d_w, float *d_g, float *d_m, float *d_v, const uint8_t *d_fro
  * by mode 2 of pf_conv3x3. */
 int pf_pack_weights(const float *d_w, void *d_Wt, void *d_Wf, void *stream);
 /* Per-tile score = max channel-0 logit over the pooled cells that tile
- * (i, j) of tile length L covers (tiling of Parflex.py:388-427); d_score
- * has n1*n2 entries in i-major order. */
+ * (i, j) of tile length L covers (tiling of Parflex.py:388-427), the region
+ * extended by ext cells on every side; d_score has n1*n2 entries in i-major
+ * order. */
 int pf_tile_scores(const float *d_Z, int64_t H, int64_t W, int pool, int64_t L, int64_t L1,
-                   int64_t L2, float *d_score, void *stream);
+                   int64_t L2, int ext, float *d_score, void *stream);
 
 /* Fill helpers used instead of hipMemsetAsync (safe under graph capture). */
 int pf_fill_f64(double *d_p, double v, int64_t n, void *stream);

@@ -7,6 +7,7 @@ this restates this project's own specification):
   pooled features  Fp[k, b] = mean over the frames i of block b of
                    F[k, i] / (sqrt(sum_k F[k, i]^2) + 1e-9)
   pooled cost      P = 1 - A^T B        (the block mean of C)
+  network input    P double-centred and divided by its standard deviation
   layer            Y[p, o] = sum_{dy, dx, c} X[p + (dy-1, dx-1), c] W[o, (dy*3+dx)*32 + c] + b[o]
                    (zero padding, 32 channels NHWC), ReLU on hidden layers
   loss             weighted BCE on the channel-0 logits
@@ -59,6 +60,14 @@ def pooled_cost(A, B):
     for k in range(A.shape[0]):
         s = (s + (A[k][:, None] * B[k][None, :]).astype(np.float32)).astype(np.float32)
     return (np.float32(1.0) - s).astype(np.float32)
+
+
+def network_input(P):
+    """Channel 0 of the network input: P double-centred and divided by its
+    standard deviation (float64; the GPU computes it in float32)."""
+    P = np.asarray(P, np.float64)
+    c = P - P.mean(axis=1, keepdims=True) - P.mean(axis=0, keepdims=True) + P.mean()
+    return c / np.sqrt(np.mean(c * c) + 1e-12)
 
 
 def im2col(X):
@@ -131,8 +140,9 @@ def backward(acts, Z, Ws, label, pos_weight, scale, round_bf16=True):
     return loss, dWs, dbs
 
 
-def tile_scores(Z, pool, L, L1, L2):
-    """Max channel-0 logit over the pooled cells each tile covers (i-major)."""
+def tile_scores(Z, pool, L, L1, L2, ext=0):
+    """Max channel-0 logit over the pooled cells each tile covers, extended
+    by ext cells on every side (i-major)."""
     hop = L - 1
     n1, n2 = -(-(L1 - 1) // hop), -(-(L2 - 1) // hop)
     H, W = Z.shape[:2]
@@ -141,6 +151,6 @@ def tile_scores(Z, pool, L, L1, L2):
         r0, r1 = i * hop, min(i * hop + L, L1) - 1
         for j in range(n2):
             c0, c1 = j * hop, min(j * hop + L, L2) - 1
-            out[i * n2 + j] = Z[r0 // pool:min(r1 // pool, H - 1) + 1,
-                                c0 // pool:min(c1 // pool, W - 1) + 1, 0].max()
+            out[i * n2 + j] = Z[max(r0 // pool - ext, 0):min(r1 // pool + ext, H - 1) + 1,
+                                max(c0 // pool - ext, 0):min(c1 // pool + ext, W - 1) + 1, 0].max()
     return out

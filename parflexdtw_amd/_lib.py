@@ -48,14 +48,15 @@ _SIGNATURES = {
     "pf_cost_tile": (_int, [_int, _vp, _vp]),
     "pf_cost_batch": (_int, [_int, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp]),
     "pf_pool_features": (_int, [_vp, _i64, _i64, _int, _vp, _vp]),
-    "pf_pooled_cost": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp]),
+    "pf_pooled_cost_workspace": (_int, [_i64, _vp]),
+    "pf_pooled_cost": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp]),
     "pf_conv3x3": (_int, [_int, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
     "pf_conv3x3_wgrad_workspace": (_int, [_i64, _i64, _vp]),
     "pf_conv3x3_wgrad": (_int, [_vp, _vp, _i64, _i64, _vp, _vp, _vp]),
     "pf_bce": (_int, [_vp, _vp, _i64, _flt, _flt, _vp, _vp, _vp]),
     "pf_adam": (_int, [_vp, _vp, _vp, _vp, _vp, _i64, _flt, _flt, _flt, _flt, _i64, _vp]),
     "pf_pack_weights": (_int, [_vp, _vp, _vp, _vp]),
-    "pf_tile_scores": (_int, [_vp, _i64, _i64, _int, _i64, _i64, _i64, _vp, _vp]),
+    "pf_tile_scores": (_int, [_vp, _i64, _i64, _int, _i64, _i64, _i64, _int, _vp, _vp]),
     "pf_fill_f64": (_int, [_vp, _dbl, _i64, _vp]),
     "pf_fill_i64": (_int, [_vp, _i64, _i64, _vp]),
 }

@@ -124,8 +124,9 @@ class BatchPlan:
     def tile_index(self, p, i, j):
         return int(self.pairs[p, 4]) + i * int(self.pairs[p, 3]) + j
 
-    def allocate(self, device, c_dtype_bytes=8):
-        """Device workspace for stage 1 and stage 2 (torch tensors)."""
+    def allocate(self, device, b_words=None):
+        """Device workspace for stage 1 and stage 2 (torch tensors); b_words
+        overrides the size of the packed back-pointer buffer."""
         import torch
         L, T, P = self.L, self.ntiles, self.npairs
         bitw = (L + 32) // 32
@@ -133,7 +134,8 @@ class BatchPlan:
         ws = {
             "tiles": torch.from_numpy(self.tiles).to(device),
             "pairs": torch.from_numpy(self.pairs).to(device),
-            "Bp": torch.empty(max(self.b_words, 1), dtype=i32, device=device),
+            "Bp": torch.empty(max(self.b_words if b_words is None else b_words, 1), dtype=i32,
+                              device=device),
             "edge_D": torch.empty(T * 2 * L, dtype=f64, device=device),
             "edge_P": torch.empty(T * 2 * L, dtype=i32, device=device),
             "edge_C": torch.empty(T * 2 * L, dtype=f64, device=device),

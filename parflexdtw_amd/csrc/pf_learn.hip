@@ -29,84 +29,172 @@ constexpr int kKdim = 9 * kCh;    // im2col width (288)
 
 __device__ __forceinline__ float bf2f(__hip_bfloat16 v) { return __bfloat162float(v); }
 
-// Fp[k*nb + b] = mean over the frames i of block b (pool frames, the last
-// block ragged) of F[k*n + i] / (sqrt(sum_k F[k*n+i]^2) + 1e-9).
-__global__ void k_pool_features(const double *__restrict__ F, int64_t K, int64_t n, int pool,
-                                float *__restrict__ Fp, int64_t nb) {
-  const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  const int64_t i0 = b * pool, i1 = (i0 + pool < n) ? i0 + pool : n;
-  for (int64_t k = 0; k < K; ++k) Fp[k * nb + b] = 0.0f;
-  for (int64_t i = i0; i < i1; ++i) {
-    double s = 0.0;
-    for (int64_t k = 0; k < K; ++k) {
-      const double v = F[k * n + i];
-      s = s + v * v;
-    }
-    const double inv = 1.0 / ((sqrt(s) + 1e-9) * (double)(i1 - i0));
-    for (int64_t k = 0; k < K; ++k) Fp[k * nb + b] += (float)(F[k * n + i] * inv);
+// Fp[k*nb + b] = mean over the frames i of block b (32 frames, the last
+// block ragged) of F[k*n + i] / (sqrt(sum_k F[k*n+i]^2) + 1e-9).  One thread
+// per frame (coalesced loads); the 32 frames of a block are 32 consecutive
+// lanes, reduced with shuffles.
+__global__ __launch_bounds__(256) void k_pool_features(const double *__restrict__ F, int64_t K,
+                                                       int64_t n, float *__restrict__ Fp,
+                                                       int64_t nb) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t b = i >> 5;
+  const bool in = i < n;
+  const int64_t cnt = (b * 32 + 32 < n) ? 32 : n - b * 32;
+  double s = 0.0;
+  for (int64_t k = 0; k < K; ++k) {
+    const double v = in ? F[k * n + i] : 0.0;
+    s = s + v * v;
+  }
+  const double inv = 1.0 / ((sqrt(s) + 1e-9) * (double)(cnt > 0 ? cnt : 1));
+  for (int64_t k = 0; k < K; ++k) {
+    float v = in ? (float)(F[k * n + i] * inv) : 0.0f;
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 31) == 0 && b < nb) Fp[k * nb + b] = v;
   }
 }
 
 // Pooled cost P[y][x] = 1 - sum_k A[k*H + y] * B[k*W + x] (the mean of C over
-// the block, C being bilinear in the normalised features), written as the
-// network input (channel 0, other channels 0) and optionally as f32.
+// the block, C being bilinear in the normalised features), f32.
 __global__ void k_pooled_cost(const float *__restrict__ A, int64_t H, const float *__restrict__ B,
-                              int64_t W, int64_t K, __hip_bfloat16 *__restrict__ X,
-                              float *__restrict__ P) {
+                              int64_t W, int64_t K, float *__restrict__ P) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= H * W) return;
   const int64_t y = p / W, x = p - y * W;
   float s = 0.0f;
   for (int64_t k = 0; k < K; ++k) s += A[k * H + y] * B[k * W + x];
-  const float c = 1.0f - s;
-  if (P) P[p] = c;
+  P[p] = 1.0f - s;
+}
+
+// Network input: the pooled cost double-centred (row and column means
+// removed, the global mean added back) and divided by its standard
+// deviation, so that a path block stands out by its local contrast.  P being
+// 1 - A^T B, the double-centred value is -(A_y - abar).(B_x - bbar) with
+// abar, bbar the means of the pooled features over the frames, and its mean
+// square is sum_{k,l} GA[k][l] GB[k][l] / (H W) with GA, GB the Gram
+// matrices of the centred features: O(K^2 (H + W)) work, no H x W pass.
+// One workgroup: st[0..K) = abar, st[K..2K) = bbar, st[2K] = 1 / std.
+__global__ __launch_bounds__(256) void k_input_stats(const float *__restrict__ A, int64_t H,
+                                                     const float *__restrict__ B, int64_t W,
+                                                     int64_t K, float *__restrict__ st) {
+  __shared__ float red[256];
+  const int t = threadIdx.x;
+  for (int64_t k = 0; k < 2 * K; ++k) {
+    const float *src = k < K ? A + k * H : B + (k - K) * W;
+    const int64_t n = k < K ? H : W;
+    float s = 0.0f;
+    for (int64_t y = t; y < n; y += 256) s += src[y];
+    red[t] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (t < h) red[t] += red[t + h];
+      __syncthreads();
+    }
+    if (t == 0) st[k] = red[0] / (float)n;
+    __syncthreads();
+  }
+  // entries (k, l), k <= l, of both Gram matrices, strided over the threads
+  float q = 0.0f;
+  const int64_t ne = K * (K + 1) / 2;
+  for (int64_t e = t; e < ne; e += 256) {
+    int64_t k = 0, rem = e;
+    while (rem >= K - k) {
+      rem -= K - k;
+      ++k;
+    }
+    const int64_t l = k + rem;
+    float ga = 0.0f, gb = 0.0f;
+    for (int64_t y = 0; y < H; ++y) ga += (A[k * H + y] - st[k]) * (A[l * H + y] - st[l]);
+    for (int64_t x = 0; x < W; ++x) gb += (B[k * W + x] - st[K + k]) * (B[l * W + x] - st[K + l]);
+    q += (k == l ? 1.0f : 2.0f) * ga * gb;
+  }
+  red[t] = q;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) red[t] += red[t + h];
+    __syncthreads();
+  }
+  if (t == 0) st[2 * K] = 1.0f / sqrtf(red[0] / ((float)H * (float)W) + 1e-12f);
+}
+
+__global__ void k_write_input(const float *__restrict__ A, int64_t H, const float *__restrict__ B,
+                              int64_t W, int64_t K, const float *__restrict__ st,
+                              __hip_bfloat16 *__restrict__ X) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= H * W) return;
+  const int64_t y = p / W, x = p - y * W;
+  float s = 0.0f;
+  for (int64_t k = 0; k < K; ++k) s += (A[k * H + y] - st[k]) * (B[k * W + x] - st[K + k]);
   __hip_bfloat16 *o = X + p * kCh;
-  o[0] = __float2bfloat16(c);
+  o[0] = __float2bfloat16(-s * st[2 * K]);
   for (int q = 1; q < kCh; ++q) o[q] = __float2bfloat16(0.0f);
 }
 
-// 3x3 convolution, 32 -> 32 channels.  A workgroup of 4 waves covers 128
-// consecutive pixels (row-major order), one wave 32 pixels x 32 channels.
+// 3x3 convolution, 32 -> 32 channels.  A workgroup of 4 waves computes a
+// block of 4 rows x 32 columns of output pixels, one wave per row (32 pixels
+// x 32 channels).  The 6 x 34 input halo of the block and the layer's
+// weights are staged in LDS once, so each input pixel is read from HBM/L2
+// once per block instead of once per tap.
 // MODE 0: Y = relu(acc + bias) as bf16 (hidden layer).
 // MODE 1: Y = acc + bias as f32 (logits).
 // MODE 2: Y = (M > 0) ? acc : 0 as bf16 (gradient through the ReLU of the
 //         layer input M, which is that layer's output activation).
+constexpr int kCvRows = 4, kCvCols = 32;
+constexpr int kPixPad = kCh + 8;      // bf16 per staged pixel (16-byte aligned rows)
+constexpr int kWPad = kKdim + 8;      // bf16 per staged weight row
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_conv3x3(const __hip_bfloat16 *__restrict__ X, int64_t H,
                                                  int64_t W, const __hip_bfloat16 *__restrict__ Wt,
                                                  const float *__restrict__ bias,
                                                  const __hip_bfloat16 *__restrict__ Mk,
                                                  void *__restrict__ Yv) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int64_t npix = H * W;
-  const int64_t p0 = (blockIdx.x * 4ll + wave) * 32;
-  if (p0 >= npix) return;  // whole wave past the image (no barrier in this kernel)
-  const int64_t p = p0 + (lane & 31);
-  const bool pin = p < npix;
-  const int64_t y = pin ? p / W : 0, x = pin ? p - (p / W) * W : 0;
+  __shared__ __attribute__((aligned(16))) __hip_bfloat16 xs[(kCvRows + 2) * (kCvCols + 2) * kPixPad];
+  __shared__ __attribute__((aligned(16))) __hip_bfloat16 wsm[kCh * kWPad];
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int64_t y0 = (int64_t)blockIdx.y * kCvRows, x0 = (int64_t)blockIdx.x * kCvCols;
+  // weights: 32 rows x 36 chunks of 8 bf16
+  for (int q = tid; q < kCh * (kKdim / 8); q += 256) {
+    const int o = q / (kKdim / 8), c8 = q - o * (kKdim / 8);
+    *reinterpret_cast<u32x4 *>(&wsm[o * kWPad + c8 * 8]) =
+        *reinterpret_cast<const u32x4 *>(&Wt[o * kKdim + c8 * 8]);
+  }
+  // input halo: (4+2) x (32+2) pixels x 4 chunks of 8 channels, zero outside
+  constexpr int kHalo = (kCvRows + 2) * (kCvCols + 2);
+  for (int q = tid; q < kHalo * 4; q += 256) {
+    const int pix = q >> 2, c8 = q & 3;
+    const int r = pix / (kCvCols + 2), c = pix - r * (kCvCols + 2);
+    const int64_t yy = y0 - 1 + r, xx = x0 - 1 + c;
+    const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
+    const int64_t src = ok ? (yy * W + xx) * kCh + c8 * 8 : 0;
+    const u32x4 v = *reinterpret_cast<const u32x4 *>(&X[src]);
+    *reinterpret_cast<u32x4 *>(&xs[pix * kPixPad + c8 * 8]) = ok ? v : u32x4{0u, 0u, 0u, 0u};
+  }
+  __syncthreads();
+  const int64_t y = y0 + wave;
+  const int px = lane & 31;
   const int kh = (lane >> 5) * 8;
-  const lbf16x8_t zero = lbf16x8_t{};
   lfloatx16_t acc = lfloatx16_t{};
 #pragma unroll
   for (int s = 0; s < 18; ++s) {
     const int dydx = s >> 1, dy = dydx / 3, dx = dydx - dy * 3;
-    const int64_t yy = y + dy - 1, xx = x + dx - 1;
-    const bool ok = pin && yy >= 0 && yy < H && xx >= 0 && xx < W;
     const int cin0 = (s & 1) * 16 + kh;
-    const int64_t src = ok ? (yy * W + xx) * kCh + cin0 : 0;
-    const lbf16x8_t a0 = *reinterpret_cast<const lbf16x8_t *>(X + src);
-    const lbf16x8_t a = ok ? a0 : zero;
-    const lbf16x8_t b = *reinterpret_cast<const lbf16x8_t *>(Wt + (lane & 31) * kKdim + s * 16 + kh);
+    const lbf16x8_t a = *reinterpret_cast<const lbf16x8_t *>(
+        &xs[((wave + dy) * (kCvCols + 2) + px + dx) * kPixPad + cin0]);
+    const lbf16x8_t b = *reinterpret_cast<const lbf16x8_t *>(&wsm[px * kWPad + s * 16 + kh]);
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
   }
+  if (y >= H) return;
   const int co = lane & 31;
   const float bv = (MODE == 2) ? 0.0f : bias[co];
 #pragma unroll
   for (int v = 0; v < 16; ++v) {
-    const int64_t q = p0 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-    if (q >= npix) continue;
+    const int64_t x = x0 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+    if (x >= W) continue;
+    const int64_t q = y * W + x;
     const float r = acc[v] + bv;
     if constexpr (MODE == 0) {
       static_cast<__hip_bfloat16 *>(Yv)[q * kCh + co] = __float2bfloat16(r > 0.0f ? r : 0.0f);
@@ -251,12 +339,13 @@ __global__ void k_pack_weights(const float *__restrict__ w, __hip_bfloat16 *__re
 }
 
 // Per-tile score: the largest channel-0 logit over the pooled cells a tile
-// covers.  Tile (i, j) of tile length L covers rows [i*hop, min(i*hop+L, L1))
-// and columns [j*hop, min(j*hop+L, L2)), hop = L-1; pooled cell (y, x)
-// covers frames [y*pool, y*pool+pool) x [x*pool, x*pool+pool).
+// covers, the region extended by `ext` cells on every side.  Tile (i, j) of
+// tile length L covers rows [i*hop, min(i*hop+L, L1)) and columns [j*hop,
+// min(j*hop+L, L2)), hop = L-1; pooled cell (y, x) covers frames
+// [y*pool, y*pool+pool) x [x*pool, x*pool+pool).
 __global__ __launch_bounds__(256) void k_tile_scores(const float *__restrict__ Z, int64_t H,
                                                      int64_t W, int pool, int64_t L, int64_t L1,
-                                                     int64_t L2, int64_t n2,
+                                                     int64_t L2, int64_t n2, int ext,
                                                      float *__restrict__ score) {
   __shared__ float red[256];
   const int64_t t = blockIdx.x;
@@ -264,8 +353,10 @@ __global__ __launch_bounds__(256) void k_tile_scores(const float *__restrict__ Z
   const int64_t hop = L - 1;
   const int64_t r0 = i * hop, r1 = (r0 + L < L1 ? r0 + L : L1) - 1;
   const int64_t c0 = j * hop, c1 = (c0 + L < L2 ? c0 + L : L2) - 1;
-  const int64_t y0 = r0 / pool, y1 = r1 / pool < H - 1 ? r1 / pool : H - 1;
-  const int64_t x0 = c0 / pool, x1 = c1 / pool < W - 1 ? c1 / pool : W - 1;
+  const int64_t y0 = r0 / pool - ext > 0 ? r0 / pool - ext : 0;
+  const int64_t y1 = r1 / pool + ext < H - 1 ? r1 / pool + ext : H - 1;
+  const int64_t x0 = c0 / pool - ext > 0 ? c0 / pool - ext : 0;
+  const int64_t x1 = c1 / pool + ext < W - 1 ? c1 / pool + ext : W - 1;
   const int64_t nx = x1 - x0 + 1, n = (y1 - y0 + 1) * nx;
   float m = -__builtin_huge_valf();
   for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
@@ -289,19 +380,35 @@ using namespace pf;
 
 extern "C" int pf_pool_features(const double *d_F, int64_t K, int64_t n, int pool, float *d_Fp,
                                 void *stream) {
-  PF_REQUIRE(d_F && d_Fp && K >= 1 && n >= 1 && pool >= 1, "pf_pool_features: bad arguments");
+  PF_REQUIRE(d_F && d_Fp && K >= 1 && n >= 1, "pf_pool_features: bad arguments");
+  PF_REQUIRE(pool == 32, "pf_pool_features: the pooling factor is 32");
   const int64_t nb = (n + pool - 1) / pool;
-  hipLaunchKernelGGL(k_pool_features, dim3(nb256(nb)), dim3(256), 0, as_stream(stream), d_F, K, n,
-                     pool, d_Fp, nb);
+  hipLaunchKernelGGL(k_pool_features, dim3(nb256(n)), dim3(256), 0, as_stream(stream), d_F, K, n,
+                     d_Fp, nb);
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
 }
 
+extern "C" int pf_pooled_cost_workspace(int64_t K, int64_t *floats) {
+  PF_REQUIRE(floats && K >= 1, "pf_pooled_cost_workspace: bad arguments");
+  *floats = 2 * K + 2;
+  return PF_OK;
+}
+
 extern "C" int pf_pooled_cost(const float *d_A, int64_t H, const float *d_B, int64_t W, int64_t K,
-                              void *d_X, float *d_P, void *stream) {
-  PF_REQUIRE(d_A && d_B && d_X && H >= 1 && W >= 1 && K >= 1, "pf_pooled_cost: bad arguments");
-  hipLaunchKernelGGL(k_pooled_cost, dim3(nb256(H * W)), dim3(256), 0, as_stream(stream), d_A, H,
-                     d_B, W, K, static_cast<__hip_bfloat16 *>(d_X), d_P);
+                              void *d_X, float *d_P, float *d_work, void *stream) {
+  PF_REQUIRE(d_A && d_B && d_X && d_work && H >= 1 && W >= 1 && K >= 1,
+             "pf_pooled_cost: bad arguments");
+  hipStream_t st = as_stream(stream);
+  if (d_P) {
+    hipLaunchKernelGGL(k_pooled_cost, dim3(nb256(H * W)), dim3(256), 0, st, d_A, H, d_B, W, K,
+                       d_P);
+    PF_CHECK_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_input_stats, dim3(1), dim3(256), 0, st, d_A, H, d_B, W, K, d_work);
+  PF_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_write_input, dim3(nb256(H * W)), dim3(256), 0, st, d_A, H, d_B, W, K,
+                     d_work, static_cast<__hip_bfloat16 *>(d_X));
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -312,8 +419,9 @@ extern "C" int pf_conv3x3(int mode, const void *d_X, int64_t H, int64_t W, const
   PF_REQUIRE(mode == 2 ? d_M != nullptr : d_bias != nullptr, "pf_conv3x3: missing bias or mask");
   PF_REQUIRE(reinterpret_cast<uintptr_t>(d_X) % 16 == 0 && reinterpret_cast<uintptr_t>(d_Wt) % 16 == 0,
              "pf_conv3x3: X and weights must be 16-byte aligned");
-  const int64_t npix = H * W;
-  const dim3 grid((unsigned)((npix + 127) / 128));
+  PF_REQUIRE((W + kCvCols - 1) / kCvCols < (1ll << 31) && (H + kCvRows - 1) / kCvRows < 65536,
+             "pf_conv3x3: image too large");
+  const dim3 grid((unsigned)((W + kCvCols - 1) / kCvCols), (unsigned)((H + kCvRows - 1) / kCvRows));
   const auto *X = static_cast<const __hip_bfloat16 *>(d_X);
   const auto *Wt = static_cast<const __hip_bfloat16 *>(d_Wt);
   const auto *M = static_cast<const __hip_bfloat16 *>(d_M);
@@ -380,15 +488,16 @@ extern "C" int pf_pack_weights(const float *d_w, void *d_Wt, void *d_Wf, void *s
 }
 
 extern "C" int pf_tile_scores(const float *d_Z, int64_t H, int64_t W, int pool, int64_t L,
-                              int64_t L1, int64_t L2, float *d_score, void *stream) {
-  PF_REQUIRE(d_Z && d_score && H >= 1 && W >= 1 && pool >= 1 && L >= 2 && L1 >= 2 && L2 >= 2,
+                              int64_t L1, int64_t L2, int ext, float *d_score, void *stream) {
+  PF_REQUIRE(d_Z && d_score && H >= 1 && W >= 1 && pool >= 1 && L >= 2 && L1 >= 2 && L2 >= 2 &&
+                 ext >= 0,
              "pf_tile_scores: bad arguments");
   PF_REQUIRE((L1 + pool - 1) / pool == H && (L2 + pool - 1) / pool == W,
              "pf_tile_scores: pooled map does not match the pair");
   const int64_t hop = L - 1;
   const int64_t n1 = (L1 - 1 + hop - 1) / hop, n2 = (L2 - 1 + hop - 1) / hop;
   hipLaunchKernelGGL(k_tile_scores, dim3((unsigned)(n1 * n2)), dim3(256), 0, as_stream(stream), d_Z,
-                     H, W, pool, L, L1, L2, n2, d_score);
+                     H, W, pool, L, L1, L2, n2, ext, d_score);
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
 }

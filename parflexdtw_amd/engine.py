@@ -188,6 +188,18 @@ def collect(plan, ws):
     return BatchResult(bc, found, paths, best_i[:, :6].copy())
 
 
+def collect_pinned(plan, ws, host):
+    """collect() through pinned host buffers kept in the dict `host` (reused
+    while the sizes match)."""
+    torch = _torch()
+    for k in ("best_f", "best_i", "path_len", "path"):
+        if k not in host or host[k].shape != ws[k].shape:
+            host[k] = torch.empty(ws[k].shape, dtype=ws[k].dtype, pin_memory=True)
+        host[k].copy_(ws[k], non_blocking=True)
+    torch.cuda.synchronize()
+    return collect(plan, host)
+
+
 def run_batch(C_dev, plan, steps, weights, stream=None, ws=None):
     """Stage 1 + stage 2 for every pair of the plan; returns (BatchResult, ws)."""
     sw = steps if isinstance(steps, StepWeights) else StepWeights(steps, weights)
@@ -384,15 +396,9 @@ class BatchRunner:
     def collect(self):
         """Best costs, endpoints and stitched paths of every pair, copied
         through pinned host buffers allocated on first use."""
-        torch = _torch()
         if getattr(self, "_host", None) is None:
-            self._host = {k: torch.empty(self.ws[k].shape, dtype=self.ws[k].dtype,
-                                         pin_memory=True)
-                          for k in ("best_f", "best_i", "path_len", "path")}
-        for k, h in self._host.items():
-            h.copy_(self.ws[k], non_blocking=True)
-        torch.cuda.synchronize()
-        return collect(self.plan, self._host)
+            self._host = {}
+        return collect_pinned(self.plan, self.ws, self._host)
 
 
 def tune_batch(pairs, Ls, steps, weights, beta=0.1, dtype=None, Gs=(32,), reps=2, log=None):

@@ -1,0 +1,307 @@
+"""Batched ParFlex with learned tile proposal and full-tile fallback.
+
+For every pair the tile network (tile_net.TileNet) scores the tiles from the
+1/32-pooled cost; the tiles whose logit reaches the threshold, dilated by a
+margin, form the selection.  Then:
+
+  * the cost is built only for selected tiles, one slot per tile
+    (pf_cost_batch with one job per tile, BF16);
+  * stage 1 runs on the selected tiles only (a work table: the selected rows
+    of the full tile table, pointing at their cost slots);
+  * stage 2 runs on the full tile grid of every pair; pruned tiles have
+    D = +inf on their edges, so no candidate or chain passes through them.
+    Their back-pointer words point at one shared all-zero region, so any
+    read of them stays in bounds.
+
+Selection: a tile is kept when its score -- the largest logit over its pooled
+cells extended by `ext` cells on every side -- reaches the threshold, so the
+margin around the predicted path is ext*32 frames, not whole tiles.
+
+A pair falls back to the plain full-tile method (engine.BatchRunner over all
+its tiles) when a tile within `guard` frames of its stitched path was pruned
+(the path reached the edge of the selection), when no endpoint is found, or
+when its normalised cost exceeds the calibrated bound.  The plain method is
+also the judge of the result.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from . import engine
+from ._lib import PF_BF16, check, lib
+from ._plan import BatchPlan
+from .engine import StepWeights, _stream
+from .tile_net import TileNet, select_tiles
+
+__all__ = ["LearnedRunner", "path_tiles", "needs_fallback"]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def path_tiles(wp, L, n1, n2, guard=0, L1=None, L2=None):
+    """Flat indices (i*n2 + j, unique) of the tiles holding a cell within
+    `guard` frames (in row and column) of a point of the path (2, n); a cell
+    on a tile overlap belongs to every tile that holds it."""
+    hop = L - 1
+    r0 = np.asarray(wp[0], np.int64)
+    c0 = np.asarray(wp[1], np.int64)
+    if r0.size == 0:
+        return np.zeros(0, np.int64)
+    offs = (0,) if guard <= 0 else (-guard, 0, guard)
+    rmax = (L1 - 1) if L1 is not None else np.iinfo(np.int64).max
+    cmax = (L2 - 1) if L2 is not None else np.iinfo(np.int64).max
+
+    def tiles_of(v, vmax, n):
+        out = []
+        for d in offs:
+            x = np.clip(v + d, 0, vmax)
+            a = np.minimum(x // hop, n - 1)
+            b = np.minimum(np.where((x % hop == 0) & (x > 0), x // hop - 1, x // hop), n - 1)
+            out.extend((a, b))
+        return out
+
+    rows = tiles_of(r0, rmax, n1)
+    cols = tiles_of(c0, cmax, n2)
+    flat = np.concatenate([ri * n2 + cj for ri in rows for cj in cols])
+    return np.unique(flat)
+
+
+def needs_fallback(wp, found, best_cost, sel, L, cost_bound, guard, L1, L2):
+    """Reason the learned result cannot be trusted, or None: no endpoint,
+    normalised cost above the bound, or a tile within `guard` frames of the
+    path was pruned (the path reached the edge of the selection)."""
+    if not found or not np.isfinite(best_cost):
+        return "no endpoint"
+    if best_cost > cost_bound:
+        return "cost bound"
+    if _pruned_near_path(wp, sel, L, guard, L1, L2):
+        return "selection edge"
+    return None
+
+
+def _pruned_near_path(wp, sel, L, guard, L1, L2):
+    """True when a pruned tile holds a cell within `guard` frames of a path
+    point.  The tiles holding frames [x0, x1] are max(x0-1, 0)//hop ..
+    min(x1//hop, n-1) (a frame on an overlap belongs to two tiles), so for
+    each point they form a rectangle of tile indices, tested with a prefix
+    sum of the pruned mask."""
+    if np.asarray(wp).size == 0:
+        return False
+    # Every step moves at most 2 frames in each coordinate, so testing every
+    # 8th point (and the last) with the guard widened by 8 frames covers all
+    # points with the original guard (a conservative test).
+    n = np.asarray(wp).shape[1]
+    idx = np.unique(np.concatenate([np.arange(0, n, 8), [n - 1]]))
+    r = np.asarray(wp[0])[idx].astype(np.int32)
+    c = np.asarray(wp[1])[idx].astype(np.int32)
+    n1, n2 = sel.shape
+    hop = np.int32(L - 1)
+    g = np.int32(guard + 8)
+    i0 = np.minimum(np.maximum(r - g - 1, 0) // hop, n1 - 1)
+    i1 = np.minimum(np.minimum(r + g, L1 - 1) // hop, n1 - 1)
+    j0 = np.minimum(np.maximum(c - g - 1, 0) // hop, n2 - 1)
+    j1 = np.minimum(np.minimum(c + g, L2 - 1) // hop, n2 - 1)
+    pr = np.zeros((n1 + 1, n2 + 1), np.int32)
+    pr[1:, 1:] = np.cumsum(np.cumsum(~sel, axis=0), axis=1)
+    w = n2 + 1
+    flat = pr.reshape(-1)
+    cnt = (flat[(i1 + 1) * w + j1 + 1] - flat[i0 * w + j1 + 1] - flat[(i1 + 1) * w + j0]
+           + flat[i0 * w + j0])
+    return bool(cnt.any())
+
+
+class LearnedRunner:
+    """One batch of feature pairs at tile length L through the learned
+    pipeline.  run() does one complete pass (scoring, selection, pruned
+    stage 1 + stage 2, fallback) and returns (BatchResult, stats)."""
+
+    def __init__(self, pairs, L, steps, weights, beta, net: TileNet, threshold, ext=8,
+                 guard=128, cost_bound=math.inf, device=None):
+        torch = _torch()
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.L, self.beta = int(L), float(beta)
+        self.steps, self.weights = steps, weights
+        self.sw = StepWeights(steps, weights)
+        self.net, self.threshold = net, float(threshold)
+        self.ext, self.guard = int(ext), int(guard)
+        if not 0 <= 2 * self.guard < self.L - 1:
+            raise ValueError("guard must be below half the tile hop (L-1)/2")
+        self.cost_bound = float(cost_bound)
+        self.pairs = pairs
+        self.shapes = [(int(F1.shape[1]), int(F2.shape[1])) for F1, F2 in pairs]
+        K = int(pairs[0][0].shape[0])
+        self.K = K
+        f64 = np.float64
+        self.Fp = [(torch.from_numpy(np.ascontiguousarray(F1, f64)).to(self.device),
+                    torch.from_numpy(np.ascontiguousarray(F2, f64)).to(self.device))
+                   for F1, F2 in pairs]
+        self.F1 = torch.cat([a for a, _ in self.Fp], dim=1).contiguous()
+        self.F2 = torch.cat([b for _, b in self.Fp], dim=1).contiguous()
+        self.f1_off = np.concatenate([[0], np.cumsum([s[0] for s in self.shapes])[:-1]])
+        self.f2_off = np.concatenate([[0], np.cumsum([s[1] for s in self.shapes])[:-1]])
+        nb = ctypes.c_int64()
+        check(lib().pf_cost_workspace(PF_BF16, K, int(self.F1.shape[1]), int(self.F2.shape[1]),
+                                      ctypes.byref(nb)), "pf_cost_workspace")
+        self.work = torch.empty(-(-nb.value // 8), dtype=torch.float64, device=self.device)
+        self.plan = BatchPlan(self.shapes, self.L, beta=self.beta)
+        tr, tc = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().pf_cost_tile(PF_BF16, ctypes.byref(tr), ctypes.byref(tc)), "pf_cost_tile")
+        self.cost_tile = (tr.value, tc.value)
+        self._host = {}
+
+    # ------------------------------------------------------------------
+    def score(self, stream=None):
+        """Tile logits of every pair, as one host array per pair (n1, n2)."""
+        torch = _torch()
+        outs = []
+        for (F1d, F2d) in self.Fp:
+            s, n1, n2 = self.net.score_pair(F1d, F2d, self.L, self.ext, stream)
+            outs.append((s, n1, n2))
+        flat = torch.cat([s for s, _, _ in outs]).cpu().numpy()
+        res, o = [], 0
+        for (_, n1, n2) in outs:
+            res.append(flat[o:o + n1 * n2].reshape(n1, n2))
+            o += n1 * n2
+        return res
+
+    def _pruned_tables(self, sels):
+        """Work table, full table with remapped back-pointer offsets, cost
+        jobs and sizes for the selection."""
+        plan = self.plan
+        full = plan.tiles.copy()
+        sel = np.concatenate([s.reshape(-1) for s in sels])
+        R, W = full[:, 2], full[:, 3]
+        words = ((R + 15) // 16) * W
+        zero_words = int(((self.L + 15) // 16) * self.L)
+        b_off = np.zeros(len(full), np.int64)
+        ws = words[sel]
+        b_off[sel] = zero_words + np.concatenate([[0], np.cumsum(ws)[:-1]]) if ws.size else []
+        full[:, 4] = b_off                    # pruned tiles read the zero region
+        work = full[sel].copy()
+        ldc = (W[sel] + 7) // 8 * 8
+        slot = R[sel] * ldc
+        c_off = np.concatenate([[0], np.cumsum(slot)[:-1]]) if slot.size else np.zeros(0, np.int64)
+        work[:, 0] = c_off
+        work[:, 1] = ldc
+        # cost jobs: the tile's frames in the concatenated feature arrays
+        pair = work[:, 6]
+        ij = work[:, 7]
+        hop = self.L - 1
+        s1 = (ij >> 32) * hop
+        s2 = (ij & 0xFFFFFFFF) * hop
+        tr, tc = self.cost_tile
+        nblk = (-(-work[:, 2] // tr)) * (-(-work[:, 3] // tc))
+        jobs = np.zeros((len(work), 8), np.int64)
+        jobs[:, 0] = self.f1_off[pair] + s1
+        jobs[:, 1] = self.f2_off[pair] + s2
+        jobs[:, 2] = work[:, 2]
+        jobs[:, 3] = work[:, 3]
+        jobs[:, 4] = c_off
+        jobs[:, 5] = ldc
+        jobs[:, 6] = np.concatenate([[0], np.cumsum(nblk)[:-1]]) if len(work) else []
+        return (sel, full, work, jobs, int(nblk.sum()), zero_words + int(ws.sum()),
+                int(slot.sum()), int((R[sel] * W[sel]).sum()))
+
+    def run(self, stream=None, timings=None):
+        """One complete pass.  With a dict for `timings`, the device is
+        synchronised after every phase and the phase times (s) are added."""
+        torch = _torch()
+        st = _stream(stream)
+        L = self.L
+        clock = [time.perf_counter()]
+
+        def mark(name):
+            if timings is not None:
+                torch.cuda.synchronize()
+                now = time.perf_counter()
+                timings[name] = timings.get(name, 0.0) + now - clock[0]
+                clock[0] = now
+
+        scores = self.score(stream)
+        mark("score")
+        sels = [select_tiles(s, s.shape[0], s.shape[1], self.threshold) for s in scores]
+        sel, full, work, jobs, nblk, b_words, c_elems, cells = self._pruned_tables(sels)
+        mark("select")
+        nsel = len(work)
+        dev = self.device
+        plan = self.plan
+        ws = {
+            "tiles": torch.from_numpy(full).to(dev),
+            "pairs": torch.from_numpy(plan.pairs).to(dev),
+        }
+        # Bp holds the zero region (read by pruned tiles) + the selected tiles
+        base = plan.allocate(dev, b_words=b_words)
+        base["tiles"], base["pairs"] = ws["tiles"], ws["pairs"]
+        base["Bp"][:(L + 15) // 16 * L].zero_()
+        ws = base
+        ws["edge_D"].fill_(math.inf)
+        ws["edge_P"].zero_()
+        ws["edge_C"].zero_()
+        mark("workspace")
+        if nsel:
+            C = torch.empty(c_elems, dtype=torch.bfloat16, device=dev)
+            d_jobs = torch.from_numpy(jobs).to(dev)
+            check(lib().pf_cost_batch(PF_BF16, self.F1.data_ptr(), int(self.F1.shape[1]),
+                                      self.F2.data_ptr(), int(self.F2.shape[1]), self.K,
+                                      d_jobs.data_ptr(), nsel, nblk, C.data_ptr(),
+                                      self.work.data_ptr(), st), "pf_cost_batch")
+            mark("cost")
+            d_work = torch.from_numpy(work).to(dev)
+            S, nw, G = self._s1_shape(nsel)
+            fl = xc = None
+            if S > 1:
+                nf, nx = ctypes.c_int64(), ctypes.c_int64()
+                check(lib().pf_stage1_workspace(PF_BF16, nsel, L, S, ctypes.byref(nf),
+                                                ctypes.byref(nx)), "pf_stage1_workspace")
+                flags = torch.empty(nf.value, dtype=torch.int32, device=dev)
+                xch = torch.empty(-(-nx.value // 8), dtype=torch.float64, device=dev)
+                fl, xc = flags.data_ptr(), xch.data_ptr()
+            check(lib().pf_stage1(PF_BF16, C.data_ptr(), d_work.data_ptr(), nsel, L,
+                                  self.sw.sp, self.sw.wp, self.sw.K, S, nw, G, fl, xc,
+                                  ws["Bp"].data_ptr(), ws["edge_D"].data_ptr(),
+                                  ws["edge_P"].data_ptr(), ws["edge_C"].data_ptr(), st),
+                  "pf_stage1")
+            mark("stage1")
+        engine.stage2(None, plan, self.sw, ws, stream)
+        mark("stage2")
+        res = engine.collect_pinned(plan, ws, self._host)
+        mark("collect")
+        # fallback decisions on the host
+        fb, why = [], {}
+        for p in range(len(self.pairs)):
+            reason = needs_fallback(res.paths[p], bool(res.found[p]), float(res.best_cost[p]),
+                                    sels[p], L, self.cost_bound, self.guard, *self.shapes[p])
+            if reason:
+                fb.append(p)
+                why[reason] = why.get(reason, 0) + 1
+        mark("check")
+        if fb:
+            sub = [self.pairs[p] for p in fb]
+            br = engine.BatchRunner(sub, L, self.steps, self.weights, self.beta,
+                                    torch.bfloat16, dev)
+            br.launch(stream)
+            r2 = br.collect()
+            for q, p in enumerate(fb):
+                res.best_cost[p] = r2.best_cost[q]
+                res.found[p] = r2.found[q]
+                res.paths[p] = r2.paths[q]
+                res.best_end[p] = r2.best_end[q]
+            mark("fallback")
+        stats = {"tiles": int(plan.ntiles), "selected_tiles": nsel,
+                 "cells_full": int(plan.cells), "cells_computed": cells,
+                 "fallback_pairs": fb, "fallback_reasons": why}
+        return res, stats
+
+    def _s1_shape(self, ntiles):
+        L = self.L
+        if ntiles >= 512:
+            nw = max(1, min(16, -(-L // 128)))
+            return -(-L // (nw * 128)), nw, 32
+        return engine.stage1_config(self.plan)

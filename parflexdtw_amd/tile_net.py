@@ -54,9 +54,9 @@ def path_block_labels(wp, L1, L2, pool=POOL):
     return lab
 
 
-def select_tiles(scores, n1, n2, threshold, margin=1):
-    """Boolean (n1, n2): score >= threshold, dilated by `margin` tiles in
-    every direction."""
+def select_tiles(scores, n1, n2, threshold, margin=0):
+    """Boolean (n1, n2): score >= threshold, dilated by `margin` whole tiles
+    in every direction."""
     s = np.asarray(scores).reshape(n1, n2) >= threshold
     if margin <= 0 or not s.any():
         return s
@@ -139,9 +139,9 @@ class TileNet:
         return net, meta
 
     # ---- input --------------------------------------------------------
-    def pooled_input(self, F1d, F2d, stream=None, want_cost=False):
-        """Network input (H, W, 32) bf16 for one pair of device features
-        (K, N) / (K, M) float64; also the f32 pooled cost if asked."""
+    def pooled_features(self, F1d, F2d, stream=None):
+        """Pooled normalised features (K*H and K*W f32) of one pair of device
+        features (K, N) / (K, M) float64."""
         torch = _torch()
         K, N = int(F1d.shape[0]), int(F1d.shape[1])
         M = int(F2d.shape[1])
@@ -153,10 +153,27 @@ class TileNet:
               "pf_pool_features")
         check(lib().pf_pool_features(F2d.data_ptr(), K, M, POOL, B.data_ptr(), st),
               "pf_pool_features")
+        return A, B, H, W, K
+
+    def input_from_pooled(self, A, B, H, W, K, stream=None, want_cost=False):
+        """Network input (H*W*32 bf16) from pooled features; also the f32
+        pooled cost (H*W) if asked."""
+        torch = _torch()
         X = torch.empty(H * W * CH, dtype=torch.bfloat16, device=self.device)
+        n = ctypes.c_int64()
+        check(lib().pf_pooled_cost_workspace(K, ctypes.byref(n)), "pf_pooled_cost_workspace")
+        work = torch.empty(n.value, dtype=torch.float32, device=self.device)
         P = torch.empty(H * W, dtype=torch.float32, device=self.device) if want_cost else None
         check(lib().pf_pooled_cost(A.data_ptr(), H, B.data_ptr(), W, K, X.data_ptr(),
-                                   P.data_ptr() if P is not None else None, st), "pf_pooled_cost")
+                                   P.data_ptr() if P is not None else None, work.data_ptr(),
+                                   _st(stream)), "pf_pooled_cost")
+        return X, P
+
+    def pooled_input(self, F1d, F2d, stream=None, want_cost=False):
+        """Network input (H, W, 32) bf16 for one pair of device features
+        (K, N) / (K, M) float64; also the f32 pooled cost if asked."""
+        A, B, H, W, K = self.pooled_features(F1d, F2d, stream)
+        X, P = self.input_from_pooled(A, B, H, W, K, stream, want_cost)
         return (X, H, W, P) if want_cost else (X, H, W)
 
     # ---- forward / backward -------------------------------------------
@@ -212,17 +229,18 @@ class TileNet:
         self.pack(stream)
 
     # ---- inference -------------------------------------------------------
-    def tile_scores(self, Z, H, W, L, L1, L2, stream=None):
+    def tile_scores(self, Z, H, W, L, L1, L2, ext=0, stream=None):
+        """Max logit over each tile's pooled cells extended by ext cells."""
         torch = _torch()
         hop = L - 1
         n1, n2 = -(-(L1 - 1) // hop), -(-(L2 - 1) // hop)
         s = torch.empty(n1 * n2, dtype=torch.float32, device=self.device)
-        check(lib().pf_tile_scores(Z.data_ptr(), H, W, POOL, L, L1, L2, s.data_ptr(),
+        check(lib().pf_tile_scores(Z.data_ptr(), H, W, POOL, L, L1, L2, int(ext), s.data_ptr(),
                                    _st(stream)), "pf_tile_scores")
         return s, n1, n2
 
-    def score_pair(self, F1d, F2d, L, stream=None):
+    def score_pair(self, F1d, F2d, L, ext=0, stream=None):
         """Tile logits (n1*n2 device tensor) of one pair of device features."""
         X, H, W = self.pooled_input(F1d, F2d, stream)
         _, Z = self.forward(X, H, W, stream)
-        return self.tile_scores(Z, H, W, L, int(F1d.shape[1]), int(F2d.shape[1]), stream)
+        return self.tile_scores(Z, H, W, L, int(F1d.shape[1]), int(F2d.shape[1]), ext, stream)

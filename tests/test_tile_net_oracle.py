@@ -93,6 +93,13 @@ def test_tile_scores_geometry(tn):
             r1 = min(i * hop + 100, 250) - 1
             c1 = min(j * hop + 100, 300) - 1
             assert s[i * n2 + j] == min(r1 // 32, 7) * 10 + min(c1 // 32, 9)
+    # extended by 2 cells: the region's bottom-right cell moves 2 further
+    s2 = tn.tile_scores(Z, 32, 100, 250, 300, ext=2)
+    for i in range(n1):
+        for j in range(n2):
+            r1 = min(i * hop + 100, 250) - 1
+            c1 = min(j * hop + 100, 300) - 1
+            assert s2[i * n2 + j] == min(r1 // 32 + 2, 7) * 10 + min(c1 // 32 + 2, 9)
 
 
 def test_bf16_rounding(tn):
@@ -100,3 +107,14 @@ def test_bf16_rounding(tn):
     assert tn.bf16(1.0 + 2 ** -9) == 1.0           # tie to even
     assert tn.bf16(1.0 + 3 * 2 ** -9) == 1.0 + 2 ** -7
     assert tn.bf16(-3.0e-3) == pytest.approx(-3.0e-3, rel=2 ** -8)
+
+
+def test_network_input_is_double_centred_and_scaled(tn):
+    rng = np.random.default_rng(9)
+    P = rng.random((17, 23)) + np.arange(17)[:, None] * 0.1 + np.arange(23)[None, :] * 0.05
+    X = tn.network_input(P)
+    assert np.allclose(X.mean(axis=0), 0, atol=1e-12)
+    assert np.allclose(X.mean(axis=1), 0, atol=1e-12)
+    assert np.mean(X * X) == pytest.approx(1.0, rel=1e-9)
+    # additive row and column offsets do not change it
+    assert np.allclose(tn.network_input(P + rng.random(17)[:, None] + rng.random(23)[None, :]), X)
