@@ -2,7 +2,18 @@
 
 Workloads (--workload):
 
-  chroma_batch256_N3000-20000 (default): 256 seeded chroma pairs (K = 12),
+  chroma_learned128_N20000-60000_L2000 (default): 128 held-out seeded chroma
+      pairs (seeds 90000+p, N ~ U[20000, 60000], M = N * U[0.8, 1.25],
+      trimmed ends), tile length 2000.  The learned form: the tile network
+      (parflexdtw_amd/data/tile_net_L2000.npz, trained on 10k other pairs)
+      scores every tile from the 1/32-pooled cost, only the selected tiles
+      run the BF16 cost build and the FP32 DP, stage 2 (FP64) runs on the
+      full grid, and a pair whose path reaches the edge of its selection
+      (or exceeds the calibrated cost bound) is redone by the full-tile
+      batched form.  One step = features (in HBM) -> paths on the host for
+      all pairs.  Beside it: the plain FP64 drop-in (parflex per pair, cost
+      build included) on every pair, which is the judge of agreement.
+  chroma_batch256_N3000-20000: 256 seeded chroma pairs (K = 12),
       N ~ U[3000, 20000], M = N * U[0.8, 1.25], trimmed ends, so every pair
       has ragged edge tiles.  The batched form: all 256 cost matrices built
       on the MFMA units in BF16, one stage-1 launch over every tile of every
@@ -45,8 +56,9 @@ METRIC = "DP GCUPS + pairs/s, batched/learned vs plain GPU at >=99.5% path agree
 TRAFFIC_BYTES = {("chroma_batch256_N3000-20000", 500): (83.97 + 12.84) * 1e9}
 HBM_PEAK_GBS = 8000.0
 STEPS, WEIGHTS, BETA = [(1, 1), (1, 2), (2, 1)], [2, 3, 3], 0.1
-WORKLOADS = ["chroma_batch256_N3000-20000", "chroma16000x19000_L4000",
-             "relu128_40000x40000_L4000", "chroma2000x2400_L500"]
+WORKLOADS = ["chroma_learned128_N20000-60000_L2000", "chroma_batch256_N3000-20000",
+             "chroma16000x19000_L4000", "relu128_40000x40000_L4000", "chroma2000x2400_L500"]
+MODEL = os.path.join(ROOT, "parflexdtw_amd", "data", "tile_net_L2000.npz")
 
 
 def _parse():
@@ -55,7 +67,8 @@ def _parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default=WORKLOADS[0], choices=WORKLOADS)
-    ap.add_argument("--pairs", type=int, default=256, help="batch size of the batched workload")
+    ap.add_argument("--pairs", type=int, default=None,
+                    help="batch size (default 128 learned, 256 batched)")
     ap.add_argument("--L", type=int, nargs="*", default=[500, 1000, 2000, 4000],
                     help="tile lengths the batched workload's tuner tries")
     ap.add_argument("--plain-sample", type=int, default=16,
@@ -96,6 +109,117 @@ def _base(args, value, unit, ms_step, dtype, config):
 
 
 # ---------------------------------------------------------------------------
+def bench_learned(args, torch):
+    from parflexdtw_amd import Parflex_gpu as PG
+    from parflexdtw_amd.gpu_flexdtw import cost_matrix_from_features_gpu
+    from parflexdtw_amd.learned import LearnedRunner
+    from parflexdtw_amd.synth import batch_pairs, path_agreement
+    from parflexdtw_amd.tile_net import TileNet
+
+    P = args.pairs or 128
+    net, meta = TileNet.load(MODEL)
+    L = int(meta["L"])
+    t0 = time.perf_counter()
+    pairs = [(a, b) for a, b, _ in batch_pairs(P, 90000, 20000, 60000)]
+    shapes = [(F1.shape[1], F2.shape[1]) for F1, F2 in pairs]
+    log(f"{P} held-out pairs (seeds 90000+, N ~ U[20000,60000]) in "
+        f"{time.perf_counter() - t0:.1f} s; {sum(a * b for a, b in shapes) / 1e9:.1f} Gcells; "
+        f"network trained on {int(meta['train_pairs'])} pairs, threshold "
+        f"{float(meta['threshold']):.3f}")
+    lr = LearnedRunner(pairs, L, STEPS, WEIGHTS, BETA, net, float(meta["threshold"]),
+                       int(meta["ext"]), int(meta["guard"]), float(meta["cost_bound"]))
+    for _ in range(args.warmup):
+        res, st = lr.run()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        res, st = lr.run(events=evs[k])
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    s1_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs])) if st["selected_tiles"] else 0.0
+    tm = {}
+    lr.run(timings=tm)
+    pairs_s = P / (ms_step * 1e-3)
+    log(f"learned: {ms_step:.2f} ms/step ({pairs_s:.1f} pairs/s); phases (synchronised run): "
+        + ", ".join(f"{k} {v * 1e3:.1f}" for k, v in tm.items()))
+    log(f"selected {st['selected_tiles']}/{st['tiles']} tiles, "
+        f"{st['cells_computed'] / st['cells_full']:.4f} of the cells; fallbacks "
+        f"{len(st['fallback_pairs'])} {st['fallback_reasons']}")
+
+    # the plain FP64 drop-in on every pair: the judge, and the plain pairs/s
+    F1d, F2d = pairs[0]
+    PG.parflex(cost_matrix_from_features_gpu(F1d, F2d), STEPS, WEIGHTS, BETA, L,
+               profile_dir=None)
+    torch.cuda.synchronize()
+    plain = []
+    t0 = time.perf_counter()
+    for F1, F2 in pairs:
+        C64 = cost_matrix_from_features_gpu(F1, F2)
+        plain.append(PG.parflex(C64, STEPS, WEIGHTS, BETA, L, profile_dir=None))
+        del C64
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    plain_pairs_s = P / plain_s
+    agree = [path_agreement(plain[p][1], res.paths[p], tol=2) for p in range(P)]
+    rel = [abs(res.best_cost[p] - plain[p][0]) / abs(plain[p][0]) for p in range(P)]
+    frames = sum(plain[p][1].shape[1] for p in range(P))
+    frames_ok = sum(path_agreement(plain[p][1], res.paths[p], tol=2) * plain[p][1].shape[1]
+                    for p in range(P))
+    log(f"plain FP64 drop-in: {plain_s:.2f} s for {P} pairs ({plain_pairs_s:.2f} pairs/s); "
+        f"agreement min {min(agree):.5f}, all frames {frames_ok / frames:.6f}")
+    cells = st["cells_full"]
+    out = _base(args, round(pairs_s, 3), "pairs/s", ms_step, "bf16 cost, f32 DP, f64 stage 2",
+                {"workload": args.workload, "pairs": P, "N": "U[20000,60000]",
+                 "M": "N*U[0.8,1.25], 0-10% trimmed each end", "K": 12, "L": L,
+                 "tiles": st["tiles"], "cells": cells, "steps": STEPS, "weights": WEIGHTS,
+                 "beta": BETA, "form": "learned tile proposal + batched BF16/FP32 pipeline, "
+                                       "fallback to the plain method"})
+    out["data"] = ("synthetic (seeded chroma pairs with known time warps, held out from the "
+                   "network's training seeds; features in HBM)")
+    out["speedup_vs_plain"] = round(pairs_s / plain_pairs_s, 3)
+    out["plain_gpu"] = {"pairs_per_s": round(plain_pairs_s, 3),
+                        "gcups": round(cells / plain_s / 1e9, 3), "pairs": P,
+                        "form": "parflex drop-in per pair, FP64 cost + FP64 DP, same L"}
+    out["effective_gcups"] = round(cells / (ms_step * 1e-3) / 1e9, 3)
+    out["dp_gcups"] = round(st["cells_computed"] / (s1_ms * 1e-3) / 1e9, 3) if s1_ms else None
+    out["stage1_ms"] = round(s1_ms, 4)
+    out["phases_ms"] = {k: round(v * 1e3, 3) for k, v in tm.items()}
+    out["selection"] = {"tiles_selected": st["selected_tiles"], "tiles": st["tiles"],
+                        "cells_fraction": round(st["cells_computed"] / cells, 5),
+                        "fallback_pairs": len(st["fallback_pairs"]),
+                        "fallback_reasons": st["fallback_reasons"]}
+    out["agreement"] = {"frames_within_2_all": round(frames_ok / frames, 6),
+                        "frames_within_2_min_pair": round(min(agree), 6),
+                        "best_cost_rel_err_max": float(max(rel)), "pairs_compared": P,
+                        "judge": "plain FP64 path of the same pair at the same L"}
+    out["model"] = {"file": "parflexdtw_amd/data/tile_net_L2000.npz",
+                    "train_pairs": int(meta["train_pairs"]),
+                    "train_N": f"U[{int(meta['n_lo'])},{int(meta['n_hi'])}]",
+                    "epochs": int(meta["epochs_done"]), "threshold": float(meta["threshold"]),
+                    "ext_cells": int(meta["ext"]), "guard_frames": int(meta["guard"])}
+    out["roofline"] = _roofline(st["cells_computed"], s1_ms, 2.25,
+                                "k_stage1<2,__hip_bfloat16,float,false,true>") if s1_ms else None
+    if not args.no_cpu:
+        from oracle import oracle
+        nthreads = args.cpu_threads
+        C = cost_matrix_from_features_gpu(*pairs[0]).cpu().numpy()
+        tc = time.perf_counter()
+        bo, wpo = oracle.parflex(C, STEPS, WEIGHTS, BETA, L, nthreads=nthreads)
+        t_cpu = time.perf_counter() - tc
+        out["cpu_baseline"] = {
+            "value": round(1.0 / t_cpu, 4), "unit": "pairs/s", "cores": nthreads, "kind": "port",
+            "sample": f"pair 0 of the held-out set ({shapes[0][0]}x{shapes[0][1]}, "
+                      f"{shapes[0][0] * shapes[0][1] / 1e9:.2f} Gcells), FP64 parflex end to end "
+                      f"at L={L}, {nthreads} worker threads (cpu_parflex.py:392 schedule); "
+                      f"{_cpu_model()}",
+            "seconds": round(t_cpu, 4)}
+        out["parity"] = {"plain_gpu_vs_cpu_oracle_exact":
+                         bool(bo == plain[0][0] and np.array_equal(wpo, plain[0][1]))}
+    return out
+
+
+# ---------------------------------------------------------------------------
 def bench_batch(args, torch):
     from parflexdtw_amd import Parflex_gpu as PG
     from parflexdtw_amd import engine
@@ -103,6 +227,7 @@ def bench_batch(args, torch):
     from parflexdtw_amd.synth import batch_pairs, path_agreement
 
     t0 = time.perf_counter()
+    args.pairs = args.pairs or 256
     pairs = [(F1, F2) for F1, F2, _ in batch_pairs(args.pairs)]
     shapes = [(F1.shape[1], F2.shape[1]) for F1, F2 in pairs]
     log(f"{len(pairs)} pairs generated in {time.perf_counter() - t0:.1f} s; "
@@ -406,7 +531,9 @@ def main():
         torch.cuda.synchronize()
         log("copy probe: 1073741824 bytes read, 1073741824 bytes written")
         del src, dst
-    if args.workload.startswith("chroma_batch"):
+    if args.workload.startswith("chroma_learned"):
+        out = bench_learned(args, torch)
+    elif args.workload.startswith("chroma_batch"):
         out = bench_batch(args, torch)
     elif args.workload.startswith("relu"):
         out = bench_precision(args, torch)

@@ -209,9 +209,11 @@ class LearnedRunner:
         return (sel, full, work, jobs, int(nblk.sum()), zero_words + int(ws.sum()),
                 int(slot.sum()), int((R[sel] * W[sel]).sum()))
 
-    def run(self, stream=None, timings=None):
+    def run(self, stream=None, timings=None, events=None):
         """One complete pass.  With a dict for `timings`, the device is
-        synchronised after every phase and the phase times (s) are added."""
+        synchronised after every phase and the phase times (s) are added.
+        With a pair of torch.cuda.Event for `events`, they are recorded on the
+        launch stream around the stage-1 kernel."""
         torch = _torch()
         st = _stream(stream)
         L = self.L
@@ -263,11 +265,15 @@ class LearnedRunner:
                 flags = torch.empty(nf.value, dtype=torch.int32, device=dev)
                 xch = torch.empty(-(-nx.value // 8), dtype=torch.float64, device=dev)
                 fl, xc = flags.data_ptr(), xch.data_ptr()
+            if events is not None:
+                events[0].record(stream or torch.cuda.current_stream())
             check(lib().pf_stage1(PF_BF16, C.data_ptr(), d_work.data_ptr(), nsel, L,
                                   self.sw.sp, self.sw.wp, self.sw.K, S, nw, G, fl, xc,
                                   ws["Bp"].data_ptr(), ws["edge_D"].data_ptr(),
                                   ws["edge_P"].data_ptr(), ws["edge_C"].data_ptr(), st),
                   "pf_stage1")
+            if events is not None:
+                events[1].record(stream or torch.cuda.current_stream())
             mark("stage1")
         engine.stage2(None, plan, self.sw, ws, stream)
         mark("stage2")

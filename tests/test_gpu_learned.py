@@ -1,0 +1,64 @@
+"""GPU tests of the learned pipeline: what the trained network gains over the
+method without it, judged by the plain FP64 method, and that training with
+the project's kernels lowers the loss."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+STEPS = [(1, 1), (1, 2), (2, 1)]
+WEIGHTS = [2, 3, 3]
+MODEL = os.path.join(ROOT, "parflexdtw_amd", "data", "tile_net_L2000.npz")
+
+
+@pytest.fixture(scope="module")
+def torch_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def test_trained_network_gains_over_full_tiles(torch_gpu):
+    """Held-out pairs (seeds outside training and validation): the trained
+    network's selection computes well under half of the cells, needs no
+    fallback and keeps >= 99.5% of frames within +-2 of the plain FP64
+    path; an untrained network selects nothing useful, so every pair falls
+    back to the full method (no gain, same answer)."""
+    from parflexdtw_amd.learned import LearnedRunner
+    from parflexdtw_amd.synth import batch_pairs, path_agreement
+    from parflexdtw_amd.tile_net import TileNet
+    from parflexdtw_amd.train_tile_net import plain_paths
+    net, meta = TileNet.load(MODEL)
+    L = int(meta["L"])
+    pairs = [(a, b) for a, b, _ in batch_pairs(6, 95000, 14000, 24000)]
+    cost, paths = plain_paths(pairs, L)
+    args = (float(meta["threshold"]), int(meta["ext"]), int(meta["guard"]),
+            float(meta["cost_bound"]))
+    res, st = LearnedRunner(pairs, L, STEPS, WEIGHTS, 0.1, net, *args).run()
+    frac = st["cells_computed"] / st["cells_full"]
+    assert frac < 0.5, frac
+    assert st["fallback_pairs"] == []
+    for p in range(len(pairs)):
+        assert path_agreement(paths[p], res.paths[p], tol=2) >= 0.995, p
+        assert abs(res.best_cost[p] - cost[p]) <= 2e-2 * cost[p]
+    res_u, st_u = LearnedRunner(pairs, L, STEPS, WEIGHTS, 0.1, TileNet(seed=1), *args).run()
+    assert st_u["fallback_pairs"] == list(range(len(pairs)))
+    for p in range(len(pairs)):
+        assert path_agreement(paths[p], res_u.paths[p], tol=2) >= 0.995, p
+
+
+def test_training_lowers_the_loss(torch_gpu):
+    from parflexdtw_amd.synth import batch_pairs
+    from parflexdtw_amd.tile_net import TileNet
+    from parflexdtw_amd.train_tile_net import build_dataset, plain_paths, train
+    pairs = [(a, b) for a, b, _ in batch_pairs(16, 70000, 2000, 5000)]
+    _, paths = plain_paths(pairs, 500)
+    net = TileNet(seed=4)
+    data = build_dataset(net, pairs, paths)
+    hist = train(net, data, epochs=12, lr=3e-3, batch=4, pos_weight=20.0)
+    assert hist[-1] < 0.6 * hist[0], hist
