@@ -202,17 +202,20 @@ int pf_cost_batch(int out_dtype, const double *d_F1, int64_t n1_total, const dou
  * pool-frame block (FlexDTW.py:1207-1210 normalisation); pool must be 32. */
 int pf_pool_features(const double *d_F, int64_t K, int64_t n, int pool, float *d_Fp,
                      void *stream);
-/* Network input d_X (bf16 NHWC, channel 0) from the pooled features: the
- * pooled cost P = 1 - A^T B double-centred (row and column means removed,
- * global mean added back) and divided by its standard deviation; if d_P is
- * not null, also P itself (H x W f32).  d_work: pf_pooled_cost_workspace
- * floats. */
+/* Network input from the pooled features: the pooled cost P = 1 - A^T B
+ * double-centred (row and column means removed, global mean added back) and
+ * divided by its standard deviation, written as d_X (bf16 NHWC 32, channel 0,
+ * other channels 0) and/or d_Xc (bf16 H*W, one channel); if d_P is not null,
+ * also P itself (H x W f32).  d_work: pf_pooled_cost_workspace floats. */
 int pf_pooled_cost_workspace(int64_t K, int64_t *floats);
 int pf_pooled_cost(const float *d_A, int64_t H, const float *d_B, int64_t W, int64_t K,
-                   void *d_X, float *d_P, float *d_work, void *stream);
-/* One 3x3 convolution layer on MFMA.  mode 0: relu(conv + bias) -> bf16;
- * mode 1: conv + bias -> f32; mode 2: gradient w.r.t. the layer input,
- * d_Wt = flipped weights (pf_pack_weights), masked by d_M > 0 -> bf16. */
+                   void *d_X, void *d_Xc, float *d_P, float *d_work, void *stream);
+/* One 3x3 convolution layer on MFMA.  mode 0: relu(conv + bias) -> bf16
+ * NHWC 32; mode 1: channel 0 of conv + bias -> f32 H*W (the logit map);
+ * mode 2: gradient w.r.t. the layer input, d_Wt = flipped weights
+ * (pf_pack_weights), masked by d_M > 0 -> bf16 NHWC 32; mode 3: as mode 0
+ * for the first layer from the one-channel input d_X = d_Xc of
+ * pf_pooled_cost (only input channel 0 of the weights is read). */
 int pf_conv3x3(int mode, const void *d_X, int64_t H, int64_t W, const void *d_Wt,
                const float *d_bias, const void *d_M, void *d_Y, void *stream);
 /* Weight and bias gradient of one layer, accumulated into d_grad (f32
@@ -220,9 +223,9 @@ int pf_conv3x3(int mode, const void *d_X, int64_t H, int64_t W, const void *d_Wt
 int pf_conv3x3_wgrad_workspace(int64_t H, int64_t W, int64_t *floats);
 int pf_conv3x3_wgrad(const void *d_dY, const void *d_X, int64_t H, int64_t W, float *d_grad,
                      float *d_part, void *stream);
-/* Weighted binary cross-entropy on the channel-0 logits of d_Z (f32 NHWC
- * 32): writes the logit gradient d_dZ (bf16 NHWC 32, scaled by `scale`) and
- * adds the scaled loss to *d_loss. */
+/* Weighted binary cross-entropy on the logit map d_Z (f32 H*W): writes the
+ * logit gradient d_dZ (bf16 NHWC 32, channel 0, scaled by `scale`) and adds
+ * the scaled loss to *d_loss. */
 int pf_bce(const float *d_Z, const uint8_t *d_label, int64_t npix, float pos_weight,
            float scale, void *d_dZ, float *d_loss, void *stream);
 /* Adam step (bias-corrected, step >= 1) on f32 master weights; clears d_g.
@@ -232,7 +235,7 @@ int pf_adam(float *d_w, float *d_g, float *d_m, float *d_v, const uint8_t *d_fro
 /* bf16 weights of one layer and, if d_Wf is not null, the flipped copy used
  * by mode 2 of pf_conv3x3. */
 int pf_pack_weights(const float *d_w, void *d_Wt, void *d_Wf, void *stream);
-/* Per-tile score = max channel-0 logit over the pooled cells that tile
+/* Per-tile score = max logit (map d_Z, f32 H*W) over the pooled cells that tile
  * (i, j) of tile length L covers (tiling of Parflex.py:388-427), the region
  * extended by ext cells on every side; d_score has n1*n2 entries in i-major
  * order. */

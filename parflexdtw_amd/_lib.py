@@ -49,7 +49,7 @@ _SIGNATURES = {
     "pf_cost_batch": (_int, [_int, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp]),
     "pf_pool_features": (_int, [_vp, _i64, _i64, _int, _vp, _vp]),
     "pf_pooled_cost_workspace": (_int, [_i64, _vp]),
-    "pf_pooled_cost": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp]),
+    "pf_pooled_cost": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
     "pf_conv3x3": (_int, [_int, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
     "pf_conv3x3_wgrad_workspace": (_int, [_i64, _i64, _vp]),
     "pf_conv3x3_wgrad": (_int, [_vp, _vp, _i64, _i64, _vp, _vp, _vp]),

@@ -74,7 +74,7 @@ __global__ void k_pooled_cost(const float *__restrict__ A, int64_t H, const floa
 // square is sum_{k,l} GA[k][l] GB[k][l] / (H W) with GA, GB the Gram
 // matrices of the centred features: O(K^2 (H + W)) work, no H x W pass.
 // One workgroup: st[0..K) = abar, st[K..2K) = bbar, st[2K] = 1 / std.
-__global__ __launch_bounds__(256) void k_input_stats(const float *__restrict__ A, int64_t H,
+__global__ __launch_bounds__(256) void k_input_stats_any(const float *__restrict__ A, int64_t H,
                                                      const float *__restrict__ B, int64_t W,
                                                      int64_t K, float *__restrict__ st) {
   __shared__ float red[256];
@@ -117,31 +117,108 @@ __global__ __launch_bounds__(256) void k_input_stats(const float *__restrict__ A
   if (t == 0) st[2 * K] = 1.0f / sqrtf(red[0] / ((float)H * (float)W) + 1e-12f);
 }
 
+// Same statistics for K <= KM with every thread accumulating all means and
+// Gram entries of its frames in registers; partial sums are reduced with
+// wave shuffles, then over the 4 waves in a fixed order (deterministic).
+template <int KM>
+__global__ __launch_bounds__(256) void k_input_stats_small(const float *__restrict__ A, int64_t H,
+                                                           const float *__restrict__ B, int64_t W,
+                                                           int K, float *__restrict__ st) {
+  constexpr int E = KM * (KM + 1) / 2;
+  __shared__ float part[4][E];
+  __shared__ float mean[2][KM];
+  __shared__ float G[2][E];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  for (int side = 0; side < 2; ++side) {
+    const float *S = side ? B : A;
+    const int64_t n = side ? W : H;
+    float m[KM];
+#pragma unroll
+    for (int k = 0; k < KM; ++k) m[k] = 0.0f;
+    for (int64_t y = t; y < n; y += 256)
+#pragma unroll
+      for (int k = 0; k < KM; ++k)
+        if (k < K) m[k] += S[k * n + y];
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      float v = m[k];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      if (lane == 0) part[wave][k] = v;
+    }
+    __syncthreads();
+    if (t < KM) {
+      mean[side][t] = ((part[0][t] + part[1][t]) + (part[2][t] + part[3][t])) / (float)n;
+      if (t < K) st[side * K + t] = mean[side][t];
+    }
+    __syncthreads();
+  }
+  for (int side = 0; side < 2; ++side) {
+    const float *S = side ? B : A;
+    const int64_t n = side ? W : H;
+    float g[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) g[e] = 0.0f;
+    for (int64_t y = t; y < n; y += 256) {
+      float c[KM];
+#pragma unroll
+      for (int k = 0; k < KM; ++k) c[k] = k < K ? S[k * n + y] - mean[side][k] : 0.0f;
+      int e = 0;
+#pragma unroll
+      for (int k = 0; k < KM; ++k)
+#pragma unroll
+        for (int l = k; l < KM; ++l) g[e++] += c[k] * c[l];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      float v = g[e];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      if (lane == 0) part[wave][e] = v;
+    }
+    __syncthreads();
+    for (int e = t; e < E; e += 256) G[side][e] = (part[0][e] + part[1][e]) + (part[2][e] + part[3][e]);
+    __syncthreads();
+  }
+  if (t == 0) {
+    float q = 0.0f;
+    int e = 0;
+    for (int k = 0; k < KM; ++k)
+      for (int l = k; l < KM; ++l, ++e) q += (k == l ? 1.0f : 2.0f) * G[0][e] * G[1][e];
+    st[2 * K] = 1.0f / sqrtf(q / ((float)H * (float)W) + 1e-12f);
+  }
+}
+
 __global__ void k_write_input(const float *__restrict__ A, int64_t H, const float *__restrict__ B,
                               int64_t W, int64_t K, const float *__restrict__ st,
-                              __hip_bfloat16 *__restrict__ X) {
+                              __hip_bfloat16 *__restrict__ X, __hip_bfloat16 *__restrict__ Xc) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= H * W) return;
   const int64_t y = p / W, x = p - y * W;
   float s = 0.0f;
   for (int64_t k = 0; k < K; ++k) s += (A[k * H + y] - st[k]) * (B[k * W + x] - st[K + k]);
-  __hip_bfloat16 *o = X + p * kCh;
-  o[0] = __float2bfloat16(-s * st[2 * K]);
-  for (int q = 1; q < kCh; ++q) o[q] = __float2bfloat16(0.0f);
+  const __hip_bfloat16 v = __float2bfloat16(-s * st[2 * K]);
+  if (Xc) Xc[p] = v;
+  if (X) {
+    __hip_bfloat16 *o = X + p * kCh;
+    o[0] = v;
+    for (int q = 1; q < kCh; ++q) o[q] = __float2bfloat16(0.0f);
+  }
 }
 
 // 3x3 convolution, 32 -> 32 channels.  A workgroup of 4 waves computes a
-// block of 4 rows x 32 columns of output pixels, one wave per row (32 pixels
-// x 32 channels).  The 6 x 34 input halo of the block and the layer's
-// weights are staged in LDS once, so each input pixel is read from HBM/L2
-// once per block instead of once per tap.
+// block of kCvRows x kCvCols output pixels; each wave takes kCvRows/4 rows
+// as MFMA tiles of 32 pixels x 32 channels sharing the weight operand.  The
+// input halo of the block and the layer's weights are staged in LDS once per
+// block.  4 x 32 (35 KB of LDS, 4 workgroups per CU) measured faster than
+// 8 x 64 (72 KB, 2 per CU).
 // MODE 0: Y = relu(acc + bias) as bf16 (hidden layer).
-// MODE 1: Y = acc + bias as f32 (logits).
+// MODE 1: channel 0 of acc + bias as f32, H*W (the logit map).
 // MODE 2: Y = (M > 0) ? acc : 0 as bf16 (gradient through the ReLU of the
 //         layer input M, which is that layer's output activation).
 constexpr int kCvRows = 4, kCvCols = 32;
+constexpr int kTR = kCvRows / 4, kTC = kCvCols / 32, kTiles = kTR * kTC;  // per wave
 constexpr int kPixPad = kCh + 8;      // bf16 per staged pixel (16-byte aligned rows)
 constexpr int kWPad = kKdim + 8;      // bf16 per staged weight row
+typedef unsigned int lu32x4_t __attribute__((ext_vector_type(4)));
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_conv3x3(const __hip_bfloat16 *__restrict__ X, int64_t H,
@@ -149,9 +226,9 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __hip_bfloat16 *__restric
                                                  const float *__restrict__ bias,
                                                  const __hip_bfloat16 *__restrict__ Mk,
                                                  void *__restrict__ Yv) {
-  __shared__ __attribute__((aligned(16))) __hip_bfloat16 xs[(kCvRows + 2) * (kCvCols + 2) * kPixPad];
+  constexpr int HR = kCvRows + 2, HC = kCvCols + 2;
+  __shared__ __attribute__((aligned(16))) __hip_bfloat16 xs[HR * HC * kPixPad];
   __shared__ __attribute__((aligned(16))) __hip_bfloat16 wsm[kCh * kWPad];
-  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -159,51 +236,108 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __hip_bfloat16 *__restric
   // weights: 32 rows x 36 chunks of 8 bf16
   for (int q = tid; q < kCh * (kKdim / 8); q += 256) {
     const int o = q / (kKdim / 8), c8 = q - o * (kKdim / 8);
-    *reinterpret_cast<u32x4 *>(&wsm[o * kWPad + c8 * 8]) =
-        *reinterpret_cast<const u32x4 *>(&Wt[o * kKdim + c8 * 8]);
+    *reinterpret_cast<lu32x4_t *>(&wsm[o * kWPad + c8 * 8]) =
+        *reinterpret_cast<const lu32x4_t *>(&Wt[o * kKdim + c8 * 8]);
   }
-  // input halo: (4+2) x (32+2) pixels x 4 chunks of 8 channels, zero outside
-  constexpr int kHalo = (kCvRows + 2) * (kCvCols + 2);
-  for (int q = tid; q < kHalo * 4; q += 256) {
+  // input halo: HR x HC pixels x 4 chunks of 8 channels, zero outside
+  for (int q = tid; q < HR * HC * 4; q += 256) {
     const int pix = q >> 2, c8 = q & 3;
-    const int r = pix / (kCvCols + 2), c = pix - r * (kCvCols + 2);
+    const int r = pix / HC, c = pix - r * HC;
     const int64_t yy = y0 - 1 + r, xx = x0 - 1 + c;
     const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
     const int64_t src = ok ? (yy * W + xx) * kCh + c8 * 8 : 0;
-    const u32x4 v = *reinterpret_cast<const u32x4 *>(&X[src]);
-    *reinterpret_cast<u32x4 *>(&xs[pix * kPixPad + c8 * 8]) = ok ? v : u32x4{0u, 0u, 0u, 0u};
+    const lu32x4_t v = *reinterpret_cast<const lu32x4_t *>(&X[src]);
+    *reinterpret_cast<lu32x4_t *>(&xs[pix * kPixPad + c8 * 8]) =
+        ok ? v : lu32x4_t{0u, 0u, 0u, 0u};
   }
   __syncthreads();
-  const int64_t y = y0 + wave;
   const int px = lane & 31;
   const int kh = (lane >> 5) * 8;
-  lfloatx16_t acc = lfloatx16_t{};
+  lfloatx16_t acc[kTiles];
+#pragma unroll
+  for (int t = 0; t < kTiles; ++t) acc[t] = lfloatx16_t{};
 #pragma unroll
   for (int s = 0; s < 18; ++s) {
     const int dydx = s >> 1, dy = dydx / 3, dx = dydx - dy * 3;
     const int cin0 = (s & 1) * 16 + kh;
-    const lbf16x8_t a = *reinterpret_cast<const lbf16x8_t *>(
-        &xs[((wave + dy) * (kCvCols + 2) + px + dx) * kPixPad + cin0]);
     const lbf16x8_t b = *reinterpret_cast<const lbf16x8_t *>(&wsm[px * kWPad + s * 16 + kh]);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) {
+      const int rr = kTR * wave + t / kTC, cc = (t % kTC) * 32;
+      const lbf16x8_t a = *reinterpret_cast<const lbf16x8_t *>(
+          &xs[((rr + dy) * HC + cc + px + dx) * kPixPad + cin0]);
+      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[t], 0, 0, 0);
+    }
   }
-  if (y >= H) return;
   const int co = lane & 31;
   const float bv = (MODE == 2) ? 0.0f : bias[co];
+#pragma unroll
+  for (int t = 0; t < kTiles; ++t) {
+    const int64_t y = y0 + kTR * wave + t / kTC;
+    if (y >= H) continue;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int64_t x = x0 + (t % kTC) * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+      if (x >= W) continue;
+      const int64_t q = y * W + x;
+      const float r = acc[t][v] + bv;
+      if constexpr (MODE == 0) {
+        static_cast<__hip_bfloat16 *>(Yv)[q * kCh + co] = __float2bfloat16(r > 0.0f ? r : 0.0f);
+      } else if constexpr (MODE == 1) {
+        if (co == 0) static_cast<float *>(Yv)[q] = r;   // channel 0 only: the logit map
+      } else {
+        const float m = bf2f(Mk[q * kCh + co]);
+        static_cast<__hip_bfloat16 *>(Yv)[q * kCh + co] = __float2bfloat16(m > 0.0f ? r : 0.0f);
+      }
+    }
+  }
+}
+
+// First layer from a compact one-channel input Xc (bf16, H*W): the 9 taps of
+// channel 0, padded to 16, are one MFMA step (k = dy*3 + dx; the other input
+// channels of the layer are zero).  Output as MODE 0: relu(acc + bias), bf16
+// NHWC 32.  A workgroup of 4 waves covers 4 rows x 32 columns.
+constexpr int kC1Rows = 4, kC1Cols = 32;
+__global__ __launch_bounds__(256) void k_conv3x3_first(const __hip_bfloat16 *__restrict__ Xc,
+                                                       int64_t H, int64_t W,
+                                                       const __hip_bfloat16 *__restrict__ Wt,
+                                                       const float *__restrict__ bias,
+                                                       __hip_bfloat16 *__restrict__ Y) {
+  __shared__ __hip_bfloat16 xs[(kC1Rows + 2) * (kC1Cols + 2)];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int64_t y0 = (int64_t)blockIdx.y * kC1Rows, x0 = (int64_t)blockIdx.x * kC1Cols;
+  for (int q = tid; q < (kC1Rows + 2) * (kC1Cols + 2); q += 256) {
+    const int r = q / (kC1Cols + 2), c = q - r * (kC1Cols + 2);
+    const int64_t yy = y0 - 1 + r, xx = x0 - 1 + c;
+    const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
+    const __hip_bfloat16 v = Xc[ok ? yy * W + xx : 0];
+    xs[q] = ok ? v : __float2bfloat16(0.0f);
+  }
+  __syncthreads();
+  const int64_t y = y0 + wave;
+  const int px = lane & 31, kh = (lane >> 5) * 8;
+  lbf16x8_t a, b;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int k = kh + t;                       // tap dy*3 + dx, or padding
+    const int dy = k / 3, dx = k - (k / 3) * 3;
+    const __hip_bfloat16 xv = xs[(k < 9 ? (wave + dy) * (kC1Cols + 2) + px + dx : 0)];
+    const __hip_bfloat16 wv = Wt[px * kKdim + (k < 9 ? k : 0) * kCh];
+    a[t] = k < 9 ? (__bf16)bf2f(xv) : (__bf16)0.0f;
+    b[t] = k < 9 ? (__bf16)bf2f(wv) : (__bf16)0.0f;
+  }
+  lfloatx16_t acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, lfloatx16_t{}, 0, 0, 0);
+  if (y >= H) return;
+  const int co = lane & 31;
+  const float bv = bias[co];
 #pragma unroll
   for (int v = 0; v < 16; ++v) {
     const int64_t x = x0 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
     if (x >= W) continue;
-    const int64_t q = y * W + x;
     const float r = acc[v] + bv;
-    if constexpr (MODE == 0) {
-      static_cast<__hip_bfloat16 *>(Yv)[q * kCh + co] = __float2bfloat16(r > 0.0f ? r : 0.0f);
-    } else if constexpr (MODE == 1) {
-      static_cast<float *>(Yv)[q * kCh + co] = r;
-    } else {
-      const float m = bf2f(Mk[q * kCh + co]);
-      static_cast<__hip_bfloat16 *>(Yv)[q * kCh + co] = __float2bfloat16(m > 0.0f ? r : 0.0f);
-    }
+    Y[(y * W + x) * kCh + co] = __float2bfloat16(r > 0.0f ? r : 0.0f);
   }
 }
 
@@ -280,7 +414,7 @@ __global__ void k_reduce_add(const float *__restrict__ part, int64_t nblk, int64
   g[i] += s;
 }
 
-// Weighted BCE on the channel-0 logits of Z (f32, NHWC 32): dZ channel 0 =
+// Weighted BCE on the logit map Z (f32, H*W): dZ channel 0 =
 // scale * w * (sigmoid(z) - y), other channels 0; loss_acc += scale * sum w*bce.
 __global__ __launch_bounds__(256) void k_bce(const float *__restrict__ Z,
                                              const uint8_t *__restrict__ label, int64_t npix,
@@ -291,7 +425,7 @@ __global__ __launch_bounds__(256) void k_bce(const float *__restrict__ Z,
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   float l = 0.0f;
   if (p < npix) {
-    const float z = Z[p * kCh];
+    const float z = Z[p];
     const float y = label[p] ? 1.0f : 0.0f;
     const float w = label[p] ? pos_weight : 1.0f;
     l = w * (fmaxf(z, 0.0f) - z * y + log1pf(expf(-fabsf(z)))) * scale;
@@ -338,7 +472,7 @@ __global__ void k_pack_weights(const float *__restrict__ w, __hip_bfloat16 *__re
   if (Wf) Wf[cin * kKdim + (8 - dydx) * kCh + o] = b;
 }
 
-// Per-tile score: the largest channel-0 logit over the pooled cells a tile
+// Per-tile score: the largest logit (map Z, f32 H*W) over the pooled cells a tile
 // covers, the region extended by `ext` cells on every side.  Tile (i, j) of
 // tile length L covers rows [i*hop, min(i*hop+L, L1)) and columns [j*hop,
 // min(j*hop+L, L2)), hop = L-1; pooled cell (y, x) covers frames
@@ -361,7 +495,7 @@ __global__ __launch_bounds__(256) void k_tile_scores(const float *__restrict__ Z
   float m = -__builtin_huge_valf();
   for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
     const int64_t y = y0 + q / nx, x = x0 + q % nx;
-    m = fmaxf(m, Z[(y * W + x) * kCh]);
+    m = fmaxf(m, Z[y * W + x]);
   }
   red[threadIdx.x] = m;
   __syncthreads();
@@ -396,8 +530,8 @@ extern "C" int pf_pooled_cost_workspace(int64_t K, int64_t *floats) {
 }
 
 extern "C" int pf_pooled_cost(const float *d_A, int64_t H, const float *d_B, int64_t W, int64_t K,
-                              void *d_X, float *d_P, float *d_work, void *stream) {
-  PF_REQUIRE(d_A && d_B && d_X && d_work && H >= 1 && W >= 1 && K >= 1,
+                              void *d_X, void *d_Xc, float *d_P, float *d_work, void *stream) {
+  PF_REQUIRE(d_A && d_B && (d_X || d_Xc) && d_work && H >= 1 && W >= 1 && K >= 1,
              "pf_pooled_cost: bad arguments");
   hipStream_t st = as_stream(stream);
   if (d_P) {
@@ -405,10 +539,15 @@ extern "C" int pf_pooled_cost(const float *d_A, int64_t H, const float *d_B, int
                        d_P);
     PF_CHECK_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_input_stats, dim3(1), dim3(256), 0, st, d_A, H, d_B, W, K, d_work);
+  if (K <= 16)
+    hipLaunchKernelGGL(k_input_stats_small<16>, dim3(1), dim3(256), 0, st, d_A, H, d_B, W, (int)K,
+                       d_work);
+  else
+    hipLaunchKernelGGL(k_input_stats_any, dim3(1), dim3(256), 0, st, d_A, H, d_B, W, K, d_work);
   PF_CHECK_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_write_input, dim3(nb256(H * W)), dim3(256), 0, st, d_A, H, d_B, W, K,
-                     d_work, static_cast<__hip_bfloat16 *>(d_X));
+                     d_work, static_cast<__hip_bfloat16 *>(d_X),
+                     static_cast<__hip_bfloat16 *>(d_Xc));
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -419,9 +558,11 @@ extern "C" int pf_conv3x3(int mode, const void *d_X, int64_t H, int64_t W, const
   PF_REQUIRE(mode == 2 ? d_M != nullptr : d_bias != nullptr, "pf_conv3x3: missing bias or mask");
   PF_REQUIRE(reinterpret_cast<uintptr_t>(d_X) % 16 == 0 && reinterpret_cast<uintptr_t>(d_Wt) % 16 == 0,
              "pf_conv3x3: X and weights must be 16-byte aligned");
-  PF_REQUIRE((W + kCvCols - 1) / kCvCols < (1ll << 31) && (H + kCvRows - 1) / kCvRows < 65536,
+  PF_REQUIRE((W + kC1Cols - 1) / kC1Cols < (1ll << 31) && (H + kC1Rows - 1) / kC1Rows < 65536,
              "pf_conv3x3: image too large");
   const dim3 grid((unsigned)((W + kCvCols - 1) / kCvCols), (unsigned)((H + kCvRows - 1) / kCvRows));
+  const dim3 grid1((unsigned)((W + kC1Cols - 1) / kC1Cols),
+                   (unsigned)((H + kC1Rows - 1) / kC1Rows));
   const auto *X = static_cast<const __hip_bfloat16 *>(d_X);
   const auto *Wt = static_cast<const __hip_bfloat16 *>(d_Wt);
   const auto *M = static_cast<const __hip_bfloat16 *>(d_M);
@@ -432,8 +573,11 @@ extern "C" int pf_conv3x3(int mode, const void *d_X, int64_t H, int64_t W, const
     hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(256), 0, st, X, H, W, Wt, d_bias, M, d_Y);
   else if (mode == 2)
     hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, st, X, H, W, Wt, d_bias, M, d_Y);
+  else if (mode == 3)
+    hipLaunchKernelGGL(k_conv3x3_first, grid1, dim3(256), 0, st, X, H, W, Wt, d_bias,
+                       static_cast<__hip_bfloat16 *>(d_Y));
   else
-    PF_REQUIRE(false, "pf_conv3x3: mode must be 0, 1 or 2");
+    PF_REQUIRE(false, "pf_conv3x3: mode must be 0, 1, 2 or 3");
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
 }

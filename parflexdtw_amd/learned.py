@@ -123,7 +123,7 @@ class LearnedRunner:
     stage 1 + stage 2, fallback) and returns (BatchResult, stats)."""
 
     def __init__(self, pairs, L, steps, weights, beta, net: TileNet, threshold, ext=8,
-                 guard=128, cost_bound=math.inf, device=None):
+                 guard=128, cost_bound=math.inf, device=None, graph=True):
         torch = _torch()
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.L, self.beta = int(L), float(beta)
@@ -155,18 +155,50 @@ class LearnedRunner:
         check(lib().pf_cost_tile(PF_BF16, ctypes.byref(tr), ctypes.byref(tc)), "pf_cost_tile")
         self.cost_tile = (tr.value, tc.value)
         self._host = {}
+        self.graph, self._graph, self._scores = bool(graph), None, None
 
     # ------------------------------------------------------------------
-    def score(self, stream=None):
-        """Tile logits of every pair, as one host array per pair (n1, n2)."""
-        torch = _torch()
-        outs = []
+    def _score_into(self, out, stream=None):
+        """Tile logits of every pair into the device buffer `out` (the
+        pairs' grids concatenated in i-major order)."""
+        o = 0
         for (F1d, F2d) in self.Fp:
-            s, n1, n2 = self.net.score_pair(F1d, F2d, self.L, self.ext, stream)
-            outs.append((s, n1, n2))
-        flat = torch.cat([s for s, _, _ in outs]).cpu().numpy()
+            X, H, W = self.net.pooled_input(F1d, F2d, stream, compact=True)
+            _, Z = self.net.forward(X, H, W, stream)
+            L1, L2 = int(F1d.shape[1]), int(F2d.shape[1])
+            n1, n2 = -(-(L1 - 1) // (self.L - 1)), -(-(L2 - 1) // (self.L - 1))
+            check(lib().pf_tile_scores(Z.data_ptr(), H, W, 32, self.L, L1, L2, self.ext,
+                                       out[o:o + n1 * n2].data_ptr(), _stream(stream)),
+                  "pf_tile_scores")
+            o += n1 * n2
+
+    def score(self, stream=None):
+        """Tile logits of every pair, as one host array per pair (n1, n2).
+        With graph=True the scoring of all pairs is recorded once as a HIP
+        graph (after one eager pass) and replayed; the network's weights are
+        then fixed for this runner."""
+        torch = _torch()
+        ntot = int(self.plan.ntiles)
+        if self._scores is None:
+            self._scores = torch.empty(ntot, dtype=torch.float32, device=self.device)
+        if not self.graph:
+            self._score_into(self._scores, stream)
+        else:
+            if self._graph is None:
+                self._score_into(self._scores, stream)
+                torch.cuda.synchronize()
+                g = torch.cuda.CUDAGraph()
+                s = torch.cuda.Stream()
+                s.wait_stream(torch.cuda.current_stream())
+                with torch.cuda.stream(s):
+                    with torch.cuda.graph(g, stream=s):
+                        self._score_into(self._scores, s)
+                torch.cuda.current_stream().wait_stream(s)
+                self._graph = g
+            self._graph.replay()
+        flat = self._scores.cpu().numpy()
         res, o = [], 0
-        for (_, n1, n2) in outs:
+        for (n1, n2) in self.plan.grids:
             res.append(flat[o:o + n1 * n2].reshape(n1, n2))
             o += n1 * n2
         return res

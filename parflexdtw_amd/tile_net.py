@@ -155,42 +155,49 @@ class TileNet:
               "pf_pool_features")
         return A, B, H, W, K
 
-    def input_from_pooled(self, A, B, H, W, K, stream=None, want_cost=False):
-        """Network input (H*W*32 bf16) from pooled features; also the f32
-        pooled cost (H*W) if asked."""
+    def input_from_pooled(self, A, B, H, W, K, stream=None, want_cost=False, compact=False):
+        """Network input from pooled features: H*W*32 bf16 (the layout the
+        training passes use) or, with compact=True, the one-channel H*W bf16
+        map the inference pass starts from; also the f32 pooled cost (H*W)
+        if asked."""
         torch = _torch()
-        X = torch.empty(H * W * CH, dtype=torch.bfloat16, device=self.device)
+        X = torch.empty(H * W * (1 if compact else CH), dtype=torch.bfloat16, device=self.device)
         n = ctypes.c_int64()
         check(lib().pf_pooled_cost_workspace(K, ctypes.byref(n)), "pf_pooled_cost_workspace")
         work = torch.empty(n.value, dtype=torch.float32, device=self.device)
         P = torch.empty(H * W, dtype=torch.float32, device=self.device) if want_cost else None
-        check(lib().pf_pooled_cost(A.data_ptr(), H, B.data_ptr(), W, K, X.data_ptr(),
+        check(lib().pf_pooled_cost(A.data_ptr(), H, B.data_ptr(), W, K,
+                                   None if compact else X.data_ptr(),
+                                   X.data_ptr() if compact else None,
                                    P.data_ptr() if P is not None else None, work.data_ptr(),
                                    _st(stream)), "pf_pooled_cost")
         return X, P
 
-    def pooled_input(self, F1d, F2d, stream=None, want_cost=False):
-        """Network input (H, W, 32) bf16 for one pair of device features
-        (K, N) / (K, M) float64; also the f32 pooled cost if asked."""
+    def pooled_input(self, F1d, F2d, stream=None, want_cost=False, compact=False):
+        """Network input for one pair of device features (K, N) / (K, M)
+        float64 (see input_from_pooled); also the f32 pooled cost if asked."""
         A, B, H, W, K = self.pooled_features(F1d, F2d, stream)
-        X, P = self.input_from_pooled(A, B, H, W, K, stream, want_cost)
+        X, P = self.input_from_pooled(A, B, H, W, K, stream, want_cost, compact)
         return (X, H, W, P) if want_cost else (X, H, W)
 
     # ---- forward / backward -------------------------------------------
     def forward(self, X, H, W, stream=None):
-        """Activations [X, A1, A2, A3] (bf16) and logits Z (f32, NHWC 32)."""
+        """Activations [X, A1, A2, A3] (bf16) and the logit map Z (f32 H*W).
+        X is the 32-channel input, or the one-channel compact input (then the
+        first layer reads channel 0 only)."""
         torch = _torch()
         st = _st(stream)
         acts = [X]
         A = X
+        first = 3 if X.numel() == H * W else 0
         for i in range(3):
             Y = torch.empty(H * W * CH, dtype=torch.bfloat16, device=self.device)
-            check(lib().pf_conv3x3(0, A.data_ptr(), H, W, self.Wt[i].data_ptr(),
-                                   self.layer(i)[CH * KDIM:].data_ptr(), None, Y.data_ptr(), st),
-                  "pf_conv3x3")
+            check(lib().pf_conv3x3(first if i == 0 else 0, A.data_ptr(), H, W,
+                                   self.Wt[i].data_ptr(), self.layer(i)[CH * KDIM:].data_ptr(),
+                                   None, Y.data_ptr(), st), "pf_conv3x3")
             acts.append(Y)
             A = Y
-        Z = torch.empty(H * W * CH, dtype=torch.float32, device=self.device)
+        Z = torch.empty(H * W, dtype=torch.float32, device=self.device)
         check(lib().pf_conv3x3(1, A.data_ptr(), H, W, self.Wt[3].data_ptr(),
                                self.layer(3)[CH * KDIM:].data_ptr(), None, Z.data_ptr(), st),
               "pf_conv3x3")
@@ -241,6 +248,6 @@ class TileNet:
 
     def score_pair(self, F1d, F2d, L, ext=0, stream=None):
         """Tile logits (n1*n2 device tensor) of one pair of device features."""
-        X, H, W = self.pooled_input(F1d, F2d, stream)
+        X, H, W = self.pooled_input(F1d, F2d, stream, compact=True)
         _, Z = self.forward(X, H, W, stream)
         return self.tile_scores(Z, H, W, L, int(F1d.shape[1]), int(F2d.shape[1]), ext, stream)

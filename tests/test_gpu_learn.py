@@ -48,6 +48,10 @@ def test_pooled_input_matches_oracle(env):
     ref = tn.network_input(Pref)
     assert np.all(np.abs(Xh[..., 0] - ref) <= 2 ** -8 * np.abs(ref) + 1e-4)
     assert not Xh[..., 1:].any()
+    Xc, _ = net.input_from_pooled(*net.pooled_features(torch.from_numpy(F1).cuda(),
+                                                       torch.from_numpy(F2).cuda()),
+                                  compact=True)
+    assert np.array_equal(Xc.float().cpu().numpy().reshape(H, W), Xh[..., 0])
 
 
 @pytest.mark.parametrize("H,W", [(13, 70), (1, 5), (40, 33)])
@@ -74,8 +78,8 @@ def test_conv_modes_match_oracle(env, H, W):
     G = torch.empty(H * W * 32, dtype=torch.bfloat16, device="cuda")
     check(L.pf_conv3x3(2, Xd.data_ptr(), H, W, Wfd.data_ptr(), None, Md.data_ptr(), G.data_ptr(),
                        st), "conv2")
-    Zh = Z.cpu().numpy().reshape(H, W, 32)
-    assert np.max(np.abs(Zh - ref)) <= tol
+    Zh = Z.cpu().numpy()[:H * W].reshape(H, W)        # mode 1: channel 0 only
+    assert np.max(np.abs(Zh - ref[..., 0])) <= tol
     Yh = Y.float().cpu().numpy().reshape(H, W, 32)
     relu = np.maximum(ref, 0)
     assert np.all(np.abs(Yh - relu) <= 2 ** -7 * np.abs(relu) + tol)
@@ -114,7 +118,7 @@ def test_bce_and_tile_scores_match_oracle(env):
     H, W = 37, 45
     Z = rng.standard_normal((H, W, 32)).astype(np.float32) * 3
     lab = (rng.random((H, W)) < 0.1).astype(np.uint8)
-    Zd = torch.from_numpy(Z).cuda()
+    Zd = torch.from_numpy(np.ascontiguousarray(Z[..., 0])).cuda()   # the logit map
     labd = torch.from_numpy(lab).cuda()
     dZ = torch.empty(H * W * 32, dtype=torch.bfloat16, device="cuda")
     loss = torch.zeros(1, dtype=torch.float32, device="cuda")
@@ -159,8 +163,15 @@ def test_network_gradient_matches_oracle(env):
     net.backward(acts, Z, H, W, torch.from_numpy(lab).cuda(), 4.0, 1.0 / (H * W))
     torch.cuda.synchronize()
     racts, rZ = tn.forward(X0, Ws, bs)
-    Zh = Z.cpu().numpy().reshape(H, W, 32)[..., 0]
+    Zh = Z.cpu().numpy().reshape(H, W)
     assert np.max(np.abs(Zh - rZ[..., 0])) <= 2e-2 * np.abs(rZ[..., 0]).max() + 1e-4
+    # inference path: compact one-channel input and the first-layer kernel
+    acts_c, Zc = net.forward(Xd.reshape(H, W, 32)[..., 0].contiguous().reshape(-1), H, W)
+    torch.cuda.synchronize()
+    assert np.max(np.abs(Zc.cpu().numpy().reshape(H, W) - Zh)) <= 1e-2 * np.abs(Zh).max() + 1e-3
+    A1 = acts_c[1].float().cpu().numpy().reshape(H, W, 32)
+    A1ref = acts[1].float().cpu().numpy().reshape(H, W, 32)
+    assert np.all(np.abs(A1 - A1ref) <= 2 ** -7 * np.abs(A1ref) + 1e-5)
     rloss, dWs, dbs = tn.backward(racts, rZ, Ws, lab, 4.0, 1.0 / (H * W))
     assert float(net.loss.item()) == pytest.approx(rloss, rel=1e-2)
     g = net.g.cpu().numpy().reshape(4, -1).astype(np.float64)

@@ -62,3 +62,22 @@ def test_training_lowers_the_loss(torch_gpu):
     data = build_dataset(net, pairs, paths)
     hist = train(net, data, epochs=12, lr=3e-3, batch=4, pos_weight=20.0)
     assert hist[-1] < 0.6 * hist[0], hist
+
+
+def test_scoring_graph_replays_equal_eager(torch_gpu):
+    """The scoring of all pairs replayed from its HIP graph equals the eager
+    pass, replay after replay, with eager work in between."""
+    torch = torch_gpu
+    from parflexdtw_amd.learned import LearnedRunner
+    from parflexdtw_amd.synth import batch_pairs
+    from parflexdtw_amd.tile_net import TileNet
+    net, meta = TileNet.load(MODEL)
+    pairs = [(a, b) for a, b, _ in batch_pairs(4, 96000, 5000, 9000)]
+    eager = LearnedRunner(pairs, 2000, STEPS, WEIGHTS, 0.1, net, 0.0, graph=False).score()
+    lg = LearnedRunner(pairs, 2000, STEPS, WEIGHTS, 0.1, net, 0.0, graph=True)
+    for r in range(3):
+        got = lg.score()
+        torch.randn(1 << 20, device="cuda").sum().item()   # unrelated eager work
+        lg._scores.fill_(-1.0)
+        for a, b in zip(eager, got):
+            assert np.array_equal(a, b), r
