@@ -53,7 +53,8 @@ sys.path.insert(0, ROOT)
 METRIC = "DP GCUPS + pairs/s, batched/learned vs plain GPU at >=99.5% path agreement"
 # HBM bytes per stage-1 launch measured with rocprofv3 PMC (FETCH_SIZE +
 # WRITE_SIZE, scaled by a 1 GiB copy); profiles/pmc_traffic_*.md hold the runs.
-TRAFFIC_BYTES = {("chroma_batch256_N3000-20000", 500): (83.97 + 12.84) * 1e9}
+TRAFFIC_BYTES = {("chroma_batch256_N3000-20000", 500): (83.97 + 12.84) * 1e9,
+                 ("chroma_learned128_N20000-60000_L2000", 2000): (46.70 + 6.21) * 1e9}
 HBM_PEAK_GBS = 8000.0
 STEPS, WEIGHTS, BETA = [(1, 1), (1, 2), (2, 1)], [2, 3, 3], 0.1
 WORKLOADS = ["chroma_learned128_N20000-60000_L2000", "chroma_batch256_N3000-20000",
@@ -200,6 +201,8 @@ def bench_learned(args, torch):
                     "ext_cells": int(meta["ext"]), "guard_frames": int(meta["guard"])}
     out["roofline"] = _roofline(st["cells_computed"], s1_ms, 2.25,
                                 "k_stage1<2,__hip_bfloat16,float,false,true>") if s1_ms else None
+    if out["roofline"] and P == 128 and (args.workload, L) in TRAFFIC_BYTES:
+        out["roofline"]["traffic"] = TRAFFIC_BYTES[(args.workload, L)]
     if not args.no_cpu:
         from oracle import oracle
         nthreads = args.cpu_threads

@@ -156,21 +156,35 @@ class LearnedRunner:
         self.cost_tile = (tr.value, tc.value)
         self._host = {}
         self.graph, self._graph, self._scores = bool(graph), None, None
+        self.nstreams, self._side = 4, None
 
     # ------------------------------------------------------------------
     def _score_into(self, out, stream=None):
         """Tile logits of every pair into the device buffer `out` (the
-        pairs' grids concatenated in i-major order)."""
+        pairs' grids concatenated in i-major order).  Pairs are spread over
+        side streams (forked from and joined back to `stream`) so that the
+        small, latency-bound kernels of one pair overlap the convolutions of
+        another."""
+        torch = _torch()
+        main = stream or torch.cuda.current_stream()
+        if self._side is None:
+            self._side = [torch.cuda.Stream() for _ in range(self.nstreams)]
+        for sd in self._side:
+            sd.wait_stream(main)
         o = 0
-        for (F1d, F2d) in self.Fp:
-            X, H, W = self.net.pooled_input(F1d, F2d, stream, compact=True)
-            _, Z = self.net.forward(X, H, W, stream)
-            L1, L2 = int(F1d.shape[1]), int(F2d.shape[1])
-            n1, n2 = -(-(L1 - 1) // (self.L - 1)), -(-(L2 - 1) // (self.L - 1))
-            check(lib().pf_tile_scores(Z.data_ptr(), H, W, 32, self.L, L1, L2, self.ext,
-                                       out[o:o + n1 * n2].data_ptr(), _stream(stream)),
-                  "pf_tile_scores")
+        for p, (F1d, F2d) in enumerate(self.Fp):
+            sd = self._side[p % len(self._side)]
+            with torch.cuda.stream(sd):
+                X, H, W = self.net.pooled_input(F1d, F2d, sd, compact=True)
+                _, Z = self.net.forward(X, H, W, sd)
+                L1, L2 = int(F1d.shape[1]), int(F2d.shape[1])
+                n1, n2 = -(-(L1 - 1) // (self.L - 1)), -(-(L2 - 1) // (self.L - 1))
+                check(lib().pf_tile_scores(Z.data_ptr(), H, W, 32, self.L, L1, L2, self.ext,
+                                           out[o:o + n1 * n2].data_ptr(), sd.cuda_stream),
+                      "pf_tile_scores")
             o += n1 * n2
+        for sd in self._side:
+            main.wait_stream(sd)
 
     def score(self, stream=None):
         """Tile logits of every pair, as one host array per pair (n1, n2).
