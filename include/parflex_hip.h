@@ -242,6 +242,15 @@ int pf_pack_weights(const float *d_w, void *d_Wt, void *d_Wf, void *stream);
 int pf_tile_scores(const float *d_Z, int64_t H, int64_t W, int pool, int64_t L, int64_t L1,
                    int64_t L2, int ext, float *d_score, void *stream);
 
+/* Selection check of the learned form: d_flag[p] |= 1 (int32, cleared by
+ * the caller) when a tile holding a cell within `guard` frames of a point of
+ * pair p's stitched path (d_path / d_path_len as written by pf_stage2, pair
+ * table as for pf_stage2) is not selected in d_sel (uint8 per tile of the
+ * tile table).  max_path >= every path length; 2*guard < L-1. */
+int pf_selection_check(const int64_t *d_pairs, int64_t npairs, int64_t max_path,
+                       const int64_t *d_path, const int64_t *d_path_len, const uint8_t *d_sel,
+                       int64_t L, int64_t guard, int32_t *d_flag, void *stream);
+
 /* Fill helpers used instead of hipMemsetAsync (safe under graph capture). */
 int pf_fill_f64(double *d_p, double v, int64_t n, void *stream);
 int pf_fill_i64(int64_t *d_p, int64_t v, int64_t n, void *stream);

@@ -506,6 +506,36 @@ __global__ __launch_bounds__(256) void k_tile_scores(const float *__restrict__ Z
   if (threadIdx.x == 0) score[t] = red[0];
 }
 
+// Selection check of the learned form: flag[p] = 1 when a tile that holds a
+// cell within `guard` frames (in row and column) of a point of pair p's
+// stitched path is not selected.  The tiles holding frames [x0, x1] are
+// max(x0-1, 0)/hop .. min(x1/hop, n-1) (a frame on an overlap belongs to two
+// tiles).  Grid: (point blocks, pairs).
+__global__ __launch_bounds__(256) void k_selection_check(const PairDesc *__restrict__ pairs,
+                                                         const int64_t *__restrict__ path,
+                                                         const int64_t *__restrict__ path_len,
+                                                         const uint8_t *__restrict__ sel, int64_t L,
+                                                         int64_t guard, int32_t *__restrict__ flag) {
+  const int64_t p = blockIdx.y;
+  const PairDesc pd = pairs[p];
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= path_len[p]) return;
+  const int64_t hop = L - 1;
+  const int64_t r = path[(pd.path_off + q) * 2], c = path[(pd.path_off + q) * 2 + 1];
+  const int64_t ra = r - guard - 1 > 0 ? r - guard - 1 : 0;
+  const int64_t rb = r + guard < pd.L1 - 1 ? r + guard : pd.L1 - 1;
+  const int64_t ca = c - guard - 1 > 0 ? c - guard - 1 : 0;
+  const int64_t cb = c + guard < pd.L2 - 1 ? c + guard : pd.L2 - 1;
+  const int64_t i0 = ra / hop < pd.n1 - 1 ? ra / hop : pd.n1 - 1;
+  const int64_t i1 = rb / hop < pd.n1 - 1 ? rb / hop : pd.n1 - 1;
+  const int64_t j0 = ca / hop < pd.n2 - 1 ? ca / hop : pd.n2 - 1;
+  const int64_t j1 = cb / hop < pd.n2 - 1 ? cb / hop : pd.n2 - 1;
+  bool bad = false;
+  for (int64_t i = i0; i <= i1; ++i)
+    for (int64_t j = j0; j <= j1; ++j) bad = bad || !sel[pd.tile_base + i * pd.n2 + j];
+  if (bad) atomicOr(&flag[p], 1);
+}
+
 inline unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace pf
@@ -642,6 +672,22 @@ extern "C" int pf_tile_scores(const float *d_Z, int64_t H, int64_t W, int pool, 
   const int64_t n1 = (L1 - 1 + hop - 1) / hop, n2 = (L2 - 1 + hop - 1) / hop;
   hipLaunchKernelGGL(k_tile_scores, dim3((unsigned)(n1 * n2)), dim3(256), 0, as_stream(stream), d_Z,
                      H, W, pool, L, L1, L2, n2, ext, d_score);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_selection_check(const int64_t *d_pairs, int64_t npairs, int64_t max_path,
+                                  const int64_t *d_path, const int64_t *d_path_len,
+                                  const uint8_t *d_sel, int64_t L, int64_t guard, int32_t *d_flag,
+                                  void *stream) {
+  PF_REQUIRE(d_pairs && d_path && d_path_len && d_sel && d_flag && npairs >= 1 && L >= 2 &&
+                 guard >= 0 && 2 * guard < L - 1 && max_path >= 0,
+             "pf_selection_check: bad arguments");
+  PF_REQUIRE(npairs <= 65535, "pf_selection_check: too many pairs for one launch");
+  if (max_path == 0) return PF_OK;
+  hipLaunchKernelGGL(k_selection_check, dim3(nb256(max_path), (unsigned)npairs), dim3(256), 0,
+                     as_stream(stream), reinterpret_cast<const PairDesc *>(d_pairs), d_path,
+                     d_path_len, d_sel, L, guard, d_flag);
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
 }

@@ -323,13 +323,23 @@ class LearnedRunner:
             mark("stage1")
         engine.stage2(None, plan, self.sw, ws, stream)
         mark("stage2")
+        # selection check on the device: a pruned tile near a path
+        d_sel = torch.from_numpy(sel.astype(np.uint8)).to(dev)
+        flag = torch.zeros(len(self.pairs), dtype=torch.int32, device=dev)
+        max_path = max(a + b for a, b in self.shapes) + 8
+        check(lib().pf_selection_check(ws["pairs"].data_ptr(), len(self.pairs), max_path,
+                                       ws["path"].data_ptr(), ws["path_len"].data_ptr(),
+                                       d_sel.data_ptr(), L, self.guard, flag.data_ptr(), st),
+              "pf_selection_check")
         res = engine.collect_pinned(plan, ws, self._host)
+        edge = flag.cpu().numpy()
         mark("collect")
-        # fallback decisions on the host
         fb, why = [], {}
         for p in range(len(self.pairs)):
-            reason = needs_fallback(res.paths[p], bool(res.found[p]), float(res.best_cost[p]),
-                                    sels[p], L, self.cost_bound, self.guard, *self.shapes[p])
+            bc = float(res.best_cost[p])
+            reason = ("no endpoint" if not res.found[p] or not np.isfinite(bc) else
+                      "cost bound" if bc > self.cost_bound else
+                      "selection edge" if edge[p] else None)
             if reason:
                 fb.append(p)
                 why[reason] = why.get(reason, 0) + 1

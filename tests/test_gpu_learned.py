@@ -81,3 +81,37 @@ def test_scoring_graph_replays_equal_eager(torch_gpu):
         lg._scores.fill_(-1.0)
         for a, b in zip(eager, got):
             assert np.array_equal(a, b), r
+
+
+def test_selection_check_kernel_matches_host_rule(torch_gpu):
+    """pf_selection_check flags a pair exactly when a tile within `guard`
+    frames of its stitched path is unselected (host rule: learned.path_tiles)."""
+    torch = torch_gpu
+    from parflexdtw_amd._lib import check, lib
+    from parflexdtw_amd.engine import BatchRunner
+    from parflexdtw_amd.learned import path_tiles
+    from parflexdtw_amd.synth import batch_pairs
+    pairs = [(a, b) for a, b, _ in batch_pairs(5, 97000, 1500, 4000)]
+    L = 300
+    br = BatchRunner(pairs, L, STEPS, WEIGHTS, 0.1, torch.bfloat16)
+    br.launch()
+    res = br.collect()
+    rng = np.random.default_rng(0)
+    for trial in range(6):
+        guard = [0, 5, 40, 100, 149, 7][trial]
+        sel = rng.random(br.plan.ntiles) < [0.97, 0.9, 0.99, 0.8, 0.995, 1.0][trial]
+        flag = torch.zeros(len(pairs), dtype=torch.int32, device="cuda")
+        d_sel = torch.from_numpy(sel.astype(np.uint8)).cuda()
+        mp = max(F1.shape[1] + F2.shape[1] for F1, F2 in pairs) + 8
+        check(lib().pf_selection_check(br.ws["pairs"].data_ptr(), len(pairs), mp,
+                                       br.ws["path"].data_ptr(), br.ws["path_len"].data_ptr(),
+                                       d_sel.data_ptr(), L, guard, flag.data_ptr(),
+                                       torch.cuda.current_stream().cuda_stream), "check")
+        got = flag.cpu().numpy()
+        for p, (F1, F2) in enumerate(pairs):
+            n1, n2 = br.plan.grids[p]
+            base = int(br.plan.pairs[p, 4])
+            s = sel[base:base + n1 * n2]
+            want = not s[path_tiles(res.paths[p], L, n1, n2, guard, F1.shape[1],
+                                    F2.shape[1])].all()
+            assert bool(got[p]) == want, (trial, p)
