@@ -205,19 +205,20 @@ __global__ void k_write_input(const float *__restrict__ A, int64_t H, const floa
 }
 
 // 3x3 convolution, 32 -> 32 channels.  A workgroup of 4 waves computes a
-// block of kCvRows x kCvCols output pixels; each wave takes kCvRows/4 rows
-// as MFMA tiles of 32 pixels x 32 channels sharing the weight operand.  The
-// input halo of the block and the layer's weights are staged in LDS once per
-// block.  4 x 32 (35 KB of LDS, 4 workgroups per CU) measured faster than
-// 8 x 64 (72 KB, 2 per CU).
+// block of 16 rows x 32 columns of output pixels; wave w takes rows
+// 4w .. 4w+3 (four MFMA tiles of 32 pixels x 32 channels).  The 18 x 34
+// input halo of the block is staged in LDS once; each wave keeps its 18
+// weight fragments in registers and reads every input row of its window
+// once per (dx, channel half), feeding the up to three output rows that use
+// it (dy = input row - output row), so an LDS read serves 2 MFMAs on
+// average instead of 1 and no weight is read from LDS.
 // MODE 0: Y = relu(acc + bias) as bf16 (hidden layer).
 // MODE 1: channel 0 of acc + bias as f32, H*W (the logit map).
 // MODE 2: Y = (M > 0) ? acc : 0 as bf16 (gradient through the ReLU of the
 //         layer input M, which is that layer's output activation).
-constexpr int kCvRows = 4, kCvCols = 32;
-constexpr int kTR = kCvRows / 4, kTC = kCvCols / 32, kTiles = kTR * kTC;  // per wave
+constexpr int kRowsPerWave = 4;
+constexpr int kCvRows = 4 * kRowsPerWave, kCvCols = 32;
 constexpr int kPixPad = kCh + 8;      // bf16 per staged pixel (16-byte aligned rows)
-constexpr int kWPad = kKdim + 8;      // bf16 per staged weight row
 typedef unsigned int lu32x4_t __attribute__((ext_vector_type(4)));
 
 template <int MODE>
@@ -228,17 +229,17 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __hip_bfloat16 *__restric
                                                  void *__restrict__ Yv) {
   constexpr int HR = kCvRows + 2, HC = kCvCols + 2;
   __shared__ __attribute__((aligned(16))) __hip_bfloat16 xs[HR * HC * kPixPad];
-  __shared__ __attribute__((aligned(16))) __hip_bfloat16 wsm[kCh * kWPad];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int64_t y0 = (int64_t)blockIdx.y * kCvRows, x0 = (int64_t)blockIdx.x * kCvCols;
-  // weights: 32 rows x 36 chunks of 8 bf16
-  for (int q = tid; q < kCh * (kKdim / 8); q += 256) {
-    const int o = q / (kKdim / 8), c8 = q - o * (kKdim / 8);
-    *reinterpret_cast<lu32x4_t *>(&wsm[o * kWPad + c8 * 8]) =
-        *reinterpret_cast<const lu32x4_t *>(&Wt[o * kKdim + c8 * 8]);
-  }
+  const int px = lane & 31;
+  const int kh = (lane >> 5) * 8;
+  // weight fragments of this lane: column px (output channel), k = s*16 + kh .. +7
+  lbf16x8_t wb[18];
+#pragma unroll
+  for (int s = 0; s < 18; ++s)
+    wb[s] = *reinterpret_cast<const lbf16x8_t *>(&Wt[px * kKdim + s * 16 + kh]);
   // input halo: HR x HC pixels x 4 chunks of 8 channels, zero outside
   for (int q = tid; q < HR * HC * 4; q += 256) {
     const int pix = q >> 2, c8 = q & 3;
@@ -251,44 +252,70 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __hip_bfloat16 *__restric
         ok ? v : lu32x4_t{0u, 0u, 0u, 0u};
   }
   __syncthreads();
-  const int px = lane & 31;
-  const int kh = (lane >> 5) * 8;
-  lfloatx16_t acc[kTiles];
+  lfloatx16_t acc[kRowsPerWave];
 #pragma unroll
-  for (int t = 0; t < kTiles; ++t) acc[t] = lfloatx16_t{};
+  for (int t = 0; t < kRowsPerWave; ++t) acc[t] = lfloatx16_t{};
+  const int r0 = kRowsPerWave * wave;   // first output row of the wave, halo row r0 + dy
 #pragma unroll
-  for (int s = 0; s < 18; ++s) {
-    const int dydx = s >> 1, dy = dydx / 3, dx = dydx - dy * 3;
-    const int cin0 = (s & 1) * 16 + kh;
-    const lbf16x8_t b = *reinterpret_cast<const lbf16x8_t *>(&wsm[px * kWPad + s * 16 + kh]);
+  for (int ch = 0; ch < 2; ++ch)
 #pragma unroll
-    for (int t = 0; t < kTiles; ++t) {
-      const int rr = kTR * wave + t / kTC, cc = (t % kTC) * 32;
-      const lbf16x8_t a = *reinterpret_cast<const lbf16x8_t *>(
-          &xs[((rr + dy) * HC + cc + px + dx) * kPixPad + cin0]);
-      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[t], 0, 0, 0);
-    }
-  }
+    for (int xr = 0; xr < kRowsPerWave + 2; ++xr)
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx) {
+        const lbf16x8_t a = *reinterpret_cast<const lbf16x8_t *>(
+            &xs[((r0 + xr) * HC + px + dx) * kPixPad + ch * 16 + kh]);
+#pragma unroll
+        for (int t = 0; t < kRowsPerWave; ++t) {
+          const int dy = xr - t;
+          if (dy >= 0 && dy <= 2)
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, wb[(dy * 3 + dx) * 2 + ch],
+                                                             acc[t], 0, 0, 0);
+        }
+      }
   const int co = lane & 31;
   const float bv = (MODE == 2) ? 0.0f : bias[co];
+  if constexpr (MODE == 1) {
 #pragma unroll
-  for (int t = 0; t < kTiles; ++t) {
-    const int64_t y = y0 + kTR * wave + t / kTC;
-    if (y >= H) continue;
+    for (int t = 0; t < kRowsPerWave; ++t) {
+      const int64_t y = y0 + r0 + t;
+      if (y >= H || co != 0) continue;
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      const int64_t x = x0 + (t % kTC) * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-      if (x >= W) continue;
-      const int64_t q = y * W + x;
-      const float r = acc[t][v] + bv;
-      if constexpr (MODE == 0) {
-        static_cast<__hip_bfloat16 *>(Yv)[q * kCh + co] = __float2bfloat16(r > 0.0f ? r : 0.0f);
-      } else if constexpr (MODE == 1) {
-        if (co == 0) static_cast<float *>(Yv)[q] = r;   // channel 0 only: the logit map
-      } else {
-        const float m = bf2f(Mk[q * kCh + co]);
-        static_cast<__hip_bfloat16 *>(Yv)[q * kCh + co] = __float2bfloat16(m > 0.0f ? r : 0.0f);
+      for (int v = 0; v < 16; ++v) {
+        const int64_t x = x0 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+        if (x < W) static_cast<float *>(Yv)[y * W + x] = acc[t][v] + bv;   // the logit map
       }
+    }
+  } else {
+    // Stage the 16 x 32 x 32 output block (bf16) in LDS, then write it as
+    // contiguous 16-byte chunks (a block row is 2 KB contiguous in NHWC).
+    __syncthreads();   // every wave is done reading the halo
+    __hip_bfloat16 *ys = xs;
+#pragma unroll
+    for (int t = 0; t < kRowsPerWave; ++t) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int xl = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+        float r = acc[t][v] + bv;
+        if constexpr (MODE == 2) {
+          const int64_t y = y0 + r0 + t, x = x0 + xl;
+          const bool in = y < H && x < W;
+          const float m = bf2f(Mk[(in ? y * W + x : 0) * kCh + co]);
+          r = (in && m > 0.0f) ? r : 0.0f;
+        } else {
+          r = r > 0.0f ? r : 0.0f;
+        }
+        ys[((r0 + t) * kCvCols + xl) * kCh + co] = __float2bfloat16(r);
+      }
+    }
+    __syncthreads();
+    __hip_bfloat16 *Y = static_cast<__hip_bfloat16 *>(Yv);
+    for (int q = tid; q < kCvRows * kCvCols * (kCh / 8); q += 256) {
+      const int pix = q >> 2, c8 = q & 3;
+      const int rr = pix / kCvCols, xl = pix - rr * kCvCols;
+      const int64_t y = y0 + rr, x = x0 + xl;
+      if (y < H && x < W)
+        *reinterpret_cast<lu32x4_t *>(&Y[(y * W + x) * kCh + c8 * 8]) =
+            *reinterpret_cast<const lu32x4_t *>(&ys[pix * kCh + c8 * 8]);
     }
   }
 }
@@ -316,7 +343,6 @@ __global__ __launch_bounds__(256) void k_conv3x3_first(const __hip_bfloat16 *__r
     xs[q] = ok ? v : __float2bfloat16(0.0f);
   }
   __syncthreads();
-  const int64_t y = y0 + wave;
   const int px = lane & 31, kh = (lane >> 5) * 8;
   lbf16x8_t a, b;
 #pragma unroll
@@ -329,15 +355,25 @@ __global__ __launch_bounds__(256) void k_conv3x3_first(const __hip_bfloat16 *__r
     b[t] = k < 9 ? (__bf16)bf2f(wv) : (__bf16)0.0f;
   }
   lfloatx16_t acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, lfloatx16_t{}, 0, 0, 0);
-  if (y >= H) return;
+  // stage the 4 x 32 x 32 output block in LDS, then write 16-byte chunks
+  __shared__ __attribute__((aligned(16))) __hip_bfloat16 ys[kC1Rows * kC1Cols * kCh];
   const int co = lane & 31;
   const float bv = bias[co];
 #pragma unroll
   for (int v = 0; v < 16; ++v) {
-    const int64_t x = x0 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-    if (x >= W) continue;
+    const int xl = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
     const float r = acc[v] + bv;
-    Y[(y * W + x) * kCh + co] = __float2bfloat16(r > 0.0f ? r : 0.0f);
+    ys[(wave * kC1Cols + xl) * kCh + co] = __float2bfloat16(r > 0.0f ? r : 0.0f);
+  }
+  __syncthreads();
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  for (int q = tid; q < kC1Rows * kC1Cols * (kCh / 8); q += 256) {
+    const int pix = q >> 2, c8 = q & 3;
+    const int rr = pix / kC1Cols, xl = pix - rr * kC1Cols;
+    const int64_t yy = y0 + rr, x = x0 + xl;
+    if (yy < H && x < W)
+      *reinterpret_cast<u32x4 *>(&Y[(yy * W + x) * kCh + c8 * 8]) =
+          *reinterpret_cast<const u32x4 *>(&ys[pix * kCh + c8 * 8]);
   }
 }
 
