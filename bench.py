@@ -7,17 +7,19 @@ Workloads (--workload):
       trimmed ends), tile length 2000.  The learned form: the tile network
       (parflexdtw_amd/data/tile_net_L2000.npz, trained on 10k other pairs)
       scores every tile from the 1/32-pooled cost, only the selected tiles
-      run the BF16 cost build and the FP32 DP, stage 2 (FP64) runs on the
-      full grid, and a pair whose path reaches the edge of its selection
+      run stage 1 (FP32 DP, the BF16 MFMA cost computed inside the sweep
+      from the features; --cost materialised builds it first), stage 2
+      (FP64) runs on the full grid, and a pair whose path reaches the edge of its selection
       (or exceeds the calibrated cost bound) is redone by the full-tile
       batched form.  One step = features (in HBM) -> paths on the host for
       all pairs.  Beside it: the plain FP64 drop-in (parflex per pair, cost
       build included) on every pair, which is the judge of agreement.
   chroma_batch256_N3000-20000: 256 seeded chroma pairs (K = 12),
       N ~ U[3000, 20000], M = N * U[0.8, 1.25], trimmed ends, so every pair
-      has ragged edge tiles.  The batched form: all 256 cost matrices built
-      on the MFMA units in BF16, one stage-1 launch over every tile of every
-      pair (FP32 DP), one stage-2 pass (FP64 scan), captured as HIP graphs;
+      has ragged edge tiles.  The batched form: one stage-1 launch over every
+      tile of every pair (FP32 DP, the BF16 MFMA cost computed inside the
+      sweep; --cost materialised builds all 256 cost matrices first), one
+      stage-2 pass (FP64 scan), captured as HIP graphs;
       the tuner picks L in {500, 1000, 2000, 4000} and the stage-1 launch
       shape.  One step = features (resident in HBM) -> cost -> stage 1 ->
       stage 2 -> best costs and paths on the host for all 256 pairs.
@@ -53,8 +55,9 @@ sys.path.insert(0, ROOT)
 METRIC = "DP GCUPS + pairs/s, batched/learned vs plain GPU at >=99.5% path agreement"
 # HBM bytes per stage-1 launch measured with rocprofv3 PMC (FETCH_SIZE +
 # WRITE_SIZE, scaled by a 1 GiB copy); profiles/pmc_traffic_*.md hold the runs.
-TRAFFIC_BYTES = {("chroma_batch256_N3000-20000", 500): (83.97 + 12.84) * 1e9,
-                 ("chroma_learned128_N20000-60000_L2000", 2000): (46.70 + 6.21) * 1e9}
+TRAFFIC_BYTES = {("chroma_batch256_N3000-20000", 500, "materialised"): (83.97 + 12.84) * 1e9,
+                 ("chroma_learned128_N20000-60000_L2000", 2000, "materialised"):
+                     (46.70 + 6.21) * 1e9}
 HBM_PEAK_GBS = 8000.0
 STEPS, WEIGHTS, BETA = [(1, 1), (1, 2), (2, 1)], [2, 3, 3], 0.1
 WORKLOADS = ["chroma_learned128_N20000-60000_L2000", "chroma_batch256_N3000-20000",
@@ -74,6 +77,9 @@ def _parse():
                     help="tile lengths the batched workload's tuner tries")
     ap.add_argument("--plain-sample", type=int, default=16,
                     help="pairs of the batch also run through the plain FP64 form")
+    ap.add_argument("--cost", choices=["fused", "materialised"], default="fused",
+                    help="precision forms: compute C inside stage 1 from BF16 features (fused) "
+                         "or build the BF16 cost matrix first (materialised)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-trials", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -128,7 +134,8 @@ def bench_learned(args, torch):
         f"network trained on {int(meta['train_pairs'])} pairs, threshold "
         f"{float(meta['threshold']):.3f}")
     lr = LearnedRunner(pairs, L, STEPS, WEIGHTS, BETA, net, float(meta["threshold"]),
-                       int(meta["ext"]), int(meta["guard"]), float(meta["cost_bound"]))
+                       int(meta["ext"]), int(meta["guard"]), float(meta["cost_bound"]),
+                       fused=args.cost == "fused")
     for _ in range(args.warmup):
         res, st = lr.run()
     torch.cuda.synchronize()
@@ -174,8 +181,8 @@ def bench_learned(args, torch):
                 {"workload": args.workload, "pairs": P, "N": "U[20000,60000]",
                  "M": "N*U[0.8,1.25], 0-10% trimmed each end", "K": 12, "L": L,
                  "tiles": st["tiles"], "cells": cells, "steps": STEPS, "weights": WEIGHTS,
-                 "beta": BETA, "form": "learned tile proposal + batched BF16/FP32 pipeline, "
-                                       "fallback to the plain method"})
+                 "beta": BETA, "form": "learned tile proposal + batched BF16/FP32 pipeline "
+                                       f"({args.cost} cost), fallback to the plain method"})
     out["data"] = ("synthetic (seeded chroma pairs with known time warps, held out from the "
                    "network's training seeds; features in HBM)")
     out["speedup_vs_plain"] = round(pairs_s / plain_pairs_s, 3)
@@ -199,10 +206,13 @@ def bench_learned(args, torch):
                     "train_N": f"U[{int(meta['n_lo'])},{int(meta['n_hi'])}]",
                     "epochs": int(meta["epochs_done"]), "threshold": float(meta["threshold"]),
                     "ext_cells": int(meta["ext"]), "guard_frames": int(meta["guard"])}
-    out["roofline"] = _roofline(st["cells_computed"], s1_ms, 2.25,
-                                "k_stage1<2,__hip_bfloat16,float,false,true>") if s1_ms else None
-    if out["roofline"] and P == 128 and (args.workload, L) in TRAFFIC_BYTES:
-        out["roofline"]["traffic"] = TRAFFIC_BYTES[(args.workload, L)]
+    fused = args.cost == "fused"
+    out["roofline"] = _roofline(
+        st["cells_computed"], s1_ms, 0.25 if fused else 2.25,
+        f"k_stage1<2,__hip_bfloat16,float,false,true{',true' if fused else ''}>") if s1_ms else None
+    key = (args.workload, L, args.cost)
+    if out["roofline"] and P == 128 and key in TRAFFIC_BYTES:
+        out["roofline"]["traffic"] = TRAFFIC_BYTES[key]
     if not args.no_cpu:
         from oracle import oracle
         nthreads = args.cpu_threads
@@ -236,7 +246,8 @@ def bench_batch(args, torch):
     log(f"{len(pairs)} pairs generated in {time.perf_counter() - t0:.1f} s; "
         f"{sum(a * b for a, b in shapes) / 1e9:.2f} Gcells")
     t0 = time.perf_counter()
-    br, report = engine.tune_batch(pairs, args.L, STEPS, WEIGHTS, BETA, torch.bfloat16, log=log)
+    br, report = engine.tune_batch(pairs, args.L, STEPS, WEIGHTS, BETA, torch.bfloat16, log=log,
+                                   fused=args.cost == "fused")
     L = br.plan.L
     log(f"tuned in {time.perf_counter() - t0:.1f} s: L={L} config={br.ws['s1_config']}")
     br.capture()
@@ -297,7 +308,9 @@ def bench_batch(args, torch):
                  "L_tried": sorted(report), "stage1_shape": list(br.ws["s1_config"]),
                  "tiles": br.plan.ntiles, "cells": cells, "steps": STEPS, "weights": WEIGHTS,
                  "beta": BETA,
-                 "form": "batched: BF16 MFMA cost, FP32 DP, FP64 stage 2, HIP graphs"})
+                 "form": ("batched: BF16 MFMA cost computed inside stage 1 (fused), FP32 DP, "
+                          "FP64 stage 2, HIP graphs" if args.cost == "fused" else
+                          "batched: BF16 MFMA cost matrix, FP32 DP, FP64 stage 2, HIP graphs")})
     out["data"] = "synthetic (seeded chroma pairs with known time warps, features in HBM)"
     out["dp_gcups"] = round(cells / (s1_ms * 1e-3) / 1e9, 3)
     out["gcups_end_to_end"] = round(cells / (ms_step * 1e-3) / 1e9, 3)
@@ -314,9 +327,12 @@ def bench_batch(args, torch):
                         "frames_within_2_mean": round(float(np.mean(agree)), 6),
                         "best_cost_rel_err_max": float(max(rel_err)),
                         "pairs_compared": ns, "judge": "plain FP64 path, same pair and L"}
-    out["roofline"] = _roofline(cells, s1_ms, 2.25, "k_stage1<2,__hip_bfloat16,float,false,true>")
-    if args.pairs == 256 and (args.workload, L) in TRAFFIC_BYTES:
-        out["roofline"]["traffic"] = TRAFFIC_BYTES[(args.workload, L)]
+    fused = args.cost == "fused"
+    out["roofline"] = _roofline(
+        cells, s1_ms, 0.25 if fused else 2.25,
+        f"k_stage1<2,__hip_bfloat16,float,false,true{',true' if fused else ''}>")
+    if args.pairs == 256 and (args.workload, L, args.cost) in TRAFFIC_BYTES:
+        out["roofline"]["traffic"] = TRAFFIC_BYTES[(args.workload, L, args.cost)]
 
     if not args.no_cpu:
         from oracle import oracle

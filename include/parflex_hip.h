@@ -81,6 +81,26 @@ int pf_stage1(int dtype, const void *d_C, const int64_t *d_tiles, int64_t ntiles
               void *d_xch, uint32_t *d_Bpacked, double *d_edge_D, int32_t *d_edge_P,
               double *d_edge_C, void *stream);
 
+/* Stage 1 computing the cost inside the sweep from features (the fused
+ * form of tiled_stage1_from_features, Parflex_gpu.py:610-653): C = 1 -
+ * F1n^T F2n is never stored; every 32 rows each wave computes its block of C
+ * on the MFMA units from d_F1b / d_F2b (frames x Kp bf16, the L2-normalised
+ * features from pf_features_bf16), with the operands and rounding of
+ * pf_cost(PF_BF16), so the results equal pf_stage1(PF_BF16) on that cost.
+ * Tile table as for pf_stage1 except columns 0 and 1, which hold the frame
+ * (row in d_F1b / d_F2b) of the tile's first row and first column.  FP32 DP;
+ * 2 or 4 columns per lane (strips*waves*64*4 >= Lmax).  Other arguments as
+ * for pf_stage1 with dtype PF_BF16. */
+int pf_stage1_features(const void *d_F1b, const void *d_F2b, int Kp, const int64_t *d_tiles,
+                       int64_t ntiles, int64_t Lmax, const int32_t *steps, const double *weights,
+                       int K, int strips, int waves, int rows_per_block, int32_t *d_flags,
+                       void *d_xch, uint32_t *d_Bpacked, double *d_edge_D, int32_t *d_edge_P,
+                       double *d_edge_C, void *stream);
+
+/* L2-normalised features as frames x Kp bf16 (Kp = K rounded up to 16, zero
+ * padded) from F (K x n f64, FlexDTW.py:1207-1210 normalisation). */
+int pf_features_bf16(const double *d_F, int64_t K, int64_t n, void *d_out, void *stream);
+
 /* Sizes of the strip-exchange workspace of pf_stage1: int32 count of
  * d_flags and byte size of d_xch (both 0 when strips == 1). */
 int pf_stage1_workspace(int dtype, int64_t ntiles, int64_t Lmax, int strips,

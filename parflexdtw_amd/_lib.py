@@ -34,6 +34,9 @@ _SIGNATURES = {
     "pf_stage1": (_int, [_int, _vp, _vp, _i64, _i64, _vp, _vp, _int, _int, _int, _int, _vp, _vp,
                          _vp, _vp, _vp, _vp, _vp]),
     "pf_stage1_workspace": (_int, [_int, _i64, _i64, _int, _vp, _vp]),
+    "pf_stage1_features": (_int, [_vp, _vp, _int, _vp, _i64, _i64, _vp, _vp, _int, _int, _int,
+                                  _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pf_features_bf16": (_int, [_vp, _i64, _i64, _vp, _vp]),
     "pf_tile_full": (_int, [_vp, _i64, _i64, _i64, _i64, _i64, _vp, _vp, _int, _vp, _vp, _vp,
                             _vp]),
     "pf_tile_endpoint": (_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp]),

@@ -332,3 +332,15 @@ extern "C" int pf_cost_batch(int out_dtype, const double *d_F1, int64_t n1_total
                            reinterpret_cast<const CostJob *>(d_jobs), njobs, total_blocks, d_C,
                            d_work, as_stream(stream));
 }
+
+extern "C" int pf_features_bf16(const double *d_F, int64_t K, int64_t n, void *d_out,
+                                void *stream) {
+  PF_REQUIRE(d_F && d_out && K >= 1 && n >= 1, "pf_features_bf16: bad arguments");
+  PF_REQUIRE(reinterpret_cast<uintptr_t>(d_out) % 16 == 0,
+             "pf_features_bf16: output must be 16-byte aligned");
+  const int64_t Kp = (K + 15) / 16 * 16;
+  hipLaunchKernelGGL(k_cost_prep_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     as_stream(stream), d_F, K, n, Kp, static_cast<__hip_bfloat16 *>(d_out));
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}

@@ -150,14 +150,30 @@ struct MaxThreads {
   static constexpr int value = (FULL || CPL <= 2) ? 1024 : (CPL == 4 ? 512 : 256);
 };
 
-template <int CPL, typename CT, typename DT, bool FULL, bool STD>
+typedef __bf16 s1bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float s1floatx16_t __attribute__((ext_vector_type(16)));
+
+// FEAT: the cost is not read from HBM but computed inside the sweep from
+// BF16 frame-major features (Fa: rows' frames, Fb: columns' frames, Kp
+// padded to 16; tile table columns 0/1 hold the tile's first row / column
+// frame).  Every 32 rows each wave computes the 32 x (64*CPL) block of C of
+// its columns on v_mfma_f32_32x32x16_bf16 (the operand layout and rounding
+// of the MFMA cost kernel, so the values are those of the BF16 cost matrix)
+// into a wave-private LDS block, which the row sweep then reads.  One block
+// buffer suffices: the C prefetch kPF rows ahead finishes reading block b
+// one row before it starts reading block b+1, and block b+1 is computed in
+// between.
+template <int CPL, typename CT, typename DT, bool FULL, bool STD, bool FEAT = false>
 __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
     const CT *__restrict__ Cb, const TileDesc *__restrict__ tiles, TileDesc td0, StepSet ss,
     int64_t Lmax, int S, int G, int *__restrict__ flags, Xch<DT> *__restrict__ xch,
     uint32_t *__restrict__ Bp, double *__restrict__ eD, int32_t *__restrict__ eP,
     double *__restrict__ eC, double *__restrict__ Df, int32_t *__restrict__ Pf,
-    int8_t *__restrict__ Bf) {
+    int8_t *__restrict__ Bf, const __hip_bfloat16 *__restrict__ Fa,
+    const __hip_bfloat16 *__restrict__ Fb, int Kp) {
   static_assert(CPL >= 2, "a lane needs two columns of left context");
+  static_assert(!FEAT || (!FULL && sizeof(DT) == 4), "features form: packed FP32 DP only");
+  extern __shared__ __attribute__((aligned(16))) __hip_bfloat16 cdyn[];
   __shared__ DT shD[kRing][16][2];
   __shared__ DT shX[kRing][16][2];
   __shared__ int prog[16];
@@ -192,7 +208,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
   const Xch<DT> *xin = consumer ? xch + (t * S + s - 1) * Lmax : nullptr;
   Xch<DT> *xout = producer ? xch + (t * S + s) * Lmax : nullptr;
 
-  const CT *__restrict__ C = Cb + td.c_off;
+  const CT *__restrict__ C = Cb + (FEAT ? 0 : td.c_off);
   const int64_t ldc = td.ldc;
   const int c0 = cbeg + tid * CPL;
   const int jlast = W - 1 - c0;  // this lane's index of the tile's last column
@@ -215,11 +231,51 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
 #pragma unroll
   for (int j = 0; j < CPL; ++j) cl[j] = (c0 + j < W) ? c0 + j : W - 1;
 
+  // features form: this wave's block of C, 32 rows x (64*CPL) columns
+  constexpr int kWC = 64 * CPL;
+  __hip_bfloat16 *cblk = cdyn + (FEAT ? (int64_t)(tid >> 6) * 32 * kWC : 0);
+  auto fill_block = [&](const int b) {
+    if constexpr (FEAT) {
+      const int lr = lane & 31, kh = (lane >> 5) * 8;
+      const int ar = (32 * b + lr < R) ? 32 * b + lr : R - 1;
+      const __hip_bfloat16 *ap = Fa + (td.c_off + ar) * (int64_t)Kp + kh;
+      const int wc0 = cbeg + (tid >> 6) * kWC;
+#pragma unroll
+      for (int cb = 0; cb < kWC / 32; ++cb) {
+        const int gc = (wc0 + cb * 32 + lr < W) ? wc0 + cb * 32 + lr : W - 1;
+        const __hip_bfloat16 *bp = Fb + (td.ldc + gc) * (int64_t)Kp + kh;
+        s1floatx16_t acc = s1floatx16_t{};
+        for (int kk = 0; kk < Kp; kk += 16) {
+          const s1bf16x8_t a = *reinterpret_cast<const s1bf16x8_t *>(ap + kk);
+          const s1bf16x8_t bb = *reinterpret_cast<const s1bf16x8_t *>(bp + kk);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 16; ++v)
+          cblk[((v >> 2) * 8 + (lane >> 5) * 4 + (v & 3)) * kWC + cb * 32 + lr] =
+              __float2bfloat16(1.0f - acc[v]);
+      }
+      // the block is read by other lanes of this wave
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  };
+  // C of row `row`, lane slot j (a column clamped into the tile)
+  auto c_at = [&](const int row, const int j) -> DT {
+    if constexpr (FEAT) {
+      return (DT)__bfloat162float(cblk[(row & 31) * kWC + lane * CPL + j]);
+    } else {
+      return load_c<DT>(C + (int64_t)row * ldc + cl[j]);
+    }
+  };
+
   // Row 0: boundary cells, P = 0, X = +col.
+  fill_block(0);
 #pragma unroll
   for (int j = 0; j < CPL; ++j) {
     const int c = c0 + j;
-    const DT v = load_c<DT>(C + cl[j]);
+    const DT v = c_at(0, j);
     D1[j] = (c < W) ? v : INF;
     X1[j] = (DT)c;
     D2[j] = INF;
@@ -250,7 +306,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
   for (int u = 0; u < kPF; ++u) {
     const int row = (1 + u < R) ? 1 + u : R - 1;
 #pragma unroll
-    for (int j = 0; j < CPL; ++j) Cq[u][j] = load_c<DT>(C + (int64_t)row * ldc + cl[j]);
+    for (int j = 0; j < CPL; ++j) Cq[u][j] = c_at(row, j);
   }
   if (producer) {
     xout[0] = Xch<DT>{D1[CPL - 2], D1[CPL - 1], X1[CPL - 2], X1[CPL - 1]};
@@ -582,9 +638,12 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
       // Refill the slot only after its row is done, so the load targets the
       // same registers (no copy of an in-flight load at the loop edge).
       const int rp = (r + kPF < R) ? r + kPF : R - 1;  // clamped: always a valid row
-      const CT *row = C + (int64_t)rp * ldc;
+      if constexpr (FEAT) {
+        // rows of the previous block were all read by earlier refills
+        if ((rp & 31) == 0 && rp == r + kPF) fill_block(rp >> 5);
+      }
 #pragma unroll
-      for (int j = 0; j < CPL; ++j) Cq[u][j] = load_c<DT>(row + cl[j]);
+      for (int j = 0; j < CPL; ++j) Cq[u][j] = c_at(rp, j);
     }
   }
   // Remaining rows (fewer than kPF), already prefetched.
@@ -616,10 +675,28 @@ static void launch_cpl(dim3 grid, dim3 block, hipStream_t st, const CT *C, const
   Xch<DT> *x = static_cast<Xch<DT> *>(xch);
   if (full)
     hipLaunchKernelGGL((k_stage1<CPL, CT, DT, true, STD>), grid, block, 0, st, C, tiles, td0, ss,
-                       Lmax, S, G, flags, x, Bp, eD, eP, eC, Df, Pf, Bf);
+                       Lmax, S, G, flags, x, Bp, eD, eP, eC, Df, Pf, Bf, nullptr, nullptr, 0);
   else
     hipLaunchKernelGGL((k_stage1<CPL, CT, DT, false, STD>), grid, block, 0, st, C, tiles, td0, ss,
-                       Lmax, S, G, flags, x, Bp, eD, eP, eC, Df, Pf, Bf);
+                       Lmax, S, G, flags, x, Bp, eD, eP, eC, Df, Pf, Bf, nullptr, nullptr, 0);
+}
+
+// Features form: LDS for the per-wave C blocks is dynamic (nw * 32 * 64*CPL bf16).
+template <bool STD, int CPL>
+static int launch_feat_cpl(dim3 grid, dim3 block, size_t lds, hipStream_t st,
+                           const TileDesc *tiles, const StepSet &ss, int64_t Lmax, int S, int G,
+                           int *flags, void *xch, uint32_t *Bp, double *eD, int32_t *eP,
+                           double *eC, const __hip_bfloat16 *Fa, const __hip_bfloat16 *Fb, int Kp) {
+  auto kern = k_stage1<CPL, __hip_bfloat16, float, false, STD, true>;
+  if (lds > 64 * 1024)
+    PF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const TileDesc td0{};
+  hipLaunchKernelGGL(kern, grid, block, lds, st, static_cast<const __hip_bfloat16 *>(nullptr),
+                     tiles, td0, ss, Lmax, S, G, flags, static_cast<Xch<float> *>(xch), Bp, eD, eP,
+                     eC, nullptr, nullptr, nullptr, Fa, Fb, Kp);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
 }
 
 template <typename CT, typename DT>
@@ -716,6 +793,55 @@ extern "C" int pf_stage1(int dtype, const void *d_C, const int64_t *d_tiles, int
     default:
       PF_REQUIRE(false, "pf_stage1: unknown dtype");
   }
+}
+
+extern "C" int pf_stage1_features(const void *d_F1b, const void *d_F2b, int Kp,
+                                  const int64_t *d_tiles, int64_t ntiles, int64_t Lmax,
+                                  const int32_t *steps, const double *weights, int K, int strips,
+                                  int waves, int rows_per_block, int32_t *d_flags, void *d_xch,
+                                  uint32_t *d_Bpacked, double *d_edge_D, int32_t *d_edge_P,
+                                  double *d_edge_C, void *stream) {
+  StepSet ss;
+  int rc = make_stepset(steps, weights, K, &ss);
+  if (rc) return rc;
+  PF_REQUIRE(ntiles >= 0 && Lmax >= 2 && Lmax <= 8192, "pf_stage1_features: bad tile length");
+  PF_REQUIRE(Kp >= 16 && Kp % 16 == 0, "pf_stage1_features: Kp must be a positive multiple of 16");
+  PF_REQUIRE(strips >= 1 && waves >= 1 && waves <= 16, "pf_stage1_features: 1 <= waves <= 16");
+  PF_REQUIRE(rows_per_block >= 1 && rows_per_block <= kMaxG, "pf_stage1_features: 1 <= G <= 64");
+  const int cpl = pick_cpl(Lmax, strips, waves);
+  PF_REQUIRE(cpl == 2 || cpl == 4, "pf_stage1_features: strips*waves*64*4 must cover the tile");
+  PF_REQUIRE(waves * 64 <= (cpl == 2 ? 1024 : 512), "pf_stage1_features: too many waves");
+  const size_t lds = (size_t)waves * 32 * 64 * cpl * sizeof(__hip_bfloat16);
+  PF_REQUIRE(lds <= 160 * 1024, "pf_stage1_features: C blocks exceed the LDS");
+  if (ntiles == 0) return PF_OK;
+  PF_REQUIRE(d_F1b && d_F2b && d_tiles && d_Bpacked && d_edge_D && d_edge_P && d_edge_C,
+             "pf_stage1_features: null buffer");
+  PF_REQUIRE(reinterpret_cast<uintptr_t>(d_F1b) % 16 == 0 &&
+                 reinterpret_cast<uintptr_t>(d_F2b) % 16 == 0,
+             "pf_stage1_features: features must be 16-byte aligned");
+  PF_REQUIRE(strips == 1 || (d_flags && d_xch), "pf_stage1_features: strips > 1 need flags and xch");
+  const int64_t n_items = ntiles * strips;
+  PF_REQUIRE(n_items < (1ll << 31), "pf_stage1_features: too many work items");
+  hipStream_t st = as_stream(stream);
+  int *flags = nullptr;
+  if (strips > 1) {
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n_items + 1 + 255) / 256)), dim3(256), 0,
+                       st, d_flags, 0, n_items + 1);
+    PF_CHECK_HIP(hipGetLastError());
+    flags = d_flags + 1;
+  }
+  const bool std3 = ss.K == 3 && ss.dr[0] == 1 && ss.dc[0] == 1 && ss.dr[1] == 1 &&
+                    ss.dc[1] == 2 && ss.dr[2] == 2 && ss.dc[2] == 1;
+  const TileDesc *tiles = reinterpret_cast<const TileDesc *>(d_tiles);
+  const auto *Fa = static_cast<const __hip_bfloat16 *>(d_F1b);
+  const auto *Fb = static_cast<const __hip_bfloat16 *>(d_F2b);
+  const dim3 grid((unsigned)n_items), block((unsigned)(waves * 64));
+#define PF_FEAT(STDV, CPLV)                                                                  \
+  launch_feat_cpl<STDV, CPLV>(grid, block, lds, st, tiles, ss, Lmax, strips, rows_per_block, \
+                              flags, d_xch, d_Bpacked, d_edge_D, d_edge_P, d_edge_C, Fa, Fb, Kp)
+  if (std3) return cpl == 2 ? PF_FEAT(true, 2) : PF_FEAT(true, 4);
+  return cpl == 2 ? PF_FEAT(false, 2) : PF_FEAT(false, 4);
+#undef PF_FEAT
 }
 
 extern "C" int pf_stage1_workspace(int dtype, int64_t ntiles, int64_t Lmax, int strips,

@@ -98,24 +98,38 @@ def stage1_config(plan, n_cu=256):
     return S, nw, int(env.get("PARFLEX_S1_G", "32"))
 
 
-def stage1(C_dev, plan, sw, ws, stream=None, config=None):
+def stage1(C_dev, plan, sw, ws, stream=None, config=None, feat=None):
     """Per-tile DP for every tile of the plan: packed back-pointers plus edge
-    D/P (replaces the tile loop at Parflex_gpu.py:461-563)."""
+    D/P (replaces the tile loop at Parflex_gpu.py:461-563).  With feat =
+    (F1b, F2b, Kp) the cost is computed inside the sweep from BF16 features
+    (pf_stage1_features; ws["tiles"] columns 0/1 then hold frame indices)
+    and C_dev is not used."""
     S, nw, G = config or ws.get("s1_config") or stage1_config(plan)
     torch = _torch()
+    dev = ws["tiles"].device
+    code = PF_BF16 if feat is not None else dtype_code(C_dev)
     if S > 1:
         nf, nx = ctypes.c_int64(), ctypes.c_int64()
-        check(lib().pf_stage1_workspace(dtype_code(C_dev), plan.ntiles, plan.L, S,
+        check(lib().pf_stage1_workspace(code, plan.ntiles, plan.L, S,
                                         ctypes.byref(nf), ctypes.byref(nx)), "pf_stage1_workspace")
         need_f = nf.value
         need_x = -(-nx.value // 8)
         if ws.get("s1_flags") is None or ws["s1_flags"].numel() < need_f:
-            ws["s1_flags"] = torch.empty(need_f, dtype=torch.int32, device=C_dev.device)
+            ws["s1_flags"] = torch.empty(need_f, dtype=torch.int32, device=dev)
         if ws.get("s1_xch") is None or ws["s1_xch"].numel() < need_x:
-            ws["s1_xch"] = torch.empty(need_x, dtype=torch.float64, device=C_dev.device)
+            ws["s1_xch"] = torch.empty(need_x, dtype=torch.float64, device=dev)
         fl, xc = ws["s1_flags"].data_ptr(), ws["s1_xch"].data_ptr()
     else:
         fl = xc = None
+    if feat is not None:
+        F1b, F2b, Kp = feat
+        rc = lib().pf_stage1_features(F1b.data_ptr(), F2b.data_ptr(), int(Kp),
+                                      ws["tiles"].data_ptr(), plan.ntiles, plan.L, sw.sp, sw.wp,
+                                      sw.K, S, nw, G, fl, xc, ws["Bp"].data_ptr(),
+                                      ws["edge_D"].data_ptr(), ws["edge_P"].data_ptr(),
+                                      ws["edge_C"].data_ptr(), _stream(stream))
+        check(rc, "pf_stage1_features")
+        return
     rc = lib().pf_stage1(dtype_code(C_dev), C_dev.data_ptr(), ws["tiles"].data_ptr(),
                          plan.ntiles, plan.L, sw.sp, sw.wp, sw.K, S, nw, G, fl, xc,
                          ws["Bp"].data_ptr(), ws["edge_D"].data_ptr(), ws["edge_P"].data_ptr(),
@@ -221,35 +235,36 @@ def single_pair_plan(L1, L2, L, beta):
 _TUNED = {}  # (L, dtype code, ntiles bucket) -> stage-1 launch shape
 
 
-def _config_candidates(L):
-    """(strips, waves) shapes with 2 columns per lane and no wave idle."""
+def _config_candidates(L, feat=False):
+    """(strips, waves) shapes with 2 columns per lane and no wave idle (the
+    features form keeps its C blocks in LDS: 8 KB per wave, at most 160 KB)."""
     out = []
     for nw in (1, 2, 4, 8, 16):
-        if (nw - 1) * 128 < L:
+        if (nw - 1) * 128 < L and (not feat or nw * 8 <= 160):
             out.append((-(-L // (nw * 128)), nw))
     return out
 
 
-def autotune_stage1(C_dev, plan, sw, ws, Gs=(32,), reps=2, stream=None):
+def autotune_stage1(C_dev, plan, sw, ws, Gs=(32,), reps=2, stream=None, feat=None):
     """Time stage 1 for the candidate launch shapes on this plan and keep the
     fastest in ws['s1_config'] (and a process-wide cache keyed by tile length,
     dtype and tile count).  Returns (config, {config: ms})."""
     torch = _torch()
-    key = (plan.L, dtype_code(C_dev), plan.ntiles // 64)
+    key = (plan.L, "feat" if feat is not None else dtype_code(C_dev), plan.ntiles // 64)
     if key in _TUNED:
         ws["s1_config"] = _TUNED[key]
         return _TUNED[key], {}
     times = {}
-    for (S, nw) in _config_candidates(plan.L):
+    for (S, nw) in _config_candidates(plan.L, feat is not None):
         for G in Gs:
             cfg = (S, nw, G)
-            stage1(C_dev, plan, sw, ws, stream, config=cfg)  # warm-up / allocation
+            stage1(C_dev, plan, sw, ws, stream, config=cfg, feat=feat)  # warm-up / allocation
             torch.cuda.synchronize()
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
             for _ in range(reps):
-                stage1(C_dev, plan, sw, ws, stream, config=cfg)
+                stage1(C_dev, plan, sw, ws, stream, config=cfg, feat=feat)
             e1.record()
             torch.cuda.synchronize()
             times[cfg] = e0.elapsed_time(e1) / reps
@@ -266,12 +281,15 @@ class BatchRunner:
     can be captured in a HIP graph (capture()) and replayed (replay()): every
     buffer is allocated up front and every fill is a kernel."""
 
-    def __init__(self, pairs, L, steps, weights, beta=0.1, dtype=None, device=None, share=None):
+    def __init__(self, pairs, L, steps, weights, beta=0.1, dtype=None, device=None, share=None,
+                 fused=False):
         """pairs: list of (F1 (K, N), F2 (K, M)) host feature arrays.  dtype of
         the cost matrices: torch.bfloat16 (default; FP32 DP) or float32 or
         float64 (exact plain form).  share: another runner over the same
         pairs and dtype whose features and cost buffer are reused (a sweep
-        over tile lengths builds the cost matrices once)."""
+        over tile lengths builds the cost matrices once).  fused=True (BF16
+        only): no cost matrix is stored; stage 1 computes C from the BF16
+        features inside the sweep (pf_stage1_features)."""
         torch = _torch()
         self.sw = StepWeights(steps, weights)
         shapes = [(int(F1.shape[1]), int(F2.shape[1])) for F1, F2 in pairs]
@@ -289,10 +307,16 @@ class BatchRunner:
             self.work, self.code, self.jobs = share.work, share.code, share.jobs
             self.cost_blocks = share.cost_blocks
             self.c_ready = share.c_ready
+            self.fused, self.feat = share.fused, share.feat
         else:
             self.dtype = torch.bfloat16 if dtype is None else dtype
             self.device = device or torch.device("cuda", torch.cuda.current_device())
-            self.C = torch.empty(int(offs[-1]), dtype=self.dtype, device=self.device)
+            self.fused = bool(fused)
+            if self.fused and self.dtype != torch.bfloat16:
+                raise ValueError("the fused form computes the cost in BF16")
+            # the fused form stores no cost matrix
+            self.C = torch.empty(1 if self.fused else int(offs[-1]), dtype=self.dtype,
+                                 device=self.device)
             self.code = dtype_code(self.C)
             K = int(pairs[0][0].shape[0])
             if any(int(F1.shape[0]) != K or int(F2.shape[0]) != K for F1, F2 in pairs):
@@ -319,11 +343,33 @@ class BatchRunner:
                   "pf_cost_workspace")
             self.work = torch.empty(-(-nb.value // 8), dtype=torch.float64, device=self.device)
             self.c_ready = [False]
+            self.feat = None
+            if self.fused:
+                Kp = -(-K // 16) * 16
+                self.feat = (torch.empty(f1 * Kp, dtype=torch.bfloat16, device=self.device),
+                             torch.empty(f2 * Kp, dtype=torch.bfloat16, device=self.device), Kp)
         self.ws = self.plan.allocate(self.device)
+        if self.fused:
+            # tile table columns 0/1: frames of the tile's first row / column
+            t = self.plan.tiles.copy()
+            f1o = np.concatenate([[0], np.cumsum([a for a, _ in shapes])[:-1]])
+            f2o = np.concatenate([[0], np.cumsum([b for _, b in shapes])[:-1]])
+            hop = int(L) - 1
+            t[:, 0] = f1o[t[:, 6]] + (t[:, 7] >> 32) * hop
+            t[:, 1] = f2o[t[:, 6]] + (t[:, 7] & 0xFFFFFFFF) * hop
+            self.ws["tiles"] = torch.from_numpy(t).to(self.device)
         self.graph = None
 
     def cost(self, stream=None):
-        """Cost matrices of every pair: one batched launch (pf_cost_batch)."""
+        """Cost matrices of every pair: one batched launch (pf_cost_batch);
+        in the fused form, the BF16 normalised features instead."""
+        if self.fused:
+            F1b, F2b, _ = self.feat
+            for F, Fb in ((self.F1, F1b), (self.F2, F2b)):
+                check(lib().pf_features_bf16(F.data_ptr(), self.K, int(F.shape[1]), Fb.data_ptr(),
+                                             _stream(stream)), "pf_features_bf16")
+            self.c_ready[0] = True
+            return
         check(lib().pf_cost_batch(self.code, self.F1.data_ptr(), int(self.F1.shape[1]),
                                   self.F2.data_ptr(), int(self.F2.shape[1]), self.K,
                                   self.jobs.data_ptr(), int(self.jobs.shape[0]), self.cost_blocks,
@@ -334,27 +380,27 @@ class BatchRunner:
     def launch(self, stream=None):
         """Cost build, stage 1 and stage 2 for every pair (asynchronous)."""
         self.cost(stream)
-        stage1(self.C, self.plan, self.sw, self.ws, stream)
+        stage1(self.C, self.plan, self.sw, self.ws, stream, feat=self.feat)
         stage2(self.C, self.plan, self.sw, self.ws, stream)
 
     def autotune(self, Gs=(32,)):
         if not self.c_ready[0]:
             self.cost()
-        return autotune_stage1(self.C, self.plan, self.sw, self.ws, Gs=Gs)
+        return autotune_stage1(self.C, self.plan, self.sw, self.ws, Gs=Gs, feat=self.feat)
 
     def time_launch(self, reps=2):
         """Mean ms of stage 1 + stage 2 (the cost build does not depend on L)."""
         torch = _torch()
         if not self.c_ready[0]:
             self.cost()
-        stage1(self.C, self.plan, self.sw, self.ws)
+        stage1(self.C, self.plan, self.sw, self.ws, feat=self.feat)
         stage2(self.C, self.plan, self.sw, self.ws)
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(reps):
-            stage1(self.C, self.plan, self.sw, self.ws)
+            stage1(self.C, self.plan, self.sw, self.ws, feat=self.feat)
             stage2(self.C, self.plan, self.sw, self.ws)
         e1.record()
         torch.cuda.synchronize()
@@ -371,7 +417,7 @@ class BatchRunner:
         graphs = []
         with torch.cuda.stream(s):
             for phase in (self.cost,
-                          lambda st: stage1(self.C, self.plan, self.sw, self.ws, st),
+                          lambda st: stage1(self.C, self.plan, self.sw, self.ws, st, feat=self.feat),
                           lambda st: stage2(self.C, self.plan, self.sw, self.ws, st)):
                 g = torch.cuda.CUDAGraph()
                 with torch.cuda.graph(g, stream=s):
@@ -401,7 +447,8 @@ class BatchRunner:
         return collect_pinned(self.plan, self.ws, self._host)
 
 
-def tune_batch(pairs, Ls, steps, weights, beta=0.1, dtype=None, Gs=(32,), reps=2, log=None):
+def tune_batch(pairs, Ls, steps, weights, beta=0.1, dtype=None, Gs=(32,), reps=2, log=None,
+               fused=False):
     """Autotuner over the tile length L and the stage-1 launch shape
     (strips, waves, exchange block G) for one batch: for every L the fastest
     launch shape is found (autotune_stage1), then the L whose stage 1 +
@@ -409,7 +456,7 @@ def tune_batch(pairs, Ls, steps, weights, beta=0.1, dtype=None, Gs=(32,), reps=2
     Returns (runner, {L: (config, ms)})."""
     best, best_ms, report, base = None, None, {}, None
     for L in Ls:
-        br = BatchRunner(pairs, L, steps, weights, beta, dtype, share=base)
+        br = BatchRunner(pairs, L, steps, weights, beta, dtype, share=base, fused=fused)
         if base is None:
             base = br
         cfg, _ = br.autotune(Gs=Gs)

@@ -123,7 +123,7 @@ class LearnedRunner:
     stage 1 + stage 2, fallback) and returns (BatchResult, stats)."""
 
     def __init__(self, pairs, L, steps, weights, beta, net: TileNet, threshold, ext=8,
-                 guard=128, cost_bound=math.inf, device=None, graph=True):
+                 guard=128, cost_bound=math.inf, device=None, graph=True, fused=True):
         torch = _torch()
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.L, self.beta = int(L), float(beta)
@@ -157,6 +157,15 @@ class LearnedRunner:
         self._host = {}
         self.graph, self._graph, self._scores = bool(graph), None, None
         self.nstreams, self._side = 4, None
+        # fused: the selected tiles' cost is computed inside stage 1 from the
+        # BF16 features (pf_stage1_features) instead of per-tile cost slots
+        self.fused = bool(fused)
+        if self.fused:
+            Kp = -(-K // 16) * 16
+            self.feat = (torch.empty(int(self.F1.shape[1]) * Kp, dtype=torch.bfloat16,
+                                     device=self.device),
+                         torch.empty(int(self.F2.shape[1]) * Kp, dtype=torch.bfloat16,
+                                     device=self.device), Kp)
 
     # ------------------------------------------------------------------
     def _score_into(self, out, stream=None):
@@ -234,14 +243,18 @@ class LearnedRunner:
         ldc = (W[sel] + 7) // 8 * 8
         slot = R[sel] * ldc
         c_off = np.concatenate([[0], np.cumsum(slot)[:-1]]) if slot.size else np.zeros(0, np.int64)
-        work[:, 0] = c_off
-        work[:, 1] = ldc
-        # cost jobs: the tile's frames in the concatenated feature arrays
+        # the tile's frames in the concatenated feature arrays
         pair = work[:, 6]
         ij = work[:, 7]
         hop = self.L - 1
         s1 = (ij >> 32) * hop
         s2 = (ij & 0xFFFFFFFF) * hop
+        if self.fused:
+            work[:, 0] = self.f1_off[pair] + s1      # frame of the tile's first row
+            work[:, 1] = self.f2_off[pair] + s2      # frame of the tile's first column
+        else:
+            work[:, 0] = c_off
+            work[:, 1] = ldc
         tr, tc = self.cost_tile
         nblk = (-(-work[:, 2] // tr)) * (-(-work[:, 3] // tc))
         jobs = np.zeros((len(work), 8), np.int64)
@@ -294,12 +307,19 @@ class LearnedRunner:
         ws["edge_C"].zero_()
         mark("workspace")
         if nsel:
-            C = torch.empty(c_elems, dtype=torch.bfloat16, device=dev)
-            d_jobs = torch.from_numpy(jobs).to(dev)
-            check(lib().pf_cost_batch(PF_BF16, self.F1.data_ptr(), int(self.F1.shape[1]),
-                                      self.F2.data_ptr(), int(self.F2.shape[1]), self.K,
-                                      d_jobs.data_ptr(), nsel, nblk, C.data_ptr(),
-                                      self.work.data_ptr(), st), "pf_cost_batch")
+            if self.fused:
+                C = None
+                F1b, F2b, Kp = self.feat
+                for F, Fb in ((self.F1, F1b), (self.F2, F2b)):
+                    check(lib().pf_features_bf16(F.data_ptr(), self.K, int(F.shape[1]),
+                                                 Fb.data_ptr(), st), "pf_features_bf16")
+            else:
+                C = torch.empty(c_elems, dtype=torch.bfloat16, device=dev)
+                d_jobs = torch.from_numpy(jobs).to(dev)
+                check(lib().pf_cost_batch(PF_BF16, self.F1.data_ptr(), int(self.F1.shape[1]),
+                                          self.F2.data_ptr(), int(self.F2.shape[1]), self.K,
+                                          d_jobs.data_ptr(), nsel, nblk, C.data_ptr(),
+                                          self.work.data_ptr(), st), "pf_cost_batch")
             mark("cost")
             d_work = torch.from_numpy(work).to(dev)
             S, nw, G = self._s1_shape(nsel)
@@ -313,11 +333,18 @@ class LearnedRunner:
                 fl, xc = flags.data_ptr(), xch.data_ptr()
             if events is not None:
                 events[0].record(stream or torch.cuda.current_stream())
-            check(lib().pf_stage1(PF_BF16, C.data_ptr(), d_work.data_ptr(), nsel, L,
-                                  self.sw.sp, self.sw.wp, self.sw.K, S, nw, G, fl, xc,
-                                  ws["Bp"].data_ptr(), ws["edge_D"].data_ptr(),
-                                  ws["edge_P"].data_ptr(), ws["edge_C"].data_ptr(), st),
-                  "pf_stage1")
+            if self.fused:
+                check(lib().pf_stage1_features(F1b.data_ptr(), F2b.data_ptr(), Kp,
+                                               d_work.data_ptr(), nsel, L, self.sw.sp, self.sw.wp,
+                                               self.sw.K, S, nw, G, fl, xc, ws["Bp"].data_ptr(),
+                                               ws["edge_D"].data_ptr(), ws["edge_P"].data_ptr(),
+                                               ws["edge_C"].data_ptr(), st), "pf_stage1_features")
+            else:
+                check(lib().pf_stage1(PF_BF16, C.data_ptr(), d_work.data_ptr(), nsel, L,
+                                      self.sw.sp, self.sw.wp, self.sw.K, S, nw, G, fl, xc,
+                                      ws["Bp"].data_ptr(), ws["edge_D"].data_ptr(),
+                                      ws["edge_P"].data_ptr(), ws["edge_C"].data_ptr(), st),
+                      "pf_stage1")
             if events is not None:
                 events[1].record(stream or torch.cuda.current_stream())
             mark("stage1")

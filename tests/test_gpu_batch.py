@@ -139,3 +139,38 @@ def test_batched_cost_equals_single(torch_gpu, dtype):
         assert torch.equal(got, one), p
         if dtype == "float64":
             assert np.array_equal(got.cpu().numpy(), cost_matrix_det(F1, F2)), p
+
+
+@pytest.mark.parametrize("K,kind,L,config", [(12, "chroma", 300, None), (12, "chroma", 700, (3, 2, 16)),
+                                             (20, "relu", 257, (2, 1, 32)), (12, "chroma", 1000, (1, 8, 32)),
+                                             (128, "relu", 520, (2, 2, 7))])
+def test_fused_stage1_equals_materialised_bf16(torch_gpu, K, kind, L, config):
+    """Stage 1 computing C inside the sweep (pf_stage1_features) gives the
+    same back-pointers, edges and paths, bit for bit, as stage 1 reading the
+    materialised BF16 cost matrix (same MFMA operands and rounding)."""
+    from parflexdtw_amd import engine
+    from parflexdtw_amd.engine import BatchRunner
+    torch = torch_gpu
+    pairs = _pairs([(700, 800), (433, 521), (1100, 950), (260, 301)], 900 + K, K=K, kind=kind)
+    ref = BatchRunner(pairs, L, STEPS, WEIGHTS, beta=0.1, dtype=torch.bfloat16)
+    fus = BatchRunner(pairs, L, STEPS, WEIGHTS, beta=0.1, dtype=torch.bfloat16, fused=True)
+    for br in (ref, fus):
+        if config is not None:
+            br.ws["s1_config"] = config
+        br.launch()
+    torch.cuda.synchronize()
+    assert torch.equal(ref.ws["Bp"], fus.ws["Bp"])
+    # edge slots that exist (position < edge length); the rest are never written
+    tiles = ref.plan.tiles
+    valid = np.zeros((len(tiles), 2, L), bool)
+    for t, row in enumerate(tiles):
+        valid[t, 0, :row[3]] = True
+        valid[t, 1, :row[2]] = True
+    valid = valid.reshape(-1)
+    for k in ("edge_D", "edge_P", "edge_C"):
+        assert np.array_equal(ref.ws[k].cpu().numpy()[valid], fus.ws[k].cpu().numpy()[valid]), k
+    a, b = ref.collect(), fus.collect()
+    assert np.array_equal(a.best_cost, b.best_cost, equal_nan=True)
+    for p in range(len(pairs)):
+        assert np.array_equal(a.paths[p], b.paths[p]), p
+    assert fus.C.numel() == 1   # no cost matrix is stored
