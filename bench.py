@@ -57,7 +57,8 @@ METRIC = "DP GCUPS + pairs/s, batched/learned vs plain GPU at >=99.5% path agree
 # WRITE_SIZE, scaled by a 1 GiB copy); profiles/pmc_traffic_*.md hold the runs.
 TRAFFIC_BYTES = {("chroma_batch256_N3000-20000", 500, "materialised"): (83.97 + 12.84) * 1e9,
                  ("chroma_learned128_N20000-60000_L2000", 2000, "materialised"):
-                     (46.70 + 6.21) * 1e9}
+                     (46.70 + 6.21) * 1e9,
+                 ("chroma_learned128_N20000-60000_L2000", 2000, "fused"): (1.04 + 6.19) * 1e9}
 HBM_PEAK_GBS = 8000.0
 STEPS, WEIGHTS, BETA = [(1, 1), (1, 2), (2, 1)], [2, 3, 3], 0.1
 WORKLOADS = ["chroma_learned128_N20000-60000_L2000", "chroma_batch256_N3000-20000",
