@@ -222,6 +222,28 @@ int pf_cost_batch(int out_dtype, const double *d_F1, int64_t n1_total, const dou
  * pool-frame block (FlexDTW.py:1207-1210 normalisation); pool must be 32. */
 int pf_pool_features(const double *d_F, int64_t K, int64_t n, int pool, float *d_Fp,
                      void *stream);
+/* The same for many frame runs in one launch (every pair of a batch):
+ * d_seg holds nseg rows of int64 {f_off, frames, b_off, 0}; run s pools
+ * frames [f_off, f_off+frames) of d_F (K x n, row stride n) into columns
+ * [b_off, b_off + ceil(frames/32)) of d_Fp (K x nb, row stride nb).  Rows
+ * are in increasing b_off; blocks of one run are contiguous. */
+int pf_pool_features_batch(const double *d_F, int64_t K, int64_t n, const int64_t *d_seg,
+                           int64_t nseg, int pool, float *d_Fp, int64_t nb, void *stream);
+/* Input statistics of many pairs in one launch (K <= 16): pair p's pooled
+ * features are columns [a_off, a_off+H) of d_A (K x lda) and [b_off, b_off+W)
+ * of d_B (K x ldb), d_pairs rows int64 {a_off, H, b_off, W}; d_st + p*(2K+1)
+ * receives the feature means of both sides and 1 / std of the double-centred
+ * pooled cost (the statistics pf_pooled_cost computes, in the same order). */
+int pf_input_stats_batch(const float *d_A, int64_t lda, const float *d_B, int64_t ldb,
+                         int64_t K, const int64_t *d_pairs, int64_t npairs, float *d_st,
+                         void *stream);
+/* First layer (as pf_conv3x3 mode 3) computed from one pair's pooled
+ * features and statistics, without an input map: equal bit for bit to
+ * pf_pooled_cost(d_Xc) followed by pf_conv3x3 mode 3.  K <= 16. */
+int pf_conv3x3_first_pooled(const float *d_A, int64_t lda, int64_t a_off, int64_t H,
+                            const float *d_B, int64_t ldb, int64_t b_off, int64_t W, int64_t K,
+                            const float *d_st, const void *d_Wt, const float *d_bias, void *d_Y,
+                            void *stream);
 /* Network input from the pooled features: the pooled cost P = 1 - A^T B
  * double-centred (row and column means removed, global mean added back) and
  * divided by its standard deviation, written as d_X (bf16 NHWC 32, channel 0,
@@ -270,6 +292,15 @@ int pf_tile_scores(const float *d_Z, int64_t H, int64_t W, int pool, int64_t L, 
 int pf_selection_check(const int64_t *d_pairs, int64_t npairs, int64_t max_path,
                        const int64_t *d_path, const int64_t *d_path_len, const uint8_t *d_sel,
                        int64_t L, int64_t guard, int32_t *d_flag, void *stream);
+
+/* Stitched paths of every pair (d_path / d_path_len as written by
+ * pf_stage2, pair table as for pf_stage2) packed back to back as int64 (2, n)
+ * arrays: pair p at offset 2 * sum_{q<p} len_q, rows then columns, so that
+ * one copy of 2 * sum(len) values brings every path to the host in the
+ * layout the reference returns (wp of shape (2, n), Parflex.py:1543-1548).
+ * max_path >= every path length; d_out holds at least 2 * sum(len). */
+int pf_pack_paths(const int64_t *d_pairs, int64_t npairs, int64_t max_path,
+                  const int64_t *d_path, const int64_t *d_path_len, int64_t *d_out, void *stream);
 
 /* Fill helpers used instead of hipMemsetAsync (safe under graph capture). */
 int pf_fill_f64(double *d_p, double v, int64_t n, void *stream);

@@ -51,6 +51,11 @@ _SIGNATURES = {
     "pf_cost_tile": (_int, [_int, _vp, _vp]),
     "pf_cost_batch": (_int, [_int, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp]),
     "pf_pool_features": (_int, [_vp, _i64, _i64, _int, _vp, _vp]),
+    "pf_pack_paths": (_int, [_vp, _i64, _i64, _vp, _vp, _vp, _vp]),
+    "pf_pool_features_batch": (_int, [_vp, _i64, _i64, _vp, _i64, _int, _vp, _i64, _vp]),
+    "pf_input_stats_batch": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _vp]),
+    "pf_conv3x3_first_pooled": (_int, [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp,
+                                       _vp, _vp, _vp, _vp]),
     "pf_pooled_cost_workspace": (_int, [_i64, _vp]),
     "pf_pooled_cost": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
     "pf_conv3x3": (_int, [_int, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),

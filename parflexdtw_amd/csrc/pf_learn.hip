@@ -29,28 +29,51 @@ constexpr int kKdim = 9 * kCh;    // im2col width (288)
 
 __device__ __forceinline__ float bf2f(__hip_bfloat16 v) { return __bfloat162float(v); }
 
-// Fp[k*nb + b] = mean over the frames i of block b (32 frames, the last
-// block ragged) of F[k*n + i] / (sqrt(sum_k F[k*n+i]^2) + 1e-9).  One thread
-// per frame (coalesced loads); the 32 frames of a block are 32 consecutive
-// lanes, reduced with shuffles.
+// A run of frames pooled into consecutive blocks: frames [f_off, f_off + n)
+// of the feature array into pooled blocks [b_off, b_off + ceil(n / 32)).
+struct PoolSeg {
+  int64_t f_off, n, b_off, pad;
+};
+
+// Fp[k*ldp + g] = mean over the frames i of pooled block g (32 frames, the
+// last block of a segment ragged) of F[k*ldf + i] / (sqrt(sum_k F[k*ldf+i]^2)
+// + 1e-9).  One thread per frame (coalesced loads); the 32 frames of a block
+// are 32 consecutive lanes, reduced with shuffles.  The segment of block g is
+// the last one with b_off <= g (binary search over `segs`, or `one` when segs
+// is null), so one launch pools every pair of a batch.
 __global__ __launch_bounds__(256) void k_pool_features(const double *__restrict__ F, int64_t K,
-                                                       int64_t n, float *__restrict__ Fp,
+                                                       int64_t ldf,
+                                                       const PoolSeg *__restrict__ segs,
+                                                       int64_t nseg, PoolSeg one,
+                                                       float *__restrict__ Fp, int64_t ldp,
                                                        int64_t nb) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t b = i >> 5;
-  const bool in = i < n;
-  const int64_t cnt = (b * 32 + 32 < n) ? 32 : n - b * 32;
+  const int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t g = gt >> 5;
+  PoolSeg sg = one;
+  if (segs) {
+    int64_t lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (segs[mid].b_off <= g) lo = mid; else hi = mid - 1;
+    }
+    sg = segs[lo];
+  }
+  const int64_t b = g - sg.b_off;
+  const int64_t fi = b * 32 + (gt & 31);
+  const bool in = g < nb && fi < sg.n;
+  const int64_t i = sg.f_off + fi;
+  const int64_t cnt = (b * 32 + 32 < sg.n) ? 32 : sg.n - b * 32;
   double s = 0.0;
   for (int64_t k = 0; k < K; ++k) {
-    const double v = in ? F[k * n + i] : 0.0;
+    const double v = in ? F[k * ldf + i] : 0.0;
     s = s + v * v;
   }
   const double inv = 1.0 / ((sqrt(s) + 1e-9) * (double)(cnt > 0 ? cnt : 1));
   for (int64_t k = 0; k < K; ++k) {
-    float v = in ? (float)(F[k * n + i] * inv) : 0.0f;
+    float v = in ? (float)(F[k * ldf + i] * inv) : 0.0f;
 #pragma unroll
     for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if ((threadIdx.x & 31) == 0 && b < nb) Fp[k * nb + b] = v;
+    if ((threadIdx.x & 31) == 0 && g < nb) Fp[k * ldp + g] = v;
   }
 }
 
@@ -117,73 +140,90 @@ __global__ __launch_bounds__(256) void k_input_stats_any(const float *__restrict
   if (t == 0) st[2 * K] = 1.0f / sqrtf(red[0] / ((float)H * (float)W) + 1e-12f);
 }
 
-// Same statistics for K <= KM with every thread accumulating all means and
-// Gram entries of its frames in registers; partial sums are reduced with
-// wave shuffles, then over the 4 waves in a fixed order (deterministic).
+// Same statistics for K <= KM, one workgroup per pair of a batch: pair p's
+// pooled features are A[k*lda + a_off + y] (y < H) and B[k*ldb + b_off + x]
+// (x < W), its statistics st + p*(2K+1) (`one` when `pairs` is null).  Wave w
+// sums the means of k = w, w+4, ...; the centred features are then staged in
+// LDS in chunks of kStChunk frames and wave w accumulates the Gram entries
+// e = w, w+4, ... (lanes over frames).  Every sum has a fixed order, so the
+// result does not depend on the launch or on other pairs.
+struct StatPair {
+  int64_t a_off, H, b_off, W;
+};
+constexpr int kStChunk = 512;
+
 template <int KM>
-__global__ __launch_bounds__(256) void k_input_stats_small(const float *__restrict__ A, int64_t H,
-                                                           const float *__restrict__ B, int64_t W,
-                                                           int K, float *__restrict__ st) {
+__global__ __launch_bounds__(256) void k_input_stats_seg(const float *__restrict__ A, int64_t lda,
+                                                         const float *__restrict__ B, int64_t ldb,
+                                                         int K, const StatPair *__restrict__ pairs,
+                                                         StatPair one, float *__restrict__ st_all) {
   constexpr int E = KM * (KM + 1) / 2;
-  __shared__ float part[4][E];
-  __shared__ float mean[2][KM];
+  constexpr int EW = (E + 3) / 4;
+  __shared__ float cs[KM * kStChunk];
+  __shared__ float mean[KM];
   __shared__ float G[2][E];
+  __shared__ unsigned char ek[E], el[E];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  for (int side = 0; side < 2; ++side) {
-    const float *S = side ? B : A;
-    const int64_t n = side ? W : H;
-    float m[KM];
-#pragma unroll
-    for (int k = 0; k < KM; ++k) m[k] = 0.0f;
-    for (int64_t y = t; y < n; y += 256)
-#pragma unroll
-      for (int k = 0; k < KM; ++k)
-        if (k < K) m[k] += S[k * n + y];
-#pragma unroll
-    for (int k = 0; k < KM; ++k) {
-      float v = m[k];
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-      if (lane == 0) part[wave][k] = v;
-    }
-    __syncthreads();
-    if (t < KM) {
-      mean[side][t] = ((part[0][t] + part[1][t]) + (part[2][t] + part[3][t])) / (float)n;
-      if (t < K) st[side * K + t] = mean[side][t];
-    }
-    __syncthreads();
+  const StatPair pr = pairs ? pairs[blockIdx.x] : one;
+  float *st = st_all + (int64_t)blockIdx.x * (2 * K + 1);
+  const int Ek = K * (K + 1) / 2;
+  if (t == 0) {
+    int e = 0;
+    for (int k = 0; k < K; ++k)
+      for (int l = k; l < K; ++l, ++e) {
+        ek[e] = (unsigned char)k;
+        el[e] = (unsigned char)l;
+      }
   }
   for (int side = 0; side < 2; ++side) {
-    const float *S = side ? B : A;
-    const int64_t n = side ? W : H;
-    float g[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) g[e] = 0.0f;
-    for (int64_t y = t; y < n; y += 256) {
-      float c[KM];
-#pragma unroll
-      for (int k = 0; k < KM; ++k) c[k] = k < K ? S[k * n + y] - mean[side][k] : 0.0f;
-      int e = 0;
-#pragma unroll
-      for (int k = 0; k < KM; ++k)
-#pragma unroll
-        for (int l = k; l < KM; ++l) g[e++] += c[k] * c[l];
-    }
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      float v = g[e];
+    const float *S = side ? B + pr.b_off : A + pr.a_off;
+    const int64_t ld = side ? ldb : lda;
+    const int64_t n = side ? pr.W : pr.H;
+    for (int k = wave; k < K; k += 4) {
+      float v = 0.0f;
+      for (int64_t f = lane; f < n; f += 64) v += S[k * ld + f];
       for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-      if (lane == 0) part[wave][e] = v;
+      if (lane == 0) {
+        mean[k] = v / (float)n;
+        st[side * K + k] = v / (float)n;
+      }
     }
     __syncthreads();
-    for (int e = t; e < E; e += 256) G[side][e] = (part[0][e] + part[1][e]) + (part[2][e] + part[3][e]);
+    float g[EW];
+#pragma unroll
+    for (int j = 0; j < EW; ++j) g[j] = 0.0f;
+    for (int64_t c0 = 0; c0 < n; c0 += kStChunk) {
+      const int m = (int)(n - c0 < kStChunk ? n - c0 : kStChunk);
+      for (int q = t; q < K * kStChunk; q += 256) {
+        const int k = q / kStChunk, f = q - k * kStChunk;
+        cs[q] = f < m ? S[k * ld + c0 + f] - mean[k] : 0.0f;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < EW; ++j) {
+        const int e = wave + 4 * j;
+        if (e < Ek) {
+          const float *ck = cs + ek[e] * kStChunk, *cl = cs + el[e] * kStChunk;
+          float v = g[j];
+          for (int f = lane; f < m; f += 64) v += ck[f] * cl[f];
+          g[j] = v;
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < EW; ++j) {
+      const int e = wave + 4 * j;
+      float v = g[j];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      if (lane == 0 && e < Ek) G[side][e] = v;
+    }
     __syncthreads();
   }
   if (t == 0) {
     float q = 0.0f;
-    int e = 0;
-    for (int k = 0; k < KM; ++k)
-      for (int l = k; l < KM; ++l, ++e) q += (k == l ? 1.0f : 2.0f) * G[0][e] * G[1][e];
-    st[2 * K] = 1.0f / sqrtf(q / ((float)H * (float)W) + 1e-12f);
+    for (int e = 0; e < Ek; ++e) q += (ek[e] == el[e] ? 1.0f : 2.0f) * G[0][e] * G[1][e];
+    st[2 * K] = 1.0f / sqrtf(q / ((float)pr.H * (float)pr.W) + 1e-12f);
   }
 }
 
@@ -374,6 +414,86 @@ __global__ __launch_bounds__(256) void k_conv3x3_first(const __hip_bfloat16 *__r
     if (yy < H && x < W)
       *reinterpret_cast<u32x4 *>(&Y[(yy * W + x) * kCh + c8 * 8]) =
           *reinterpret_cast<const u32x4 *>(&ys[pix * kCh + c8 * 8]);
+  }
+}
+
+// First layer straight from a pair's pooled features and input statistics
+// (no input map): the halo value at (y, x) is the network input
+// bf16(-sum_k (A[k][y] - abar_k) (B[k][x] - bbar_k) * inv_std), computed in the
+// same order as k_write_input, so the output equals k_write_input followed by
+// k_conv3x3_first bit for bit.  A workgroup of 4 waves covers 16 rows x 32
+// columns, wave w rows 4w .. 4w+3 (one MFMA each); K <= 16.
+__global__ __launch_bounds__(256) void k_conv3x3_first_pooled(
+    const float *__restrict__ A, int64_t lda, int64_t a_off, int64_t H,
+    const float *__restrict__ B, int64_t ldb, int64_t b_off, int64_t W, int K,
+    const float *__restrict__ st, const __hip_bfloat16 *__restrict__ Wt,
+    const float *__restrict__ bias, __hip_bfloat16 *__restrict__ Y) {
+  constexpr int HR = kCvRows + 2, HC = kCvCols + 2;
+  __shared__ float ca[HR * 16], cb[HC * 16];
+  __shared__ __hip_bfloat16 xs[HR * HC];
+  __shared__ __attribute__((aligned(16))) __hip_bfloat16 ys[kCvRows * kCvCols * kCh];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int64_t y0 = (int64_t)blockIdx.y * kCvRows, x0 = (int64_t)blockIdx.x * kCvCols;
+  // centred features of the halo rows and columns
+  for (int q = tid; q < (HR + HC) * K; q += 256) {
+    const int r = q / K, k = q - r * K;
+    if (r < HR) {
+      const int64_t yy = y0 - 1 + r;
+      ca[r * 16 + k] = (yy >= 0 && yy < H) ? A[k * lda + a_off + yy] - st[k] : 0.0f;
+    } else {
+      const int64_t xx = x0 - 1 + (r - HR);
+      cb[(r - HR) * 16 + k] = (xx >= 0 && xx < W) ? B[k * ldb + b_off + xx] - st[K + k] : 0.0f;
+    }
+  }
+  __syncthreads();
+  const float inv = st[2 * K];
+  for (int q = tid; q < HR * HC; q += 256) {
+    const int r = q / HC, c = q - r * HC;
+    const int64_t yy = y0 - 1 + r, xx = x0 - 1 + c;
+    const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
+    float s = 0.0f;
+    for (int k = 0; k < K; ++k) s += ca[r * 16 + k] * cb[c * 16 + k];
+    xs[q] = ok ? __float2bfloat16(-s * inv) : __float2bfloat16(0.0f);
+  }
+  __syncthreads();
+  const int px = lane & 31, kh = (lane >> 5) * 8;
+  lbf16x8_t b;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int k = kh + i;                       // tap dy*3 + dx, or padding
+    b[i] = k < 9 ? (__bf16)bf2f(Wt[px * kKdim + k * kCh]) : (__bf16)0.0f;
+  }
+  const int co = lane & 31;
+  const float bv = bias[co];
+#pragma unroll
+  for (int t = 0; t < kRowsPerWave; ++t) {
+    const int orow = kRowsPerWave * wave + t;
+    lbf16x8_t a;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int k = kh + i;
+      const int dy = k / 3, dx = k - (k / 3) * 3;
+      const __hip_bfloat16 xv = xs[k < 9 ? (orow + dy) * HC + px + dx : 0];
+      a[i] = k < 9 ? (__bf16)bf2f(xv) : (__bf16)0.0f;
+    }
+    const lfloatx16_t acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, lfloatx16_t{}, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int xl = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+      const float r = acc[v] + bv;
+      ys[(orow * kCvCols + xl) * kCh + co] = __float2bfloat16(r > 0.0f ? r : 0.0f);
+    }
+  }
+  __syncthreads();
+  for (int q = tid; q < kCvRows * kCvCols * (kCh / 8); q += 256) {
+    const int pix = q >> 2, c8 = q & 3;
+    const int rr = pix / kCvCols, xl = pix - rr * kCvCols;
+    const int64_t yy = y0 + rr, x = x0 + xl;
+    if (yy < H && x < W)
+      *reinterpret_cast<lu32x4_t *>(&Y[(yy * W + x) * kCh + c8 * 8]) =
+          *reinterpret_cast<const lu32x4_t *>(&ys[pix * kCh + c8 * 8]);
   }
 }
 
@@ -583,8 +703,56 @@ extern "C" int pf_pool_features(const double *d_F, int64_t K, int64_t n, int poo
   PF_REQUIRE(d_F && d_Fp && K >= 1 && n >= 1, "pf_pool_features: bad arguments");
   PF_REQUIRE(pool == 32, "pf_pool_features: the pooling factor is 32");
   const int64_t nb = (n + pool - 1) / pool;
-  hipLaunchKernelGGL(k_pool_features, dim3(nb256(n)), dim3(256), 0, as_stream(stream), d_F, K, n,
-                     d_Fp, nb);
+  const PoolSeg one{0, n, 0, 0};
+  hipLaunchKernelGGL(k_pool_features, dim3(nb256(nb * 32)), dim3(256), 0, as_stream(stream), d_F,
+                     K, n, (const PoolSeg *)nullptr, (int64_t)1, one, d_Fp, nb, nb);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_pool_features_batch(const double *d_F, int64_t K, int64_t n,
+                                      const int64_t *d_seg, int64_t nseg, int pool, float *d_Fp,
+                                      int64_t nb, void *stream) {
+  PF_REQUIRE(d_F && d_seg && d_Fp && K >= 1 && n >= 1 && nseg >= 1 && nb >= 1,
+             "pf_pool_features_batch: bad arguments");
+  PF_REQUIRE(pool == 32, "pf_pool_features_batch: the pooling factor is 32");
+  hipLaunchKernelGGL(k_pool_features, dim3(nb256(nb * 32)), dim3(256), 0, as_stream(stream), d_F,
+                     K, n, reinterpret_cast<const PoolSeg *>(d_seg), nseg, PoolSeg{0, 0, 0, 0},
+                     d_Fp, nb, nb);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_input_stats_batch(const float *d_A, int64_t lda, const float *d_B, int64_t ldb,
+                                    int64_t K, const int64_t *d_pairs, int64_t npairs,
+                                    float *d_st, void *stream) {
+  PF_REQUIRE(d_A && d_B && d_pairs && d_st && K >= 1 && npairs >= 1,
+             "pf_input_stats_batch: bad arguments");
+  PF_REQUIRE(K <= 16, "pf_input_stats_batch: at most 16 features per frame");
+  PF_REQUIRE(npairs < (1ll << 31), "pf_input_stats_batch: too many pairs");
+  hipLaunchKernelGGL(k_input_stats_seg<16>, dim3((unsigned)npairs), dim3(256), 0,
+                     as_stream(stream), d_A, lda, d_B, ldb, (int)K,
+                     reinterpret_cast<const StatPair *>(d_pairs), StatPair{0, 0, 0, 0}, d_st);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_conv3x3_first_pooled(const float *d_A, int64_t lda, int64_t a_off, int64_t H,
+                                       const float *d_B, int64_t ldb, int64_t b_off, int64_t W,
+                                       int64_t K, const float *d_st, const void *d_Wt,
+                                       const float *d_bias, void *d_Y, void *stream) {
+  PF_REQUIRE(d_A && d_B && d_st && d_Wt && d_bias && d_Y && H >= 1 && W >= 1 && K >= 1 &&
+                 a_off >= 0 && b_off >= 0 && a_off + H <= lda && b_off + W <= ldb,
+             "pf_conv3x3_first_pooled: bad arguments");
+  PF_REQUIRE(K <= 16, "pf_conv3x3_first_pooled: at most 16 features per frame");
+  PF_REQUIRE(reinterpret_cast<uintptr_t>(d_Y) % 16 == 0,
+             "pf_conv3x3_first_pooled: Y must be 16-byte aligned");
+  PF_REQUIRE((H + kCvRows - 1) / kCvRows < 65536, "pf_conv3x3_first_pooled: image too large");
+  const dim3 grid((unsigned)((W + kCvCols - 1) / kCvCols), (unsigned)((H + kCvRows - 1) / kCvRows));
+  hipLaunchKernelGGL(k_conv3x3_first_pooled, grid, dim3(256), 0, as_stream(stream), d_A, lda,
+                     a_off, H, d_B, ldb, b_off, W, (int)K, d_st,
+                     static_cast<const __hip_bfloat16 *>(d_Wt), d_bias,
+                     static_cast<__hip_bfloat16 *>(d_Y));
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -606,8 +774,8 @@ extern "C" int pf_pooled_cost(const float *d_A, int64_t H, const float *d_B, int
     PF_CHECK_HIP(hipGetLastError());
   }
   if (K <= 16)
-    hipLaunchKernelGGL(k_input_stats_small<16>, dim3(1), dim3(256), 0, st, d_A, H, d_B, W, (int)K,
-                       d_work);
+    hipLaunchKernelGGL(k_input_stats_seg<16>, dim3(1), dim3(256), 0, st, d_A, H, d_B, W, (int)K,
+                       (const StatPair *)nullptr, StatPair{0, H, 0, W}, d_work);
   else
     hipLaunchKernelGGL(k_input_stats_any, dim3(1), dim3(256), 0, st, d_A, H, d_B, W, K, d_work);
   PF_CHECK_HIP(hipGetLastError());

@@ -730,6 +730,50 @@ static inline unsigned nblk(int64_t n, int bs) {
 
 using namespace pf;
 
+namespace pf {
+// Paths of every pair packed back to back in the (2, n) layout the host
+// returns: pair p starts at off_p = 2 * sum_{q<p} len_q, its rows at
+// [off_p, off_p + len_p) and its columns at [off_p + len_p, off_p + 2 len_p).
+// Grid (point blocks, pairs); every block sums the lengths before its pair.
+__global__ __launch_bounds__(256) void k_pack_paths(const PairDesc *__restrict__ pairs,
+                                                    const int64_t *__restrict__ path,
+                                                    const int64_t *__restrict__ path_len,
+                                                    int64_t *__restrict__ out) {
+  __shared__ int64_t red[256];
+  const int64_t p = blockIdx.y;
+  const int64_t n = path_len[p];
+  if ((int64_t)blockIdx.x * blockDim.x >= n) return;
+  int64_t s = 0;
+  for (int64_t q = threadIdx.x; q < p; q += blockDim.x) s += path_len[q];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  const int64_t off = 2 * red[0];
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t *src = path + (pairs[p].path_off + t) * 2;
+  out[off + t] = src[0];
+  out[off + n + t] = src[1];
+}
+}  // namespace pf
+
+extern "C" int pf_pack_paths(const int64_t *d_pairs, int64_t npairs, int64_t max_path,
+                             const int64_t *d_path, const int64_t *d_path_len, int64_t *d_out,
+                             void *stream) {
+  PF_REQUIRE(d_pairs && d_path && d_path_len && d_out && npairs >= 1 && max_path >= 0,
+             "pf_pack_paths: bad arguments");
+  PF_REQUIRE(npairs <= 65535, "pf_pack_paths: too many pairs for one launch");
+  if (max_path == 0) return PF_OK;
+  hipLaunchKernelGGL(k_pack_paths, dim3((unsigned)((max_path + 255) / 256), (unsigned)npairs),
+                     dim3(256), 0, as_stream(stream), reinterpret_cast<const PairDesc *>(d_pairs),
+                     d_path, d_path_len, d_out);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
 extern "C" int pf_fill_f64(double *d_p, double v, int64_t n, void *stream) {
   if (n <= 0) return PF_OK;
   hipLaunchKernelGGL(k_fill_f64, dim3(nblk(n, 256)), dim3(256), 0, as_stream(stream), d_p, v, n);

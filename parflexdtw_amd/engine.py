@@ -214,6 +214,47 @@ def collect_pinned(plan, ws, host):
     return collect(plan, host)
 
 
+def collect_packed(plan, ws, host, stream=None):
+    """collect() with the paths packed on the device (pf_pack_paths): the
+    small per-pair results go through pinned buffers kept in `host`, then one
+    copy brings the 2 * sum(len) path values into a fresh pinned buffer, and
+    every returned path is a (2, n) int64 view of it (no host-side copy)."""
+    torch = _torch()
+    st = stream or torch.cuda.current_stream()
+    n = plan.npairs
+    if "packed" not in ws:
+        ws["packed"] = torch.empty(ws["path"].numel(), dtype=torch.int64,
+                                   device=ws["path"].device)
+    max_path = int((plan.pairs[:, 0] + plan.pairs[:, 1]).max()) + 8
+    check(lib().pf_pack_paths(ws["pairs"].data_ptr(), n, max_path, ws["path"].data_ptr(),
+                              ws["path_len"].data_ptr(), ws["packed"].data_ptr(), st.cuda_stream),
+          "pf_pack_paths")
+    with torch.cuda.stream(st):
+        for k in ("best_f", "best_i", "path_len"):
+            if k not in host or host[k].shape != ws[k].shape:
+                host[k] = torch.empty(ws[k].shape, dtype=ws[k].dtype, pin_memory=True)
+            host[k].copy_(ws[k], non_blocking=True)
+    st.synchronize()
+    plen = host["path_len"].numpy().copy()
+    total = 2 * int(plen.sum())
+    buf = torch.empty(max(total, 1), dtype=torch.int64, pin_memory=True)
+    if total:
+        with torch.cuda.stream(st):
+            buf[:total].copy_(ws["packed"][:total], non_blocking=True)
+        st.synchronize()
+    arr = buf.numpy()
+    best_f = host["best_f"].numpy().copy()
+    best_i = host["best_i"].numpy().reshape(n, 8).copy()
+    found = best_i[:, 6] == 1
+    paths, o = [], 0
+    for p in range(n):
+        m = int(plen[p])
+        paths.append(arr[o:o + 2 * m].reshape(2, m))
+        o += 2 * m
+    bc = np.where(found, best_f, np.nan)
+    return BatchResult(bc, found, paths, best_i[:, :6].copy())
+
+
 def run_batch(C_dev, plan, steps, weights, stream=None, ws=None):
     """Stage 1 + stage 2 for every pair of the plan; returns (BatchResult, ws)."""
     sw = steps if isinstance(steps, StepWeights) else StepWeights(steps, weights)
@@ -444,7 +485,7 @@ class BatchRunner:
         through pinned host buffers allocated on first use."""
         if getattr(self, "_host", None) is None:
             self._host = {}
-        return collect_pinned(self.plan, self.ws, self._host)
+        return collect_packed(self.plan, self.ws, self._host)
 
 
 def tune_batch(pairs, Ls, steps, weights, beta=0.1, dtype=None, Gs=(32,), reps=2, log=None,

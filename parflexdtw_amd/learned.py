@@ -35,7 +35,7 @@ from . import engine
 from ._lib import PF_BF16, check, lib
 from ._plan import BatchPlan
 from .engine import StepWeights, _stream
-from .tile_net import TileNet, select_tiles
+from .tile_net import POOL, TileNet, select_tiles
 
 __all__ = ["LearnedRunner", "path_tiles", "needs_fallback"]
 
@@ -137,6 +137,8 @@ class LearnedRunner:
         self.pairs = pairs
         self.shapes = [(int(F1.shape[1]), int(F2.shape[1])) for F1, F2 in pairs]
         K = int(pairs[0][0].shape[0])
+        if K > 16:
+            raise ValueError("the learned form takes at most 16 features per frame")
         self.K = K
         f64 = np.float64
         self.Fp = [(torch.from_numpy(np.ascontiguousarray(F1, f64)).to(self.device),
@@ -156,7 +158,24 @@ class LearnedRunner:
         self.cost_tile = (tr.value, tc.value)
         self._host = {}
         self.graph, self._graph, self._scores = bool(graph), None, None
-        self.nstreams, self._side = 4, None
+        self.nstreams, self._side, self._bufs = 4, None, None
+        # pooled blocks of every pair, concatenated: pair p's rows are columns
+        # [hb[p], hb[p] + H_p) of the K x sum(H) array, its columns likewise
+        self.HW = [(-(-a // POOL), -(-b // POOL)) for a, b in self.shapes]
+        H_, W_ = np.array([h for h, _ in self.HW]), np.array([w for _, w in self.HW])
+        self.hb = np.concatenate([[0], np.cumsum(H_)[:-1]]).astype(np.int64)
+        self.wb = np.concatenate([[0], np.cumsum(W_)[:-1]]).astype(np.int64)
+        self.nbA, self.nbB = int(H_.sum()), int(W_.sum())
+        n = len(pairs)
+        seg1 = np.zeros((n, 4), np.int64)
+        seg1[:, 0], seg1[:, 1], seg1[:, 2] = self.f1_off, [a for a, _ in self.shapes], self.hb
+        seg2 = np.zeros((n, 4), np.int64)
+        seg2[:, 0], seg2[:, 1], seg2[:, 2] = self.f2_off, [b for _, b in self.shapes], self.wb
+        stp = np.stack([self.hb, H_, self.wb, W_], axis=1).astype(np.int64)
+        self._seg = [torch.from_numpy(t).to(self.device) for t in (seg1, seg2, stp)]
+        self.Ap = torch.empty(K * self.nbA, dtype=torch.float32, device=self.device)
+        self.Bp = torch.empty(K * self.nbB, dtype=torch.float32, device=self.device)
+        self.stats = torch.empty(n * (2 * K + 1), dtype=torch.float32, device=self.device)
         # fused: the selected tiles' cost is computed inside stage 1 from the
         # BF16 features (pf_stage1_features) instead of per-tile cost slots
         self.fused = bool(fused)
@@ -170,27 +189,51 @@ class LearnedRunner:
     # ------------------------------------------------------------------
     def _score_into(self, out, stream=None):
         """Tile logits of every pair into the device buffer `out` (the
-        pairs' grids concatenated in i-major order).  Pairs are spread over
-        side streams (forked from and joined back to `stream`) so that the
-        small, latency-bound kernels of one pair overlap the convolutions of
-        another."""
+        pairs' grids concatenated in i-major order).  Pooled features and
+        input statistics of all pairs take three launches on `stream`; the
+        network layers of the pairs are spread over side streams (forked
+        from and joined back to `stream`) so that one pair's layers overlap
+        the tail of another's."""
         torch = _torch()
         main = stream or torch.cuda.current_stream()
+        K, n = self.K, len(self.pairs)
+        seg1, seg2, stp = self._seg
+        ms = main.cuda_stream
+        check(lib().pf_pool_features_batch(self.F1.data_ptr(), K, int(self.F1.shape[1]),
+                                           seg1.data_ptr(), n, POOL, self.Ap.data_ptr(),
+                                           self.nbA, ms), "pf_pool_features_batch")
+        check(lib().pf_pool_features_batch(self.F2.data_ptr(), K, int(self.F2.shape[1]),
+                                           seg2.data_ptr(), n, POOL, self.Bp.data_ptr(),
+                                           self.nbB, ms), "pf_pool_features_batch")
+        check(lib().pf_input_stats_batch(self.Ap.data_ptr(), self.nbA, self.Bp.data_ptr(),
+                                         self.nbB, K, stp.data_ptr(), n, self.stats.data_ptr(),
+                                         ms), "pf_input_stats_batch")
+        ns = self.nstreams
         if self._side is None:
-            self._side = [torch.cuda.Stream() for _ in range(self.nstreams)]
+            self._side = [torch.cuda.Stream() for _ in range(ns)]
+        if self._bufs is None:
+            # activation buffers per side stream, sized for its largest pair
+            self._bufs = []
+            for q in range(ns):
+                hw = max([h * w for p, (h, w) in enumerate(self.HW) if p % ns == q] or [1])
+                self._bufs.append((torch.empty(hw * 32, dtype=torch.bfloat16, device=self.device),
+                                   torch.empty(hw * 32, dtype=torch.bfloat16, device=self.device),
+                                   torch.empty(hw, dtype=torch.float32, device=self.device)))
         for sd in self._side:
             sd.wait_stream(main)
+        st0 = self.stats.data_ptr()
         o = 0
-        for p, (F1d, F2d) in enumerate(self.Fp):
-            sd = self._side[p % len(self._side)]
-            with torch.cuda.stream(sd):
-                X, H, W = self.net.pooled_input(F1d, F2d, sd, compact=True)
-                _, Z = self.net.forward(X, H, W, sd)
-                L1, L2 = int(F1d.shape[1]), int(F2d.shape[1])
-                n1, n2 = -(-(L1 - 1) // (self.L - 1)), -(-(L2 - 1) // (self.L - 1))
-                check(lib().pf_tile_scores(Z.data_ptr(), H, W, 32, self.L, L1, L2, self.ext,
-                                           out[o:o + n1 * n2].data_ptr(), sd.cuda_stream),
-                      "pf_tile_scores")
+        for p in range(n):
+            sd = self._side[p % ns]
+            H, W = self.HW[p]
+            L1, L2 = self.shapes[p]
+            Z = self.net.forward_pooled(self.Ap, self.nbA, int(self.hb[p]), H, self.Bp, self.nbB,
+                                        int(self.wb[p]), W, K, st0 + 4 * p * (2 * K + 1),
+                                        self._bufs[p % ns], sd)
+            n1, n2 = -(-(L1 - 1) // (self.L - 1)), -(-(L2 - 1) // (self.L - 1))
+            check(lib().pf_tile_scores(Z.data_ptr(), H, W, POOL, self.L, L1, L2, self.ext,
+                                       out[o:o + n1 * n2].data_ptr(), sd.cuda_stream),
+                  "pf_tile_scores")
             o += n1 * n2
         for sd in self._side:
             main.wait_stream(sd)
@@ -358,7 +401,7 @@ class LearnedRunner:
                                        ws["path"].data_ptr(), ws["path_len"].data_ptr(),
                                        d_sel.data_ptr(), L, self.guard, flag.data_ptr(), st),
               "pf_selection_check")
-        res = engine.collect_pinned(plan, ws, self._host)
+        res = engine.collect_packed(plan, ws, self._host, stream)
         edge = flag.cpu().numpy()
         mark("collect")
         fb, why = [], {}

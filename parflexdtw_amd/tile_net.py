@@ -235,6 +235,28 @@ class TileNet:
                             float(b1), float(b2), float(eps), self.step, _st(stream)), "pf_adam")
         self.pack(stream)
 
+    def forward_pooled(self, Ap, lda, a_off, H, Bp, ldb, b_off, W, K, st, bufs, stream=None):
+        """Logit map of one pair of a batch, from the batch's pooled features
+        (Ap: K x lda, Bp: K x ldb f32; the pair's columns start at a_off,
+        b_off) and its statistics (device pointer `st` to 2K+1 floats, as
+        written by pf_input_stats_batch).  The first layer builds its input
+        on the fly.  bufs = (X1, X2, Z): two bf16 activation buffers of at
+        least H*W*32 elements and an f32 buffer of at least H*W; returns Z."""
+        X1, X2, Z = bufs
+        sp = _st(stream)
+        check(lib().pf_conv3x3_first_pooled(Ap.data_ptr(), lda, a_off, H, Bp.data_ptr(), ldb,
+                                            b_off, W, K, st, self.Wt[0].data_ptr(),
+                                            self.layer(0)[CH * KDIM:].data_ptr(), X1.data_ptr(),
+                                            sp), "pf_conv3x3_first_pooled")
+        for i, (src, dst) in ((1, (X1, X2)), (2, (X2, X1))):
+            check(lib().pf_conv3x3(0, src.data_ptr(), H, W, self.Wt[i].data_ptr(),
+                                   self.layer(i)[CH * KDIM:].data_ptr(), None, dst.data_ptr(), sp),
+                  "pf_conv3x3")
+        check(lib().pf_conv3x3(1, X1.data_ptr(), H, W, self.Wt[3].data_ptr(),
+                               self.layer(3)[CH * KDIM:].data_ptr(), None, Z.data_ptr(), sp),
+              "pf_conv3x3")
+        return Z
+
     # ---- inference -------------------------------------------------------
     def tile_scores(self, Z, H, W, L, L1, L2, ext=0, stream=None):
         """Max logit over each tile's pooled cells extended by ext cells."""

@@ -54,6 +54,104 @@ def test_pooled_input_matches_oracle(env):
     assert np.array_equal(Xc.float().cpu().numpy().reshape(H, W), Xh[..., 0])
 
 
+def _batch_pairs(shapes, seed):
+    """Nonnegative random features (12, n) of the given ragged lengths."""
+    rng = np.random.default_rng(seed)
+    return [(rng.random((12, a)) ** 3, rng.random((12, b)) ** 3) for a, b in shapes]
+
+
+BATCH_SHAPES = [(333, 300), (32, 65), (31, 1), (1500, 2100), (1, 40), (700, 517)]
+
+
+def test_batched_pooling_and_statistics_equal_per_pair(env):
+    """pf_pool_features_batch over concatenated ragged pairs equals
+    pf_pool_features on each pair, and pf_input_stats_batch equals the
+    statistics pf_pooled_cost computes for the pair alone, bit for bit."""
+    torch, tn, tile_net = env
+    L, check = _lib()
+    import ctypes
+    pairs = _batch_pairs(BATCH_SHAPES, 50)
+    K = 12
+    st = torch.cuda.current_stream().cuda_stream
+    F1 = torch.from_numpy(np.concatenate([a for a, _ in pairs], axis=1)).cuda()
+    F2 = torch.from_numpy(np.concatenate([b for _, b in pairs], axis=1)).cuda()
+    n1 = [a.shape[1] for a, _ in pairs]
+    n2 = [b.shape[1] for _, b in pairs]
+    H = [-(-v // 32) for v in n1]
+    W = [-(-v // 32) for v in n2]
+    f1o, f2o = np.cumsum([0] + n1[:-1]), np.cumsum([0] + n2[:-1])
+    hb, wb = np.cumsum([0] + H[:-1]), np.cumsum([0] + W[:-1])
+    nbA, nbB = sum(H), sum(W)
+    seg1 = torch.from_numpy(np.stack([f1o, n1, hb, 0 * hb], 1).astype(np.int64)).cuda()
+    seg2 = torch.from_numpy(np.stack([f2o, n2, wb, 0 * wb], 1).astype(np.int64)).cuda()
+    Ap = torch.empty(K * nbA, dtype=torch.float32, device="cuda")
+    Bp = torch.empty(K * nbB, dtype=torch.float32, device="cuda")
+    check(L.pf_pool_features_batch(F1.data_ptr(), K, F1.shape[1], seg1.data_ptr(), len(pairs), 32,
+                                   Ap.data_ptr(), nbA, st), "pool A")
+    check(L.pf_pool_features_batch(F2.data_ptr(), K, F2.shape[1], seg2.data_ptr(), len(pairs), 32,
+                                   Bp.data_ptr(), nbB, st), "pool B")
+    stp = torch.from_numpy(np.stack([hb, H, wb, W], 1).astype(np.int64)).cuda()
+    stats = torch.empty(len(pairs) * (2 * K + 1), dtype=torch.float32, device="cuda")
+    check(L.pf_input_stats_batch(Ap.data_ptr(), nbA, Bp.data_ptr(), nbB, K, stp.data_ptr(),
+                                 len(pairs), stats.data_ptr(), st), "stats")
+    Ah = Ap.cpu().numpy().reshape(K, nbA)
+    Bh = Bp.cpu().numpy().reshape(K, nbB)
+    sh = stats.cpu().numpy().reshape(len(pairs), 2 * K + 1)
+    net = tile_net.TileNet(seed=0)
+    n = ctypes.c_int64()
+    check(L.pf_pooled_cost_workspace(K, ctypes.byref(n)), "ws")
+    for p, (a, b) in enumerate(pairs):
+        A1, B1, h, w, _ = net.pooled_features(torch.from_numpy(a).cuda(),
+                                              torch.from_numpy(b).cuda())
+        assert np.array_equal(Ah[:, hb[p]:hb[p] + h], A1.cpu().numpy().reshape(K, h)), p
+        assert np.array_equal(Bh[:, wb[p]:wb[p] + w], B1.cpu().numpy().reshape(K, w)), p
+        work = torch.empty(n.value, dtype=torch.float32, device="cuda")
+        Xc = torch.empty(h * w, dtype=torch.bfloat16, device="cuda")
+        check(L.pf_pooled_cost(A1.data_ptr(), h, B1.data_ptr(), w, K, None, Xc.data_ptr(), None,
+                               work.data_ptr(), st), "pooled cost")
+        assert np.array_equal(sh[p], work.cpu().numpy()[:2 * K + 1]), p
+
+
+def test_first_layer_from_pooled_features_equals_input_map_path(env):
+    """pf_conv3x3_first_pooled on a pair inside a batch equals the input map
+    of pf_pooled_cost followed by pf_conv3x3 mode 3, bit for bit."""
+    torch, tn, tile_net = env
+    L, check = _lib()
+    import ctypes
+    pairs = _batch_pairs([(333, 300), (1500, 2100), (31, 1)], 60)
+    K = 12
+    st = torch.cuda.current_stream().cuda_stream
+    net = tile_net.TileNet(seed=2)
+    pooled = [net.pooled_features(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda())
+              for a, b in pairs]
+    # a batch layout with the pairs' pooled columns side by side
+    H = [q[2] for q in pooled]
+    W = [q[3] for q in pooled]
+    lda, ldb = sum(H) + 3, sum(W) + 5
+    Ap = torch.zeros((K, lda), dtype=torch.float32, device="cuda")
+    Bp = torch.zeros((K, ldb), dtype=torch.float32, device="cuda")
+    hb, wb = np.cumsum([0] + H[:-1]) + 2, np.cumsum([0] + W[:-1]) + 1
+    for p, (A1, B1, h, w, _) in enumerate(pooled):
+        Ap[:, hb[p]:hb[p] + h] = A1.reshape(K, h)
+        Bp[:, wb[p]:wb[p] + w] = B1.reshape(K, w)
+    n = ctypes.c_int64()
+    check(L.pf_pooled_cost_workspace(K, ctypes.byref(n)), "ws")
+    bias = net.layer(0)[32 * 288:]
+    for p, (A1, B1, h, w, _) in enumerate(pooled):
+        work = torch.empty(n.value, dtype=torch.float32, device="cuda")
+        Xc = torch.empty(h * w, dtype=torch.bfloat16, device="cuda")
+        check(L.pf_pooled_cost(A1.data_ptr(), h, B1.data_ptr(), w, K, None, Xc.data_ptr(), None,
+                               work.data_ptr(), st), "pooled cost")
+        Yref = torch.empty(h * w * 32, dtype=torch.bfloat16, device="cuda")
+        check(L.pf_conv3x3(3, Xc.data_ptr(), h, w, net.Wt[0].data_ptr(), bias.data_ptr(), None,
+                           Yref.data_ptr(), st), "first")
+        Y = torch.empty(h * w * 32, dtype=torch.bfloat16, device="cuda")
+        check(L.pf_conv3x3_first_pooled(Ap.data_ptr(), lda, int(hb[p]), h, Bp.data_ptr(), ldb,
+                                        int(wb[p]), w, K, work.data_ptr(), net.Wt[0].data_ptr(),
+                                        bias.data_ptr(), Y.data_ptr(), st), "first pooled")
+        assert torch.equal(Y.view(torch.int16), Yref.view(torch.int16)), p
+
+
 @pytest.mark.parametrize("H,W", [(13, 70), (1, 5), (40, 33)])
 def test_conv_modes_match_oracle(env, H, W):
     torch, tn, _ = env

@@ -83,6 +83,25 @@ def test_scoring_graph_replays_equal_eager(torch_gpu):
             assert np.array_equal(a, b), r
 
 
+def test_batched_scoring_equals_per_pair_scoring(torch_gpu):
+    """The runner's scoring (batched pooling and statistics, the first layer
+    from pooled features, pairs over side streams) gives each pair the tile
+    logits of TileNet.score_pair on that pair alone, bit for bit."""
+    torch = torch_gpu
+    from parflexdtw_amd.learned import LearnedRunner
+    from parflexdtw_amd.synth import batch_pairs
+    from parflexdtw_amd.tile_net import TileNet
+    net, meta = TileNet.load(MODEL)
+    pairs = [(a, b) for a, b, _ in batch_pairs(7, 98000, 800, 7000)]
+    L, ext = 600, 5
+    got = LearnedRunner(pairs, L, STEPS, WEIGHTS, 0.1, net, 0.0, ext=ext, guard=100,
+                        graph=False).score()
+    for p, (F1, F2) in enumerate(pairs):
+        s, n1, n2 = net.score_pair(torch.from_numpy(F1).cuda(), torch.from_numpy(F2).cuda(), L,
+                                   ext)
+        assert np.array_equal(got[p], s.cpu().numpy().reshape(n1, n2)), p
+
+
 def test_selection_check_kernel_matches_host_rule(torch_gpu):
     """pf_selection_check flags a pair exactly when a tile within `guard`
     frames of its stitched path is unselected (host rule: learned.path_tiles)."""
