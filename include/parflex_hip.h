@@ -257,7 +257,8 @@ int pf_pooled_cost(const float *d_A, int64_t H, const float *d_B, int64_t W, int
  * mode 2: gradient w.r.t. the layer input, d_Wt = flipped weights
  * (pf_pack_weights), masked by d_M > 0 -> bf16 NHWC 32; mode 3: as mode 0
  * for the first layer from the one-channel input d_X = d_Xc of
- * pf_pooled_cost (only input channel 0 of the weights is read). */
+ * pf_pooled_cost (only input channel 0 of the weights is read).  X, the
+ * weights and the bias are 16-byte aligned, Y and M 8-byte aligned. */
 int pf_conv3x3(int mode, const void *d_X, int64_t H, int64_t W, const void *d_Wt,
                const float *d_bias, const void *d_M, void *d_Y, void *stream);
 /* Weight and bias gradient of one layer, accumulated into d_grad (f32

@@ -47,6 +47,18 @@ struct TileDesc {
 };
 static_assert(sizeof(TileDesc) == 64, "tile table rows are int64[8]");
 
+// A value every lane of the wave holds, moved to scalar registers.
+__device__ __forceinline__ int64_t wave_uniform(int64_t v) {
+  const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return ((int64_t)hi << 32) | (uint32_t)lo;
+}
+__device__ __forceinline__ TileDesc wave_uniform(const TileDesc &d) {
+  return TileDesc{wave_uniform(d.c_off), wave_uniform(d.ldc),  wave_uniform(d.R),
+                  wave_uniform(d.W),     wave_uniform(d.b_off), wave_uniform(d.e_off),
+                  wave_uniform(d.pair),  wave_uniform(d.ij)};
+}
+
 // Pair table row (int64[16]), see parflex_hip.h.
 struct PairDesc {
   int64_t L1, L2, n1, n2, tile_base, c_off, ldc, buf, lk_off, seg_off, max_segs,

@@ -246,12 +246,17 @@ __global__ void k_write_input(const float *__restrict__ A, int64_t H, const floa
 
 // 3x3 convolution, 32 -> 32 channels.  A workgroup of 4 waves computes a
 // block of 16 rows x 32 columns of output pixels; wave w takes rows
-// 4w .. 4w+3 (four MFMA tiles of 32 pixels x 32 channels).  The 18 x 34
+// 4w .. 4w+3 (four MFMA tiles of 32 channels x 32 pixels).  The 18 x 34
 // input halo of the block is staged in LDS once; each wave keeps its 18
 // weight fragments in registers and reads every input row of its window
 // once per (dx, channel half), feeding the up to three output rows that use
 // it (dy = input row - output row), so an LDS read serves 2 MFMAs on
 // average instead of 1 and no weight is read from LDS.
+// The MFMA computes the transposed tile D[cout][pixel] = W * im2col(X)^T, so
+// lane l holds pixel l%32 and output channels (v/4)*8 + (l/32)*4 + v%4: four
+// consecutive channels per accumulator group, stored straight to HBM as one
+// 8-byte write (the two half-waves together write 16 contiguous bytes of
+// every pixel), without staging the tile through LDS.
 // MODE 0: Y = relu(acc + bias) as bf16 (hidden layer).
 // MODE 1: channel 0 of acc + bias as f32, H*W (the logit map).
 // MODE 2: Y = (M > 0) ? acc : 0 as bf16 (gradient through the ReLU of the
@@ -260,6 +265,72 @@ constexpr int kRowsPerWave = 4;
 constexpr int kCvRows = 4 * kRowsPerWave, kCvCols = 32;
 constexpr int kPixPad = kCh + 8;      // bf16 per staged pixel (16-byte aligned rows)
 typedef unsigned int lu32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int lu32x2_t __attribute__((ext_vector_type(2)));
+typedef float lfloatx4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned int pack_bf16x2(float lo, float hi) {
+  const unsigned int a = (unsigned int)__bfloat16_as_ushort(__float2bfloat16(lo));
+  const unsigned int b = (unsigned int)__bfloat16_as_ushort(__float2bfloat16(hi));
+  return a | (b << 16);
+}
+
+// Epilogue of the transposed tile acc (channels x 32 pixels) of output row y,
+// pixels x0 .. x0+31 (lane l holds pixel x0 + l%32).  MODE 1 writes the logit
+// straight to HBM (32 consecutive floats per wave); the other modes write the
+// row's bf16 outputs into the LDS staging row ys (pixel stride kPixPad, four
+// channels = 8 bytes per lane and accumulator group), which flush_block then
+// copies to HBM in 16-byte chunks.
+template <int MODE>
+__device__ __forceinline__ void conv_epilogue(const lfloatx16_t &acc, const float *__restrict__ bias,
+                                              int64_t y, int64_t x0, int64_t H, int64_t W, int lane,
+                                              const __hip_bfloat16 *__restrict__ Mk, void *Yv,
+                                              __hip_bfloat16 *ys) {
+  const int64_t x = x0 + (lane & 31);
+  const bool in = y < H && x < W;
+  const int64_t pix = in ? y * W + x : 0;
+  if constexpr (MODE == 1) {
+    if (in && lane < 32) static_cast<float *>(Yv)[pix] = acc[0] + bias[0];   // the logit map
+  } else {
+    const int cq = (lane >> 5) * 4;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const lfloatx4_t b4 = (MODE == 2) ? lfloatx4_t{0.0f, 0.0f, 0.0f, 0.0f}
+                                        : *reinterpret_cast<const lfloatx4_t *>(&bias[g * 8 + cq]);
+      float r[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        r[i] = acc[g * 4 + i] + b4[i];
+        if constexpr (MODE == 0) r[i] = r[i] > 0.0f ? r[i] : 0.0f;
+      }
+      if constexpr (MODE == 2) {
+        // the four mask values (bf16) as floats: their bits in the high half
+        const lu32x2_t mv = in ? *reinterpret_cast<const lu32x2_t *>(&Mk[pix * kCh + g * 8 + cq])
+                               : lu32x2_t{0u, 0u};
+        const float m[4] = {__uint_as_float(mv[0] << 16), __uint_as_float(mv[0] & 0xffff0000u),
+                            __uint_as_float(mv[1] << 16), __uint_as_float(mv[1] & 0xffff0000u)};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r[i] = m[i] > 0.0f ? r[i] : 0.0f;
+      }
+      *reinterpret_cast<lu32x2_t *>(&ys[(lane & 31) * kPixPad + g * 8 + cq]) =
+          lu32x2_t{pack_bf16x2(r[0], r[1]), pack_bf16x2(r[2], r[3])};
+    }
+  }
+}
+
+// Copy a staged block of `rows` x 32 pixels (bf16 NHWC 32, pixel stride
+// kPixPad in LDS) to Y in 16-byte chunks: a block row is 2 KB contiguous.
+__device__ __forceinline__ void flush_block(const __hip_bfloat16 *ys, __hip_bfloat16 *__restrict__ Y,
+                                            int64_t y0, int64_t x0, int64_t H, int64_t W, int rows,
+                                            int tid) {
+  for (int q = tid; q < rows * kCvCols * (kCh / 8); q += 256) {
+    const int pix = q >> 2, c8 = q & 3;
+    const int rr = pix / kCvCols, xl = pix - rr * kCvCols;
+    const int64_t y = y0 + rr, x = x0 + xl;
+    if (y < H && x < W)
+      *reinterpret_cast<lu32x4_t *>(&Y[(y * W + x) * kCh + c8 * 8]) =
+          *reinterpret_cast<const lu32x4_t *>(&ys[pix * kPixPad + c8 * 8]);
+  }
+}
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_conv3x3(const __hip_bfloat16 *__restrict__ X, int64_t H,
@@ -275,7 +346,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __hip_bfloat16 *__restric
   const int64_t y0 = (int64_t)blockIdx.y * kCvRows, x0 = (int64_t)blockIdx.x * kCvCols;
   const int px = lane & 31;
   const int kh = (lane >> 5) * 8;
-  // weight fragments of this lane: column px (output channel), k = s*16 + kh .. +7
+  // weight fragments of this lane: row px (output channel), k = s*16 + kh .. +7
   lbf16x8_t wb[18];
 #pragma unroll
   for (int s = 0; s < 18; ++s)
@@ -308,55 +379,18 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __hip_bfloat16 *__restric
         for (int t = 0; t < kRowsPerWave; ++t) {
           const int dy = xr - t;
           if (dy >= 0 && dy <= 2)
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, wb[(dy * 3 + dx) * 2 + ch],
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wb[(dy * 3 + dx) * 2 + ch], a,
                                                              acc[t], 0, 0, 0);
         }
       }
-  const int co = lane & 31;
-  const float bv = (MODE == 2) ? 0.0f : bias[co];
-  if constexpr (MODE == 1) {
+  if constexpr (MODE != 1) __syncthreads();   // every wave is done reading the halo
 #pragma unroll
-    for (int t = 0; t < kRowsPerWave; ++t) {
-      const int64_t y = y0 + r0 + t;
-      if (y >= H || co != 0) continue;
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int64_t x = x0 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-        if (x < W) static_cast<float *>(Yv)[y * W + x] = acc[t][v] + bv;   // the logit map
-      }
-    }
-  } else {
-    // Stage the 16 x 32 x 32 output block (bf16) in LDS, then write it as
-    // contiguous 16-byte chunks (a block row is 2 KB contiguous in NHWC).
-    __syncthreads();   // every wave is done reading the halo
-    __hip_bfloat16 *ys = xs;
-#pragma unroll
-    for (int t = 0; t < kRowsPerWave; ++t) {
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int xl = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-        float r = acc[t][v] + bv;
-        if constexpr (MODE == 2) {
-          const int64_t y = y0 + r0 + t, x = x0 + xl;
-          const bool in = y < H && x < W;
-          const float m = bf2f(Mk[(in ? y * W + x : 0) * kCh + co]);
-          r = (in && m > 0.0f) ? r : 0.0f;
-        } else {
-          r = r > 0.0f ? r : 0.0f;
-        }
-        ys[((r0 + t) * kCvCols + xl) * kCh + co] = __float2bfloat16(r);
-      }
-    }
+  for (int t = 0; t < kRowsPerWave; ++t)
+    conv_epilogue<MODE>(acc[t], bias, y0 + r0 + t, x0, H, W, lane, Mk, Yv,
+                        xs + (r0 + t) * kCvCols * kPixPad);
+  if constexpr (MODE != 1) {
     __syncthreads();
-    __hip_bfloat16 *Y = static_cast<__hip_bfloat16 *>(Yv);
-    for (int q = tid; q < kCvRows * kCvCols * (kCh / 8); q += 256) {
-      const int pix = q >> 2, c8 = q & 3;
-      const int rr = pix / kCvCols, xl = pix - rr * kCvCols;
-      const int64_t y = y0 + rr, x = x0 + xl;
-      if (y < H && x < W)
-        *reinterpret_cast<lu32x4_t *>(&Y[(y * W + x) * kCh + c8 * 8]) =
-            *reinterpret_cast<const lu32x4_t *>(&ys[pix * kCh + c8 * 8]);
-    }
+    flush_block(xs, static_cast<__hip_bfloat16 *>(Yv), y0, x0, H, W, kCvRows, tid);
   }
 }
 
@@ -371,6 +405,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_first(const __hip_bfloat16 *__r
                                                        const float *__restrict__ bias,
                                                        __hip_bfloat16 *__restrict__ Y) {
   __shared__ __hip_bfloat16 xs[(kC1Rows + 2) * (kC1Cols + 2)];
+  __shared__ __attribute__((aligned(16))) __hip_bfloat16 ys[kC1Rows * kCvCols * kPixPad];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -394,27 +429,11 @@ __global__ __launch_bounds__(256) void k_conv3x3_first(const __hip_bfloat16 *__r
     a[t] = k < 9 ? (__bf16)bf2f(xv) : (__bf16)0.0f;
     b[t] = k < 9 ? (__bf16)bf2f(wv) : (__bf16)0.0f;
   }
-  lfloatx16_t acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, lfloatx16_t{}, 0, 0, 0);
-  // stage the 4 x 32 x 32 output block in LDS, then write 16-byte chunks
-  __shared__ __attribute__((aligned(16))) __hip_bfloat16 ys[kC1Rows * kC1Cols * kCh];
-  const int co = lane & 31;
-  const float bv = bias[co];
-#pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    const int xl = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-    const float r = acc[v] + bv;
-    ys[(wave * kC1Cols + xl) * kCh + co] = __float2bfloat16(r > 0.0f ? r : 0.0f);
-  }
+  // transposed tile: channels x pixels (see k_conv3x3)
+  const lfloatx16_t acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, lfloatx16_t{}, 0, 0, 0);
+  conv_epilogue<0>(acc, bias, y0 + wave, x0, H, W, lane, nullptr, Y, ys + wave * kCvCols * kPixPad);
   __syncthreads();
-  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-  for (int q = tid; q < kC1Rows * kC1Cols * (kCh / 8); q += 256) {
-    const int pix = q >> 2, c8 = q & 3;
-    const int rr = pix / kC1Cols, xl = pix - rr * kC1Cols;
-    const int64_t yy = y0 + rr, x = x0 + xl;
-    if (yy < H && x < W)
-      *reinterpret_cast<u32x4 *>(&Y[(yy * W + x) * kCh + c8 * 8]) =
-          *reinterpret_cast<const u32x4 *>(&ys[pix * kCh + c8 * 8]);
-  }
+  flush_block(ys, Y, y0, x0, H, W, kC1Rows, tid);
 }
 
 // First layer straight from a pair's pooled features and input statistics
@@ -429,9 +448,10 @@ __global__ __launch_bounds__(256) void k_conv3x3_first_pooled(
     const float *__restrict__ st, const __hip_bfloat16 *__restrict__ Wt,
     const float *__restrict__ bias, __hip_bfloat16 *__restrict__ Y) {
   constexpr int HR = kCvRows + 2, HC = kCvCols + 2;
-  __shared__ float ca[HR * 16], cb[HC * 16];
+  constexpr int KS = 17;   // row stride of ca / cb (odd: no bank conflicts)
+  __shared__ float ca[HR * KS], cb[HC * KS];
   __shared__ __hip_bfloat16 xs[HR * HC];
-  __shared__ __attribute__((aligned(16))) __hip_bfloat16 ys[kCvRows * kCvCols * kCh];
+  __shared__ __attribute__((aligned(16))) __hip_bfloat16 ys[kCvRows * kCvCols * kPixPad];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -441,10 +461,10 @@ __global__ __launch_bounds__(256) void k_conv3x3_first_pooled(
     const int r = q / K, k = q - r * K;
     if (r < HR) {
       const int64_t yy = y0 - 1 + r;
-      ca[r * 16 + k] = (yy >= 0 && yy < H) ? A[k * lda + a_off + yy] - st[k] : 0.0f;
+      ca[r * KS + k] = (yy >= 0 && yy < H) ? A[k * lda + a_off + yy] - st[k] : 0.0f;
     } else {
       const int64_t xx = x0 - 1 + (r - HR);
-      cb[(r - HR) * 16 + k] = (xx >= 0 && xx < W) ? B[k * ldb + b_off + xx] - st[K + k] : 0.0f;
+      cb[(r - HR) * KS + k] = (xx >= 0 && xx < W) ? B[k * ldb + b_off + xx] - st[K + k] : 0.0f;
     }
   }
   __syncthreads();
@@ -454,7 +474,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_first_pooled(
     const int64_t yy = y0 - 1 + r, xx = x0 - 1 + c;
     const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
     float s = 0.0f;
-    for (int k = 0; k < K; ++k) s += ca[r * 16 + k] * cb[c * 16 + k];
+    for (int k = 0; k < K; ++k) s += ca[r * KS + k] * cb[c * KS + k];
     xs[q] = ok ? __float2bfloat16(-s * inv) : __float2bfloat16(0.0f);
   }
   __syncthreads();
@@ -465,8 +485,6 @@ __global__ __launch_bounds__(256) void k_conv3x3_first_pooled(
     const int k = kh + i;                       // tap dy*3 + dx, or padding
     b[i] = k < 9 ? (__bf16)bf2f(Wt[px * kKdim + k * kCh]) : (__bf16)0.0f;
   }
-  const int co = lane & 31;
-  const float bv = bias[co];
 #pragma unroll
   for (int t = 0; t < kRowsPerWave; ++t) {
     const int orow = kRowsPerWave * wave + t;
@@ -478,23 +496,13 @@ __global__ __launch_bounds__(256) void k_conv3x3_first_pooled(
       const __hip_bfloat16 xv = xs[k < 9 ? (orow + dy) * HC + px + dx : 0];
       a[i] = k < 9 ? (__bf16)bf2f(xv) : (__bf16)0.0f;
     }
-    const lfloatx16_t acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, lfloatx16_t{}, 0, 0, 0);
-#pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      const int xl = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-      const float r = acc[v] + bv;
-      ys[(orow * kCvCols + xl) * kCh + co] = __float2bfloat16(r > 0.0f ? r : 0.0f);
-    }
+    // transposed tile: channels x pixels (see k_conv3x3)
+    const lfloatx16_t acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, lfloatx16_t{}, 0, 0, 0);
+    conv_epilogue<0>(acc, bias, y0 + orow, x0, H, W, lane, nullptr, Y,
+                     ys + orow * kCvCols * kPixPad);
   }
   __syncthreads();
-  for (int q = tid; q < kCvRows * kCvCols * (kCh / 8); q += 256) {
-    const int pix = q >> 2, c8 = q & 3;
-    const int rr = pix / kCvCols, xl = pix - rr * kCvCols;
-    const int64_t yy = y0 + rr, x = x0 + xl;
-    if (yy < H && x < W)
-      *reinterpret_cast<lu32x4_t *>(&Y[(yy * W + x) * kCh + c8 * 8]) =
-          *reinterpret_cast<const lu32x4_t *>(&ys[pix * kCh + c8 * 8]);
-  }
+  flush_block(ys, Y, y0, x0, H, W, kCvRows, tid);
 }
 
 // Weight gradient partials: one wave per chunk of kWgChunk pixels;
@@ -745,8 +753,8 @@ extern "C" int pf_conv3x3_first_pooled(const float *d_A, int64_t lda, int64_t a_
                  a_off >= 0 && b_off >= 0 && a_off + H <= lda && b_off + W <= ldb,
              "pf_conv3x3_first_pooled: bad arguments");
   PF_REQUIRE(K <= 16, "pf_conv3x3_first_pooled: at most 16 features per frame");
-  PF_REQUIRE(reinterpret_cast<uintptr_t>(d_Y) % 16 == 0,
-             "pf_conv3x3_first_pooled: Y must be 16-byte aligned");
+  PF_REQUIRE(reinterpret_cast<uintptr_t>(d_Y) % 16 == 0 && reinterpret_cast<uintptr_t>(d_bias) % 16 == 0,
+             "pf_conv3x3_first_pooled: Y and bias must be 16-byte aligned");
   PF_REQUIRE((H + kCvRows - 1) / kCvRows < 65536, "pf_conv3x3_first_pooled: image too large");
   const dim3 grid((unsigned)((W + kCvCols - 1) / kCvCols), (unsigned)((H + kCvRows - 1) / kCvRows));
   hipLaunchKernelGGL(k_conv3x3_first_pooled, grid, dim3(256), 0, as_stream(stream), d_A, lda,
@@ -790,8 +798,10 @@ extern "C" int pf_conv3x3(int mode, const void *d_X, int64_t H, int64_t W, const
                           const float *d_bias, const void *d_M, void *d_Y, void *stream) {
   PF_REQUIRE(d_X && d_Wt && d_Y && H >= 1 && W >= 1, "pf_conv3x3: bad arguments");
   PF_REQUIRE(mode == 2 ? d_M != nullptr : d_bias != nullptr, "pf_conv3x3: missing bias or mask");
-  PF_REQUIRE(reinterpret_cast<uintptr_t>(d_X) % 16 == 0 && reinterpret_cast<uintptr_t>(d_Wt) % 16 == 0,
-             "pf_conv3x3: X and weights must be 16-byte aligned");
+  PF_REQUIRE(reinterpret_cast<uintptr_t>(d_X) % 16 == 0 && reinterpret_cast<uintptr_t>(d_Wt) % 16 == 0 &&
+                 reinterpret_cast<uintptr_t>(d_bias) % 16 == 0 &&
+                 reinterpret_cast<uintptr_t>(d_Y) % 8 == 0 && reinterpret_cast<uintptr_t>(d_M) % 8 == 0,
+             "pf_conv3x3: X, weights and bias must be 16-byte aligned, Y and M 8-byte aligned");
   PF_REQUIRE((W + kC1Cols - 1) / kC1Cols < (1ll << 31) && (H + kC1Rows - 1) / kC1Rows < 65536,
              "pf_conv3x3: image too large");
   const dim3 grid((unsigned)((W + kCvCols - 1) / kCvCols), (unsigned)((H + kCvRows - 1) / kCvRows));

@@ -179,6 +179,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
   __shared__ int prog[16];
   __shared__ DT lxD[kMaxG][2];
   __shared__ DT lxX[kMaxG][2];
+  __shared__ DT shNone[2][2];   // D = +inf, X = 0 of the missing columns -2, -1
   __shared__ int sh_item;
 
   // Work item from the ticket counter in strip order.
@@ -187,16 +188,18 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
     // flags[-1] is the ticket counter (see pf_stage1).
     if (threadIdx.x == 0) sh_item = atomicAdd(&flags[-1], 1);
     __syncthreads();
-    item = sh_item;
+    item = __builtin_amdgcn_readfirstlane(sh_item);
   } else {
     item = blockIdx.x;
   }
   const int64_t t = item / S;
   const int s = (int)(item - t * S);
-  const TileDesc td = tiles ? tiles[t] : td0;
+  // the tile's geometry is the same for every lane: scalar registers, so
+  // per-row conditions on R and W are scalar branches
+  const TileDesc td = wave_uniform(tiles ? tiles[t] : td0);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int SW = (int)blockDim.x * CPL;
   const int R = (int)td.R, W = (int)td.W;
   const int cbeg = s * SW;
@@ -240,20 +243,42 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
       const int ar = (32 * b + lr < R) ? 32 * b + lr : R - 1;
       const __hip_bfloat16 *ap = Fa + (td.c_off + ar) * (int64_t)Kp + kh;
       const int wc0 = cbeg + (tid >> 6) * kWC;
-#pragma unroll
-      for (int cb = 0; cb < kWC / 32; ++cb) {
-        const int gc = (wc0 + cb * 32 + lr < W) ? wc0 + cb * 32 + lr : W - 1;
-        const __hip_bfloat16 *bp = Fb + (td.ldc + gc) * (int64_t)Kp + kh;
-        s1floatx16_t acc = s1floatx16_t{};
-        for (int kk = 0; kk < Kp; kk += 16) {
-          const s1bf16x8_t a = *reinterpret_cast<const s1bf16x8_t *>(ap + kk);
-          const s1bf16x8_t bb = *reinterpret_cast<const s1bf16x8_t *>(bp + kk);
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc, 0, 0, 0);
-        }
+      auto store_block = [&](const int cb, const s1floatx16_t &acc) {
 #pragma unroll
         for (int v = 0; v < 16; ++v)
           cblk[((v >> 2) * 8 + (lane >> 5) * 4 + (v & 3)) * kWC + cb * 32 + lr] =
               __float2bfloat16(1.0f - acc[v]);
+      };
+      if (Kp == 16) {
+        // one MFMA per 32 x 32 sub-block: every feature load is issued
+        // before the first MFMA, so the block costs one load latency
+        const s1bf16x8_t a = *reinterpret_cast<const s1bf16x8_t *>(ap);
+        s1bf16x8_t bb[kWC / 32];
+#pragma unroll
+        for (int cb = 0; cb < kWC / 32; ++cb) {
+          const int gc = (wc0 + cb * 32 + lr < W) ? wc0 + cb * 32 + lr : W - 1;
+          bb[cb] = *reinterpret_cast<const s1bf16x8_t *>(Fb + (td.ldc + gc) * (int64_t)16 + kh);
+        }
+#pragma unroll
+        for (int cb = 0; cb < kWC / 32; ++cb) {
+          store_block(cb, __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb[cb], s1floatx16_t{}, 0,
+                                                                  0, 0));
+          // one accumulator live at a time (register budget of the sweep)
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      } else {
+#pragma unroll
+        for (int cb = 0; cb < kWC / 32; ++cb) {
+          const int gc = (wc0 + cb * 32 + lr < W) ? wc0 + cb * 32 + lr : W - 1;
+          const __hip_bfloat16 *bp = Fb + (td.ldc + gc) * (int64_t)Kp + kh;
+          s1floatx16_t acc = s1floatx16_t{};
+          for (int kk = 0; kk < Kp; kk += 16) {
+            const s1bf16x8_t a = *reinterpret_cast<const s1bf16x8_t *>(ap + kk);
+            const s1bf16x8_t bv = *reinterpret_cast<const s1bf16x8_t *>(bp + kk);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc, 0, 0, 0);
+          }
+          store_block(cb, acc);
+        }
       }
       // the block is read by other lanes of this wave
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -312,6 +337,12 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
     xout[0] = Xch<DT>{D1[CPL - 2], D1[CPL - 1], X1[CPL - 2], X1[CPL - 1]};
     if (R == 1) __hip_atomic_store(fout, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
+  if (tid == 0) {
+    shNone[0][0] = INF;
+    shNone[0][1] = INF;
+    shNone[1][0] = 0;
+    shNone[1][1] = 0;
+  }
   if (lane == 63) {
     shD[0][wave][0] = D1[CPL - 2];
     shD[0][wave][1] = D1[CPL - 1];
@@ -344,28 +375,30 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
       // Row r-1 of columns c0-2, c0-1: from lane-1 by DPP; lane 0 from the
       // previous wave (LDS ring), the previous strip (exchange rows) or the
       // missing columns -2, -1 of the tile.
-      DT oD1a = INF, oD1b = INF, oX1a = 0, oX1b = 0;  // only lane 0's copy is used
       if (wave > 0 && seen_left < r - 1) {
         // wait until the wave on the left has handed over row r-1
         do {
-          seen_left = __hip_atomic_load(&prog[wave - 1], __ATOMIC_ACQUIRE,
-                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+          seen_left = __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+              &prog[wave - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
           if (seen_left < r - 1) __builtin_amdgcn_s_sleep(1);
         } while (seen_left < r - 1);
       }
+      // The source is the same for the whole wave (only lane 0's copy is
+      // used): an unconditional LDS read, no per-row register setup.
+      const DT *srcD, *srcX;
       if (wave > 0) {
         const int s1 = (r - 1) & (kRing - 1);  // slot of row r-1
-        oD1a = shD[s1][wave - 1][0];
-        oD1b = shD[s1][wave - 1][1];
-        oX1a = shX[s1][wave - 1][0];
-        oX1b = shX[s1][wave - 1][1];
+        srcD = &shD[s1][wave - 1][0];
+        srcX = &shX[s1][wave - 1][0];
       } else if (consumer) {
         const int i1 = r - blk0;
-        oD1a = lxD[i1][0];
-        oD1b = lxD[i1][1];
-        oX1a = lxX[i1][0];
-        oX1b = lxX[i1][1];
+        srcD = &lxD[i1][0];
+        srcX = &lxX[i1][0];
+      } else {
+        srcD = &shNone[0][0];
+        srcX = &shNone[1][0];
       }
+      const DT oD1a = srcD[0], oD1b = srcD[1], oX1a = srcX[0], oX1b = srcX[1];
       const DT LD1a = shr1(D1[CPL - 2], oD1a), LD1b = shr1(D1[CPL - 1], oD1b);
       const DT LX1a = shr1(X1[CPL - 2], oX1a), LX1b = shr1(X1[CPL - 1], oX1b);
 
@@ -588,8 +621,8 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
       // Hand row r to the wave on the right once it has read row r - kRing.
       if (wave + 1 < nwaves && seen_right < r - kRing + 1) {
         do {
-          seen_right = __hip_atomic_load(&prog[wave + 1], __ATOMIC_ACQUIRE,
-                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+          seen_right = __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+              &prog[wave + 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
           if (seen_right < r - kRing + 1) __builtin_amdgcn_s_sleep(1);
         } while (seen_right < r - kRing + 1);
       }

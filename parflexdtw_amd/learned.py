@@ -158,7 +158,7 @@ class LearnedRunner:
         self.cost_tile = (tr.value, tc.value)
         self._host = {}
         self.graph, self._graph, self._scores = bool(graph), None, None
-        self.nstreams, self._side, self._bufs = 4, None, None
+        self.nstreams, self._side, self._bufs = 2, None, None
         # pooled blocks of every pair, concatenated: pair p's rows are columns
         # [hb[p], hb[p] + H_p) of the K x sum(H) array, its columns likewise
         self.HW = [(-(-a // POOL), -(-b // POOL)) for a, b in self.shapes]
