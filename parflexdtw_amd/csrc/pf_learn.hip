@@ -280,8 +280,17 @@ __device__ __forceinline__ unsigned int pack_bf16x2(float lo, float hi) {
 // row's bf16 outputs into the LDS staging row ys (pixel stride kPixPad, four
 // channels = 8 bytes per lane and accumulator group), which flush_block then
 // copies to HBM in 16-byte chunks.
+// Bias of the output channels lane `lane` holds: b4[g][i] = bias[g*8 + (lane/32)*4 + i].
 template <int MODE>
-__device__ __forceinline__ void conv_epilogue(const lfloatx16_t &acc, const float *__restrict__ bias,
+__device__ __forceinline__ void load_bias4(const float *bias, int lane, lfloatx4_t (&b4)[4]) {
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+    b4[g] = (MODE == 2) ? lfloatx4_t{0.0f, 0.0f, 0.0f, 0.0f}
+                        : *reinterpret_cast<const lfloatx4_t *>(&bias[g * 8 + (lane >> 5) * 4]);
+}
+
+template <int MODE>
+__device__ __forceinline__ void conv_epilogue(const lfloatx16_t &acc, const lfloatx4_t (&b4)[4],
                                               int64_t y, int64_t x0, int64_t H, int64_t W, int lane,
                                               const __hip_bfloat16 *__restrict__ Mk, void *Yv,
                                               __hip_bfloat16 *ys) {
@@ -289,17 +298,15 @@ __device__ __forceinline__ void conv_epilogue(const lfloatx16_t &acc, const floa
   const bool in = y < H && x < W;
   const int64_t pix = in ? y * W + x : 0;
   if constexpr (MODE == 1) {
-    if (in && lane < 32) static_cast<float *>(Yv)[pix] = acc[0] + bias[0];   // the logit map
+    if (in && lane < 32) static_cast<float *>(Yv)[pix] = acc[0] + b4[0][0];   // the logit map
   } else {
     const int cq = (lane >> 5) * 4;
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-      const lfloatx4_t b4 = (MODE == 2) ? lfloatx4_t{0.0f, 0.0f, 0.0f, 0.0f}
-                                        : *reinterpret_cast<const lfloatx4_t *>(&bias[g * 8 + cq]);
       float r[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        r[i] = acc[g * 4 + i] + b4[i];
+        r[i] = acc[g * 4 + i] + b4[g][i];
         if constexpr (MODE == 0) r[i] = r[i] > 0.0f ? r[i] : 0.0f;
       }
       if constexpr (MODE == 2) {
@@ -332,18 +339,29 @@ __device__ __forceinline__ void flush_block(const __hip_bfloat16 *ys, __hip_bflo
   }
 }
 
+constexpr int kHaloChunks = (kCvRows + 2) * (kCvCols + 2) * 4;   // 16-byte chunks of a halo
+constexpr int kHaloPerThread = (kHaloChunks + 255) / 256;
+
+// A workgroup computes `seg` consecutive blocks of one 32-column strip,
+// top to bottom.  The next block's input halo is loaded into registers
+// (clamped addresses, no branch around the loads) while the current block
+// is computed and written, and is moved into LDS after it: the HBM latency
+// of a halo overlaps the MFMA work and the stores of the previous block.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_conv3x3(const __hip_bfloat16 *__restrict__ X, int64_t H,
                                                  int64_t W, const __hip_bfloat16 *__restrict__ Wt,
                                                  const float *__restrict__ bias,
                                                  const __hip_bfloat16 *__restrict__ Mk,
-                                                 void *__restrict__ Yv) {
+                                                 void *__restrict__ Yv, int seg) {
   constexpr int HR = kCvRows + 2, HC = kCvCols + 2;
   __shared__ __attribute__((aligned(16))) __hip_bfloat16 xs[HR * HC * kPixPad];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int64_t y0 = (int64_t)blockIdx.y * kCvRows, x0 = (int64_t)blockIdx.x * kCvCols;
+  const int64_t x0 = (int64_t)blockIdx.x * kCvCols;
+  const int64_t ya = (int64_t)blockIdx.y * seg * kCvRows;
+  const int64_t left = (H - ya + kCvRows - 1) / kCvRows;
+  const int nb = (int)(left < seg ? left : seg);
   const int px = lane & 31;
   const int kh = (lane >> 5) * 8;
   // weight fragments of this lane: row px (output channel), k = s*16 + kh .. +7
@@ -351,46 +369,89 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __hip_bfloat16 *__restric
 #pragma unroll
   for (int s = 0; s < 18; ++s)
     wb[s] = *reinterpret_cast<const lbf16x8_t *>(&Wt[px * kKdim + s * 16 + kh]);
-  // input halo: HR x HC pixels x 4 chunks of 8 channels, zero outside
-  for (int q = tid; q < HR * HC * 4; q += 256) {
-    const int pix = q >> 2, c8 = q & 3;
-    const int r = pix / HC, c = pix - r * HC;
-    const int64_t yy = y0 - 1 + r, xx = x0 - 1 + c;
-    const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
-    const int64_t src = ok ? (yy * W + xx) * kCh + c8 * 8 : 0;
-    const lu32x4_t v = *reinterpret_cast<const lu32x4_t *>(&X[src]);
-    *reinterpret_cast<lu32x4_t *>(&xs[pix * kPixPad + c8 * 8]) =
-        ok ? v : lu32x4_t{0u, 0u, 0u, 0u};
-  }
-  __syncthreads();
-  lfloatx16_t acc[kRowsPerWave];
+  // bias in LDS (read per block: no registers held across the sweep)
+  __shared__ __attribute__((aligned(16))) float sbias[kCh];
+  if (tid < kCh) sbias[tid] = (MODE == 2) ? 0.0f : bias[tid];
+  // input halo of the block at row y0: HR x HC pixels x 4 chunks of 8
+  // channels; thread tid owns chunks tid + 256*i
+  lu32x4_t P[kHaloPerThread];
+  // The chunk indices are recomputed from an opaque copy of tid in every
+  // block, so the compiler does not keep 2 x kHaloPerThread address
+  // registers alive across the sweep (the halo registers need the room).
+  auto halo_load = [&](const int64_t y0) {
+    int tq = tid;
+    asm volatile("" : "+v"(tq));
 #pragma unroll
-  for (int t = 0; t < kRowsPerWave; ++t) acc[t] = lfloatx16_t{};
-  const int r0 = kRowsPerWave * wave;   // first output row of the wave, halo row r0 + dy
+    for (int i = 0; i < kHaloPerThread; ++i) {
+      int q = tq + 256 * i;
+      q = q < kHaloChunks ? q : 0;
+      const int pix = q >> 2, c8 = q & 3;
+      const int r = pix / HC, c = pix - r * HC;
+      int64_t yy = y0 - 1 + r, xx = x0 - 1 + c;
+      yy = yy < 0 ? 0 : (yy >= H ? H - 1 : yy);
+      xx = xx < 0 ? 0 : (xx >= W ? W - 1 : xx);
+      P[i] = *reinterpret_cast<const lu32x4_t *>(&X[(yy * W + xx) * kCh + c8 * 8]);
+    }
+  };
+  auto halo_store = [&](const int64_t y0) {
+    int tq = tid;
+    asm volatile("" : "+v"(tq));
 #pragma unroll
-  for (int ch = 0; ch < 2; ++ch)
-#pragma unroll
-    for (int xr = 0; xr < kRowsPerWave + 2; ++xr)
-#pragma unroll
-      for (int dx = 0; dx < 3; ++dx) {
-        const lbf16x8_t a = *reinterpret_cast<const lbf16x8_t *>(
-            &xs[((r0 + xr) * HC + px + dx) * kPixPad + ch * 16 + kh]);
-#pragma unroll
-        for (int t = 0; t < kRowsPerWave; ++t) {
-          const int dy = xr - t;
-          if (dy >= 0 && dy <= 2)
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wb[(dy * 3 + dx) * 2 + ch], a,
-                                                             acc[t], 0, 0, 0);
-        }
+    for (int i = 0; i < kHaloPerThread; ++i) {
+      const int q = tq + 256 * i;
+      if (q < kHaloChunks) {
+        const int pix = q >> 2, c8 = q & 3;
+        const int r = pix / HC, c = pix - r * HC;
+        const int64_t yy = y0 - 1 + r, xx = x0 - 1 + c;
+        const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;   // zero padding
+        *reinterpret_cast<lu32x4_t *>(&xs[pix * kPixPad + c8 * 8]) =
+            ok ? P[i] : lu32x4_t{0u, 0u, 0u, 0u};
       }
-  if constexpr (MODE != 1) __syncthreads();   // every wave is done reading the halo
+    }
+  };
+  halo_load(ya);
+  halo_store(ya);
+  __syncthreads();
+  const int r0 = kRowsPerWave * wave;   // first output row of the wave, halo row r0 + dy
+  for (int b = 0; b < nb; ++b) {
+    const int64_t y0 = ya + (int64_t)b * kCvRows;
+    const bool more = b + 1 < nb;
+    if (more) halo_load(y0 + kCvRows);
+    lfloatx16_t acc[kRowsPerWave];
 #pragma unroll
-  for (int t = 0; t < kRowsPerWave; ++t)
-    conv_epilogue<MODE>(acc[t], bias, y0 + r0 + t, x0, H, W, lane, Mk, Yv,
-                        xs + (r0 + t) * kCvCols * kPixPad);
-  if constexpr (MODE != 1) {
-    __syncthreads();
-    flush_block(xs, static_cast<__hip_bfloat16 *>(Yv), y0, x0, H, W, kCvRows, tid);
+    for (int t = 0; t < kRowsPerWave; ++t) acc[t] = lfloatx16_t{};
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch)
+#pragma unroll
+      for (int xr = 0; xr < kRowsPerWave + 2; ++xr)
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+          const lbf16x8_t a = *reinterpret_cast<const lbf16x8_t *>(
+              &xs[((r0 + xr) * HC + px + dx) * kPixPad + ch * 16 + kh]);
+#pragma unroll
+          for (int t = 0; t < kRowsPerWave; ++t) {
+            const int dy = xr - t;
+            if (dy >= 0 && dy <= 2)
+              acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wb[(dy * 3 + dx) * 2 + ch], a,
+                                                               acc[t], 0, 0, 0);
+          }
+        }
+    if constexpr (MODE != 1) __syncthreads();   // every wave is done reading the halo
+    lfloatx4_t b4[4];
+    load_bias4<MODE>(sbias, lane, b4);
+#pragma unroll
+    for (int t = 0; t < kRowsPerWave; ++t)
+      conv_epilogue<MODE>(acc[t], b4, y0 + r0 + t, x0, H, W, lane, Mk, Yv,
+                          xs + (r0 + t) * kCvCols * kPixPad);
+    if constexpr (MODE != 1) {
+      __syncthreads();
+      flush_block(xs, static_cast<__hip_bfloat16 *>(Yv), y0, x0, H, W, kCvRows, tid);
+    }
+    __syncthreads();   // the halo (or the staged block) has been read
+    if (more) {
+      halo_store(y0 + kCvRows);
+      __syncthreads();
+    }
   }
 }
 
@@ -431,7 +492,9 @@ __global__ __launch_bounds__(256) void k_conv3x3_first(const __hip_bfloat16 *__r
   }
   // transposed tile: channels x pixels (see k_conv3x3)
   const lfloatx16_t acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, lfloatx16_t{}, 0, 0, 0);
-  conv_epilogue<0>(acc, bias, y0 + wave, x0, H, W, lane, nullptr, Y, ys + wave * kCvCols * kPixPad);
+  lfloatx4_t b4[4];
+  load_bias4<0>(bias, lane, b4);
+  conv_epilogue<0>(acc, b4, y0 + wave, x0, H, W, lane, nullptr, Y, ys + wave * kCvCols * kPixPad);
   __syncthreads();
   flush_block(ys, Y, y0, x0, H, W, kC1Rows, tid);
 }
@@ -485,6 +548,8 @@ __global__ __launch_bounds__(256) void k_conv3x3_first_pooled(
     const int k = kh + i;                       // tap dy*3 + dx, or padding
     b[i] = k < 9 ? (__bf16)bf2f(Wt[px * kKdim + k * kCh]) : (__bf16)0.0f;
   }
+  lfloatx4_t b4[4];
+  load_bias4<0>(bias, lane, b4);
 #pragma unroll
   for (int t = 0; t < kRowsPerWave; ++t) {
     const int orow = kRowsPerWave * wave + t;
@@ -498,7 +563,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_first_pooled(
     }
     // transposed tile: channels x pixels (see k_conv3x3)
     const lfloatx16_t acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, lfloatx16_t{}, 0, 0, 0);
-    conv_epilogue<0>(acc, bias, y0 + orow, x0, H, W, lane, nullptr, Y,
+    conv_epilogue<0>(acc, b4, y0 + orow, x0, H, W, lane, nullptr, Y,
                      ys + orow * kCvCols * kPixPad);
   }
   __syncthreads();
@@ -804,7 +869,12 @@ extern "C" int pf_conv3x3(int mode, const void *d_X, int64_t H, int64_t W, const
              "pf_conv3x3: X, weights and bias must be 16-byte aligned, Y and M 8-byte aligned");
   PF_REQUIRE((W + kC1Cols - 1) / kC1Cols < (1ll << 31) && (H + kC1Rows - 1) / kC1Rows < 65536,
              "pf_conv3x3: image too large");
-  const dim3 grid((unsigned)((W + kCvCols - 1) / kCvCols), (unsigned)((H + kCvRows - 1) / kCvRows));
+  // blocks of one column strip per workgroup: about 768 workgroups or more
+  const int64_t ncol = (W + kCvCols - 1) / kCvCols, nrow = (H + kCvRows - 1) / kCvRows;
+  int64_t segl = (ncol * nrow) / 768;
+  segl = segl < 1 ? 1 : (segl > 8 ? 8 : segl);
+  const int seg = (int)segl;
+  const dim3 grid((unsigned)ncol, (unsigned)((nrow + seg - 1) / seg));
   const dim3 grid1((unsigned)((W + kC1Cols - 1) / kC1Cols),
                    (unsigned)((H + kC1Rows - 1) / kC1Rows));
   const auto *X = static_cast<const __hip_bfloat16 *>(d_X);
@@ -812,11 +882,11 @@ extern "C" int pf_conv3x3(int mode, const void *d_X, int64_t H, int64_t W, const
   const auto *M = static_cast<const __hip_bfloat16 *>(d_M);
   hipStream_t st = as_stream(stream);
   if (mode == 0)
-    hipLaunchKernelGGL(k_conv3x3<0>, grid, dim3(256), 0, st, X, H, W, Wt, d_bias, M, d_Y);
+    hipLaunchKernelGGL(k_conv3x3<0>, grid, dim3(256), 0, st, X, H, W, Wt, d_bias, M, d_Y, seg);
   else if (mode == 1)
-    hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(256), 0, st, X, H, W, Wt, d_bias, M, d_Y);
+    hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(256), 0, st, X, H, W, Wt, d_bias, M, d_Y, seg);
   else if (mode == 2)
-    hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, st, X, H, W, Wt, d_bias, M, d_Y);
+    hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, st, X, H, W, Wt, d_bias, M, d_Y, seg);
   else if (mode == 3)
     hipLaunchKernelGGL(k_conv3x3_first, grid1, dim3(256), 0, st, X, H, W, Wt, d_bias,
                        static_cast<__hip_bfloat16 *>(d_Y));
