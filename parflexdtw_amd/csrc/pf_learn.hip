@@ -503,13 +503,16 @@ __global__ __launch_bounds__(256) void k_conv3x3_first(const __hip_bfloat16 *__r
 // (no input map): the halo value at (y, x) is the network input
 // bf16(-sum_k (A[k][y] - abar_k) (B[k][x] - bbar_k) * inv_std), computed in the
 // same order as k_write_input, so the output equals k_write_input followed by
-// k_conv3x3_first bit for bit.  A workgroup of 4 waves covers 16 rows x 32
-// columns, wave w rows 4w .. 4w+3 (one MFMA each); K <= 16.
+// k_conv3x3_first bit for bit.  A workgroup sweeps `seg` blocks of 16 rows x
+// 32 columns down one column strip (wave w rows 4w .. 4w+3 of a block, one
+// MFMA each): the strip's column features are staged once, the next block's
+// row features are loaded into registers while the current block is
+// computed and written; K <= 16.
 __global__ __launch_bounds__(256) void k_conv3x3_first_pooled(
     const float *__restrict__ A, int64_t lda, int64_t a_off, int64_t H,
     const float *__restrict__ B, int64_t ldb, int64_t b_off, int64_t W, int K,
     const float *__restrict__ st, const __hip_bfloat16 *__restrict__ Wt,
-    const float *__restrict__ bias, __hip_bfloat16 *__restrict__ Y) {
+    const float *__restrict__ bias, __hip_bfloat16 *__restrict__ Y, int seg) {
   constexpr int HR = kCvRows + 2, HC = kCvCols + 2;
   constexpr int KS = 17;   // row stride of ca / cb (odd: no bank conflicts)
   __shared__ float ca[HR * KS], cb[HC * KS];
@@ -518,29 +521,35 @@ __global__ __launch_bounds__(256) void k_conv3x3_first_pooled(
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int64_t y0 = (int64_t)blockIdx.y * kCvRows, x0 = (int64_t)blockIdx.x * kCvCols;
-  // centred features of the halo rows and columns
-  for (int q = tid; q < (HR + HC) * K; q += 256) {
-    const int r = q / K, k = q - r * K;
-    if (r < HR) {
-      const int64_t yy = y0 - 1 + r;
-      ca[r * KS + k] = (yy >= 0 && yy < H) ? A[k * lda + a_off + yy] - st[k] : 0.0f;
-    } else {
-      const int64_t xx = x0 - 1 + (r - HR);
-      cb[(r - HR) * KS + k] = (xx >= 0 && xx < W) ? B[k * ldb + b_off + xx] - st[K + k] : 0.0f;
-    }
+  const int64_t x0 = (int64_t)blockIdx.x * kCvCols;
+  const int64_t ya = (int64_t)blockIdx.y * seg * kCvRows;
+  const int64_t left = (H - ya + kCvRows - 1) / kCvRows;
+  const int nb = (int)(left < seg ? left : seg);
+  // centred features of the strip's halo columns
+  for (int q = tid; q < HC * K; q += 256) {
+    const int c = q / K, k = q - c * K;
+    const int64_t xx = x0 - 1 + c;
+    cb[c * KS + k] = (xx >= 0 && xx < W) ? B[k * ldb + b_off + xx] - st[K + k] : 0.0f;
   }
-  __syncthreads();
-  const float inv = st[2 * K];
-  for (int q = tid; q < HR * HC; q += 256) {
-    const int r = q / HC, c = q - r * HC;
-    const int64_t yy = y0 - 1 + r, xx = x0 - 1 + c;
-    const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
-    float s = 0.0f;
-    for (int k = 0; k < K; ++k) s += ca[r * KS + k] * cb[c * KS + k];
-    xs[q] = ok ? __float2bfloat16(-s * inv) : __float2bfloat16(0.0f);
+  // row features of a block's halo rows: entry q < HR*K, (r, k) = (q / K,
+  // q % K), is held in a register by thread q % 256 (slot q / 256); clamped
+  // loads, masked when stored
+  constexpr int kRowSlots = (HR * 16 + 255) / 256;
+  int ra[kRowSlots], ka[kRowSlots];
+#pragma unroll
+  for (int j = 0; j < kRowSlots; ++j) {
+    const int q = tid + 256 * j < HR * K ? tid + 256 * j : 0;
+    ra[j] = q / K;
+    ka[j] = q - ra[j] * K;
   }
-  __syncthreads();
+  auto row_load = [&](const int64_t y0, const int j) {
+    int64_t yy = y0 - 1 + ra[j];
+    yy = yy < 0 ? 0 : (yy >= H ? H - 1 : yy);
+    return A[ka[j] * lda + a_off + yy];
+  };
+  float pa[kRowSlots];
+#pragma unroll
+  for (int j = 0; j < kRowSlots; ++j) pa[j] = row_load(ya, j);
   const int px = lane & 31, kh = (lane >> 5) * 8;
   lbf16x8_t b;
 #pragma unroll
@@ -550,24 +559,50 @@ __global__ __launch_bounds__(256) void k_conv3x3_first_pooled(
   }
   lfloatx4_t b4[4];
   load_bias4<0>(bias, lane, b4);
+  const float inv = st[2 * K];
+  for (int blk = 0; blk < nb; ++blk) {
+    const int64_t y0 = ya + (int64_t)blk * kCvRows;
 #pragma unroll
-  for (int t = 0; t < kRowsPerWave; ++t) {
-    const int orow = kRowsPerWave * wave + t;
-    lbf16x8_t a;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int k = kh + i;
-      const int dy = k / 3, dx = k - (k / 3) * 3;
-      const __hip_bfloat16 xv = xs[k < 9 ? (orow + dy) * HC + px + dx : 0];
-      a[i] = k < 9 ? (__bf16)bf2f(xv) : (__bf16)0.0f;
+    for (int j = 0; j < kRowSlots; ++j) {
+      if (tid + 256 * j < HR * K) {
+        const int64_t yy = y0 - 1 + ra[j];
+        ca[ra[j] * KS + ka[j]] = (yy >= 0 && yy < H) ? pa[j] - st[ka[j]] : 0.0f;
+      }
     }
-    // transposed tile: channels x pixels (see k_conv3x3)
-    const lfloatx16_t acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, lfloatx16_t{}, 0, 0, 0);
-    conv_epilogue<0>(acc, b4, y0 + orow, x0, H, W, lane, nullptr, Y,
-                     ys + orow * kCvCols * kPixPad);
+    __syncthreads();
+    if (blk + 1 < nb) {   // in flight during this block
+#pragma unroll
+      for (int j = 0; j < kRowSlots; ++j) pa[j] = row_load(y0 + kCvRows, j);
+    }
+    for (int q = tid; q < HR * HC; q += 256) {
+      const int r = q / HC, c = q - r * HC;
+      const int64_t yy = y0 - 1 + r, xx = x0 - 1 + c;
+      const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
+      float s = 0.0f;
+      for (int k = 0; k < K; ++k) s += ca[r * KS + k] * cb[c * KS + k];
+      xs[q] = ok ? __float2bfloat16(-s * inv) : __float2bfloat16(0.0f);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < kRowsPerWave; ++t) {
+      const int orow = kRowsPerWave * wave + t;
+      lbf16x8_t a;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int k = kh + i;
+        const int dy = k / 3, dx = k - (k / 3) * 3;
+        const __hip_bfloat16 xv = xs[k < 9 ? (orow + dy) * HC + px + dx : 0];
+        a[i] = k < 9 ? (__bf16)bf2f(xv) : (__bf16)0.0f;
+      }
+      // transposed tile: channels x pixels (see k_conv3x3)
+      const lfloatx16_t acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, lfloatx16_t{}, 0, 0, 0);
+      conv_epilogue<0>(acc, b4, y0 + orow, x0, H, W, lane, nullptr, Y,
+                       ys + orow * kCvCols * kPixPad);
+    }
+    __syncthreads();
+    flush_block(ys, Y, y0, x0, H, W, kCvRows, tid);
+    __syncthreads();   // ys, xs and ca are rewritten by the next block
   }
-  __syncthreads();
-  flush_block(ys, Y, y0, x0, H, W, kCvRows, tid);
 }
 
 // Weight gradient partials: one wave per chunk of kWgChunk pixels;
@@ -820,12 +855,16 @@ extern "C" int pf_conv3x3_first_pooled(const float *d_A, int64_t lda, int64_t a_
   PF_REQUIRE(K <= 16, "pf_conv3x3_first_pooled: at most 16 features per frame");
   PF_REQUIRE(reinterpret_cast<uintptr_t>(d_Y) % 16 == 0 && reinterpret_cast<uintptr_t>(d_bias) % 16 == 0,
              "pf_conv3x3_first_pooled: Y and bias must be 16-byte aligned");
-  PF_REQUIRE((H + kCvRows - 1) / kCvRows < 65536, "pf_conv3x3_first_pooled: image too large");
-  const dim3 grid((unsigned)((W + kCvCols - 1) / kCvCols), (unsigned)((H + kCvRows - 1) / kCvRows));
+  const int64_t ncol = (W + kCvCols - 1) / kCvCols, nrow = (H + kCvRows - 1) / kCvRows;
+  int64_t segl = (ncol * nrow) / 768;
+  segl = segl < 1 ? 1 : (segl > 8 ? 8 : segl);
+  const int seg = (int)segl;
+  PF_REQUIRE((nrow + seg - 1) / seg < 65536, "pf_conv3x3_first_pooled: image too large");
+  const dim3 grid((unsigned)ncol, (unsigned)((nrow + seg - 1) / seg));
   hipLaunchKernelGGL(k_conv3x3_first_pooled, grid, dim3(256), 0, as_stream(stream), d_A, lda,
                      a_off, H, d_B, ldb, b_off, W, (int)K, d_st,
                      static_cast<const __hip_bfloat16 *>(d_Wt), d_bias,
-                     static_cast<__hip_bfloat16 *>(d_Y));
+                     static_cast<__hip_bfloat16 *>(d_Y), seg);
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
 }

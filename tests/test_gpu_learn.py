@@ -186,6 +186,65 @@ def test_conv_modes_match_oracle(env, H, W):
     assert np.all(np.abs(Gh - gref) <= 2 ** -7 * np.abs(gref) + 1e-5 * np.abs(gref).max() + 1e-6)
 
 
+def test_layer_sweeps_equal_band_by_band(env):
+    """On an image large enough for a workgroup to sweep several blocks of a
+    column strip (1100 x 1560 pooled cells), every layer kernel equals the
+    same kernel run band by band (bands of at most 400 rows with one halo
+    row on each side, so one block per workgroup), bit for bit: the sweep
+    and the prefetch of the next halo change nothing in the arithmetic."""
+    torch, tn, tile_net = env
+    lib_, check = _lib()
+    H, W, K = 1100, 1560, 12
+    rng = np.random.default_rng(77)
+    st = torch.cuda.current_stream().cuda_stream
+    X = torch.from_numpy(np.maximum(rng.standard_normal((H, W, 32)), 0).astype(np.float32))
+    X = X.to(torch.bfloat16).cuda().contiguous()
+    M = torch.from_numpy(rng.standard_normal((H, W, 32)).astype(np.float32)).to(
+        torch.bfloat16).cuda().contiguous()
+    net = tile_net.TileNet(seed=9)
+    Wt, Wf = net.Wt[1], net.Wf[1]
+    bias = net.layer(1)[32 * 288:]
+
+    def run(mode, Xs, Ms, h):
+        out = torch.empty((h, W) if mode == 1 else (h, W, 32),
+                          dtype=torch.float32 if mode == 1 else torch.bfloat16, device="cuda")
+        check(lib_.pf_conv3x3(mode, Xs.data_ptr(), h, W, (Wf if mode == 2 else Wt).data_ptr(),
+                              None if mode == 2 else bias.data_ptr(),
+                              Ms.data_ptr() if mode == 2 else None, out.data_ptr(), st), "conv")
+        return out
+
+    bands = [(a, min(a + 400, H)) for a in range(0, H, 400)]
+    for mode in (0, 1, 2):
+        full = run(mode, X, M, H)
+        for a, b in bands:
+            lo, hi = max(a - 1, 0), min(b + 1, H)
+            part = run(mode, X[lo:hi].contiguous(), M[lo:hi].contiguous(), hi - lo)
+            assert torch.equal(full[a:b].view(torch.int16) if mode != 1 else full[a:b],
+                               part[a - lo:b - lo].view(torch.int16) if mode != 1
+                               else part[a - lo:b - lo]), (mode, a)
+    # first layer from pooled features
+    A = torch.from_numpy(rng.random((K, H)).astype(np.float32) / 12).cuda()
+    B = torch.from_numpy(rng.random((K, W)).astype(np.float32) / 12).cuda()
+    stats = torch.empty(2 * K + 1, dtype=torch.float32, device="cuda")
+    pairs = torch.tensor([[0, H, 0, W]], dtype=torch.int64, device="cuda")
+    check(lib_.pf_input_stats_batch(A.data_ptr(), H, B.data_ptr(), W, K, pairs.data_ptr(), 1,
+                                    stats.data_ptr(), st), "stats")
+    b0 = net.layer(0)[32 * 288:]
+
+    def first(a_off, h):
+        out = torch.empty((h, W, 32), dtype=torch.bfloat16, device="cuda")
+        check(lib_.pf_conv3x3_first_pooled(A.data_ptr(), H, a_off, h, B.data_ptr(), W, 0, W, K,
+                                           stats.data_ptr(), net.Wt[0].data_ptr(), b0.data_ptr(),
+                                           out.data_ptr(), st), "first")
+        return out
+
+    full = first(0, H)
+    for a, b in bands:
+        lo, hi = max(a - 1, 0), min(b + 1, H)
+        part = first(lo, hi - lo)
+        assert torch.equal(full[a:b].view(torch.int16), part[a - lo:b - lo].view(torch.int16)), a
+
+
 @pytest.mark.parametrize("H,W", [(9, 31), (20, 70)])
 def test_wgrad_matches_oracle(env, H, W):
     torch, tn, _ = env
