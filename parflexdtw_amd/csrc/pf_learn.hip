@@ -268,10 +268,12 @@ typedef unsigned int lu32x4_t __attribute__((ext_vector_type(4)));
 typedef unsigned int lu32x2_t __attribute__((ext_vector_type(2)));
 typedef float lfloatx4_t __attribute__((ext_vector_type(4)));
 
+typedef __bf16 lbf16x2_t __attribute__((ext_vector_type(2)));
+typedef float lfloatx2_t __attribute__((ext_vector_type(2)));
+// Two floats rounded to bf16 (nearest even, as __float2bfloat16) in one
+// packed conversion; lo in the low half.
 __device__ __forceinline__ unsigned int pack_bf16x2(float lo, float hi) {
-  const unsigned int a = (unsigned int)__bfloat16_as_ushort(__float2bfloat16(lo));
-  const unsigned int b = (unsigned int)__bfloat16_as_ushort(__float2bfloat16(hi));
-  return a | (b << 16);
+  return __builtin_bit_cast(unsigned int, __builtin_convertvector(lfloatx2_t{lo, hi}, lbf16x2_t));
 }
 
 // Epilogue of the transposed tile acc (channels x 32 pixels) of output row y,
@@ -296,7 +298,7 @@ __device__ __forceinline__ void conv_epilogue(const lfloatx16_t &acc, const lflo
                                               __hip_bfloat16 *ys) {
   const int64_t x = x0 + (lane & 31);
   const bool in = y < H && x < W;
-  const int64_t pix = in ? y * W + x : 0;
+  const uint32_t pix = in ? (uint32_t)(y * W + x) : 0u;
   if constexpr (MODE == 1) {
     if (in && lane < 32) static_cast<float *>(Yv)[pix] = acc[0] + b4[0][0];   // the logit map
   } else {
@@ -311,7 +313,7 @@ __device__ __forceinline__ void conv_epilogue(const lfloatx16_t &acc, const lflo
       }
       if constexpr (MODE == 2) {
         // the four mask values (bf16) as floats: their bits in the high half
-        const lu32x2_t mv = in ? *reinterpret_cast<const lu32x2_t *>(&Mk[pix * kCh + g * 8 + cq])
+        const lu32x2_t mv = in ? *reinterpret_cast<const lu32x2_t *>(&Mk[pix * (uint32_t)kCh + g * 8 + cq])
                                : lu32x2_t{0u, 0u};
         const float m[4] = {__uint_as_float(mv[0] << 16), __uint_as_float(mv[0] & 0xffff0000u),
                             __uint_as_float(mv[1] << 16), __uint_as_float(mv[1] & 0xffff0000u)};
@@ -324,18 +326,22 @@ __device__ __forceinline__ void conv_epilogue(const lfloatx16_t &acc, const lflo
   }
 }
 
-// Copy a staged block of `rows` x 32 pixels (bf16 NHWC 32, pixel stride
-// kPixPad in LDS) to Y in 16-byte chunks: a block row is 2 KB contiguous.
+// Copy a staged block of `rows` (even) x 32 pixels (bf16 NHWC 32, pixel
+// stride kPixPad in LDS) to Y in 16-byte chunks: a block row is 2 KB
+// contiguous.  Thread tid copies chunk tid % 4 of pixel column (tid / 4) % 32
+// on rows tid / 128 + 2 i.  32-bit element offsets (H * W * 32 < 2^31).
 __device__ __forceinline__ void flush_block(const __hip_bfloat16 *ys, __hip_bfloat16 *__restrict__ Y,
                                             int64_t y0, int64_t x0, int64_t H, int64_t W, int rows,
                                             int tid) {
-  for (int q = tid; q < rows * kCvCols * (kCh / 8); q += 256) {
-    const int pix = q >> 2, c8 = q & 3;
-    const int rr = pix / kCvCols, xl = pix - rr * kCvCols;
-    const int64_t y = y0 + rr, x = x0 + xl;
-    if (y < H && x < W)
-      *reinterpret_cast<lu32x4_t *>(&Y[(y * W + x) * kCh + c8 * 8]) =
-          *reinterpret_cast<const lu32x4_t *>(&ys[pix * kPixPad + c8 * 8]);
+  const int c8 = tid & 3, xl = (tid >> 2) & 31, rr0 = tid >> 7;
+  const int x = (int)x0 + xl;
+  if (x >= (int)W) return;
+  for (int i = 0; i < rows / 2; ++i) {
+    const int rr = rr0 + 2 * i;
+    const int y = (int)y0 + rr;
+    if (y < (int)H)
+      *reinterpret_cast<lu32x4_t *>(&Y[(uint32_t)(y * (int)W + x) * (uint32_t)kCh + c8 * 8]) =
+          *reinterpret_cast<const lu32x4_t *>(&ys[(rr * kCvCols + xl) * kPixPad + c8 * 8]);
   }
 }
 
@@ -373,50 +379,62 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __hip_bfloat16 *__restric
   __shared__ __attribute__((aligned(16))) float sbias[kCh];
   if (tid < kCh) sbias[tid] = (MODE == 2) ? 0.0f : bias[tid];
   // input halo of the block at row y0: HR x HC pixels x 4 chunks of 8
-  // channels; thread tid owns chunks tid + 256*i
+  // channels; thread tid owns chunk c8 = tid % 4 of halo pixels
+  // (tid / 4) + 64 i = r_i * HC + c_i, stepped without division.  Element
+  // offsets fit in 32 bits (the host checks H * W * 32 < 2^31).
   lu32x4_t P[kHaloPerThread];
-  // The chunk indices are recomputed from an opaque copy of tid in every
-  // block, so the compiler does not keep 2 x kHaloPerThread address
-  // registers alive across the sweep (the halo registers need the room).
-  auto halo_load = [&](const int64_t y0) {
-    int tq = tid;
-    asm volatile("" : "+v"(tq));
+  const int c8 = tid & 3, pix0 = tid >> 2;
+  const int rh0 = pix0 / HC, ch0 = pix0 - rh0 * HC;
+  const int Hi = (int)H, Wi = (int)W, x0i = (int)x0;
+  // (r, c) restart from an opaque copy in every block, so the compiler steps
+  // them again instead of keeping kHaloPerThread addresses alive across the
+  // sweep (the halo registers need the room)
+  auto halo_load = [&](const int y0) {
+    int r = rh0, c = ch0;
+    asm volatile("" : "+v"(r), "+v"(c));
 #pragma unroll
     for (int i = 0; i < kHaloPerThread; ++i) {
-      int q = tq + 256 * i;
-      q = q < kHaloChunks ? q : 0;
-      const int pix = q >> 2, c8 = q & 3;
-      const int r = pix / HC, c = pix - r * HC;
-      int64_t yy = y0 - 1 + r, xx = x0 - 1 + c;
-      yy = yy < 0 ? 0 : (yy >= H ? H - 1 : yy);
-      xx = xx < 0 ? 0 : (xx >= W ? W - 1 : xx);
-      P[i] = *reinterpret_cast<const lu32x4_t *>(&X[(yy * W + xx) * kCh + c8 * 8]);
-    }
-  };
-  auto halo_store = [&](const int64_t y0) {
-    int tq = tid;
-    asm volatile("" : "+v"(tq));
-#pragma unroll
-    for (int i = 0; i < kHaloPerThread; ++i) {
-      const int q = tq + 256 * i;
-      if (q < kHaloChunks) {
-        const int pix = q >> 2, c8 = q & 3;
-        const int r = pix / HC, c = pix - r * HC;
-        const int64_t yy = y0 - 1 + r, xx = x0 - 1 + c;
-        const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;   // zero padding
-        *reinterpret_cast<lu32x4_t *>(&xs[pix * kPixPad + c8 * 8]) =
-            ok ? P[i] : lu32x4_t{0u, 0u, 0u, 0u};
+      int yy = y0 - 1 + r, xx = x0i - 1 + c;   // clamped: every load is in bounds
+      yy = yy < 0 ? 0 : (yy >= Hi ? Hi - 1 : yy);
+      xx = xx < 0 ? 0 : (xx >= Wi ? Wi - 1 : xx);
+      P[i] = *reinterpret_cast<const lu32x4_t *>(
+          &X[(uint32_t)(yy * Wi + xx) * (uint32_t)kCh + (uint32_t)(c8 * 8)]);
+      c += 64 - HC;   // next pixel: 64 = HC + 30
+      r += 1;
+      if (c >= HC) {
+        c -= HC;
+        r += 1;
       }
     }
   };
-  halo_load(ya);
-  halo_store(ya);
+  auto halo_store = [&](const int y0) {
+    int r = rh0, c = ch0;
+    asm volatile("" : "+v"(r), "+v"(c));
+#pragma unroll
+    for (int i = 0; i < kHaloPerThread; ++i) {
+      const int pix = pix0 + 64 * i;
+      if (pix < HR * HC) {
+        const int yy = y0 - 1 + r, xx = x0i - 1 + c;
+        const bool ok = (uint32_t)yy < (uint32_t)Hi && (uint32_t)xx < (uint32_t)Wi;   // zero padding
+        *reinterpret_cast<lu32x4_t *>(&xs[pix * kPixPad + c8 * 8]) =
+            ok ? P[i] : lu32x4_t{0u, 0u, 0u, 0u};
+      }
+      c += 64 - HC;
+      r += 1;
+      if (c >= HC) {
+        c -= HC;
+        r += 1;
+      }
+    }
+  };
+  halo_load((int)ya);
+  halo_store((int)ya);
   __syncthreads();
   const int r0 = kRowsPerWave * wave;   // first output row of the wave, halo row r0 + dy
   for (int b = 0; b < nb; ++b) {
     const int64_t y0 = ya + (int64_t)b * kCvRows;
     const bool more = b + 1 < nb;
-    if (more) halo_load(y0 + kCvRows);
+    if (more) halo_load((int)y0 + kCvRows);
     lfloatx16_t acc[kRowsPerWave];
 #pragma unroll
     for (int t = 0; t < kRowsPerWave; ++t) acc[t] = lfloatx16_t{};
@@ -449,7 +467,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __hip_bfloat16 *__restric
     }
     __syncthreads();   // the halo (or the staged block) has been read
     if (more) {
-      halo_store(y0 + kCvRows);
+      halo_store((int)y0 + kCvRows);
       __syncthreads();
     }
   }
@@ -853,6 +871,8 @@ extern "C" int pf_conv3x3_first_pooled(const float *d_A, int64_t lda, int64_t a_
                  a_off >= 0 && b_off >= 0 && a_off + H <= lda && b_off + W <= ldb,
              "pf_conv3x3_first_pooled: bad arguments");
   PF_REQUIRE(K <= 16, "pf_conv3x3_first_pooled: at most 16 features per frame");
+  PF_REQUIRE(H * W * kCh < (1ll << 31),
+             "pf_conv3x3_first_pooled: image too large (H * W * 32 must be below 2^31)");
   PF_REQUIRE(reinterpret_cast<uintptr_t>(d_Y) % 16 == 0 && reinterpret_cast<uintptr_t>(d_bias) % 16 == 0,
              "pf_conv3x3_first_pooled: Y and bias must be 16-byte aligned");
   const int64_t ncol = (W + kCvCols - 1) / kCvCols, nrow = (H + kCvRows - 1) / kCvRows;
@@ -906,8 +926,8 @@ extern "C" int pf_conv3x3(int mode, const void *d_X, int64_t H, int64_t W, const
                  reinterpret_cast<uintptr_t>(d_bias) % 16 == 0 &&
                  reinterpret_cast<uintptr_t>(d_Y) % 8 == 0 && reinterpret_cast<uintptr_t>(d_M) % 8 == 0,
              "pf_conv3x3: X, weights and bias must be 16-byte aligned, Y and M 8-byte aligned");
-  PF_REQUIRE((W + kC1Cols - 1) / kC1Cols < (1ll << 31) && (H + kC1Rows - 1) / kC1Rows < 65536,
-             "pf_conv3x3: image too large");
+  PF_REQUIRE(H * W * kCh < (1ll << 31) && (H + kC1Rows - 1) / kC1Rows < 65536,
+             "pf_conv3x3: image too large (H * W * 32 must be below 2^31)");
   // blocks of one column strip per workgroup: about 768 workgroups or more
   const int64_t ncol = (W + kCvCols - 1) / kCvCols, nrow = (H + kCvRows - 1) / kCvRows;
   int64_t segl = (ncol * nrow) / 768;
