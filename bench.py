@@ -59,6 +59,13 @@ TRAFFIC_BYTES = {("chroma_batch256_N3000-20000", 500, "materialised"): (83.97 + 
                  ("chroma_learned128_N20000-60000_L2000", 2000, "materialised"):
                      (46.70 + 6.21) * 1e9,
                  ("chroma_learned128_N20000-60000_L2000", 2000, "fused"): (1.04 + 6.19) * 1e9}
+# VALU wave instructions per computed cell of the stage-1 kernel, measured
+# with rocprofv3 PMC (SQ_INSTS_VALU / cells; profiles/pmc_valu_*.md): the
+# kernel's real ceiling is VALU issue, 1024 SIMDs x one wave64 VALU
+# instruction per 4 cycles at 2.4 GHz.
+VALU_INSTS_PER_CELL = {("chroma_learned128_N20000-60000_L2000", 2000, "fused"):
+                           1.309e10 / 2.2747e10}
+VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
 HBM_PEAK_GBS = 8000.0
 STEPS, WEIGHTS, BETA = [(1, 1), (1, 2), (2, 1)], [2, 3, 3], 0.1
 WORKLOADS = ["chroma_learned128_N20000-60000_L2000", "chroma_batch256_N3000-20000",
@@ -214,6 +221,13 @@ def bench_learned(args, torch):
     key = (args.workload, L, args.cost)
     if out["roofline"] and P == 128 and key in TRAFFIC_BYTES:
         out["roofline"]["traffic"] = TRAFFIC_BYTES[key]
+    if out["roofline"] and P == 128 and key in VALU_INSTS_PER_CELL:
+        ipc = VALU_INSTS_PER_CELL[key]
+        out["roofline"]["valu_issue"] = {
+            "insts_per_cell": round(ipc, 4),
+            "frac": round(ipc * st["cells_computed"] / (s1_ms * 1e-3) / VALU_ISSUE_PER_S, 4),
+            "note": "fraction of the VALU issue rate (the kernel's ceiling; insts per cell "
+                    "from a PMC run of this workload)"}
     if not args.no_cpu:
         from oracle import oracle
         nthreads = args.cpu_threads
