@@ -138,6 +138,11 @@ int pf_tile_backtrace(const int8_t *d_B, int64_t R, int64_t W,
  *   [9] seg_off (offset of the pair's 2*max_segs segment slots) [10] max_segs
  *   [11] path_off (offset, in points, of the pair's path buffer; capacity L1+L2+8)
  *   [12..15] reserved (0)
+ * Tile table: as for pf_stage1.  A tile whose column 0 is negative had no
+ * stage 1 (the learned form's pruned tiles): its edges read as D = +inf,
+ * P = 0, C = 0, and its slots of d_edge_D / d_edge_P / d_edge_C are never
+ * read (they need no initialisation); its back-pointer offset must point
+ * at readable zero words.
  * Workspace (device, caller-allocated, sizes from parflexdtw_amd/_plan.py):
  *   d_D_arr, d_L_arr, d_olap  f64 [ntiles*2*L]      (Parflex.py:509-564,631-855)
  *   d_bits                    uint32 [ntiles*2*bitw], bitw = (L+32)/32

@@ -93,8 +93,10 @@ __global__ __launch_bounds__(256) void k_s2_prep(const TileDesc *__restrict__ ti
     if (pos >= elen) {
       olap[q] = 0.0;
     } else {
-      const int64_t s = eP[q];
-      olap[q] = s > 0 ? eC[td.e_off + s] : eC[td.e_off + L - s];
+      // a tile without stage 1 (column 0 < 0): P = 0, C = 0, arrays not read
+      const bool live = td.c_off >= 0;
+      const int64_t s = live ? eP[q] : 0;
+      olap[q] = !live ? 0.0 : (s > 0 ? eC[td.e_off + s] : eC[td.e_off + L - s]);
       if (s > 0) atomicOr(&lb[0][s >> 5], 1u << (s & 31));
       else atomicOr(&lb[1][(-s) >> 5], 1u << ((-s) & 31));
     }
@@ -116,7 +118,8 @@ __device__ __forceinline__ bool bit_at(const uint32_t *bits, int64_t bitw, int64
 // One tile anti-diagonal d of _nb_initialize_chunks_sparse (Parflex.py:631-760)
 // and _nb_dp_fill_chunks_sparse (Parflex.py:763-855), with the valid mask of
 // _build_valid_mask (Parflex.py:582-624) evaluated per slot.
-__global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs, int64_t L,
+__global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs,
+                          const TileDesc *__restrict__ tiles, int64_t L,
                           const double *__restrict__ eD, const int32_t *__restrict__ eP,
                           const double *__restrict__ olap, const uint32_t *__restrict__ bits,
                           int64_t bitw, double *__restrict__ Darr, double *__restrict__ Larr) {
@@ -148,7 +151,10 @@ __global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs, int64_t
   }
 
   const int64_t lr = e == 0 ? g.rows - 1 : pos, lc = e == 0 ? pos : g.cols - 1;
-  const int64_t s = eP[slot];
+  // a tile without stage 1 (column 0 < 0): D = +inf, P = 0, arrays not read
+  const bool live = tiles[t].c_off >= 0;
+  const int64_t s = live ? eP[slot] : 0;
+  const double eDv = live ? eD[slot] : __builtin_huge_val();
   const int64_t sr = s < 0 ? -s : 0, sc = s > 0 ? s : 0;
   const double seg_len = (double)(iabs64(lr - sr) + iabs64(lc - sc));
 
@@ -173,13 +179,13 @@ __global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs, int64_t
     const int64_t pslot = ((pd.tile_base + pi * n2 + pj) * 2 + pedge) * L + ppos;
     const double prev = Darr[pslot];
     if (!isfinite(prev)) return;
-    Darr[slot] = (prev + eD[slot]) - olap[slot];
+    Darr[slot] = (prev + eDv) - olap[slot];
     Larr[slot] = Larr[pslot] + seg_len;
   };
 
   if (i == 0 && j == 0) {
     if (valid && (sr == 0 || sc == 0)) {
-      Darr[slot] = eD[slot];
+      Darr[slot] = eDv;
       Larr[slot] = seg_len;
     }
     return;
@@ -196,7 +202,7 @@ __global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs, int64_t
     }
     if (!valid) return;
     if (sr == 0) {
-      Darr[slot] = eD[slot];
+      Darr[slot] = eDv;
       Larr[slot] = seg_len;
     } else if (sc == 0) {
       chain(0, j - 1);
@@ -215,7 +221,7 @@ __global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs, int64_t
     }
     if (!valid) return;
     if (sc == 0) {
-      Darr[slot] = eD[slot];
+      Darr[slot] = eDv;
       Larr[slot] = seg_len;
     } else if (sr == 0) {
       chain(i - 1, 0);
@@ -244,7 +250,7 @@ __global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs, int64_t
   } else if (sc == 0) {
     chain(i, j - 1);
   } else {
-    Darr[slot] = eD[slot];
+    Darr[slot] = eDv;
     Larr[slot] = seg_len;
   }
 }
@@ -461,7 +467,8 @@ __global__ __launch_bounds__(256) void k_s2_best(const PairDesc *__restrict__ pa
 // goes to cnt[q].
 __global__ void k_s2_chain(int from_best, const int64_t *__restrict__ req, int64_t nreq,
                            const int64_t *__restrict__ besti, const PairDesc *__restrict__ pairs,
-                           int64_t L, const int32_t *__restrict__ eP, int64_t *__restrict__ chain,
+                           const TileDesc *__restrict__ tiles, int64_t L,
+                           const int32_t *__restrict__ eP, int64_t *__restrict__ chain,
                            int64_t *__restrict__ cnt) {
   const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (q >= nreq) return;
@@ -499,7 +506,8 @@ __global__ void k_s2_chain(int from_best, const int64_t *__restrict__ req, int64
       ec = g.cols - 1;
     }
     if (er < 0 || ec < 0 || er >= g.rows || ec >= g.cols) break;
-    const int64_t S = (er == g.rows - 1) ? eP[(t * 2 + 0) * L + ec] : eP[(t * 2 + 1) * L + er];
+    const int64_t S = tiles[t].c_off < 0 ? 0
+                      : (er == g.rows - 1) ? eP[(t * 2 + 0) * L + ec] : eP[(t * 2 + 1) * L + er];
     const int64_t sr = S >= 0 ? 0 : -S, sc = S >= 0 ? S : 0;
     int64_t *en = ch + ns * 8;
     en[0] = t;
@@ -794,7 +802,7 @@ static int stitch_common(hipStream_t st, int from_best, const int64_t *d_req, in
                          int64_t *chain, int64_t *pts, int use_pair_off, int64_t path_cap,
                          int64_t *path, int64_t *cnt) {
   hipLaunchKernelGGL(k_s2_chain, dim3(nblk(nreq, 64)), dim3(64), 0, st, from_best, d_req, nreq,
-                     d_besti, pairs, L, eP, chain, cnt);
+                     d_besti, pairs, tiles, L, eP, chain, cnt);
   PF_CHECK_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_s2_walk, dim3(101, (unsigned)nreq), dim3(64), 0, st, tiles, L, ss, Bp,
                      chain, cnt, pts);
@@ -844,7 +852,7 @@ extern "C" int pf_stage2(const double *d_edge_C, const int64_t *d_pairs, int64_t
     const int64_t nth = ntd * 2 * L;
     PF_REQUIRE((nth + 255) / 256 <= (1ll << 31) - 1, "pf_stage2: diagonal too large");
     hipLaunchKernelGGL(k_s2_diag, dim3((unsigned)((nth + 255) / 256), (unsigned)npairs),
-                       dim3(256), 0, st, d, pairs, L, d_edge_D, d_edge_P, d_olap, d_bits, bitw,
+                       dim3(256), 0, st, d, pairs, tiles, L, d_edge_D, d_edge_P, d_olap, d_bits, bitw,
                        d_D_arr, d_L_arr);
     PF_CHECK_HIP(hipGetLastError());
   }

@@ -282,6 +282,7 @@ class LearnedRunner:
         ws = words[sel]
         b_off[sel] = zero_words + np.concatenate([[0], np.cumsum(ws)[:-1]]) if ws.size else []
         full[:, 4] = b_off                    # pruned tiles read the zero region
+        full[~sel, 0] = -1                    # pruned: no stage 1, edges +inf (pf_stage2)
         work = full[sel].copy()
         ldc = (W[sel] + 7) // 8 * 8
         slot = R[sel] * ldc
@@ -344,10 +345,7 @@ class LearnedRunner:
         base = plan.allocate(dev, b_words=b_words)
         base["tiles"], base["pairs"] = ws["tiles"], ws["pairs"]
         base["Bp"][:(L + 15) // 16 * L].zero_()
-        ws = base
-        ws["edge_D"].fill_(math.inf)
-        ws["edge_P"].zero_()
-        ws["edge_C"].zero_()
+        ws = base   # edge arrays need no fill: stage 2 does not read pruned tiles' slots
         mark("workspace")
         if nsel:
             if self.fused:

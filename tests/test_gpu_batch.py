@@ -174,3 +174,52 @@ def test_fused_stage1_equals_materialised_bf16(torch_gpu, K, kind, L, config):
     for p in range(len(pairs)):
         assert np.array_equal(a.paths[p], b.paths[p]), p
     assert fus.C.numel() == 1   # no cost matrix is stored
+
+
+def test_stage2_reads_no_edges_of_tiles_marked_without_stage1(torch_gpu):
+    """A tile table entry with column 0 < 0 (a tile whose stage 1 did not run)
+    makes stage 2 treat the tile's edges as D = +inf, P = 0, C = 0 without
+    reading them: the same outputs as filling those edges with +inf, 0, 0,
+    whatever the unread slots hold."""
+    from parflexdtw_amd import engine
+    from parflexdtw_amd.engine import BatchRunner
+    torch = torch_gpu
+    pairs = _pairs([(1500, 1700), (900, 1250), (2100, 1900)], 4242)
+    L = 300
+    br = BatchRunner(pairs, L, STEPS, WEIGHTS, beta=0.1, dtype=torch.bfloat16)
+    br.launch()
+    torch.cuda.synchronize()
+    plan, ws = br.plan, br.ws
+    rng = np.random.default_rng(5)
+    pruned = rng.random(plan.ntiles) < 0.4
+    slots = np.repeat(pruned, 2 * L)
+    idx = torch.from_numpy(np.nonzero(slots)[0]).cuda()
+    outs = []
+    for marked in (False, True):
+        w = dict(ws)
+        for k in ("edge_D", "edge_P", "edge_C"):
+            w[k] = ws[k].clone()
+        if marked:
+            # garbage in the slots that must not be read
+            w["edge_D"][idx] = -7.0
+            w["edge_P"][idx] = 12345
+            w["edge_C"][idx] = 3.5
+            tiles = plan.tiles.copy()
+            tiles[pruned, 0] = -1
+            w["tiles"] = torch.from_numpy(tiles).cuda()
+        else:
+            w["edge_D"][idx] = float("inf")
+            w["edge_P"][idx] = 0
+            w["edge_C"][idx] = 0.0
+        for k in ("D_arr", "L_arr", "olap", "bits", "path", "path_len", "best_f", "best_i"):
+            w[k] = torch.empty_like(ws[k])
+        engine.stage2(None, plan, br.sw, w, None)
+        torch.cuda.synchronize()
+        outs.append(w)
+    a, b = outs
+    for k in ("D_arr", "L_arr", "olap", "bits", "best_f", "best_i", "path_len"):
+        assert torch.equal(a[k].view(torch.int64) if a[k].dtype == torch.float64 else a[k],
+                           b[k].view(torch.int64) if b[k].dtype == torch.float64 else b[k]), k
+    ra, rb = engine.collect(plan, a), engine.collect(plan, b)
+    for p in range(len(pairs)):
+        assert np.array_equal(ra.paths[p], rb.paths[p]), p
