@@ -55,19 +55,13 @@ __global__ void k_fill_i64(int64_t *p, int64_t v, int64_t n) {
        i += (int64_t)gridDim.x * blockDim.x)
     p[i] = v;
 }
-__global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = v;
-}
 
 // _build_edge_data (Parflex.py:509-564) + _edge_starts_from_S
 // (Parflex.py:347-373): overlap cost C[start] per slot (from the first row /
 // first column of C that stage 1 stored per tile), start-set bitmaps of
 // every tile (bit s of map 0: some edge cell starts at (0, s), s > 0; bit a
 // of map 1: some edge cell starts at (a, 0), a >= 0), and D_arr = L_arr =
-// inf.  Workgroup (t, y) covers slots y*256 .. y*256+255 of tile t; the
-// bitmaps are gathered in LDS and merged into global memory once per word.
+// inf.
 constexpr int kMaxBitw = 8192 / 32 + 1;
 
 __global__ __launch_bounds__(256) void k_s2_prep(const TileDesc *__restrict__ tiles, int64_t L,
@@ -77,24 +71,26 @@ __global__ __launch_bounds__(256) void k_s2_prep(const TileDesc *__restrict__ ti
                                                  double *__restrict__ Larr,
                                                  double *__restrict__ olap,
                                                  uint32_t *__restrict__ bits, int64_t bitw) {
+  // One workgroup per tile: its 2L slots, and its two start bitmaps built in
+  // LDS and stored whole (the workgroup owns them: no global atomics and no
+  // clearing pass).
   __shared__ uint32_t lb[2][kMaxBitw];
   const int64_t t = blockIdx.x;
-  for (int w = threadIdx.x; w < 2 * kMaxBitw; w += blockDim.x) lb[w / kMaxBitw][w % kMaxBitw] = 0u;
+  for (int w = threadIdx.x; w < 2 * bitw; w += blockDim.x) lb[w >= bitw][w - (w >= bitw) * bitw] = 0u;
   __syncthreads();
-  const int64_t rem = blockIdx.y * (int64_t)blockDim.x + threadIdx.x;
-  if (rem < 2 * L) {
+  const TileDesc td = tiles[t];
+  // a tile without stage 1 (column 0 < 0): P = 0, C = 0, arrays not read
+  const bool live = td.c_off >= 0;
+  for (int64_t rem = threadIdx.x; rem < 2 * L; rem += blockDim.x) {
     const int64_t q = t * 2 * L + rem;
     const int e = rem >= L ? 1 : 0;
     const int64_t pos = rem - (int64_t)e * L;
     Darr[q] = __builtin_huge_val();
     Larr[q] = __builtin_huge_val();
-    const TileDesc td = tiles[t];
     const int64_t elen = e == 0 ? td.W : td.R;
     if (pos >= elen) {
       olap[q] = 0.0;
     } else {
-      // a tile without stage 1 (column 0 < 0): P = 0, C = 0, arrays not read
-      const bool live = td.c_off >= 0;
       const int64_t s = live ? eP[q] : 0;
       olap[q] = !live ? 0.0 : (s > 0 ? eC[td.e_off + s] : eC[td.e_off + L - s]);
       if (s > 0) atomicOr(&lb[0][s >> 5], 1u << (s & 31));
@@ -105,8 +101,7 @@ __global__ __launch_bounds__(256) void k_s2_prep(const TileDesc *__restrict__ ti
   for (int w = threadIdx.x; w < 2 * bitw; w += blockDim.x) {
     const int m = w >= bitw ? 1 : 0;
     const int64_t k = w - m * bitw;
-    const uint32_t v = lb[m][k];
-    if (v) atomicOr(bits + (t * 2 + m) * bitw + k, v);
+    bits[(t * 2 + m) * bitw + k] = lb[m][k];
   }
 }
 
@@ -839,13 +834,9 @@ extern "C" int pf_stage2(const double *d_edge_C, const int64_t *d_pairs, int64_t
   const PairDesc *pairs = reinterpret_cast<const PairDesc *>(d_pairs);
   const TileDesc *tiles = reinterpret_cast<const TileDesc *>(d_tiles);
   const int64_t bitw = (L + 32) / 32;
-  const int64_t nbits = ntiles * 2 * bitw;
-  hipLaunchKernelGGL(k_fill_u32, dim3(nblk(nbits, 256)), dim3(256), 0, st, d_bits, 0u, nbits);
-  PF_CHECK_HIP(hipGetLastError());
   PF_REQUIRE(ntiles < (1ll << 31), "pf_stage2: too many tiles");
-  hipLaunchKernelGGL(k_s2_prep, dim3((unsigned)ntiles, (unsigned)((2 * L + 255) / 256)),
-                     dim3(256), 0, st, tiles, L, d_edge_P, d_edge_C, d_D_arr, d_L_arr, d_olap,
-                     d_bits, bitw);
+  hipLaunchKernelGGL(k_s2_prep, dim3((unsigned)ntiles), dim3(256), 0, st, tiles, L, d_edge_P,
+                     d_edge_C, d_D_arr, d_L_arr, d_olap, d_bits, bitw);
   PF_CHECK_HIP(hipGetLastError());
   for (int64_t d = 0; d < max_diag; ++d) {
     const int64_t ntd = (d + 1) < max_tile_diag ? (d + 1) : max_tile_diag;
