@@ -543,12 +543,16 @@ __global__ __launch_bounds__(256) void k_conv3x3_first_pooled(
   const int64_t ya = (int64_t)blockIdx.y * seg * kCvRows;
   const int64_t left = (H - ya + kCvRows - 1) / kCvRows;
   const int nb = (int)(left < seg ? left : seg);
-  // centred features of the strip's halo columns
-  for (int q = tid; q < HC * K; q += 256) {
-    const int c = q / K, k = q - c * K;
+  // centred features of the strip's halo columns; entries k >= K of ca and
+  // cb are zero, so the dot products below run over 16 terms unrolled (the
+  // extra +0.0 terms leave every sum bit-identical)
+  for (int q = tid; q < HC * 16; q += 256) {
+    const int c = q >> 4, k = q & 15;
     const int64_t xx = x0 - 1 + c;
-    cb[c * KS + k] = (xx >= 0 && xx < W) ? B[k * ldb + b_off + xx] - st[K + k] : 0.0f;
+    cb[c * KS + k] = (k < K && xx >= 0 && xx < W) ? B[k * ldb + b_off + xx] - st[K + k] : 0.0f;
   }
+  for (int q = tid; q < HR * 16; q += 256)
+    if ((q & 15) >= K) ca[(q >> 4) * KS + (q & 15)] = 0.0f;
   // row features of a block's halo rows: entry q < HR*K, (r, k) = (q / K,
   // q % K), is held in a register by thread q % 256 (slot q / 256); clamped
   // loads, masked when stored
@@ -597,7 +601,8 @@ __global__ __launch_bounds__(256) void k_conv3x3_first_pooled(
       const int64_t yy = y0 - 1 + r, xx = x0 - 1 + c;
       const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
       float s = 0.0f;
-      for (int k = 0; k < K; ++k) s += ca[r * KS + k] * cb[c * KS + k];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) s += ca[r * KS + k] * cb[c * KS + k];
       xs[q] = ok ? __float2bfloat16(-s * inv) : __float2bfloat16(0.0f);
     }
     __syncthreads();
@@ -609,8 +614,8 @@ __global__ __launch_bounds__(256) void k_conv3x3_first_pooled(
       for (int i = 0; i < 8; ++i) {
         const int k = kh + i;
         const int dy = k / 3, dx = k - (k / 3) * 3;
-        const __hip_bfloat16 xv = xs[k < 9 ? (orow + dy) * HC + px + dx : 0];
-        a[i] = k < 9 ? (__bf16)bf2f(xv) : (__bf16)0.0f;
+        const __bf16 xv = reinterpret_cast<const __bf16 *>(xs)[k < 9 ? (orow + dy) * HC + px + dx : 0];
+        a[i] = k < 9 ? xv : (__bf16)0.0f;
       }
       // transposed tile: channels x pixels (see k_conv3x3)
       const lfloatx16_t acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, lfloatx16_t{}, 0, 0, 0);
