@@ -124,18 +124,57 @@ def _base(args, value, unit, ms_step, dtype, config):
 
 
 # ---------------------------------------------------------------------------
+EVAL_TOL_MS = (25, 50, 100, 200, 500)
+
+
+def _evaluation(torch, gts, hyps, runner=None):
+    """Error rate versus tolerance (05_Evaluate_Benchmarks_L.ipynb#cell23)
+    of each set of paths in `hyps` against the synthetic ground-truth beats,
+    computed by the device evaluator; with `runner`, also the evaluator read
+    in place from the runner's device paths, its launch timed with HIP
+    events (median of 5)."""
+    from parflexdtw_amd.evaluate import HOP_SEC, evaluate_batch, evaluate_paths
+    out = {"metric": "error rate at tolerance (fraction of beats with |error| > t), "
+                     "05_Evaluate_Benchmarks_L.ipynb#cell23",
+           "ground_truth": "synthetic beats every 22 F2 frames (0.51 s) with their F1 "
+                           "position from the known warp; hop 512/22050 s"}
+    for name, (paths, g) in hyps.items():
+        r = evaluate_paths(paths, g, HOP_SEC, 1000)
+        out[name] = {"beats": r.beats,
+                     **{f"{t}ms": round(float(r.rates[t]), 6) for t in EVAL_TOL_MS}}
+    if runner is not None:
+        st = torch.cuda.current_stream()
+        ts = []
+        for _ in range(5):
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            r = evaluate_batch(runner, gts, HOP_SEC, 1000, stream=st, events=ev)
+            ts.append(ev[0].elapsed_time(ev[1]))
+        us = float(np.median(ts)) * 1e3
+        out["device_paths"] = {"beats": r.beats,
+                               **{f"{t}ms": round(float(r.rates[t]), 6) for t in EVAL_TOL_MS}}
+        out["kernel"] = {"name": "k_eval_paths", "us": round(us, 2),
+                         "beats_per_s": round(r.beats / (us * 1e-6), 1),
+                         "note": "paths read in place from stage 2's device buffer; one "
+                                 "bisection per beat, 1001 thresholds"}
+    return out
+
+
 def bench_learned(args, torch):
     from parflexdtw_amd import Parflex_gpu as PG
     from parflexdtw_amd.gpu_flexdtw import cost_matrix_from_features_gpu
     from parflexdtw_amd.learned import LearnedRunner
-    from parflexdtw_amd.synth import batch_pairs, path_agreement
+    from parflexdtw_amd.evaluate import HOP_SEC
+    from parflexdtw_amd.synth import batch_pairs, beat_times, path_agreement
     from parflexdtw_amd.tile_net import TileNet
 
     P = args.pairs or 128
     net, meta = TileNet.load(MODEL)
     L = int(meta["L"])
     t0 = time.perf_counter()
-    pairs = [(a, b) for a, b, _ in batch_pairs(P, 90000, 20000, 60000)]
+    trip = batch_pairs(P, 90000, 20000, 60000)
+    pairs = [(a, b) for a, b, _ in trip]
+    gts = [beat_times(w, HOP_SEC, every=22) for _, _, w in trip]
+    del trip
     shapes = [(F1.shape[1], F2.shape[1]) for F1, F2 in pairs]
     log(f"{P} held-out pairs (seeds 90000+, N ~ U[20000,60000]) in "
         f"{time.perf_counter() - t0:.1f} s; {sum(a * b for a, b in shapes) / 1e9:.1f} Gcells; "
@@ -214,6 +253,8 @@ def bench_learned(args, torch):
                     "train_N": f"U[{int(meta['n_lo'])},{int(meta['n_hi'])}]",
                     "epochs": int(meta["epochs_done"]), "threshold": float(meta["threshold"]),
                     "ext_cells": int(meta["ext"]), "guard_frames": int(meta["guard"])}
+    out["evaluation"] = _evaluation(torch, gts, {
+        "learned": (res.paths, gts), "plain_fp64": ([wp for _, wp in plain], gts)})
     fused = args.cost == "fused"
     out["roofline"] = _roofline(
         st["cells_computed"], s1_ms, 0.25 if fused else 2.25,
@@ -252,11 +293,15 @@ def bench_batch(args, torch):
     from parflexdtw_amd import Parflex_gpu as PG
     from parflexdtw_amd import engine
     from parflexdtw_amd.gpu_flexdtw import cost_matrix_from_features_gpu
-    from parflexdtw_amd.synth import batch_pairs, path_agreement
+    from parflexdtw_amd.evaluate import HOP_SEC
+    from parflexdtw_amd.synth import batch_pairs, beat_times, path_agreement
 
     t0 = time.perf_counter()
     args.pairs = args.pairs or 256
-    pairs = [(F1, F2) for F1, F2, _ in batch_pairs(args.pairs)]
+    trip = batch_pairs(args.pairs)
+    pairs = [(F1, F2) for F1, F2, _ in trip]
+    gts = [beat_times(w, HOP_SEC, every=22) for _, _, w in trip]
+    del trip
     shapes = [(F1.shape[1], F2.shape[1]) for F1, F2 in pairs]
     log(f"{len(pairs)} pairs generated in {time.perf_counter() - t0:.1f} s; "
         f"{sum(a * b for a, b in shapes) / 1e9:.2f} Gcells")
@@ -348,6 +393,9 @@ def bench_batch(args, torch):
         f"k_stage1<2,__hip_bfloat16,float,false,true{',true' if fused else ''}>")
     if args.pairs == 256 and (args.workload, L, args.cost) in TRAFFIC_BYTES:
         out["roofline"]["traffic"] = TRAFFIC_BYTES[(args.workload, L, args.cost)]
+    out["evaluation"] = _evaluation(torch, gts, {
+        "batched_sample": (res.paths[:ns], gts[:ns]),
+        "plain_fp64_sample": ([wp for _, wp in plain], gts[:ns])}, runner=br)
 
     if not args.no_cpu:
         from oracle import oracle

@@ -308,6 +308,32 @@ int pf_selection_check(const int64_t *d_pairs, int64_t npairs, int64_t max_path,
 int pf_pack_paths(const int64_t *d_pairs, int64_t npairs, int64_t max_path,
                   const int64_t *d_path, const int64_t *d_path_len, int64_t *d_out, void *stream);
 
+/* Alignment errors of many warping paths against ground-truth time pairs,
+ * and the error-rate-versus-tolerance histogram, in one launch.  Replaces
+ * the per-file loop of 05_Evaluate_Benchmarks_L.ipynb#cell7-8 (and
+ * eval_tools.py:33-46 with hop = 1) and the tolerance sweep of #cell23.
+ *
+ * Query q's path has d_lens[q] points; point j is (d_path[d_offs[2q] +
+ * j*stride], d_path[d_offs[2q+1] + j*stride]) in frames (int64), rows
+ * non-decreasing.  Stage 2's interleaved buffer is stride 2 with offsets
+ * (2*path_off, 2*path_off + 1); a packed (2, n) array is stride 1 with
+ * offsets (o, o + n).  Query q's beats are [d_gt_off[q], d_gt_off[q+1]) of
+ * d_gt_x (query times, seconds) and d_gt_y (reference times, seconds);
+ * max_beats >= every query's beat count.  For each beat b:
+ *   d_err[b] = np.interp(gt_x[b], rows * hop, cols * hop) - gt_y[b]
+ * with np.interp's semantics (left/right clamping, NaN x gives NaN, one
+ * sample point gives that point), bit for bit; +inf for a query with an
+ * empty path (05_Evaluate_Benchmarks_L.ipynb#cell8's value for a missing
+ * hypothesis).  If d_hist is not null (uint64, ntol + 1 bins, accumulated:
+ * the caller clears it), d_hist[k] += 1 for k = #{i : |err| > d_tol[i]}
+ * (d_tol ascending, 1 <= ntol <= PF_EVAL_MAX_TOL), so that the error rate
+ * at threshold i is sum_{k > i} d_hist[k] / #beats. */
+#define PF_EVAL_MAX_TOL 4096
+int pf_eval_paths(const int64_t *d_path, int64_t stride, const int64_t *d_offs,
+                  const int64_t *d_lens, int64_t nq, const int64_t *d_gt_off, int64_t max_beats,
+                  const double *d_gt_x, const double *d_gt_y, double hop, const double *d_tol,
+                  int64_t ntol, double *d_err, uint64_t *d_hist, void *stream);
+
 /* Fill helpers used instead of hipMemsetAsync (safe under graph capture). */
 int pf_fill_f64(double *d_p, double v, int64_t n, void *stream);
 int pf_fill_i64(int64_t *d_p, int64_t v, int64_t n, void *stream);

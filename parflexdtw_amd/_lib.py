@@ -66,6 +66,8 @@ _SIGNATURES = {
     "pf_pack_weights": (_int, [_vp, _vp, _vp, _vp]),
     "pf_tile_scores": (_int, [_vp, _i64, _i64, _int, _i64, _i64, _i64, _int, _vp, _vp]),
     "pf_selection_check": (_int, [_vp, _i64, _i64, _vp, _vp, _vp, _i64, _i64, _vp, _vp]),
+    "pf_eval_paths": (_int, [_vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _dbl, _vp, _i64,
+                             _vp, _vp, _vp]),
     "pf_fill_f64": (_int, [_vp, _dbl, _i64, _vp]),
     "pf_fill_i64": (_int, [_vp, _i64, _i64, _vp]),
 }

@@ -130,3 +130,12 @@ def path_agreement(wp_ref, wp, tol=2):
     pred = np.interp(wp_ref[0].astype(np.float64), wp[0].astype(np.float64),
                      wp[1].astype(np.float64))
     return float(np.mean(np.abs(pred - wp_ref[1]) <= tol))
+
+
+def beat_times(warp, hop, every=22, offset=0):
+    """Ground-truth beat pairs of a synthetic pair in seconds: reference
+    (F2) frames offset, offset+every, ... and the fractional F1 frame each
+    was sampled at (make_pair's warp), i.e. (query times, reference times)
+    as the .beat annotations of a real pair give them."""
+    m = np.arange(offset, len(warp), every)
+    return np.asarray(warp, dtype=np.float64)[m] * hop, m.astype(np.float64) * hop
