@@ -122,21 +122,29 @@ __global__ __launch_bounds__(256) void k_cost_f64(const double *__restrict__ F1n
 // BF16 MFMA form (precision form): features normalised in FP64, rounded to
 // BF16 and stored frame-major with K padded to a multiple of 16; the product
 // runs on v_mfma_f32_32x32x16_bf16 with FP32 accumulation.  A workgroup of
-// 4 waves computes a 32 x 512 block of C (wide, so that each row of the
-// block is one long contiguous run in HBM), each wave 32 x 128 as four MFMA
-// blocks of 32 x 32 sharing the A operand.  Operand layout (per lane l of a
-// wave): A row l%32, k-offset (l/32)*8, 8 consecutive k; B column l%32,
-// same k; D column l%32, rows (v/4)*8 + (l/32)*4 + v%4 for the 16
-// accumulators v.  The block is staged through LDS and written row by row,
-// 16 bytes per lane where the row is 16-byte aligned in C.
-
+// 4 waves computes a 128 x 256 block of C, the waves in a 2 x 2 arrangement,
+// each 64 x 128 as 2 x 4 MFMA blocks of 32 x 32: every A fragment feeds four
+// MFMAs and every B fragment two, and the workgroup reads (128 + 256) rows
+// of K features for 32768 cells (about 0.023 K bytes per cell), so large K
+// stays bound by the C write rather than by operand traffic.  Operand layout
+// (per lane l of a wave): A row l%32, k-offset (l/32)*8, 8 consecutive k;
+// B column l%32, same k; D column l%32, rows (v/4)*8 + (l/32)*4 + v%4 for
+// the 16 accumulators v.  Every cell sees the same instruction sequence over
+// k as in the stage-1 sweep that computes C itself, so both give the same
+// bits.  Each half of the block (64 rows) is staged through LDS and written
+// row by row, 16 bytes per lane where the row is 16-byte aligned in C.
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx16_t __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
-// Fb[i*Kp + k] = bf16(F[k*n + i] / (sqrt(sum_k F[k*n+i]^2) + 1e-9)), zero pad.
+// Frame i's feature k, bf16(F[k*n + i] / (sqrt(sum_k F[k*n+i]^2) + 1e-9)),
+// zero padded to Kp, goes to Fb[i*Kp + k] (frame-major, the layout stage 1
+// reads) or, when `blocked`, to Fb[((k/16)*n + i)*16 + k%16]: chunks of 16
+// features, all frames of a chunk together, so that an MFMA fragment of 32
+// frames x 8 features is one contiguous 512-byte run per half wave.  For
+// Kp = 16 the two layouts are the same.
 __global__ void k_cost_prep_bf16(const double *__restrict__ F, int64_t K, int64_t n, int64_t Kp,
-                                 __hip_bfloat16 *__restrict__ Fb) {
+                                 __hip_bfloat16 *__restrict__ Fb, int blocked) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
@@ -145,8 +153,10 @@ __global__ void k_cost_prep_bf16(const double *__restrict__ F, int64_t K, int64_
     s = s + v * v;
   }
   const double nrm = sqrt(s) + 1e-9;
-  for (int64_t k = 0; k < Kp; ++k)
-    Fb[i * Kp + k] = __float2bfloat16(k < K ? (float)(F[k * n + i] / nrm) : 0.0f);
+  for (int64_t k = 0; k < Kp; ++k) {
+    const int64_t at = blocked ? ((k >> 4) * n + i) * 16 + (k & 15) : i * Kp + k;
+    Fb[at] = __float2bfloat16(k < K ? (float)(F[k * n + i] / nrm) : 0.0f);
+  }
 }
 
 __device__ __forceinline__ float cvt_out(float v, float *) { return v; }
@@ -154,77 +164,145 @@ __device__ __forceinline__ __hip_bfloat16 cvt_out(float v, __hip_bfloat16 *) {
   return __float2bfloat16(v);
 }
 
-constexpr int kMfmaRows = 32, kMfmaCols = 512;
+constexpr int kMfmaRows = 128, kMfmaCols = 256;
 constexpr int kTilePad = 8;  // keeps staged rows 16-byte aligned, no bank conflicts
 
 template <typename OT, bool BATCH>
 __global__ __launch_bounds__(256) void k_cost_mfma_bf16(const __hip_bfloat16 *__restrict__ A0,
                                                         const __hip_bfloat16 *__restrict__ B0,
-                                                        int64_t Kp, CostJob one,
+                                                        int64_t n1, int64_t n2, int64_t Kp,
+                                                        CostJob one,
                                                         const CostJob *__restrict__ jobs,
                                                         int64_t njobs, OT *__restrict__ C,
                                                         int vec_ok) {
-  __shared__ __attribute__((aligned(16))) OT tile[kMfmaRows][kMfmaCols + kTilePad];
+  constexpr int kHalf = kMfmaRows / 2;
+  __shared__ __attribute__((aligned(16))) OT tile[kHalf][kMfmaCols + kTilePad];
   int64_t blk;
   const CostJob jb = find_job<BATCH>(one, jobs, njobs, blk);
   const int64_t N = jb.N, M = jb.M;
+  const int64_t nbr = (N + kMfmaRows - 1) / kMfmaRows;
   const int64_t nbc = (M + kMfmaCols - 1) / kMfmaCols;
-  const int64_t row0 = (blk / nbc) * kMfmaRows, col0 = (blk % nbc) * kMfmaCols;
-  const __hip_bfloat16 *A = A0 + jb.f1_off * Kp;
-  const __hip_bfloat16 *B = B0 + jb.f2_off * Kp;
+  // XCD-aware order.  Workgroups go to the 8 XCDs round robin, so the
+  // workgroups with equal blk % 8 share one XCD (and its L2).  Each such
+  // class takes one contiguous eighth of the sequence `lin` (the blocks past
+  // the last multiple of 8 keep their place).  `lin` runs over strips of
+  // kStrip column blocks, each strip row by row: consecutive workgroups of
+  // an XCD reuse one A row block kStrip times, and the strip's B columns
+  // (kStrip x 256 frames) stay in that XCD's L2.  A bijection on
+  // [0, nbr*nbc).
+  constexpr int64_t kStrip = 16;
+  const int64_t nblk = nbr * nbc, q = nblk >> 3;
+  const int64_t lin = blk < (q << 3) ? (blk & 7) * q + (blk >> 3) : blk;
+  const int64_t wide = nbc / kStrip * kStrip, in_wide = nbr * wide;
+  int64_t rb, cbk;
+  if (lin < in_wide) {
+    const int64_t t = lin % (nbr * kStrip);
+    rb = t / kStrip;
+    cbk = lin / (nbr * kStrip) * kStrip + t % kStrip;
+  } else {
+    const int64_t t = lin - in_wide, w = nbc - wide;
+    rb = t / w;
+    cbk = wide + t % w;
+  }
+  const int64_t row0 = rb * kMfmaRows, col0 = cbk * kMfmaCols;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int wc = wave * 128;
+  const int wr = wave >> 1, wc = (wave & 1) * 128;
   const int lr = lane & 31, kh = (lane >> 5) * 8;
-  // rows/columns past the matrix read the last valid frame (results unused)
-  const int64_t ra = row0 + lr < N ? row0 + lr : N - 1;
-  int64_t cb[4];
+  // Operands in the blocked layout of k_cost_prep_bf16: frame f, features
+  // [kk + kh, kk + kh + 8) at ((kk/16)*n + f)*16 + kh.  Rows/columns past
+  // the matrix read the last valid frame (results unused).  Element offsets
+  // fit 32 bits: (n1 + n2) * Kp < 2^31 (host).
+  uint32_t ra[2], cb[4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) cb[j] = col0 + wc + j * 32 + lr < M ? col0 + wc + j * 32 + lr : M - 1;
-  floatx16_t acc[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) acc[j] = floatx16_t{};
-  for (int64_t kk = 0; kk < Kp; kk += 16) {
-    const bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(A + ra * Kp + kk + kh);
-    bf16x8_t b[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8_t *>(B + cb[j] * Kp + kk + kh);
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[j], acc[j], 0, 0, 0);
+  for (int i = 0; i < 2; ++i) {
+    const int64_t r = row0 + wr * 64 + i * 32 + lr;
+    ra[i] = (uint32_t)((jb.f1_off + (r < N ? r : N - 1)) * 16 + kh);
   }
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
+  for (int j = 0; j < 4; ++j) {
+    const int64_t c = col0 + wc + j * 32 + lr;
+    cb[j] = (uint32_t)((jb.f2_off + (c < M ? c : M - 1)) * 16 + kh);
+  }
+  const uint32_t sa = (uint32_t)(n1 * 16), sb = (uint32_t)(n2 * 16);
+  floatx16_t acc[2][4];
 #pragma unroll
-    for (int v = 0; v < 16; ++v)
-      tile[(v >> 2) * 8 + (lane >> 5) * 4 + (v & 3)][wc + j * 32 + lr] =
-          cvt_out(1.0f - acc[j][v], (OT *)nullptr);
-  __syncthreads();
-  // One row per wave per step: 16-byte chunks of E elements when the row's
-  // first element is 16-byte aligned in C, else one element per lane.
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = floatx16_t{};
+  // Operands of step k+16 are loaded before the MFMAs of step k are issued.
+  bf16x8_t a[2], b[4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8_t *>(A0 + ra[i]);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8_t *>(B0 + cb[j]);
+  const uint32_t nk = (uint32_t)(Kp >> 4);
+  for (uint32_t kc = 0; kc < nk; ++kc) {
+    bf16x8_t an[2], bn[4];
+    const bool more = kc + 1 < nk;
+    if (more) {
+      const __hip_bfloat16 *An = A0 + (kc + 1) * sa;
+      const __hip_bfloat16 *Bn = B0 + (kc + 1) * sb;
+#pragma unroll
+      for (int i = 0; i < 2; ++i) an[i] = *reinterpret_cast<const bf16x8_t *>(An + ra[i]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bn[j] = *reinterpret_cast<const bf16x8_t *>(Bn + cb[j]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    if (more) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = an[i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = bn[j];
+    }
+  }
   constexpr int E = 16 / (int)sizeof(OT);
-  const int64_t rows = N - row0 < kMfmaRows ? N - row0 : kMfmaRows;
   const int64_t cols = M - col0 < kMfmaCols ? M - col0 : kMfmaCols;
-  for (int rr = wave; rr < rows; rr += 4) {
-    const int64_t e0 = jb.c_off + (row0 + rr) * jb.ldc + col0;
-    OT *dst = C + e0;
-    if (vec_ok && (e0 % E) == 0) {
 #pragma unroll
-      for (int h = 0; h < kMfmaCols; h += 64 * E) {
-        const int c = h + lane * E;
-        if (c + E <= cols) {
-          const u32x4_t v = *reinterpret_cast<const u32x4_t *>(&tile[rr][c]);
-          __builtin_nontemporal_store(v, reinterpret_cast<u32x4_t *>(dst + c));
-        } else {
+  for (int h = 0; h < 2; ++h) {
+    if (h) __syncthreads();  // the previous half has been written out
+    if (wr == h) {
 #pragma unroll
-          for (int e = 0; e < E; ++e)
-            if (c + e < cols) st_nt(dst + c + e, tile[rr][c + e]);
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int v = 0; v < 16; ++v)
+            tile[i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3)][wc + j * 32 + lr] =
+                cvt_out(1.0f - acc[i][j][v], (OT *)nullptr);
+    }
+    __syncthreads();
+    // One row per wave per step: 16-byte chunks of E elements when the row's
+    // first element is 16-byte aligned in C, else one element per lane.
+    const int64_t r0 = row0 + h * kHalf;
+    const int64_t rows = N - r0 < kHalf ? N - r0 : kHalf;
+    for (int rr = wave; rr < rows; rr += 4) {
+      const int64_t e0 = jb.c_off + (r0 + rr) * jb.ldc + col0;
+      OT *dst = C + e0;
+      if (vec_ok && (e0 % E) == 0) {
+#pragma unroll
+        for (int g = 0; g < kMfmaCols; g += 64 * E) {
+          const int c = g + lane * E;
+          if (c < kMfmaCols) {
+            if (c + E <= cols) {
+              const u32x4_t v = *reinterpret_cast<const u32x4_t *>(&tile[rr][c]);
+              __builtin_nontemporal_store(v, reinterpret_cast<u32x4_t *>(dst + c));
+            } else {
+#pragma unroll
+              for (int e = 0; e < E; ++e)
+                if (c + e < cols) st_nt(dst + c + e, tile[rr][c + e]);
+            }
+          }
         }
-      }
-    } else {
+      } else {
 #pragma unroll
-      for (int h = 0; h < kMfmaCols; h += 64)
-        if (h + lane < cols) st_nt(dst + h + lane, tile[rr][h + lane]);
+        for (int g = 0; g < kMfmaCols; g += 64)
+          if (g + lane < cols) st_nt(dst + g + lane, tile[rr][g + lane]);
+      }
     }
   }
 }
@@ -269,23 +347,25 @@ int cost_launch(int out_dtype, const double *d_F1, int64_t n1, const double *d_F
   PF_REQUIRE(nblocks >= 1 && nblocks < (1ll << 31), "pf_cost: grid size out of range");
   if (out_dtype == PF_F32 || out_dtype == PF_BF16) {
     const int64_t Kp = (K + 15) / 16 * 16;
+    PF_REQUIRE((n1 + n2) * Kp < (1ll << 31), "pf_cost: (frames of both sides) x K too large");
     PF_REQUIRE(reinterpret_cast<uintptr_t>(d_work) % 16 == 0,
                "pf_cost: d_work must be 16-byte aligned");
     __hip_bfloat16 *A = reinterpret_cast<__hip_bfloat16 *>(d_work);
     __hip_bfloat16 *B = A + n1 * Kp;
     hipLaunchKernelGGL(k_cost_prep_bf16, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, st,
-                       d_F1, K, n1, Kp, A);
+                       d_F1, K, n1, Kp, A, 1);
     PF_CHECK_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_cost_prep_bf16, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st,
-                       d_F2, K, n2, Kp, B);
+                       d_F2, K, n2, Kp, B, 1);
     PF_CHECK_HIP(hipGetLastError());
     const int vec_ok = reinterpret_cast<uintptr_t>(d_C) % 16 == 0;
     if (out_dtype == PF_F32)
       hipLaunchKernelGGL((k_cost_mfma_bf16<float, BATCH>), dim3((unsigned)nblocks), dim3(256), 0,
-                         st, A, B, Kp, one, d_jobs, njobs, static_cast<float *>(d_C), vec_ok);
+                         st, A, B, n1, n2, Kp, one, d_jobs, njobs, static_cast<float *>(d_C),
+                         vec_ok);
     else
       hipLaunchKernelGGL((k_cost_mfma_bf16<__hip_bfloat16, BATCH>), dim3((unsigned)nblocks),
-                         dim3(256), 0, st, A, B, Kp, one, d_jobs, njobs,
+                         dim3(256), 0, st, A, B, n1, n2, Kp, one, d_jobs, njobs,
                          static_cast<__hip_bfloat16 *>(d_C), vec_ok);
     PF_CHECK_HIP(hipGetLastError());
     return PF_OK;
@@ -340,7 +420,7 @@ extern "C" int pf_features_bf16(const double *d_F, int64_t K, int64_t n, void *d
              "pf_features_bf16: output must be 16-byte aligned");
   const int64_t Kp = (K + 15) / 16 * 16;
   hipLaunchKernelGGL(k_cost_prep_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     as_stream(stream), d_F, K, n, Kp, static_cast<__hip_bfloat16 *>(d_out));
+                     as_stream(stream), d_F, K, n, Kp, static_cast<__hip_bfloat16 *>(d_out), 0);
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
 }
