@@ -764,33 +764,45 @@ __global__ void k_pack_weights(const float *__restrict__ w, __hip_bfloat16 *__re
 // tile length L covers rows [i*hop, min(i*hop+L, L1)) and columns [j*hop,
 // min(j*hop+L, L2)), hop = L-1; pooled cell (y, x) covers frames
 // [y*pool, y*pool+pool) x [x*pool, x*pool+pool).
+// A wave walks rows wave, wave+4, ... of the region with its lanes along
+// the row (coalesced), four rows in flight per step; max is exact in any
+// order.  Element offsets fit 32 bits (the host checks H * W < 2^31).
 __global__ __launch_bounds__(256) void k_tile_scores(const float *__restrict__ Z, int64_t H,
                                                      int64_t W, int pool, int64_t L, int64_t L1,
                                                      int64_t L2, int64_t n2, int ext,
                                                      float *__restrict__ score) {
-  __shared__ float red[256];
+  __shared__ float red[4];
   const int64_t t = blockIdx.x;
   const int64_t i = t / n2, j = t - i * n2;
   const int64_t hop = L - 1;
   const int64_t r0 = i * hop, r1 = (r0 + L < L1 ? r0 + L : L1) - 1;
   const int64_t c0 = j * hop, c1 = (c0 + L < L2 ? c0 + L : L2) - 1;
-  const int64_t y0 = r0 / pool - ext > 0 ? r0 / pool - ext : 0;
-  const int64_t y1 = r1 / pool + ext < H - 1 ? r1 / pool + ext : H - 1;
-  const int64_t x0 = c0 / pool - ext > 0 ? c0 / pool - ext : 0;
-  const int64_t x1 = c1 / pool + ext < W - 1 ? c1 / pool + ext : W - 1;
-  const int64_t nx = x1 - x0 + 1, n = (y1 - y0 + 1) * nx;
-  float m = -__builtin_huge_valf();
-  for (int64_t q = threadIdx.x; q < n; q += blockDim.x) {
-    const int64_t y = y0 + q / nx, x = x0 + q % nx;
-    m = fmaxf(m, Z[y * W + x]);
+  const int y0 = (int)(r0 / pool - ext > 0 ? r0 / pool - ext : 0);
+  const int y1 = (int)(r1 / pool + ext < H - 1 ? r1 / pool + ext : H - 1);
+  const int x0 = (int)(c0 / pool - ext > 0 ? c0 / pool - ext : 0);
+  const int x1 = (int)(c1 / pool + ext < W - 1 ? c1 / pool + ext : W - 1);
+  const int Wi = (int)W;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const float ninf = -__builtin_huge_valf();
+  float m0 = ninf, m1 = ninf, m2 = ninf, m3 = ninf;
+  for (int x = x0 + lane; x <= x1; x += 64) {
+    int y = y0 + wave;
+    for (; y + 12 <= y1; y += 16) {
+      const uint32_t o = (uint32_t)(y * Wi + x), s4 = (uint32_t)(4 * Wi);
+      const float z0 = Z[o], z1 = Z[o + s4], z2 = Z[o + 2 * s4], z3 = Z[o + 3 * s4];
+      m0 = fmaxf(m0, z0);
+      m1 = fmaxf(m1, z1);
+      m2 = fmaxf(m2, z2);
+      m3 = fmaxf(m3, z3);
+    }
+    for (; y <= y1; y += 4) m0 = fmaxf(m0, Z[(uint32_t)(y * Wi + x)]);
   }
-  red[threadIdx.x] = m;
+  float m = fmaxf(fmaxf(m0, m1), fmaxf(m2, m3));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if (lane == 0) red[wave] = m;
   __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + st]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) score[t] = red[0];
+  if (threadIdx.x == 0) score[t] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
 }
 
 // Selection check of the learned form: flag[p] = 1 when a tile that holds a
@@ -1016,6 +1028,7 @@ extern "C" int pf_tile_scores(const float *d_Z, int64_t H, int64_t W, int pool, 
              "pf_tile_scores: bad arguments");
   PF_REQUIRE((L1 + pool - 1) / pool == H && (L2 + pool - 1) / pool == W,
              "pf_tile_scores: pooled map does not match the pair");
+  PF_REQUIRE(H * W < (1ll << 31), "pf_tile_scores: pooled map too large");
   const int64_t hop = L - 1;
   const int64_t n1 = (L1 - 1 + hop - 1) / hop, n2 = (L2 - 1 + hop - 1) / hop;
   hipLaunchKernelGGL(k_tile_scores, dim3((unsigned)(n1 * n2)), dim3(256), 0, as_stream(stream), d_Z,

@@ -358,3 +358,21 @@ def test_adam_matches_numpy(env):
         w = np.where(fz, w, w - upd)
     assert np.allclose(net.weights_host(), w, rtol=1e-5, atol=1e-6)
     assert not net.g.any()
+
+
+def test_tile_scores_large_map_match_oracle(env):
+    """Regions of 60-90 pooled rows and columns (the 4-rows-in-flight loop,
+    rows left over, a second pass along x) give the oracle's maxima."""
+    torch, tn, _ = env
+    L, check = _lib()
+    rng = np.random.default_rng(12)
+    for (L_, L1, L2, ext) in [(2000, 6390, 6300, 8), (1000, 6390, 6300, 0), (2000, 4001, 6300, 3)]:
+        H, W = -(-L1 // 32), -(-L2 // 32)
+        Z = rng.standard_normal((H, W)).astype(np.float32) * 3
+        Zd = torch.from_numpy(Z).cuda()
+        s = torch.empty(-(-(L1 - 1) // (L_ - 1)) * -(-(L2 - 1) // (L_ - 1)), dtype=torch.float32,
+                        device="cuda")
+        check(L.pf_tile_scores(Zd.data_ptr(), H, W, 32, L_, L1, L2, ext, s.data_ptr(),
+                               torch.cuda.current_stream().cuda_stream), "scores")
+        want = tn.tile_scores(Z[..., None], 32, L_, L1, L2, ext).astype(np.float32)
+        assert np.array_equal(s.cpu().numpy(), want), (L_, L1, L2, ext)
