@@ -615,10 +615,10 @@ __global__ __launch_bounds__(64) void k_s2_walk(const TileDesc *__restrict__ til
     while (true) {
       const int qi = (r >> 4) - rb0, ci = c - wc0;
       // out of steps, or out of the window (then the next one is loaded)
-      if (((qi | ci) < 0) | (m >= max_iters)) break;
+      if (__builtin_expect(((qi | ci) < 0) | (m >= max_iters), 0)) break;
       ++m;
       const int ns = qi * 2 + (ci >> 6);
-      if (ns != sel) {
+      if (__builtin_expect(ns != sel, 0)) {
         sel = ns;
         wcur = w[0];
 #pragma unroll
@@ -631,13 +631,13 @@ __global__ __launch_bounds__(64) void k_s2_walk(const TileDesc *__restrict__ til
       // the start reached (no tail), or an invalid code or a step off the
       // tile (dr >= 1 and dc >= 0: only lower bounds): (sr, sc) is appended
       const bool home = (r == sr) & (c == sc);
-      if (home | (code >= ss.K) | ((pr | pc) < 0)) {
+      if (__builtin_expect(home | (code >= ss.K) | ((pr | pc) < 0), 0)) {
         tail = home ? 0 : 1;
         done = true;
         break;
       }
       acc |= (uint32_t)code << (2 * (nc & 15));
-      if ((nc & 15) == 15) {
+      if (__builtin_expect((nc & 15) == 15, 0)) {
         cw = (lane == ((nc >> 4) & 63)) ? acc : cw;
         acc = 0;
         if (((nc >> 4) & 63) == 63) codes[(nc >> 10) * 64 + lane] = cw;
