@@ -353,6 +353,10 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
   const int nwaves = (int)blockDim.x >> 6;
   int seen_left = 0, seen_right = 0;  // last observed progress of the neighbour waves
   int blk0 = 1;  // first row of the current exchange block
+  // next row r with (r + 1) % G == 0, from row 1 on: the producer publishes
+  // its progress there (a running value, no division per row)
+  int next_pub = G - 1;
+  while (next_pub < 1) next_pub += G;
   // Rows 0 .. G-1 of the strip on the left feed rows 1 .. G.
   if (consumer && wave == 0 && R > 1) {
     const int need = (G < R) ? G : R;
@@ -615,8 +619,9 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
 
       if (producer) {
         xout[r] = Xch<DT>{D1[CPL - 2], D1[CPL - 1], X1[CPL - 2], X1[CPL - 1]};
-        if ((r + 1) % G == 0 || r == R - 1)
+        if (r == next_pub || r == R - 1)
           __hip_atomic_store(fout, r + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (r == next_pub) next_pub += G;
       }
       // Hand row r to the wave on the right once it has read row r - kRing.
       if (wave + 1 < nwaves && seen_right < r - kRing + 1) {
