@@ -35,7 +35,7 @@ from . import engine
 from ._lib import PF_BF16, check, lib
 from ._plan import BatchPlan
 from .engine import StepWeights, _stream
-from .tile_net import POOL, TileNet, select_tiles
+from .tile_net import POOL, TileNet
 
 __all__ = ["LearnedRunner", "path_tiles", "needs_fallback"]
 
@@ -243,6 +243,16 @@ class LearnedRunner:
         With graph=True the scoring of all pairs is recorded once as a HIP
         graph (after one eager pass) and replayed; the network's weights are
         then fixed for this runner."""
+        flat = self._score_flat(stream)
+        res, o = [], 0
+        for (n1, n2) in self.plan.grids:
+            res.append(flat[o:o + n1 * n2].reshape(n1, n2))
+            o += n1 * n2
+        return res
+
+    def _score_flat(self, stream=None):
+        """Tile logits of every pair as one host array, the pairs' grids
+        concatenated in i-major order (the plan's tile order)."""
         torch = _torch()
         ntot = int(self.plan.ntiles)
         if self._scores is None:
@@ -262,19 +272,16 @@ class LearnedRunner:
                 torch.cuda.current_stream().wait_stream(s)
                 self._graph = g
             self._graph.replay()
-        flat = self._scores.cpu().numpy()
-        res, o = [], 0
-        for (n1, n2) in self.plan.grids:
-            res.append(flat[o:o + n1 * n2].reshape(n1, n2))
-            o += n1 * n2
-        return res
+        return self._scores.cpu().numpy()
 
     def _pruned_tables(self, sels):
         """Work table, full table with remapped back-pointer offsets, cost
-        jobs and sizes for the selection."""
+        jobs and sizes for the selection (one boolean per tile of the plan,
+        flat in the plan's order, or one (n1, n2) array per pair)."""
         plan = self.plan
         full = plan.tiles.copy()
-        sel = np.concatenate([s.reshape(-1) for s in sels])
+        sel = (np.asarray(sels, bool) if isinstance(sels, np.ndarray)
+               else np.concatenate([np.asarray(s).reshape(-1) for s in sels]))
         R, W = full[:, 2], full[:, 3]
         words = ((R + 15) // 16) * W
         zero_words = int(((self.L + 15) // 16) * self.L)
@@ -329,10 +336,11 @@ class LearnedRunner:
                 timings[name] = timings.get(name, 0.0) + now - clock[0]
                 clock[0] = now
 
-        scores = self.score(stream)
+        flat = self._score_flat(stream)
         mark("score")
-        sels = [select_tiles(s, s.shape[0], s.shape[1], self.threshold) for s in scores]
-        sel, full, work, jobs, nblk, b_words, c_elems, cells = self._pruned_tables(sels)
+        # select_tiles with no margin on every pair at once: score >= threshold
+        sel, full, work, jobs, nblk, b_words, c_elems, cells = self._pruned_tables(
+            flat >= self.threshold)
         mark("select")
         nsel = len(work)
         dev = self.device
