@@ -427,15 +427,20 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
           if constexpr (sizeof(DT) == 4) {
             // FP32 DP (precision form): fused multiply-add path costs and the
             // plain cross-product order a1/m1 < a0/m0 <=> a1*m0 < a0*m1.
+            // A predecessor outside the tile (column -2/-1, row -1) has
+            // D = +inf, so its cost a is +inf and a*m < a'*m' is false: the
+            // v1 / v2 masks are implied and not evaluated.
+            (void)v1;
+            (void)v2;
             const DT a0 = __builtin_fmaf(cval, wk[0], pD0), m0 = dabs(pX0) + (DT)2;
             const DT a1 = __builtin_fmaf(cval, wk[1], pD1), m1 = dabs(pX1) + (DT)3;
             const DT a2 = __builtin_fmaf(cval, wk[2], pD2), m2 = dabs(pX2) + (DT)3;
-            const bool t1 = (a1 * m0 < a0 * m1) && v1;
+            const bool t1 = a1 * m0 < a0 * m1;
             ba = t1 ? a1 : a0;
             bm = t1 ? m1 : m0;
             bx = t1 ? pX1 : pX0;
             bk = t1 ? 1 : 0;
-            const bool t2 = (a2 * bm < ba * m2) && v2;
+            const bool t2 = a2 * bm < ba * m2;
             ba = t2 ? a2 : ba;
             bm = t2 ? m2 : bm;
             bx = t2 ? pX2 : bx;
