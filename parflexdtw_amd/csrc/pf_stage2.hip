@@ -87,16 +87,20 @@ __global__ __launch_bounds__(256) void k_s2_prep(const TileDesc *__restrict__ ti
     const int64_t pos = rem - (int64_t)e * L;
     Darr[q] = __builtin_huge_val();
     Larr[q] = __builtin_huge_val();
+    // a tile without stage 1: overlap costs are not read (k_s2_diag takes 0)
+    if (!live) continue;
     const int64_t elen = e == 0 ? td.W : td.R;
     if (pos >= elen) {
       olap[q] = 0.0;
     } else {
-      const int64_t s = live ? eP[q] : 0;
-      olap[q] = !live ? 0.0 : (s > 0 ? eC[td.e_off + s] : eC[td.e_off + L - s]);
+      const int64_t s = eP[q];
+      olap[q] = s > 0 ? eC[td.e_off + s] : eC[td.e_off + L - s];
       if (s > 0) atomicOr(&lb[0][s >> 5], 1u << (s & 31));
       else atomicOr(&lb[1][(-s) >> 5], 1u << ((-s) & 31));
     }
   }
+  // a tile without stage 1 has P = 0 on every edge cell: start (0, 0) only
+  if (!live && threadIdx.x == 0) lb[1][0] = 1u;
   __syncthreads();
   for (int w = threadIdx.x; w < 2 * bitw; w += blockDim.x) {
     const int m = w >= bitw ? 1 : 0;
@@ -174,7 +178,7 @@ __global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs,
     const int64_t pslot = ((pd.tile_base + pi * n2 + pj) * 2 + pedge) * L + ppos;
     const double prev = Darr[pslot];
     if (!isfinite(prev)) return;
-    Darr[slot] = (prev + eDv) - olap[slot];
+    Darr[slot] = (prev + eDv) - (live ? olap[slot] : 0.0);
     Larr[slot] = Larr[pslot] + seg_len;
   };
 

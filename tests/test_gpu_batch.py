@@ -180,7 +180,8 @@ def test_stage2_reads_no_edges_of_tiles_marked_without_stage1(torch_gpu):
     """A tile table entry with column 0 < 0 (a tile whose stage 1 did not run)
     makes stage 2 treat the tile's edges as D = +inf, P = 0, C = 0 without
     reading them: the same outputs as filling those edges with +inf, 0, 0,
-    whatever the unread slots hold."""
+    whatever the unread slots hold (the marked tiles' overlap costs, never
+    read, are not written either)."""
     from parflexdtw_amd import engine
     from parflexdtw_amd.engine import BatchRunner
     torch = torch_gpu
@@ -217,9 +218,12 @@ def test_stage2_reads_no_edges_of_tiles_marked_without_stage1(torch_gpu):
         torch.cuda.synchronize()
         outs.append(w)
     a, b = outs
-    for k in ("D_arr", "L_arr", "olap", "bits", "best_f", "best_i", "path_len"):
+    for k in ("D_arr", "L_arr", "bits", "best_f", "best_i", "path_len"):
         assert torch.equal(a[k].view(torch.int64) if a[k].dtype == torch.float64 else a[k],
                            b[k].view(torch.int64) if b[k].dtype == torch.float64 else b[k]), k
+    # overlap costs of a marked tile are neither written nor read
+    live = torch.from_numpy(~slots).cuda()
+    assert torch.equal(a["olap"][live].view(torch.int64), b["olap"][live].view(torch.int64))
     ra, rb = engine.collect(plan, a), engine.collect(plan, b)
     for p in range(len(pairs)):
         assert np.array_equal(ra.paths[p], rb.paths[p]), p
