@@ -139,6 +139,12 @@ __global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs,
   if (pos >= elen) return;
   const int64_t t = pd.tile_base + i * n2 + j;
   const int64_t slot = (t * 2 + e) * L + pos;
+  // A tile without stage 1 (column 0 < 0) starts no path: every slot but the
+  // corner copies (pos 0) ends with D = +inf, as k_s2_prep left it, and the
+  // L of a slot with infinite D is never read (sync, scan and chaining test
+  // D first).
+  const bool live = tiles[t].c_off >= 0;
+  if (!live && pos != 0) return;
 
   bool valid;
   if (e == 0) {
@@ -150,8 +156,7 @@ __global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs,
   }
 
   const int64_t lr = e == 0 ? g.rows - 1 : pos, lc = e == 0 ? pos : g.cols - 1;
-  // a tile without stage 1 (column 0 < 0): D = +inf, P = 0, arrays not read
-  const bool live = tiles[t].c_off >= 0;
+  // a tile without stage 1: D = +inf, P = 0, arrays not read
   const int64_t s = live ? eP[slot] : 0;
   const double eDv = live ? eD[slot] : __builtin_huge_val();
   const int64_t sr = s < 0 ? -s : 0, sc = s > 0 ? s : 0;
