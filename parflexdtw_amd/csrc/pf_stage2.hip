@@ -61,7 +61,7 @@ __global__ void k_fill_i64(int64_t *p, int64_t v, int64_t n) {
 // first column of C that stage 1 stored per tile), start-set bitmaps of
 // every tile (bit s of map 0: some edge cell starts at (0, s), s > 0; bit a
 // of map 1: some edge cell starts at (a, 0), a >= 0), and D_arr = L_arr =
-// inf.
+// inf (L_arr of tiles with stage 1 only).
 constexpr int kMaxBitw = 8192 / 32 + 1;
 
 __global__ __launch_bounds__(256) void k_s2_prep(const TileDesc *__restrict__ tiles, int64_t L,
@@ -86,9 +86,11 @@ __global__ __launch_bounds__(256) void k_s2_prep(const TileDesc *__restrict__ ti
     const int e = rem >= L ? 1 : 0;
     const int64_t pos = rem - (int64_t)e * L;
     Darr[q] = __builtin_huge_val();
-    Larr[q] = __builtin_huge_val();
-    // a tile without stage 1: overlap costs are not read (k_s2_diag takes 0)
+    // A tile without stage 1: its L slots are read only where a corner copy
+    // has written them (an unreached slot, D = +inf, is never read), and its
+    // overlap costs not at all (k_s2_diag takes 0).
     if (!live) continue;
+    Larr[q] = __builtin_huge_val();
     const int64_t elen = e == 0 ? td.W : td.R;
     if (pos >= elen) {
       olap[q] = 0.0;
