@@ -603,10 +603,10 @@ __global__ __launch_bounds__(64) void k_s2_walk(const TileDesc *__restrict__ til
   steptab = __builtin_amdgcn_readfirstlane(steptab);
   int m = 0;        // cells visited (recorded in walk order)
   int nc = 0;       // step codes appended
-  int tail = 0;     // 1: (sr, sc) is recorded after the visited cells
   uint32_t acc = 0, cw = 0;
-  bool done = false;
-  while (!done) {
+  // The walk leaves both loops with a jump; why it stopped is read from the
+  // final state afterwards (no loop-carried flags).
+  for (;;) {
     const int rb0 = (r >> 4) - (kWalkRB - 1) > 0 ? (r >> 4) - (kWalkRB - 1) : 0;
     const int wc0 = c - 127 > 0 ? c - 127 : 0;
     uint32_t w[2 * kWalkRB];
@@ -623,10 +623,10 @@ __global__ __launch_bounds__(64) void k_s2_walk(const TileDesc *__restrict__ til
     }
     int sel = -1;
     uint32_t wcur = 0;
-    while (true) {
+    for (;;) {
+      if (__builtin_expect(m >= max_iters, 0)) goto out_of_steps;
       const int qi = (r >> 4) - rb0, ci = c - wc0;
-      // out of steps, or out of the window (then the next one is loaded)
-      if (__builtin_expect(((qi | ci) < 0) | (m >= max_iters), 0)) break;
+      if (__builtin_expect((qi | ci) < 0, 0)) break;   // left the window: load the next one
       ++m;
       const int ns = qi * 2 + (ci >> 6);
       if (__builtin_expect(ns != sel, 0)) {
@@ -639,15 +639,16 @@ __global__ __launch_bounds__(64) void k_s2_walk(const TileDesc *__restrict__ til
       const int code = (int)((word >> (2 * (r & 15))) & 3u);
       const int e = steptab >> (4 * code);
       const int pr = r - (e & 3), pc = c - ((e >> 2) & 3);
-      // the start reached (no tail), or an invalid code or a step off the
-      // tile (dr >= 1 and dc >= 0: only lower bounds): (sr, sc) is appended
-      const bool home = (r == sr) & (c == sc);
-      if (__builtin_expect(home | (code >= ss.K) | ((pr | pc) < 0), 0)) {
-        tail = home ? 0 : 1;
-        done = true;
-        break;
-      }
+      // the start reached, an invalid code, or a step off the tile (dr >= 1
+      // and dc >= 0: only lower bounds)
+      // integer form (no compares): the top bit of (d - 1) & ~d is set iff
+      // d == 0 for d >= 0, here d = 0 iff (r, c) == (sr, sc); a negative
+      // K-1-code, pr or pc sets it too
+      const int d = (r ^ sr) | (c ^ sc);
+      const int stop = ((d - 1) & ~d) | (ss.K - 1 - code) | pr | pc;
+      if (__builtin_expect(stop < 0, 0)) goto stopped;
       acc |= (uint32_t)code << (2 * (nc & 15));
+      asm volatile("" : "+s"(acc));   // keep the code word in a scalar register
       if (__builtin_expect((nc & 15) == 15, 0)) {
         cw = (lane == ((nc >> 4) & 63)) ? acc : cw;
         acc = 0;
@@ -657,11 +658,13 @@ __global__ __launch_bounds__(64) void k_s2_walk(const TileDesc *__restrict__ til
       r = pr;
       c = pc;
     }
-    if (!done && m >= max_iters) {
-      tail = (r != sr || c != sc) ? 1 : 0;
-      done = true;
-    }
   }
+out_of_steps:
+stopped:
+  // (sr, sc) is appended unless the walk ended on it: after the last visited
+  // cell when it stopped (an invalid code or a step off the tile), after the
+  // cells visited when it ran out of steps on another cell.
+  const int tail = (r != sr || c != sc) ? 1 : 0;
   if ((nc & 15) != 0) cw = (lane == ((nc >> 4) & 63)) ? acc : cw;
   if ((nc & 1023) != 0) codes[(nc >> 10) * 64 + lane] = cw;
   __syncthreads();
