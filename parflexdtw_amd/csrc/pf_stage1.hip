@@ -353,6 +353,31 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
   const int nwaves = (int)blockDim.x >> 6;
   int seen_left = 0, seen_right = 0;  // last observed progress of the neighbour waves
   int blk0 = 1;  // first row of the current exchange block
+  // Left context of row r (row r-1 of the two columns left of the wave) at
+  // base + ((r - ctx_off) & ctx_mask) * ctx_stride: the ring slot (r-1) % 8
+  // of the wave on the left; for wave 0 of a strip with a strip on its left
+  // the staged exchange row r - blk0; otherwise the tile's missing columns.
+  const DT *ctxD, *ctxX;
+  int ctx_off, ctx_mask, ctx_stride;
+  if (wave > 0) {
+    ctxD = &shD[0][wave - 1][0];
+    ctxX = &shX[0][wave - 1][0];
+    ctx_off = 1;
+    ctx_mask = kRing - 1;
+    ctx_stride = 16 * 2;
+  } else if (consumer) {
+    ctxD = &lxD[0][0];
+    ctxX = &lxX[0][0];
+    ctx_off = blk0;
+    ctx_mask = -1;
+    ctx_stride = 2;
+  } else {
+    ctxD = &shNone[0][0];
+    ctxX = &shNone[1][0];
+    ctx_off = 0;
+    ctx_mask = 0;
+    ctx_stride = 0;
+  }
   // next row r with (r + 1) % G == 0, from row 1 on: the producer publishes
   // its progress there (a running value, no division per row)
   int next_pub = G - 1;
@@ -388,20 +413,9 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
         } while (seen_left < r - 1);
       }
       // The source is the same for the whole wave (only lane 0's copy is
-      // used): an unconditional LDS read, no per-row register setup.
-      const DT *srcD, *srcX;
-      if (wave > 0) {
-        const int s1 = (r - 1) & (kRing - 1);  // slot of row r-1
-        srcD = &shD[s1][wave - 1][0];
-        srcX = &shX[s1][wave - 1][0];
-      } else if (consumer) {
-        const int i1 = r - blk0;
-        srcD = &lxD[i1][0];
-        srcX = &lxX[i1][0];
-      } else {
-        srcD = &shNone[0][0];
-        srcX = &shNone[1][0];
-      }
+      // used): an unconditional LDS read at a wave-uniform offset.
+      const int coff = ((r - ctx_off) & ctx_mask) * ctx_stride;
+      const DT *srcD = ctxD + coff, *srcX = ctxX + coff;
       const DT oD1a = srcD[0], oD1b = srcD[1], oX1a = srcX[0], oX1b = srcX[1];
       const DT LD1a = shr1(D1[CPL - 2], oD1a), LD1b = shr1(D1[CPL - 1], oD1b);
       const DT LX1a = shr1(X1[CPL - 2], oX1a), LX1b = shr1(X1[CPL - 1], oX1b);
@@ -649,6 +663,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
       if (consumer && r + 1 < R && (r + 1 - blk0) == G) {
         blk0 = r + 1;
         if (wave == 0) {
+          ctx_off = blk0;
           const int last = blk0 + G - 1;
           const int need = (last < R) ? last : R;
           while (__hip_atomic_load(fin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need)
