@@ -689,6 +689,10 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
   // them would make the compiler wait for all outstanding loads).
   int rb = 1;
   for (; rb + kPF <= R; rb += kPF) {
+    // rb = 1 (mod kPF): the row r = rb + u of each unrolled step has known
+    // low bits, so row conditions with period 16 or 32 (back-pointer word
+    // flush, C block fill, ring slot) fold to constants in most steps
+    __builtin_assume((rb & (kPF - 1)) == 1);
 #pragma unroll
     for (int u = 0; u < kPF; ++u) {
       const int r = rb + u;
@@ -698,7 +702,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
       const int rp = (r + kPF < R) ? r + kPF : R - 1;  // clamped: always a valid row
       if constexpr (FEAT) {
         // rows of the previous block were all read by earlier refills
-        if ((rp & 31) == 0 && rp == r + kPF) fill_block(rp >> 5);
+        if (((r + kPF) & 31) == 0 && r + kPF < R) fill_block((r + kPF) >> 5);
       }
 #pragma unroll
       for (int j = 0; j < CPL; ++j) Cq[u][j] = c_at(rp, j);
