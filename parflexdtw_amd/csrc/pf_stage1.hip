@@ -106,7 +106,12 @@ static_assert(sizeof(Xch<double>) == 32 && sizeof(Xch<float>) == 16,
               "pf_stage1_workspace reports these entry sizes");
 
 constexpr int kMaxG = 64;   // rows per exchange block, upper bound
-constexpr int kPF = 4;      // rows of C prefetched ahead
+// Rows of C prefetched ahead, and rows per unrolled step of the sweep: 8
+// for the packed FP32 DP with the standard step set and up to 4 columns per
+// lane, 4 otherwise (the wider state of the other variants would not fit
+// their register budget with 8).
+template <typename DT, int CPL, bool FULL, bool STD>
+constexpr int prefetch_rows() { return (sizeof(DT) == 4 && !FULL && STD && CPL <= 4) ? 8 : 4; }
 constexpr int kRing = 8;    // rows of wave-to-wave hand-over kept in LDS
 
 // Lane i receives lane i-1's value (DPP wave_shr:1); lane 0 keeps `old`.
@@ -224,6 +229,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
 
   DT D1[CPL], X1[CPL], D2[CPL], X2[CPL];
   uint32_t bacc[CPL];
+  constexpr int kPF = prefetch_rows<DT, CPL, FULL, STD>();
   DT Cq[kPF][CPL];
   // Left context: columns c0-2 (a), c0-1 (b) of rows r-1 (1) and r-2 (2).
   DT LD2a = INF, LD2b = INF, LX2a = 0, LX2b = 0;
