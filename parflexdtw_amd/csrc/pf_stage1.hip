@@ -221,6 +221,9 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
   const int c0 = cbeg + tid * CPL;
   const int jlast = W - 1 - c0;  // this lane's index of the tile's last column
   const bool own_last = (jlast >= 0 && jlast < CPL);
+  // the wave holding the tile's last column (wave-uniform)
+  const int wcol0 = cbeg + wave * 64 * CPL;
+  const bool own_last_wave = (W - 1 >= wcol0) && (W - 1 < wcol0 + 64 * CPL);
   const DT INF = dp_inf<DT>();
 
   DT wk[4];
@@ -242,6 +245,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
 
   // features form: this wave's block of C, 32 rows x (64*CPL) columns
   constexpr int kWC = 64 * CPL;
+  const bool has_col0 = (cbeg == 0) && (wave == 0);  // wave-uniform
   __hip_bfloat16 *cblk = cdyn + (FEAT ? (int64_t)(tid >> 6) * 32 * kWC : 0);
   auto fill_block = [&](const int b) {
     if constexpr (FEAT) {
@@ -290,6 +294,10 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // column 0 of the tile (stage 2's start cells): one 32-lane store per
+      // block instead of one single-lane store per row
+      if (has_col0 && lane < 32 && 32 * b + lane < R)
+        eC[td.e_off + Lmax + 32 * b + lane] = (double)__bfloat162float(cblk[lane * kWC]);
     }
   };
   // C of row `row`, lane slot j (a column clamped into the tile)
@@ -357,8 +365,20 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
     prog[wave] = 0;
   }
   const int nwaves = (int)blockDim.x >> 6;
+  const bool producer_wave = (S > 1) && (cbeg + SW < W) && (wave == nwaves - 1);
   int seen_left = 0, seen_right = 0;  // last observed progress of the neighbour waves
+  // Rows up to which no wait is needed: row r needs row r-1 of the wave on
+  // the left, and the ring slot of row r - kRing read by the wave on the
+  // right. Waves without that neighbour never wait.
+  constexpr int kNever = 0x7fffffff;
+  int left_ok = (wave > 0) ? seen_left + 1 : kNever;
+  int right_ok = (wave + 1 < nwaves) ? seen_right + kRing - 1 : kNever;
   int blk0 = 1;  // first row of the current exchange block
+  // rows 1-2 of every wave and all rows of the wave holding columns 0-2
+  // take the boundary rules
+  const int border_rows = (wcol0 <= 2) ? kNever : 2;
+  // first row of the next exchange block (consumer strips)
+  int next_blk = (consumer && blk0 + G < R) ? blk0 + G : kNever;
   // Left context of row r (row r-1 of the two columns left of the wave) at
   // base + ((r - ctx_off) & ctx_mask) * ctx_stride: the ring slot (r-1) % 8
   // of the wave on the left; for wave 0 of a strip with a strip on its left
@@ -410,13 +430,14 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
       // Row r-1 of columns c0-2, c0-1: from lane-1 by DPP; lane 0 from the
       // previous wave (LDS ring), the previous strip (exchange rows) or the
       // missing columns -2, -1 of the tile.
-      if (wave > 0 && seen_left < r - 1) {
+      if (r > left_ok) {
         // wait until the wave on the left has handed over row r-1
         do {
           seen_left = __builtin_amdgcn_readfirstlane(__hip_atomic_load(
               &prog[wave - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
           if (seen_left < r - 1) __builtin_amdgcn_s_sleep(1);
         } while (seen_left < r - 1);
+        left_ok = seen_left + 1;
       }
       // The source is the same for the whole wave (only lane 0's copy is
       // used): an unconditional LDS read at a wave-uniform offset.
@@ -430,7 +451,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
       int code[CPL];
       bool amb = false;  // some cell of this lane needs the exact comparison
       // rows 1-2 and columns 0-2 of the tile: predecessors on the boundary
-      const bool border = (r <= 2) || (cbeg + wave * 64 * CPL <= 2);
+      const bool border = r <= border_rows;
 #pragma unroll
       for (int j = 0; j < CPL; ++j) {
         const int c = c0 + j;
@@ -566,7 +587,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
           }
         }
       }
-      if constexpr (!FULL) {
+      if constexpr (!FULL && !FEAT) {
         if (c0 == 0) eC[td.e_off + Lmax + r] = (double)Cc[0];
       }
       if (border) {
@@ -623,7 +644,8 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
         const int sh = 2 * (r & 15);
 #pragma unroll
         for (int j = 0; j < CPL; ++j) bacc[j] |= (uint32_t)code[j] << sh;
-        if ((r & 15) == 15 || r == R - 1) {
+        // full words here; the last partial word after the sweep
+        if ((r & 15) == 15) {
           uint32_t *dst = Bp + td.b_off + (int64_t)(r >> 4) * W;
 #pragma unroll
           for (int j = 0; j < CPL; ++j) {
@@ -632,29 +654,36 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
             bacc[j] = 0u;
           }
         }
-        if (own_last) {
+        if (own_last_wave) {
+          if (own_last) {
 #pragma unroll
-          for (int j = 0; j < CPL; ++j)
-            if (j == jlast) {
-              eD[td.e_off + Lmax + r] = (double)ND[j];
-              eP[td.e_off + Lmax + r] = p_of(NX[j], r + W - 1);
-            }
+            for (int j = 0; j < CPL; ++j)
+              if (j == jlast) {
+                eD[td.e_off + Lmax + r] = (double)ND[j];
+                eP[td.e_off + Lmax + r] = p_of(NX[j], r + W - 1);
+              }
+          }
         }
       }
 
-      if (producer) {
-        xout[r] = Xch<DT>{D1[CPL - 2], D1[CPL - 1], X1[CPL - 2], X1[CPL - 1]};
-        if (r == next_pub || r == R - 1)
-          __hip_atomic_store(fout, r + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      // the producer is lane 63 of the last wave; the branch on the wave
+      // keeps the publish counter in a scalar register
+      if (producer_wave) {
+        if (lane == 63) {
+          xout[r] = Xch<DT>{D1[CPL - 2], D1[CPL - 1], X1[CPL - 2], X1[CPL - 1]};
+          if (r == next_pub || r == R - 1)
+            __hip_atomic_store(fout, r + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
         if (r == next_pub) next_pub += G;
       }
       // Hand row r to the wave on the right once it has read row r - kRing.
-      if (wave + 1 < nwaves && seen_right < r - kRing + 1) {
+      if (r > right_ok) {
         do {
           seen_right = __builtin_amdgcn_readfirstlane(__hip_atomic_load(
               &prog[wave + 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
           if (seen_right < r - kRing + 1) __builtin_amdgcn_s_sleep(1);
         } while (seen_right < r - kRing + 1);
+        right_ok = seen_right + kRing - 1;
       }
       if (lane == 63) {
         const int s0 = r & (kRing - 1);
@@ -666,8 +695,9 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
       }
       // Next exchange block: wait for the strip on the left and stage its
       // rows blk0-1 .. blk0+G-2 (row r-1 of every row r of the block).
-      if (consumer && r + 1 < R && (r + 1 - blk0) == G) {
+      if (r + 1 == next_blk) {
         blk0 = r + 1;
+        next_blk = (blk0 + G < R) ? blk0 + G : kNever;
         if (wave == 0) {
           ctx_off = blk0;
           const int last = blk0 + G - 1;
@@ -705,7 +735,10 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
       row_step(r, Cq[u]);
       // Refill the slot only after its row is done, so the load targets the
       // same registers (no copy of an in-flight load at the loop edge).
-      const int rp = (r + kPF < R) ? r + kPF : R - 1;  // clamped: always a valid row
+      // Clamped to a valid row for global loads; the features form reads
+      // its LDS block at row & 31, in bounds for any row (rows past the
+      // tile are not used).
+      const int rp = (FEAT || r + kPF < R) ? r + kPF : R - 1;
       if constexpr (FEAT) {
         // rows of the previous block were all read by earlier refills
         if (((r + kPF) & 31) == 0 && r + kPF < R) fill_block((r + kPF) >> 5);
@@ -721,6 +754,15 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
     if (r < R) row_step(r, Cq[u]);
   }
   if constexpr (!FULL) {
+    // Last back-pointer word when it holds fewer than 16 rows.
+    if (R > 1 && ((R - 1) & 15) != 15) {
+      uint32_t *dst = Bp + td.b_off + (int64_t)((R - 1) >> 4) * W;
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) {
+        const int c = c0 + j;
+        if (c < W) dst[c] = bacc[j];
+      }
+    }
     // Last row of the tile (edge 0); R == 1 was written with row 0.
     if (R > 1) {
 #pragma unroll
