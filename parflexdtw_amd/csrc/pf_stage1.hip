@@ -113,6 +113,19 @@ constexpr int kMaxG = 64;   // rows per exchange block, upper bound
 template <typename DT, int CPL, bool FULL, bool STD>
 constexpr int prefetch_rows() { return (sizeof(DT) == 4 && !FULL && STD && CPL <= 4) ? 8 : 4; }
 constexpr int kRing = 8;    // rows of wave-to-wave hand-over kept in LDS
+// Left context of every wave in one LDS array of column pairs (the two
+// columns left of a wave), the D pairs in ctx[0] and the X pairs in ctx[1]
+// at the same index:
+//   ring slot s of wave w (its last two columns of a row r, s = r % kRing)
+//     at s * 16 + w (up to 16 waves);
+//   exchange row q of the strip on the left at kCtxLx + q % kMaxG;
+//   the tile's missing columns -2, -1 (D = +inf, X = 0) at kCtxNone.
+constexpr int kCtxLx = kRing * 16;
+constexpr int kCtxNone = kCtxLx + kMaxG;
+template <typename DT>
+struct alignas(2 * sizeof(DT)) Pair {
+  DT a, b;
+};
 
 // Lane i receives lane i-1's value (DPP wave_shr:1); lane 0 keeps `old`.
 __device__ __forceinline__ int shr1(int v, int old) {
@@ -179,12 +192,8 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
   static_assert(CPL >= 2, "a lane needs two columns of left context");
   static_assert(!FEAT || (!FULL && sizeof(DT) == 4), "features form: packed FP32 DP only");
   extern __shared__ __attribute__((aligned(16))) __hip_bfloat16 cdyn[];
-  __shared__ DT shD[kRing][16][2];
-  __shared__ DT shX[kRing][16][2];
+  __shared__ Pair<DT> ctx[2][kCtxNone + 1];
   __shared__ int prog[16];
-  __shared__ DT lxD[kMaxG][2];
-  __shared__ DT lxX[kMaxG][2];
-  __shared__ DT shNone[2][2];   // D = +inf, X = 0 of the missing columns -2, -1
   __shared__ int sh_item;
 
   // Work item from the ticket counter in strip order.
@@ -218,6 +227,17 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
 
   const CT *__restrict__ C = Cb + (FEAT ? 0 : td.c_off);
   const int64_t ldc = td.ldc;
+  // This tile's edge records: the row edge (last row of D and P, first row
+  // of C) at e_off + c, the column edge (last column of D and P, first
+  // column of C) at e_off + Lmax + r. Addressed from the column-edge base,
+  // so one pointer per array stays live in the sweep.
+  double *const eDc = FULL ? nullptr : eD + td.e_off + Lmax;
+  int32_t *const ePc = FULL ? nullptr : eP + td.e_off + Lmax;
+  double *const eCc = FULL ? nullptr : eC + td.e_off + Lmax;
+  uint32_t *const Bt = FULL ? nullptr : Bp + td.b_off;
+  // features of this tile's first row and column
+  const __hip_bfloat16 *const Fat = FEAT ? Fa + td.c_off * (int64_t)Kp : nullptr;
+  const __hip_bfloat16 *const Fbt = FEAT ? Fb + td.ldc * (int64_t)Kp : nullptr;
   const int c0 = cbeg + tid * CPL;
   const int jlast = W - 1 - c0;  // this lane's index of the tile's last column
   const bool own_last = (jlast >= 0 && jlast < CPL);
@@ -251,7 +271,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
     if constexpr (FEAT) {
       const int lr = lane & 31, kh = (lane >> 5) * 8;
       const int ar = (32 * b + lr < R) ? 32 * b + lr : R - 1;
-      const __hip_bfloat16 *ap = Fa + (td.c_off + ar) * (int64_t)Kp + kh;
+      const __hip_bfloat16 *ap = Fat + ar * (int64_t)Kp + kh;
       const int wc0 = cbeg + (tid >> 6) * kWC;
       auto store_block = [&](const int cb, const s1floatx16_t &acc) {
 #pragma unroll
@@ -267,7 +287,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
 #pragma unroll
         for (int cb = 0; cb < kWC / 32; ++cb) {
           const int gc = (wc0 + cb * 32 + lr < W) ? wc0 + cb * 32 + lr : W - 1;
-          bb[cb] = *reinterpret_cast<const s1bf16x8_t *>(Fb + (td.ldc + gc) * (int64_t)16 + kh);
+          bb[cb] = *reinterpret_cast<const s1bf16x8_t *>(Fbt + gc * (int64_t)16 + kh);
         }
 #pragma unroll
         for (int cb = 0; cb < kWC / 32; ++cb) {
@@ -280,7 +300,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
 #pragma unroll
         for (int cb = 0; cb < kWC / 32; ++cb) {
           const int gc = (wc0 + cb * 32 + lr < W) ? wc0 + cb * 32 + lr : W - 1;
-          const __hip_bfloat16 *bp = Fb + (td.ldc + gc) * (int64_t)Kp + kh;
+          const __hip_bfloat16 *bp = Fbt + gc * (int64_t)Kp + kh;
           s1floatx16_t acc = s1floatx16_t{};
           for (int kk = 0; kk < Kp; kk += 16) {
             const s1bf16x8_t a = *reinterpret_cast<const s1bf16x8_t *>(ap + kk);
@@ -297,7 +317,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
       // column 0 of the tile (stage 2's start cells): one 32-lane store per
       // block instead of one single-lane store per row
       if (has_col0 && lane < 32 && 32 * b + lane < R)
-        eC[td.e_off + Lmax + 32 * b + lane] = (double)__bfloat162float(cblk[lane * kWC]);
+        eCc[32 * b + lane] = (double)__bfloat162float(cblk[lane * kWC]);
     }
   };
   // C of row `row`, lane slot j (a column clamped into the tile)
@@ -327,16 +347,16 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
         Bf[c] = 0;
       } else {
         // first row and first column of C (the start cells stage 2 reads)
-        eC[td.e_off + c] = (double)v;
-        if (c == 0) eC[td.e_off + Lmax] = (double)v;
+        eCc[c - Lmax] = (double)v;
+        if (c == 0) eCc[0] = (double)v;
         if (c == W - 1) {
-          eD[td.e_off + Lmax] = (double)D1[j];
-          eP[td.e_off + Lmax] = 0;
+          eDc[0] = (double)D1[j];
+          ePc[0] = 0;
         }
         if (R == 1) {
-          eD[td.e_off + c] = (double)D1[j];
-          eP[td.e_off + c] = 0;
-          Bp[td.b_off + c] = 0u;
+          eDc[c - Lmax] = (double)D1[j];
+          ePc[c - Lmax] = 0;
+          Bt[c] = 0u;
         }
       }
     }
@@ -352,57 +372,43 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
     if (R == 1) __hip_atomic_store(fout, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (tid == 0) {
-    shNone[0][0] = INF;
-    shNone[0][1] = INF;
-    shNone[1][0] = 0;
-    shNone[1][1] = 0;
+    ctx[0][kCtxNone] = Pair<DT>{INF, INF};
+    ctx[1][kCtxNone] = Pair<DT>{0, 0};
   }
   if (lane == 63) {
-    shD[0][wave][0] = D1[CPL - 2];
-    shD[0][wave][1] = D1[CPL - 1];
-    shX[0][wave][0] = X1[CPL - 2];
-    shX[0][wave][1] = X1[CPL - 1];
+    ctx[0][wave] = Pair<DT>{D1[CPL - 2], D1[CPL - 1]};
+    ctx[1][wave] = Pair<DT>{X1[CPL - 2], X1[CPL - 1]};
     prog[wave] = 0;
   }
   const int nwaves = (int)blockDim.x >> 6;
   const bool producer_wave = (S > 1) && (cbeg + SW < W) && (wave == nwaves - 1);
   int seen_left = 0, seen_right = 0;  // last observed progress of the neighbour waves
-  // Rows up to which no wait is needed: row r needs row r-1 of the wave on
-  // the left, and the ring slot of row r - kRing read by the wave on the
+  // Rows up to which no synchronisation is needed. Row r needs row r-1 of
+  // the wave on the left (waves > 0); wave 0 of a consumer strip stages the
+  // next exchange block at its first row (rows 1 .. G are staged below);
+  // row r overwrites the ring slot of row r - kRing, read by the wave on the
   // right. Waves without that neighbour never wait.
   constexpr int kNever = 0x7fffffff;
-  int left_ok = (wave > 0) ? seen_left + 1 : kNever;
+  int sync_ok = (wave > 0) ? seen_left + 1 : ((consumer && 1 + G < R) ? G : kNever);
   int right_ok = (wave + 1 < nwaves) ? seen_right + kRing - 1 : kNever;
-  int blk0 = 1;  // first row of the current exchange block
   // rows 1-2 of every wave and all rows of the wave holding columns 0-2
   // take the boundary rules
   const int border_rows = (wcol0 <= 2) ? kNever : 2;
-  // first row of the next exchange block (consumer strips)
-  int next_blk = (consumer && blk0 + G < R) ? blk0 + G : kNever;
-  // Left context of row r (row r-1 of the two columns left of the wave) at
-  // base + ((r - ctx_off) & ctx_mask) * ctx_stride: the ring slot (r-1) % 8
-  // of the wave on the left; for wave 0 of a strip with a strip on its left
-  // the staged exchange row r - blk0; otherwise the tile's missing columns.
-  const DT *ctxD, *ctxX;
-  int ctx_off, ctx_mask, ctx_stride;
+  // Left context of row r (row r-1 of the two columns left of the wave):
+  // for waves > 0 the ring slot (r-1) % kRing of the wave on the left, at
+  // ((r-1) % kRing) * 16 + ctx_base; otherwise at ((r-1) & ctx_mask) +
+  // ctx_base, for wave 0 of a strip with a strip on its left the exchange
+  // row r-1, else the tile's missing columns.
+  int ctx_mask, ctx_base;
   if (wave > 0) {
-    ctxD = &shD[0][wave - 1][0];
-    ctxX = &shX[0][wave - 1][0];
-    ctx_off = 1;
-    ctx_mask = kRing - 1;
-    ctx_stride = 16 * 2;
+    ctx_mask = 0;  // ring slot (r-1) % kRing, from the row's slot
+    ctx_base = wave - 1;
   } else if (consumer) {
-    ctxD = &lxD[0][0];
-    ctxX = &lxX[0][0];
-    ctx_off = blk0;
-    ctx_mask = -1;
-    ctx_stride = 2;
+    ctx_mask = kMaxG - 1;
+    ctx_base = kCtxLx;
   } else {
-    ctxD = &shNone[0][0];
-    ctxX = &shNone[1][0];
-    ctx_off = 0;
     ctx_mask = 0;
-    ctx_stride = 0;
+    ctx_base = kCtxNone;
   }
   // next row r with (r + 1) % G == 0, from row 1 on: the producer publishes
   // its progress there (a running value, no division per row)
@@ -416,34 +422,57 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
     for (int q0 = lane; q0 < G; q0 += 64) {
       if (q0 < R) {
         const Xch<DT> v = xin[q0];
-        lxD[q0][0] = v.d0;
-        lxD[q0][1] = v.d1;
-        lxX[q0][0] = v.x0;
-        lxX[q0][1] = v.x1;
+        ctx[0][kCtxLx + q0] = Pair<DT>{v.d0, v.d1};
+        ctx[1][kCtxLx + q0] = Pair<DT>{v.x0, v.x1};
       }
     }
   }
   __syncthreads();
 
   // One row of the sweep; Cc holds C of row r.
-  auto row_step = [&](const int r, const DT (&Cc)[CPL]) {
+  // slot = r % kRing (a constant in the unrolled steps when kPF is a
+  // multiple of kRing).
+  auto row_step = [&](const int r, const int slot, const DT (&Cc)[CPL]) {
       // Row r-1 of columns c0-2, c0-1: from lane-1 by DPP; lane 0 from the
       // previous wave (LDS ring), the previous strip (exchange rows) or the
       // missing columns -2, -1 of the tile.
-      if (r > left_ok) {
-        // wait until the wave on the left has handed over row r-1
-        do {
-          seen_left = __builtin_amdgcn_readfirstlane(__hip_atomic_load(
-              &prog[wave - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
-          if (seen_left < r - 1) __builtin_amdgcn_s_sleep(1);
-        } while (seen_left < r - 1);
-        left_ok = seen_left + 1;
+      if (r > sync_ok) {
+        if (wave > 0) {
+          // wait until the wave on the left has handed over row r-1
+          do {
+            seen_left = __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                &prog[wave - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+            if (seen_left < r - 1) __builtin_amdgcn_s_sleep(1);
+          } while (seen_left < r - 1);
+          sync_ok = seen_left + 1;
+        } else {
+          // first row of an exchange block: wait for the strip on the left
+          // and stage its rows r-1 .. r+G-2 (row r'-1 of every row r' of the
+          // block)
+          const int last = r + G - 1;
+          const int need = (last < R) ? last : R;
+          while (__hip_atomic_load(fin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need)
+            __builtin_amdgcn_s_sleep(2);
+          for (int q0 = lane; q0 < G; q0 += 64) {
+            const int q = r - 1 + q0;
+            if (q < R) {
+              const Xch<DT> v = xin[q];
+              const int i = kCtxLx + (q & (kMaxG - 1));
+              ctx[0][i] = Pair<DT>{v.d0, v.d1};
+              ctx[1][i] = Pair<DT>{v.x0, v.x1};
+            }
+          }
+          // lane 0 reads the staged rows from this row on
+          __threadfence_block();
+          sync_ok = (r + G < R) ? r + G - 1 : kNever;
+        }
       }
       // The source is the same for the whole wave (only lane 0's copy is
-      // used): an unconditional LDS read at a wave-uniform offset.
-      const int coff = ((r - ctx_off) & ctx_mask) * ctx_stride;
-      const DT *srcD = ctxD + coff, *srcX = ctxX + coff;
-      const DT oD1a = srcD[0], oD1b = srcD[1], oX1a = srcX[0], oX1b = srcX[1];
+      // used): an unconditional LDS read at a wave-uniform index.
+      const int ci = (wave > 0) ? ((slot - 1) & (kRing - 1)) * 16 + ctx_base
+                                : ((r - 1) & ctx_mask) + ctx_base;
+      const Pair<DT> oD = ctx[0][ci], oX = ctx[1][ci];
+      const DT oD1a = oD.a, oD1b = oD.b, oX1a = oX.a, oX1b = oX.b;
       const DT LD1a = shr1(D1[CPL - 2], oD1a), LD1b = shr1(D1[CPL - 1], oD1b);
       const DT LX1a = shr1(X1[CPL - 2], oX1a), LX1b = shr1(X1[CPL - 1], oX1b);
 
@@ -588,7 +617,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
         }
       }
       if constexpr (!FULL && !FEAT) {
-        if (c0 == 0) eC[td.e_off + Lmax + r] = (double)Cc[0];
+        if (c0 == 0) eCc[r] = (double)Cc[0];
       }
       if (border) {
         // Cells whose predecessor is on row 0 or column 0, and column 0
@@ -646,7 +675,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
         for (int j = 0; j < CPL; ++j) bacc[j] |= (uint32_t)code[j] << sh;
         // full words here; the last partial word after the sweep
         if ((r & 15) == 15) {
-          uint32_t *dst = Bp + td.b_off + (int64_t)(r >> 4) * W;
+          uint32_t *dst = Bt + (int64_t)(r >> 4) * W;
 #pragma unroll
           for (int j = 0; j < CPL; ++j) {
             const int c = c0 + j;
@@ -659,8 +688,8 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
 #pragma unroll
             for (int j = 0; j < CPL; ++j)
               if (j == jlast) {
-                eD[td.e_off + Lmax + r] = (double)ND[j];
-                eP[td.e_off + Lmax + r] = p_of(NX[j], r + W - 1);
+                eDc[r] = (double)ND[j];
+                ePc[r] = p_of(NX[j], r + W - 1);
               }
           }
         }
@@ -686,37 +715,10 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
         right_ok = seen_right + kRing - 1;
       }
       if (lane == 63) {
-        const int s0 = r & (kRing - 1);
-        shD[s0][wave][0] = D1[CPL - 2];
-        shD[s0][wave][1] = D1[CPL - 1];
-        shX[s0][wave][0] = X1[CPL - 2];
-        shX[s0][wave][1] = X1[CPL - 1];
+        const int s0 = slot * 16 + wave;
+        ctx[0][s0] = Pair<DT>{D1[CPL - 2], D1[CPL - 1]};
+        ctx[1][s0] = Pair<DT>{X1[CPL - 2], X1[CPL - 1]};
         __hip_atomic_store(&prog[wave], r, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-      }
-      // Next exchange block: wait for the strip on the left and stage its
-      // rows blk0-1 .. blk0+G-2 (row r-1 of every row r of the block).
-      if (r + 1 == next_blk) {
-        blk0 = r + 1;
-        next_blk = (blk0 + G < R) ? blk0 + G : kNever;
-        if (wave == 0) {
-          ctx_off = blk0;
-          const int last = blk0 + G - 1;
-          const int need = (last < R) ? last : R;
-          while (__hip_atomic_load(fin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need)
-            __builtin_amdgcn_s_sleep(2);
-          for (int q0 = lane; q0 < G; q0 += 64) {
-            const int q = blk0 - 1 + q0;
-            if (q < R) {
-              const Xch<DT> v = xin[q];
-              lxD[q0][0] = v.d0;
-              lxD[q0][1] = v.d1;
-              lxX[q0][0] = v.x0;
-              lxX[q0][1] = v.x1;
-            }
-          }
-          // lane 0 of this wave reads the staged rows from the next row on
-          __threadfence_block();
-        }
       }
   };
 
@@ -732,7 +734,7 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
 #pragma unroll
     for (int u = 0; u < kPF; ++u) {
       const int r = rb + u;
-      row_step(r, Cq[u]);
+      row_step(r, (kPF % kRing == 0) ? (1 + u) % kRing : r % kRing, Cq[u]);
       // Refill the slot only after its row is done, so the load targets the
       // same registers (no copy of an in-flight load at the loop edge).
       // Clamped to a valid row for global loads; the features form reads
@@ -751,12 +753,12 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
 #pragma unroll
   for (int u = 0; u < kPF; ++u) {
     const int r = rb + u;
-    if (r < R) row_step(r, Cq[u]);
+    if (r < R) row_step(r, r % kRing, Cq[u]);
   }
   if constexpr (!FULL) {
     // Last back-pointer word when it holds fewer than 16 rows.
     if (R > 1 && ((R - 1) & 15) != 15) {
-      uint32_t *dst = Bp + td.b_off + (int64_t)((R - 1) >> 4) * W;
+      uint32_t *dst = Bt + (int64_t)((R - 1) >> 4) * W;
 #pragma unroll
       for (int j = 0; j < CPL; ++j) {
         const int c = c0 + j;
@@ -769,8 +771,8 @@ __global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
       for (int j = 0; j < CPL; ++j) {
         const int c = c0 + j;
         if (c < W) {
-          eD[td.e_off + c] = (double)D1[j];
-          eP[td.e_off + c] = p_of(X1[j], R - 1 + c);
+          eDc[c - Lmax] = (double)D1[j];
+          ePc[c - Lmax] = p_of(X1[j], R - 1 + c);
         }
       }
     }
