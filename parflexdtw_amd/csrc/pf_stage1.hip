@@ -181,8 +181,11 @@ typedef float s1floatx16_t __attribute__((ext_vector_type(16)));
 // buffer suffices: the C prefetch kPF rows ahead finishes reading block b
 // one row before it starts reading block b+1, and block b+1 is computed in
 // between.
-template <int CPL, typename CT, typename DT, bool FULL, bool STD, bool FEAT = false>
-__global__ __launch_bounds__((MaxThreads<CPL, FULL>::value)) void k_stage1(
+// MAXT bounds the workgroup size, and with it the registers per lane: a
+// 1024-thread workgroup (16 waves, 4 per SIMD) leaves 128 VGPRs a lane.
+template <int CPL, typename CT, typename DT, bool FULL, bool STD, bool FEAT = false,
+          int MAXT = MaxThreads<CPL, FULL>::value>
+__global__ __launch_bounds__(MAXT) void k_stage1(
     const CT *__restrict__ Cb, const TileDesc *__restrict__ tiles, TileDesc td0, StepSet ss,
     int64_t Lmax, int S, int G, int *__restrict__ flags, Xch<DT> *__restrict__ xch,
     uint32_t *__restrict__ Bp, double *__restrict__ eD, int32_t *__restrict__ eP,
@@ -785,6 +788,16 @@ static void launch_cpl(dim3 grid, dim3 block, hipStream_t st, const CT *C, const
                        int *flags, void *xch, uint32_t *Bp, double *eD, int32_t *eP, double *eC,
                        double *Df, int32_t *Pf, int8_t *Bf, bool full) {
   Xch<DT> *x = static_cast<Xch<DT> *>(xch);
+  if constexpr (sizeof(DT) == 8 && CPL == 2) {
+    // FP64 sweep of up to 4 waves (the default shape): built for 256
+    // threads, so its state fits in registers without spills
+    if (!full && block.x <= 256) {
+      hipLaunchKernelGGL((k_stage1<CPL, CT, DT, false, STD, false, 256>), grid, block, 0, st, C,
+                         tiles, td0, ss, Lmax, S, G, flags, x, Bp, eD, eP, eC, Df, Pf, Bf,
+                         nullptr, nullptr, 0);
+      return;
+    }
+  }
   if (full)
     hipLaunchKernelGGL((k_stage1<CPL, CT, DT, true, STD>), grid, block, 0, st, C, tiles, td0, ss,
                        Lmax, S, G, flags, x, Bp, eD, eP, eC, Df, Pf, Bf, nullptr, nullptr, 0);
