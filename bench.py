@@ -58,13 +58,13 @@ METRIC = "DP GCUPS + pairs/s, batched/learned vs plain GPU at >=99.5% path agree
 TRAFFIC_BYTES = {("chroma_batch256_N3000-20000", 500, "materialised"): (83.97 + 12.84) * 1e9,
                  ("chroma_learned128_N20000-60000_L2000", 2000, "materialised"):
                      (46.70 + 6.21) * 1e9,
-                 ("chroma_learned128_N20000-60000_L2000", 2000, "fused"): (0.95 + 7.49) * 1e9}
+                 ("chroma_learned128_N20000-60000_L2000", 2000, "fused"): (0.98 + 7.87) * 1e9}
 # VALU wave instructions per computed cell of the stage-1 kernel, measured
 # with rocprofv3 PMC (SQ_INSTS_VALU / cells; profiles/pmc_valu_*.md): the
 # kernel's real ceiling is VALU issue, 1024 SIMDs x one wave64 VALU
 # instruction per 4 cycles at 2.4 GHz.
 VALU_INSTS_PER_CELL = {("chroma_learned128_N20000-60000_L2000", 2000, "fused"):
-                           1.2137e10 / 2.2747e10}
+                           1.2016e10 / 2.2747e10}
 VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
 HBM_PEAK_GBS = 8000.0
 STEPS, WEIGHTS, BETA = [(1, 1), (1, 2), (2, 1)], [2, 3, 3], 0.1
@@ -258,7 +258,7 @@ def bench_learned(args, torch):
     fused = args.cost == "fused"
     out["roofline"] = _roofline(
         st["cells_computed"], s1_ms, 0.25 if fused else 2.25,
-        f"k_stage1<2,__hip_bfloat16,float,false,true{',true' if fused else ''}>") if s1_ms else None
+        f"k_stage1<2,__hip_bfloat16,float,false,true,{'true' if fused else 'false'},1024>") if s1_ms else None
     key = (args.workload, L, args.cost)
     if out["roofline"] and P == 128 and key in TRAFFIC_BYTES:
         out["roofline"]["traffic"] = TRAFFIC_BYTES[key]
@@ -390,7 +390,7 @@ def bench_batch(args, torch):
     fused = args.cost == "fused"
     out["roofline"] = _roofline(
         cells, s1_ms, 0.25 if fused else 2.25,
-        f"k_stage1<2,__hip_bfloat16,float,false,true{',true' if fused else ''}>")
+        f"k_stage1<2,__hip_bfloat16,float,false,true,{'true' if fused else 'false'},1024>")
     if args.pairs == 256 and (args.workload, L, args.cost) in TRAFFIC_BYTES:
         out["roofline"]["traffic"] = TRAFFIC_BYTES[(args.workload, L, args.cost)]
     out["evaluation"] = _evaluation(torch, gts, {
@@ -474,7 +474,7 @@ def bench_plain(args, torch):
     out["pairs_per_s"] = round(1e3 / ms_step, 3)
     out["stage1_ms"] = round(s1_ms, 4)
     out["stage2_ms"] = round(s2_ms, 4)
-    out["roofline"] = _roofline(cells, s1_ms, 8.25, "k_stage1<2,double,double,false,true>")
+    out["roofline"] = _roofline(cells, s1_ms, 8.25, "k_stage1<2,double,double,false,true,false,256>")
     if not args.no_cpu:
         from oracle import oracle
         C = C_dev.cpu().numpy()
@@ -603,10 +603,10 @@ def bench_precision(args, torch):
     if fused:
         out["stage1_shape"] = list(cfg)
         out["roofline"] = _roofline(cells, s1_ms, 0.25,
-                                    "k_stage1<CPL,__hip_bfloat16,float,false,true,true>")
+                                    "k_stage1<CPL,__hip_bfloat16,float,false,true,true,MAXT>")
     else:
         out["roofline"] = _roofline(cells, s1_ms, 2.25,
-                                    "k_stage1<2,__hip_bfloat16,float,false,true>")
+                                    "k_stage1<2,__hip_bfloat16,float,false,true,false,1024>")
     if not args.no_cpu:
         from oracle import oracle
         C = C64.cpu().numpy()
