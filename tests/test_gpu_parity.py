@@ -184,6 +184,10 @@ S1_CONFIGS = [  # (shape, L, steps, weights, (strips, waves, rows per exchange b
     ((300, 1100), 1100, STEPS, [2, 3, 3], (1, 8, 32)),
     ((300, 2100), 2048, STEPS, [2, 3, 3], (1, 16, 16)),
     ((200, 4100), 4000, STEPS, [2, 3, 3], (2, 16, 32)),
+    # last tile row of 34 rows: its consumer strips stage every row at once
+    ((333, 410), 300, STEPS, [2, 3, 3], (2, 1, 64)),
+    # exchange block of 33 rows (not a power of two), 4-wave strips
+    ((900, 700), 450, STEPS, [2, 3, 3], (2, 2, 33)),
 ]
 
 
