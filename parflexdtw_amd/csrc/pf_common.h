@@ -66,6 +66,11 @@ struct PairDesc {
 };
 static_assert(sizeof(PairDesc) == 128, "pair table rows are int64[16]");
 
+// Launch of the stage-2 back-pointer walk (pf_walk.hip): grid (101 segments,
+// requests), one wave each, on stream st.
+void launch_s2_walk(dim3 grid, hipStream_t st, const TileDesc *tiles, int64_t L, StepSet ss,
+                    const uint32_t *Bp, int64_t *chain, const int64_t *cnt, int64_t *pts);
+
 __device__ __forceinline__ double to_f64(double v) { return v; }
 __device__ __forceinline__ double to_f64(float v) { return (double)v; }
 __device__ __forceinline__ double to_f64(__hip_bfloat16 v) { return (double)__bfloat162float(v); }
