@@ -21,8 +21,9 @@
 //    pair reduces the segment winners (global threshold 1e9);
 //  * stitch (Parflex.py:1032-1103): one thread per request walks the <= 101
 //    tiles (only edge P is needed), then one wave per segment walks the
-//    packed back-pointers through an LDS window, and one workgroup per request
-//    concatenates, removes the duplicated overlap cells and reverses.
+//    packed back-pointers through a register window (pf_walk.hip), and one
+//    workgroup per request concatenates, removes the duplicated overlap cells
+//    and reverses.
 #include "pf_common.h"
 
 namespace pf {
