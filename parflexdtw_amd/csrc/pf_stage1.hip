@@ -107,11 +107,17 @@ static_assert(sizeof(Xch<double>) == 32 && sizeof(Xch<float>) == 16,
 
 constexpr int kMaxG = 64;   // rows per exchange block, upper bound
 // Rows of C prefetched ahead, and rows per unrolled step of the sweep: 8
-// for the packed FP32 DP with the standard step set and up to 4 columns per
-// lane, 4 otherwise (the wider state of the other variants would not fit
-// their register budget with 8).
-template <typename DT, int CPL, bool FULL, bool STD>
-constexpr int prefetch_rows() { return (sizeof(DT) == 4 && !FULL && STD && CPL <= 4) ? 8 : 4; }
+// for the packed DP with the standard step set in FP32 (up to 4 columns per
+// lane) and in FP64 built for at most 256 threads (2 columns per lane, up to
+// 512 VGPRs), 4 otherwise (the wider state of the other variants would not
+// fit their register budget with 8).
+template <typename DT, int CPL, bool FULL, bool STD, int MAXT>
+constexpr int prefetch_rows() {
+  return (!FULL && STD &&
+          ((sizeof(DT) == 4 && CPL <= 4) || (sizeof(DT) == 8 && CPL == 2 && MAXT <= 256)))
+             ? 8
+             : 4;
+}
 constexpr int kRing = 8;    // rows of wave-to-wave hand-over kept in LDS
 // Left context of every wave in one LDS array of column pairs (the two
 // columns left of a wave), the D pairs in ctx[0] and the X pairs in ctx[1]
@@ -255,7 +261,7 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
 
   DT D1[CPL], X1[CPL], D2[CPL], X2[CPL];
   uint32_t bacc[CPL];
-  constexpr int kPF = prefetch_rows<DT, CPL, FULL, STD>();
+  constexpr int kPF = prefetch_rows<DT, CPL, FULL, STD, MAXT>();
   DT Cq[kPF][CPL];
   // Left context: columns c0-2 (a), c0-1 (b) of rows r-1 (1) and r-2 (2).
   DT LD2a = INF, LD2b = INF, LX2a = 0, LX2b = 0;
