@@ -286,10 +286,12 @@ def _config_candidates(L, feat=False):
     return out
 
 
-def autotune_stage1(C_dev, plan, sw, ws, Gs=(32,), reps=2, stream=None, feat=None):
+def autotune_stage1(C_dev, plan, sw, ws, Gs=(32, 64), reps=2, stream=None, feat=None):
     """Time stage 1 for the candidate launch shapes on this plan and keep the
     fastest in ws['s1_config'] (and a process-wide cache keyed by tile length,
-    dtype and tile count).  Returns (config, {config: ms})."""
+    dtype and tile count).  The exchange block G only matters with more than
+    one strip; one-strip shapes are timed with the first G.  Returns
+    (config, {config: ms})."""
     torch = _torch()
     key = (plan.L, "feat" if feat is not None else dtype_code(C_dev), plan.ntiles // 64)
     if key in _TUNED:
@@ -297,7 +299,7 @@ def autotune_stage1(C_dev, plan, sw, ws, Gs=(32,), reps=2, stream=None, feat=Non
         return _TUNED[key], {}
     times = {}
     for (S, nw) in _config_candidates(plan.L, feat is not None):
-        for G in Gs:
+        for G in (Gs if S > 1 else Gs[:1]):
             cfg = (S, nw, G)
             stage1(C_dev, plan, sw, ws, stream, config=cfg, feat=feat)  # warm-up / allocation
             torch.cuda.synchronize()
