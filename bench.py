@@ -58,13 +58,13 @@ METRIC = "DP GCUPS + pairs/s, batched/learned vs plain GPU at >=99.5% path agree
 TRAFFIC_BYTES = {("chroma_batch256_N3000-20000", 500, "materialised"): (83.97 + 12.84) * 1e9,
                  ("chroma_learned128_N20000-60000_L2000", 2000, "materialised"):
                      (46.70 + 6.21) * 1e9,
-                 ("chroma_learned128_N20000-60000_L2000", 2000, "fused"): (0.97 + 7.72) * 1e9}
+                 ("chroma_learned128_N20000-60000_L2000", 2000, "fused"): (1.08 + 6.19) * 1e9}
 # VALU wave instructions per computed cell of the stage-1 kernel, measured
 # with rocprofv3 PMC (SQ_INSTS_VALU / cells; profiles/pmc_valu_*.md): the
 # kernel's real ceiling is VALU issue, 1024 SIMDs x one wave64 VALU
 # instruction per 4 cycles at 2.4 GHz.
 VALU_INSTS_PER_CELL = {("chroma_learned128_N20000-60000_L2000", 2000, "fused"):
-                           1.2100e10 / 2.2747e10}
+                           1.2283e10 / 2.2747e10}
 VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
 HBM_PEAK_GBS = 8000.0
 STEPS, WEIGHTS, BETA = [(1, 1), (1, 2), (2, 1)], [2, 3, 3], 0.1
