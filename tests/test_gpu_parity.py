@@ -188,6 +188,9 @@ S1_CONFIGS = [  # (shape, L, steps, weights, (strips, waves, rows per exchange b
     ((333, 410), 300, STEPS, [2, 3, 3], (2, 1, 64)),
     # exchange block of 33 rows (not a power of two), 4-wave strips
     ((900, 700), 450, STEPS, [2, 3, 3], (2, 2, 33)),
+    # 8 columns per lane: one strip of 4 waves, and two strips of 2 waves
+    ((300, 2100), 2048, STEPS, [2, 3, 3], (1, 4, 32)),
+    ((500, 1500), 1200, STEPS, [2, 3, 3], (2, 2, 24)),
 ]
 
 
