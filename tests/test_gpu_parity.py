@@ -236,6 +236,49 @@ def test_stage1_packed_outputs_vs_oracle(gpu, oracle_mod, case):
             assert _same(codes.astype(np.int8), ob)
 
 
+FP32_CONFIGS = [(None, 200), ((1, 1, 16), 200), ((2, 1, 16), 200), ((2, 1, 7), 150),
+                ((1, 2, 32), 90)]
+
+
+@pytest.mark.parametrize("case", range(len(FP32_CONFIGS)))
+@pytest.mark.parametrize("dtype", ["float32", "bfloat16"])
+def test_stage1_fp32_integer_costs_vs_oracle(gpu, oracle_mod, case, dtype):
+    """The FP32 DP (standard steps) on integer costs 0..7 and tiles of at
+    most 200 rows: every sum and cross product is exact in FP32 and the FP64
+    quotients are distinct wherever they differ, so edges and every packed
+    back-pointer must equal the FP64 oracle's (ragged last rows and columns,
+    one and two strips)."""
+    import torch
+    from parflexdtw_amd import engine
+    from parflexdtw_amd._plan import BatchPlan
+    cfg, L = FP32_CONFIGS[case]
+    rng = np.random.RandomState(77 + case)
+    C = rng.randint(0, 8, size=(333, 410)).astype(np.float64)
+    plan = BatchPlan([C.shape], L)
+    sw = engine.StepWeights(STEPS, [2, 3, 3])
+    C_dev = torch.from_numpy(C).to(getattr(torch, dtype)).cuda()
+    ws = plan.allocate(C_dev.device)
+    engine.stage1(C_dev, plan, sw, ws, config=cfg)
+    s1 = oracle_mod.run_stage1(C, L, STEPS, [2, 3, 3])
+    n1, n2 = plan.grids[0]
+    eD = ws["edge_D"].cpu().numpy().reshape(n1, n2, 2, L)
+    eP = ws["edge_P"].cpu().numpy().reshape(n1, n2, 2, L)
+    Bp = ws["Bp"].cpu().numpy().view(np.uint32)
+    for i in range(n1):
+        for j in range(n2):
+            t = plan.tiles[i * n2 + j]
+            R, W, boff = int(t[2]), int(t[3]), int(t[4])
+            assert _same(eD[i, j, 0, :W], s1["edge_D"][i, j, 0, :W]), (i, j)
+            assert _same(eD[i, j, 1, :R], s1["edge_D"][i, j, 1, :R]), (i, j)
+            assert _same(eP[i, j, 0, :W], s1["edge_P"][i, j, 0, :W]), (i, j)
+            assert _same(eP[i, j, 1, :R], s1["edge_P"][i, j, 1, :R]), (i, j)
+            words = Bp[boff:boff + ((R + 15) // 16) * W].reshape(-1, W)
+            r = np.arange(R)
+            codes = (words[r // 16, :] >> (2 * (r % 16))[:, None].astype(np.uint32)) & 3
+            ob = s1["B"][i, j].reshape(-1)[:R * W].reshape(R, W)
+            assert _same(codes.astype(np.int8), ob), (i, j)
+
+
 def test_batched_pairs_match_single(gpu, oracle_mod):
     """Several ragged pairs in one stage-1 launch and one stage-2 pass."""
     import torch
