@@ -152,6 +152,16 @@ __device__ __forceinline__ float dabs(float v) { return fabsf(v); }
 __device__ __forceinline__ double copy_sign(double mag, double sgn) { return copysign(mag, sgn); }
 __device__ __forceinline__ float copy_sign(float mag, float sgn) { return copysignf(mag, sgn); }
 
+// 2 * acc + (this lane's bit of mask): one v_addc_co_u32 with the lane mask
+// as carry-in.  Appending bits this way, MSB first, and reversing the word
+// (v_bfrev_b32) puts the first appended bit at bit 0.
+__device__ __forceinline__ uint32_t shift_in(uint32_t acc, uint64_t mask) {
+  uint32_t out;
+  uint64_t carry;
+  asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(out), "=s"(carry) : "v"(acc), "s"(mask));
+  return out;
+}
+
 // Start encoding P of a cell from its signed path length X (see below).
 template <typename DT>
 __device__ __forceinline__ int p_of(DT X, int rc) {
@@ -487,6 +497,10 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
 
       DT ND[CPL], NX[CPL];
       int code[CPL];
+      // FP32 standard step set, packed output: the step codes as lane masks
+      // (code 1: mb1, code 2: mb2), shifted into bacc without forming code
+      constexpr bool kMaskCodes = sizeof(DT) == 4 && STD && !FULL;
+      uint64_t mb1[CPL], mb2[CPL];
       bool amb = false;  // some cell of this lane needs the exact comparison
       // rows 1-2 and columns 0-2 of the tile: predecessors on the boundary
       const bool border = r <= border_rows;
@@ -520,6 +534,10 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
             bx = t1 ? pX1 : pX0;
             bk = t1 ? 1 : 0;
             const bool t2 = a2 * bm < ba * m2;
+            if constexpr (kMaskCodes) {
+              mb2[j] = __builtin_amdgcn_ballot_w64(t2);
+              mb1[j] = __builtin_amdgcn_ballot_w64(t1) & ~mb2[j];
+            }
             ba = t2 ? a2 : ba;
             bm = t2 ? m2 : bm;
             bx = t2 ? pX2 : bx;
@@ -679,16 +697,25 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
           }
         }
       } else {
-        const int sh = 2 * (r & 15);
+        if constexpr (kMaskCodes) {
+          // row r's code bits, low bit first: after 16 rows and a bit
+          // reversal the code of row k sits at bits 2k, 2k+1.  Column 0
+          // (code 0 by the boundary rule) is zeroed when its word is stored.
 #pragma unroll
-        for (int j = 0; j < CPL; ++j) bacc[j] |= (uint32_t)code[j] << sh;
+          for (int j = 0; j < CPL; ++j) bacc[j] = shift_in(shift_in(bacc[j], mb1[j]), mb2[j]);
+        } else {
+          const int sh = 2 * (r & 15);
+#pragma unroll
+          for (int j = 0; j < CPL; ++j) bacc[j] |= (uint32_t)code[j] << sh;
+        }
         // full words here; the last partial word after the sweep
         if ((r & 15) == 15) {
           uint32_t *dst = Bt + (int64_t)(r >> 4) * W;
 #pragma unroll
           for (int j = 0; j < CPL; ++j) {
             const int c = c0 + j;
-            if (c < W) dst[c] = bacc[j];
+            const uint32_t wv = kMaskCodes ? (c == 0 ? 0u : __builtin_bitreverse32(bacc[j])) : bacc[j];
+            if (c < W) dst[c] = wv;
             bacc[j] = 0u;
           }
         }
@@ -768,10 +795,16 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
     // Last back-pointer word when it holds fewer than 16 rows.
     if (R > 1 && ((R - 1) & 15) != 15) {
       uint32_t *dst = Bt + (int64_t)((R - 1) >> 4) * W;
+      // rows of the word: n = ((R-1) & 15) + 1, 2n appended bits
+      const int n = ((R - 1) & 15) + 1;
 #pragma unroll
       for (int j = 0; j < CPL; ++j) {
         const int c = c0 + j;
-        if (c < W) dst[c] = bacc[j];
+        const uint32_t wv =
+            (sizeof(DT) == 4 && STD)
+                ? (c == 0 ? 0u : __builtin_bitreverse32(bacc[j]) >> (32 - 2 * n))
+                : bacc[j];
+        if (c < W) dst[c] = wv;
       }
     }
     // Last row of the tile (edge 0); R == 1 was written with row 0.
