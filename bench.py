@@ -64,7 +64,7 @@ TRAFFIC_BYTES = {("chroma_batch256_N3000-20000", 500, "materialised"): (83.97 + 
 # kernel's real ceiling is VALU issue, 1024 SIMDs x one wave64 VALU
 # instruction per 4 cycles at 2.4 GHz.
 VALU_INSTS_PER_CELL = {("chroma_learned128_N20000-60000_L2000", 2000, "fused"):
-                           1.2283e10 / 2.2747e10}
+                           1.1678e10 / 2.2747e10}
 VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
 HBM_PEAK_GBS = 8000.0
 STEPS, WEIGHTS, BETA = [(1, 1), (1, 2), (2, 1)], [2, 3, 3], 0.1
