@@ -33,7 +33,7 @@ __global__ __launch_bounds__(64) void k_s2_walk(const TileDesc *__restrict__ til
   const int64_t t = en[0];
   const TileDesc td = wave_uniform(tiles[t]);
   // Walk state is wave-uniform; readfirstlane keeps it in scalar registers.
-  const int rows = (int)td.R, cols = (int)td.W;
+  const int cols = (int)td.W;
   const int sr = __builtin_amdgcn_readfirstlane((int)en[3]);
   const int sc = __builtin_amdgcn_readfirstlane((int)en[4]);
   const int64_t gro = en[5], gco = en[6];
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(64) void k_s2_walk(const TileDesc *__restrict__ til
   select_block();
   // The loop's only exit is a stop.  Every step lowers the row (1 <= dr <= 2,
   // enforced when the step set is made), so a step off the tile (pr < 0)
-  // ends the walk within `rows` steps.
+  // ends the walk within R steps (R: the tile's rows).
   for (;;) {
     int dr_ = r - r_lo, dc_ = c - c_lo;
     if (__builtin_expect((dr_ | dc_) < 0, 0)) {
