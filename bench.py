@@ -13,7 +13,7 @@ Workloads (--workload):
       (or exceeds the calibrated cost bound) is redone by the full-tile
       batched form.  One step = features (in HBM) -> paths on the host for
       all pairs.  Beside it: the plain FP64 drop-in (parflex per pair, cost
-      build included) on every pair, which is the judge of agreement.
+      build included) on every pair, the reference for agreement.
   chroma_batch256_N3000-20000: 256 seeded chroma pairs (K = 12),
       N ~ U[3000, 20000], M = N * U[0.8, 1.25], trimmed ends, so every pair
       has ragged edge tiles.  The batched form: one stage-1 launch over every
@@ -25,7 +25,7 @@ Workloads (--workload):
       stage 2 -> best costs and paths on the host for all 256 pairs.
       Beside it: the plain FP64 drop-in (parflex per pair, cost build
       included) on a sample of the same pairs at the same L, whose paths are
-      the judge of agreement (frames within +-2), and the CPU oracle on the
+      the reference for agreement (frames within +-2), and the CPU oracle on the
       same sample.
   chroma16000x19000_L4000: the plain FP64 drop-in on one pair, exact against
       the CPU oracle (cost matrix resident in HBM).
@@ -202,7 +202,7 @@ def bench_learned(args, torch):
         f"{st['cells_computed'] / st['cells_full']:.4f} of the cells; fallbacks "
         f"{len(st['fallback_pairs'])} {st['fallback_reasons']}")
 
-    # the plain FP64 drop-in on every pair: the judge, and the plain pairs/s
+    # the plain FP64 drop-in on every pair: the reference for agreement, and the plain pairs/s
     F1d, F2d = pairs[0]
     PG.parflex(cost_matrix_from_features_gpu(F1d, F2d), STEPS, WEIGHTS, BETA, L,
                profile_dir=None)
@@ -247,7 +247,7 @@ def bench_learned(args, torch):
     out["agreement"] = {"frames_within_2_all": round(frames_ok / frames, 6),
                         "frames_within_2_min_pair": round(min(agree), 6),
                         "best_cost_rel_err_max": float(max(rel)), "pairs_compared": P,
-                        "judge": "plain FP64 path of the same pair at the same L"}
+                        "reference": "plain FP64 path of the same pair at the same L"}
     out["model"] = {"file": "parflexdtw_amd/data/tile_net_L2000.npz",
                     "train_pairs": int(meta["train_pairs"]),
                     "train_N": f"U[{int(meta['n_lo'])},{int(meta['n_hi'])}]",
@@ -337,7 +337,7 @@ def bench_batch(args, torch):
     log(f"batched: {ms_step:.2f} ms/step ({pairs_s:.1f} pairs/s), cost {cost_ms:.2f} ms, "
         f"stage1 {s1_ms:.2f} ms, stage2 {s2_ms:.2f} ms, found {int(res.found.sum())}/{P}")
 
-    # Plain FP64 drop-in on a sample of the same pairs, same L: the judge of
+    # Plain FP64 drop-in on a sample of the same pairs, same L: the reference for
     # agreement and the plain pairs/s.
     ns = min(args.plain_sample, P)
     plain, agree, rel_err = [], [], []
@@ -392,7 +392,7 @@ def bench_batch(args, torch):
     out["agreement"] = {"frames_within_2_min": round(min(agree), 6),
                         "frames_within_2_mean": round(float(np.mean(agree)), 6),
                         "best_cost_rel_err_max": float(max(rel_err)),
-                        "pairs_compared": ns, "judge": "plain FP64 path, same pair and L"}
+                        "pairs_compared": ns, "reference": "plain FP64 path, same pair and L"}
     fused = args.cost == "fused"
     out["roofline"] = _roofline(
         cells, s1_ms, 0.25 if fused else 2.25,
@@ -567,7 +567,7 @@ def bench_precision(args, torch):
     s2_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in events]))
     cells = plan.cells
 
-    # FP64 plain form on the same pair: the judge of cost error and agreement.
+    # FP64 plain form on the same pair: the reference for cost error and agreement.
     if Cb is None:
         # the fused form's cost values are those of the BF16 cost kernel
         # (bit-identical, tested); build that matrix here for the error figure
@@ -605,7 +605,7 @@ def bench_precision(args, torch):
     out["accuracy"] = {"cost_max_abs_err": cerr, "best_cost": bc, "best_cost_fp64": bc64,
                        "best_cost_abs_err": abs(bc - bc64),
                        "best_cost_rel_err": abs(bc - bc64) / abs(bc64),
-                       "frames_within_2": round(agree, 6), "judge": "plain FP64 path"}
+                       "frames_within_2": round(agree, 6), "reference": "plain FP64 path"}
     if fused:
         out["stage1_shape"] = list(cfg)
         out["roofline"] = _roofline(cells, s1_ms, 0.25,
@@ -633,8 +633,8 @@ def bench_precision(args, torch):
 def main():
     args = _parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); every GPU box here "
-              "has one GPU and the distributed path is not part of this build.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
+              "path (stage 1 sharded by tile bands) is not implemented.", file=sys.stderr)
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)

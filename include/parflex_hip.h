@@ -12,8 +12,7 @@
  * The reference (ishitajain21/ParFlexDTW) is Python: its GPU interface is the
  * set of Python functions in gpu_flexdtw.py / Parflex_gpu.py, which reach one
  * CUDA kernel through CuPy.  Each entry point below names the reference
- * function (file:line) whose work it replaces; the ctypes binding a
- * maintainer would add to the reference is shown in INTEGRATION.md.
+ * function (file:line) whose work it replaces.
  */
 #ifndef PARFLEX_HIP_H
 #define PARFLEX_HIP_H

@@ -21,7 +21,7 @@ A pair falls back to the plain full-tile method (engine.BatchRunner over all
 its tiles) when a tile within `guard` frames of its stitched path was pruned
 (the path reached the edge of the selection), when no endpoint is found, or
 when its normalised cost exceeds the calibrated bound.  The plain method is
-also the judge of the result.
+also the reference the result is checked against.
 """
 from __future__ import annotations
 

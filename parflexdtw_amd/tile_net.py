@@ -2,7 +2,7 @@
 over the 1/32-pooled cost matrix that scores every tile by how likely the
 alignment path crosses it; only tiles above a threshold, plus a margin, run
 stage 1.  The reference computes every tile, so this module has no reference
-counterpart; the plain full-tile method is its fallback and its judge.
+counterpart; the plain full-tile method is its fallback and the reference it is checked against.
 
 Every kernel is in csrc/pf_learn.hip: pooled features and pooled cost, the
 convolution layers on MFMA (BF16 operands, FP32 accumulation), the weight

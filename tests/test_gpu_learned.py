@@ -1,5 +1,5 @@
 """GPU tests of the learned pipeline: what the trained network gains over the
-method without it, judged by the plain FP64 method, and that training with
+method without it, checked against the plain FP64 method, and that training with
 the project's kernels lowers the loss."""
 import os
 
