@@ -8,6 +8,8 @@ like the reference's own:
     flexdtw(C, steps, weights, buffer)          FlexDTW.py:1320-1372
     run_stage1(C, L, steps, weights)            Parflex.py:376-469 (edges only)
     run_stage2(C, L, steps, beta, s1)           Parflex.py:862-1356
+    propagate(C, s1)                            Parflex.py:862-912 (before the sync)
+    sync_overlap(D_arr, L_arr, edge_lens)       Parflex.py:919-969
     parflex(C, steps, weights, beta, L)         Parflex.py:1510-1547
     cost_matrix(F1, F2)                         Parflex.py:1470
 """
@@ -64,6 +66,10 @@ def _load():
     lib.or_parflex.argtypes = [_dp, _i64, _i64, _i64, _i64p, _dp, ctypes.c_int,
                                ctypes.c_double, ctypes.c_int, _dp, _i64p, _i64, _i64p]
     lib.or_parflex.restype = ctypes.c_int
+    lib.or_propagate.argtypes = [_dp, _i64, _i64, _i64, _dp, _i32p, _dp, _dp, _i64p]
+    lib.or_propagate.restype = ctypes.c_int
+    lib.or_sync_overlap.argtypes = [_dp, _dp, _i64p, _i64, _i64, _i64]
+    lib.or_sync_overlap.restype = None
     lib.or_cost_matrix.argtypes = [_dp, _dp, _i64, _i64, _i64, _dp]
     lib.or_cost_matrix.restype = None
     _lib = lib
@@ -205,6 +211,32 @@ def run_stage2(C, s1, beta):
             "best_cost": bn.value, "best_out": bo, "stitched_wp": path[:plen.value].copy(),
             "top_D": tD, "top_L": tL, "right_D": rD, "right_L": rL,
             "seg_norm": sn, "seg_D": sD, "seg_L": sL, "seg_i64": si}
+
+
+def propagate(C, s1):
+    """propagate_tile_edge_costs: (D_arr, L_arr, edge_lens) before the sync."""
+    lib = _load()
+    C = np.ascontiguousarray(C, dtype=np.float64)
+    L1, L2, L, n1, n2 = s1["L1"], s1["L2"], s1["L"], s1["n1"], s1["n2"]
+    Da = np.empty((n1, n2, 2, L), np.float64)
+    La = np.empty((n1, n2, 2, L), np.float64)
+    el = np.empty((n1, n2, 2), np.int64)
+    rc = lib.or_propagate(_p(C, _dp), L1, L2, L, _p(s1["edge_D"], _dp), _p(s1["edge_P"], _i32p),
+                          _p(Da, _dp), _p(La, _dp), _p(el, _i64p))
+    if rc < 0:
+        raise MemoryError("oracle propagate failed to allocate")
+    return Da, La, el
+
+
+def sync_overlap(D_arr, L_arr, edge_lens):
+    """sync_tile_overlap_edges on copies; returns (D_arr, L_arr)."""
+    lib = _load()
+    D = np.array(D_arr, dtype=np.float64, order="C", copy=True)
+    La = np.array(L_arr, dtype=np.float64, order="C", copy=True)
+    el = np.ascontiguousarray(edge_lens, dtype=np.int64)
+    n1, n2, _, L = D.shape
+    lib.or_sync_overlap(_p(D, _dp), _p(La, _dp), _p(el, _i64p), n1, n2, L)
+    return D, La
 
 
 def backtrace_and_stitch(s1, ci, cj, et, idx):

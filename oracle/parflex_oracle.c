@@ -439,14 +439,63 @@ int64_t or_backtrace_and_stitch(int64_t L1, int64_t L2, int64_t L,
   return n;
 }
 
-int or_stage2(const double *C, int64_t L1, int64_t L2, int64_t L,
-              const int64_t *steps, int K, double beta,
-              const double *edge_D, const int32_t *edge_P, const int8_t *B_all,
-              double *D_arr, double *L_arr, int64_t *edge_lens,
-              double *top_D, double *top_L, double *right_D, double *right_L,
-              double *seg_norm, double *seg_D, double *seg_L, int64_t *seg_i64,
-              int64_t max_segs, int64_t *best_out, double *best_norm_out,
-              int64_t *path, int64_t path_cap, int64_t *path_len) {
+/* sync_tile_overlap_edges (Parflex.py:919-969): three passes in order, in
+ * place; edge_lens [n1][n2][2] (cols, rows). */
+void or_sync_overlap(double *D_arr, double *L_arr, const int64_t *edge_lens, int64_t n1,
+                     int64_t n2, int64_t L) {
+  for (int64_t i = 0; i < n1; ++i)
+    for (int64_t j = 0; j + 1 < n2; ++j) {
+      const int64_t el = edge_lens[2 * (i * n2 + j)];
+      if (el == 0) continue;
+      const double dl = D_arr[IDX4(i, j, 0, el - 1)], dr = D_arr[IDX4(i, j + 1, 0, 0)];
+      if (isfinite(dl) && !isfinite(dr)) {
+        D_arr[IDX4(i, j + 1, 0, 0)] = dl;
+        L_arr[IDX4(i, j + 1, 0, 0)] = L_arr[IDX4(i, j, 0, el - 1)];
+      } else if (!isfinite(dl) && isfinite(dr)) {
+        D_arr[IDX4(i, j, 0, el - 1)] = dr;
+        L_arr[IDX4(i, j, 0, el - 1)] = L_arr[IDX4(i, j + 1, 0, 0)];
+      }
+    }
+  for (int64_t i = 0; i + 1 < n1; ++i)
+    for (int64_t j = 0; j < n2; ++j) {
+      const int64_t el = edge_lens[2 * (i * n2 + j) + 1];
+      if (el == 0) continue;
+      const double db = D_arr[IDX4(i, j, 1, el - 1)], dt = D_arr[IDX4(i + 1, j, 1, 0)];
+      if (isfinite(db) && !isfinite(dt)) {
+        D_arr[IDX4(i + 1, j, 1, 0)] = db;
+        L_arr[IDX4(i + 1, j, 1, 0)] = L_arr[IDX4(i, j, 1, el - 1)];
+      } else if (!isfinite(db) && isfinite(dt)) {
+        D_arr[IDX4(i, j, 1, el - 1)] = dt;
+        L_arr[IDX4(i, j, 1, el - 1)] = L_arr[IDX4(i + 1, j, 1, 0)];
+      }
+    }
+  for (int64_t i = 0; i < n1; ++i)
+    for (int64_t j = 0; j < n2; ++j) {
+      const int64_t et = edge_lens[2 * (i * n2 + j)], er = edge_lens[2 * (i * n2 + j) + 1];
+      if (et == 0 || er == 0) continue;
+      const double dtc = D_arr[IDX4(i, j, 0, et - 1)], drc = D_arr[IDX4(i, j, 1, er - 1)];
+      if (isfinite(dtc) && !isfinite(drc)) {
+        D_arr[IDX4(i, j, 1, er - 1)] = dtc;
+        L_arr[IDX4(i, j, 1, er - 1)] = L_arr[IDX4(i, j, 0, et - 1)];
+      } else if (!isfinite(dtc) && isfinite(drc)) {
+        D_arr[IDX4(i, j, 0, et - 1)] = drc;
+        L_arr[IDX4(i, j, 0, et - 1)] = L_arr[IDX4(i, j, 1, er - 1)];
+      } else if (isfinite(dtc) && isfinite(drc) && fabs(dtc - drc) > 1e-10) {
+        D_arr[IDX4(i, j, 1, er - 1)] = dtc;
+        L_arr[IDX4(i, j, 1, er - 1)] = L_arr[IDX4(i, j, 0, et - 1)];
+      }
+    }
+
+}
+
+static int stage2_impl(const double *C, int64_t L1, int64_t L2, int64_t L,
+                       const int64_t *steps, int K, double beta,
+                       const double *edge_D, const int32_t *edge_P, const int8_t *B_all,
+                       double *D_arr, double *L_arr, int64_t *edge_lens,
+                       double *top_D, double *top_L, double *right_D, double *right_L,
+                       double *seg_norm, double *seg_D, double *seg_L, int64_t *seg_i64,
+                       int64_t max_segs, int64_t *best_out, double *best_norm_out,
+                       int64_t *path, int64_t path_cap, int64_t *path_len, int presync) {
   int64_t n1, n2;
   or_tile_grid(L1, L2, L, &n1, &n2);
   const size_t n4 = (size_t)n1 * n2 * 2 * L;
@@ -692,49 +741,19 @@ int or_stage2(const double *C, int64_t L1, int64_t L2, int64_t L,
       }
     }
 
-  /* sync_tile_overlap_edges (Parflex.py:919-969). */
-  for (int64_t i = 0; i < n1; ++i)
-    for (int64_t j = 0; j + 1 < n2; ++j) {
-      const int64_t el = edge_lens[2 * (i * n2 + j)];
-      if (el == 0) continue;
-      const double dl = D_arr[IDX4(i, j, 0, el - 1)], dr = D_arr[IDX4(i, j + 1, 0, 0)];
-      if (isfinite(dl) && !isfinite(dr)) {
-        D_arr[IDX4(i, j + 1, 0, 0)] = dl;
-        L_arr[IDX4(i, j + 1, 0, 0)] = L_arr[IDX4(i, j, 0, el - 1)];
-      } else if (!isfinite(dl) && isfinite(dr)) {
-        D_arr[IDX4(i, j, 0, el - 1)] = dr;
-        L_arr[IDX4(i, j, 0, el - 1)] = L_arr[IDX4(i, j + 1, 0, 0)];
-      }
-    }
-  for (int64_t i = 0; i + 1 < n1; ++i)
-    for (int64_t j = 0; j < n2; ++j) {
-      const int64_t el = edge_lens[2 * (i * n2 + j) + 1];
-      if (el == 0) continue;
-      const double db = D_arr[IDX4(i, j, 1, el - 1)], dt = D_arr[IDX4(i + 1, j, 1, 0)];
-      if (isfinite(db) && !isfinite(dt)) {
-        D_arr[IDX4(i + 1, j, 1, 0)] = db;
-        L_arr[IDX4(i + 1, j, 1, 0)] = L_arr[IDX4(i, j, 1, el - 1)];
-      } else if (!isfinite(db) && isfinite(dt)) {
-        D_arr[IDX4(i, j, 1, el - 1)] = dt;
-        L_arr[IDX4(i, j, 1, el - 1)] = L_arr[IDX4(i + 1, j, 1, 0)];
-      }
-    }
-  for (int64_t i = 0; i < n1; ++i)
-    for (int64_t j = 0; j < n2; ++j) {
-      const int64_t et = edge_lens[2 * (i * n2 + j)], er = edge_lens[2 * (i * n2 + j) + 1];
-      if (et == 0 || er == 0) continue;
-      const double dtc = D_arr[IDX4(i, j, 0, et - 1)], drc = D_arr[IDX4(i, j, 1, er - 1)];
-      if (isfinite(dtc) && !isfinite(drc)) {
-        D_arr[IDX4(i, j, 1, er - 1)] = dtc;
-        L_arr[IDX4(i, j, 1, er - 1)] = L_arr[IDX4(i, j, 0, et - 1)];
-      } else if (!isfinite(dtc) && isfinite(drc)) {
-        D_arr[IDX4(i, j, 0, et - 1)] = drc;
-        L_arr[IDX4(i, j, 0, et - 1)] = L_arr[IDX4(i, j, 1, er - 1)];
-      } else if (isfinite(dtc) && isfinite(drc) && fabs(dtc - drc) > 1e-10) {
-        D_arr[IDX4(i, j, 1, er - 1)] = dtc;
-        L_arr[IDX4(i, j, 1, er - 1)] = L_arr[IDX4(i, j, 0, et - 1)];
-      }
-    }
+  if (!presync) or_sync_overlap(D_arr, L_arr, edge_lens, n1, n2, L);
+  if (presync) {
+    free(esr);
+    free(esc);
+    free(olap);
+    free(valid);
+    free(bot);
+    free(left);
+    free(rows);
+    free(cols);
+    free(bnd);
+    return 0;
+  }
 
   /* _build_edge_lookups (Parflex.py:1110-1150). */
   int64_t *tci = (int64_t *)malloc(sizeof(int64_t) * L2 * 4);
@@ -881,6 +900,28 @@ int or_stage2(const double *C, int64_t L1, int64_t L2, int64_t L,
   free(cols);
   free(bnd);
   return rc;
+}
+
+int or_stage2(const double *C, int64_t L1, int64_t L2, int64_t L,
+              const int64_t *steps, int K, double beta,
+              const double *edge_D, const int32_t *edge_P, const int8_t *B_all,
+              double *D_arr, double *L_arr, int64_t *edge_lens,
+              double *top_D, double *top_L, double *right_D, double *right_L,
+              double *seg_norm, double *seg_D, double *seg_L, int64_t *seg_i64,
+              int64_t max_segs, int64_t *best_out, double *best_norm_out,
+              int64_t *path, int64_t path_cap, int64_t *path_len) {
+  return stage2_impl(C, L1, L2, L, steps, K, beta, edge_D, edge_P, B_all, D_arr, L_arr,
+                     edge_lens, top_D, top_L, right_D, right_L, seg_norm, seg_D, seg_L, seg_i64,
+                     max_segs, best_out, best_norm_out, path, path_cap, path_len, 0);
+}
+
+/* propagate_tile_edge_costs (Parflex.py:862-912): D_arr, L_arr before the
+ * overlap sync, and edge_lens. */
+int or_propagate(const double *C, int64_t L1, int64_t L2, int64_t L, const double *edge_D,
+                 const int32_t *edge_P, double *D_arr, double *L_arr, int64_t *edge_lens) {
+  return stage2_impl(C, L1, L2, L, NULL, 0, 0.0, edge_D, edge_P, NULL, D_arr, L_arr, edge_lens,
+                     NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, 0, NULL, NULL, NULL, 0, NULL,
+                     1);
 }
 
 int or_parflex(const double *C, int64_t L1, int64_t L2, int64_t L,

@@ -85,6 +85,16 @@ int or_stage2(const double *C, int64_t L1, int64_t L2, int64_t L,
               int64_t max_segs, int64_t *best_out, double *best_norm,
               int64_t *path, int64_t path_cap, int64_t *path_len);
 
+/* propagate_tile_edge_costs (Parflex.py:862-912): the D_arr / L_arr of
+ * stage 2 before the overlap sync, and edge_lens.  Returns 0 or -1. */
+int or_propagate(const double *C, int64_t L1, int64_t L2, int64_t L, const double *edge_D,
+                 const int32_t *edge_P, double *D_arr, double *L_arr, int64_t *edge_lens);
+
+/* sync_tile_overlap_edges (Parflex.py:919-969) in place on D_arr, L_arr
+ * [n1][n2][2][L] with edge_lens [n1][n2][2]. */
+void or_sync_overlap(double *D_arr, double *L_arr, const int64_t *edge_lens, int64_t n1,
+                     int64_t n2, int64_t L);
+
 /* backtrace_and_stitch from an arbitrary edge slot (for per-segment paths). */
 int64_t or_backtrace_and_stitch(int64_t L1, int64_t L2, int64_t L,
                                 const int64_t *steps, int K,

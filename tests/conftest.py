@@ -28,7 +28,7 @@ def load_golden(name):
 STAGE2_CASES = ["tiles_seed2", "parflex_seed3", "stage2_chroma300_L50",
                 "stage2_chroma300_L64", "stage2_chroma420_L37", "stage2_chroma250_L400",
                 "stage2_rand_L9", "stage2_rand_L5", "stage2_rand_4step", "stage2_nocand",
-                "stage2_chroma600_L100"]
+                "stage2_chroma600_L100", "stage2_chroma400_L5_cap"]
 
 
 @pytest.fixture(scope="session")

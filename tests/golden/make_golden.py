@@ -21,6 +21,19 @@ reference's outputs for them.  Cases:
   stage2_*          multi-tile stage 2 on synthetic chroma pairs and random C
                     (ragged edge tiles, several L, weights and step sets)
   cfg0 (--big)      the 2000 x 2400, L=500 chroma workload
+  stage2_chroma400_L5_cap
+                    a 400 x 440 chroma pair at L=5 (100 x 110 tiles): the
+                    stitched path crosses more than 101 tiles, so the
+                    reference's 101-tile stitch cap (Parflex.py:1041-1049)
+                    truncates it
+  stage2_sync_random
+                    (--sync) sync_tile_overlap_edges (Parflex.py:919-969) on
+                    random D_arr / L_arr with infinite slots and near-equal
+                    corners, where every pass changes values (on real stage-2
+                    arrays the fill already leaves the overlaps consistent)
+  stage2_presync    (--presync) propagate_tile_edge_costs outputs before the
+                    overlap sync (Parflex.py:862-912), D_arr and L_arr, for
+                    every stage-2 case above
 """
 from __future__ import annotations
 
@@ -116,11 +129,141 @@ def _stage2_case(P, C, L, steps, weights, beta):
     return out, chunks, n1, n2
 
 
+def _stage2_cases(steps, weights):
+    """Multi-tile stage 2 on synthetic chroma pairs and random matrices.
+    Chroma cases store the features; C is rebuilt with cost_matrix_det,
+    whose summation order is fixed, so the fixture stays small."""
+    from parflexdtw_amd.synth import make_pair
+    cases = []
+    pairs = {300: make_pair(300, 360, seed=101), 420: make_pair(420, 380, seed=102),
+             250: make_pair(250, 300, seed=103), 600: make_pair(600, 700, seed=104)}
+    cases.append(("stage2_chroma300_L50", pairs[300], 50, steps, weights, 0.1))
+    cases.append(("stage2_chroma300_L64", pairs[300], 64, steps, [1.25, 3, 3], 0.1))
+    cases.append(("stage2_chroma420_L37", pairs[420], 37, steps, weights, 0.1))
+    cases.append(("stage2_chroma250_L400", pairs[250], 400, steps, weights, 0.1))
+    rng = np.random.RandomState(7)
+    cases.append(("stage2_rand_L9", rng.rand(61, 53), 9, steps, weights, 0.1))
+    cases.append(("stage2_rand_L5", rng.rand(20, 23), 5, steps, weights, 0.3))
+    cases.append(("stage2_rand_4step", rng.rand(90, 110), 25,
+                  [(1, 1), (1, 2), (2, 1), (2, 2)], [2, 3, 3, 4], 0.1))
+    cases.append(("stage2_nocand", rng.rand(30, 30), 8, steps, weights, 1.0))
+    cases.append(("stage2_chroma600_L100", pairs[600], 100, steps, weights, 0.1))
+    # 100 x 110 tiles of 5 x 5: the path crosses more than 101 tiles
+    cases.append(("stage2_chroma400_L5_cap", make_pair(400, 440, seed=105), 5, steps, weights,
+                  0.1))
+    return cases
+
+
+def _case_C(src):
+    from parflexdtw_amd.synth import cost_matrix_det
+    if isinstance(src, tuple):
+        F1, F2, _ = src
+        return np.ascontiguousarray(cost_matrix_det(F1, F2)), F1, F2
+    return np.ascontiguousarray(src), None, None
+
+
+def _write_cases(P, cases):
+    for name, src, L, st, wt, beta in cases:
+        t0 = time.time()
+        C, F1, F2 = _case_C(src)
+        o, _, _, _ = _stage2_case(P, C, L, st, wt, beta)
+        if F1 is not None:
+            del o["C"]
+            o["F1"] = F1
+            o["F2"] = F2
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **o)
+        print(f"{name} written ({time.time() - t0:.1f}s)", flush=True)
+
+
+def _write_presync(P, only):
+    """D_arr / L_arr of propagate_tile_edge_costs (before the overlap sync)
+    for every stage-2 fixture, from the inputs stored in the fixture."""
+    names = ["tiles_seed2", "parflex_seed3"] + [c[0] for c in _stage2_cases([(1, 1)], [1])]
+    if only is not None:
+        names = [n for n in names if n in only]
+    out = {}
+    for name in names:
+        with np.load(os.path.join(HERE, name + ".npz"), allow_pickle=False) as z:
+            g = {k: z[k] for k in z.files}
+        if "C" in g:
+            C = g["C"]
+        else:
+            from parflexdtw_amd.synth import cost_matrix_det
+            C = cost_matrix_det(g["F1"], g["F2"])
+        L = int(g["L"])
+        chunks, L_out, n1, n2 = P.run_flexdtw_on_tiles(C, L=L, steps=g["steps"],
+                                                       weights=g["weights"], buffer=1,
+                                                       profile_dir=None)
+        D_arr, L_arr, edge_lens = P.propagate_tile_edge_costs(chunks, L_out, n1, n2,
+                                                              buffer_param=1, profile_dir=None)
+        out[f"{name}__D_arr"] = D_arr.copy()
+        out[f"{name}__L_arr"] = L_arr.copy()
+        out[f"{name}__edge_lens"] = np.asarray(edge_lens, dtype=np.int64)
+        # the sync of the same arrays must give the stored stage-2 result
+        D2, L2s = P.sync_tile_overlap_edges(D_arr.copy(), L_arr.copy(), edge_lens, n1, n2)
+        if "s2_D_arr" in g:
+            assert np.array_equal(D2, g["s2_D_arr"]) and np.array_equal(L2s, g["s2_L_arr"])
+        print(f"presync {name} done", flush=True)
+    np.savez_compressed(os.path.join(HERE, "stage2_presync.npz"), **out)
+    print("stage2_presync written", flush=True)
+
+
+def _write_sync_random(P):
+    """Random inputs for the overlap sync: finite values, +inf slots with
+    probability 0.4, and corner pairs that differ by 0, 5e-11 or 2e-10 (the
+    1e-10 threshold of Parflex.py:965 on both sides)."""
+    out = {}
+    specs = [(50, 61, 9, 0), (23, 40, 6, 1), (100, 80, 17, 2), (9, 9, 5, 3)]
+    for k, (L1, L2, L, seed) in enumerate(specs):
+        rng = np.random.RandomState(500 + seed)
+        hop = L - 1
+        n1, n2 = -(-(L1 - 1) // hop), -(-(L2 - 1) // hop)
+        D = np.full((n1, n2, 2, L), np.inf)
+        La = np.full((n1, n2, 2, L), np.inf)
+        el = np.zeros((n1, n2, 2), np.int64)
+        for i in range(n1):
+            for j in range(n2):
+                rows = min(i * hop + L, L1) - i * hop
+                cols = min(j * hop + L, L2) - j * hop
+                el[i, j] = (cols, rows)
+                for e, n in ((0, cols), (1, rows)):
+                    v = rng.rand(n) * 10.0
+                    ln = rng.randint(1, 200, size=n).astype(np.float64)
+                    dead = rng.rand(n) < 0.4
+                    v[dead] = np.inf
+                    ln[dead] = np.inf
+                    D[i, j, e, :n] = v
+                    La[i, j, e, :n] = ln
+                if rng.rand() < 0.5 and np.isfinite(D[i, j, 0, cols - 1]):
+                    delta = [0.0, 5e-11, -5e-11, 2e-10, -2e-10][rng.randint(5)]
+                    D[i, j, 1, rows - 1] = D[i, j, 0, cols - 1] + delta
+        out[f"c{k}_D_in"] = D.copy()
+        out[f"c{k}_L_in"] = La.copy()
+        out[f"c{k}_edge_lens"] = el
+        out[f"c{k}_shape"] = np.array([L1, L2, L, n1, n2], np.int64)
+        Do, Lo = P.sync_tile_overlap_edges(D, La, el, n1, n2)
+        out[f"c{k}_D_out"] = np.asarray(Do).copy()
+        out[f"c{k}_L_out"] = np.asarray(Lo).copy()
+        changed = int(np.sum(~((out[f"c{k}_D_in"] == out[f"c{k}_D_out"]) |
+                               (np.isinf(out[f"c{k}_D_in"]) & np.isinf(out[f"c{k}_D_out"])))))
+        print(f"sync case {k}: {changed} slots changed", flush=True)
+    out["ncases"] = np.int64(len(specs))
+    np.savez_compressed(os.path.join(HERE, "stage2_sync_random.npz"), **out)
+    print("stage2_sync_random written", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
     ap.add_argument("--big", action="store_true", help="also write the 2000x2400 workload")
+    ap.add_argument("--only", default=None,
+                    help="comma-separated fixture names: write only these")
+    ap.add_argument("--sync", action="store_true",
+                    help="write stage2_sync_random.npz (sync on random arrays)")
+    ap.add_argument("--presync", action="store_true",
+                    help="write stage2_presync.npz from the inputs of the stage-2 fixtures")
     args = ap.parse_args()
+    only = None if args.only is None else set(args.only.split(","))
 
     _install_numba_standin()
     sys.path.insert(0, args.reference)
@@ -131,6 +274,16 @@ def main():
 
     steps = [(1, 1), (1, 2), (2, 1)]
     weights = [2, 3, 3]
+
+    if args.presync:
+        _write_presync(P, only)
+        return
+    if args.sync:
+        _write_sync_random(P)
+        return
+    if only is not None:
+        _write_cases(P, [c for c in _stage2_cases(steps, weights) if c[0] in only])
+        return
 
     # FlexDTW.flexdtw single tiles (tests/test_gpu_flexdtw.py:20-39).
     rng = np.random.RandomState(1)
@@ -196,38 +349,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "parflex_seed3.npz"), **o)
     print("parflex_seed3 written", flush=True)
 
-    # Multi-tile stage 2 on synthetic chroma pairs and random matrices.
-    # Chroma cases store the features; C is rebuilt with cost_matrix_det,
-    # whose summation order is fixed, so the fixture stays small.
-    cases = []
-    pairs = {300: make_pair(300, 360, seed=101), 420: make_pair(420, 380, seed=102),
-             250: make_pair(250, 300, seed=103), 600: make_pair(600, 700, seed=104)}
-    cases.append(("stage2_chroma300_L50", pairs[300], 50, steps, weights, 0.1))
-    cases.append(("stage2_chroma300_L64", pairs[300], 64, steps, [1.25, 3, 3], 0.1))
-    cases.append(("stage2_chroma420_L37", pairs[420], 37, steps, weights, 0.1))
-    cases.append(("stage2_chroma250_L400", pairs[250], 400, steps, weights, 0.1))
-    rng = np.random.RandomState(7)
-    cases.append(("stage2_rand_L9", rng.rand(61, 53), 9, steps, weights, 0.1))
-    cases.append(("stage2_rand_L5", rng.rand(20, 23), 5, steps, weights, 0.3))
-    cases.append(("stage2_rand_4step", rng.rand(90, 110), 25,
-                  [(1, 1), (1, 2), (2, 1), (2, 2)], [2, 3, 3, 4], 0.1))
-    cases.append(("stage2_nocand", rng.rand(30, 30), 8, steps, weights, 1.0))
-    cases.append(("stage2_chroma600_L100", pairs[600], 100, steps, weights, 0.1))
-    for name, src, L, st, wt, beta in cases:
-        t0 = time.time()
-        if isinstance(src, tuple):
-            F1, F2, _ = src
-            C = cost_matrix_det(F1, F2)
-        else:
-            F1 = F2 = None
-            C = src
-        o, _, _, _ = _stage2_case(P, np.ascontiguousarray(C), L, st, wt, beta)
-        if F1 is not None:
-            del o["C"]
-            o["F1"] = F1
-            o["F2"] = F2
-        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **o)
-        print(f"{name} written ({time.time() - t0:.1f}s)", flush=True)
+    _write_cases(P, _stage2_cases(steps, weights))
 
     if args.big:
         F1, F2, _ = make_pair(2000, 2400, seed=0)

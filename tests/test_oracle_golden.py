@@ -130,6 +130,44 @@ def test_stage2_matches_reference(oracle_mod, name):
     assert np.array_equal(wp, g["parflex_wp"])
 
 
+@pytest.mark.parametrize("name", STAGE2_CASES)
+def test_propagate_matches_reference_before_sync(oracle_mod, name):
+    """propagate_tile_edge_costs (Parflex.py:862-912): D_arr and L_arr before
+    the overlap sync, and edge_lens."""
+    g = load_golden(name)
+    pre = load_golden("stage2_presync")
+    C, L = g["C"], int(g["L"])
+    s1 = oracle_mod.run_stage1(C, L, g["steps"], g["weights"])
+    D, La, el = oracle_mod.propagate(C, s1)
+    assert _same_f64(D, pre[f"{name}__D_arr"])
+    assert _same_f64(La, pre[f"{name}__L_arr"])
+    assert np.array_equal(el, pre[f"{name}__edge_lens"])
+
+
+def test_sync_matches_reference_on_random_arrays(oracle_mod):
+    """sync_tile_overlap_edges (Parflex.py:919-969) where every pass acts:
+    infinite slots on either side of an overlap, corner pairs 0, 5e-11 and
+    2e-10 apart."""
+    g = load_golden("stage2_sync_random")
+    for k in range(int(g["ncases"])):
+        D, La = oracle_mod.sync_overlap(g[f"c{k}_D_in"], g[f"c{k}_L_in"], g[f"c{k}_edge_lens"])
+        assert _same_f64(D, g[f"c{k}_D_out"]), k
+        assert _same_f64(La, g[f"c{k}_L_out"]), k
+        assert not _same_f64(g[f"c{k}_D_in"], g[f"c{k}_D_out"])
+
+
+def test_stitch_cap_case_is_truncated():
+    """The 400 x 440, L = 5 fixture reaches the 101-tile stitch cap
+    (Parflex.py:1041-1049): its stitched path does not start on row 0 or
+    column 0, and it spans at least 101 tiles."""
+    g = load_golden("stage2_chroma400_L5_cap")
+    wp = g["s2_stitched_wp"]
+    assert wp[0, 0] > 0 and wp[0, 1] > 0
+    hop = int(g["L"]) - 1
+    tiles = {(int(r) // hop, int(c) // hop) for r, c in wp}
+    assert len(tiles) >= 101
+
+
 def test_parflex_threads_do_not_change_result(oracle_mod):
     g = load_golden("stage2_chroma600_L100")
     a = oracle_mod.parflex(g["C"], g["steps"], g["weights"], 0.1, 100, nthreads=1)
