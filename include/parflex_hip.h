@@ -96,6 +96,20 @@ int pf_stage1_features(const void *d_F1b, const void *d_F2b, int Kp, const int64
                        void *d_xch, uint32_t *d_Bpacked, double *d_edge_D, int32_t *d_edge_P,
                        double *d_edge_C, void *stream);
 
+/* Bounded waits of the stage-1 kernels.  A workgroup's waves, and the strips
+ * of a tile, wait for one another (a wave for row r-1 of the wave on its
+ * left, a strip for the rows the strip on its left has published); each wait
+ * polls at most `polls` times (default 4194304, seconds of waiting).  A wave
+ * whose wait gives up sets a device error word and finishes its rows without
+ * waiting again, so every kernel ends.  pf_set_wait_limit sets the limit for
+ * later launches (process-wide) and returns the previous one in *previous
+ * (may be null).  No reference counterpart: the reference's kernel has no
+ * inter-workgroup waits. */
+int pf_set_wait_limit(int64_t polls, int64_t *previous);
+/* Synchronises `stream`, then returns PF_EHIP (and clears the word) if a
+ * stage-1 wait on the current device gave up since the last call, else 0. */
+int pf_device_status(void *stream);
+
 /* L2-normalised features as frames x Kp bf16 (Kp = K rounded up to 16, zero
  * padded) from F (K x n f64, FlexDTW.py:1207-1210 normalisation). */
 int pf_features_bf16(const double *d_F, int64_t K, int64_t n, void *d_out, void *stream);
@@ -170,6 +184,38 @@ int pf_stage2(const double *d_edge_C, const int64_t *d_pairs, int64_t npairs,
               double *d_cand_L, double *d_seg_f, int64_t *d_seg_i, double *d_best_f,
               int64_t *d_best_i, int64_t *d_chain, int64_t *d_pts, int64_t *d_path,
               int64_t *d_path_len, void *stream);
+
+/* The phases of pf_stage2 on their own, for callers that drive stage 2 step
+ * by step as the reference's Python functions do.  `phases` is a bit set,
+ * run in this order:
+ *   PF_S2_PROPAGATE  edge data, start bitmaps, valid mask, init and fill:
+ *                    D_arr, L_arr before the overlap sync
+ *                    (propagate_tile_edge_costs, Parflex_gpu.py:247-339,
+ *                    Parflex.py:862-912);
+ *   PF_S2_SYNC       the three overlap passes on D_arr, L_arr in place
+ *                    (sync_tile_overlap_edges, Parflex.py:919-969); the edge
+ *                    lengths are the tile table's W (edge 0) and R (edge 1),
+ *                    each 0 or at least 2;
+ *   PF_S2_SCAN       candidate scan, segment and global bests
+ *                    (stage2_scan_and_stitch, Parflex.py:1229-1290, with
+ *                    _nb_scan_edges, Parflex.py:1153-1222);
+ *   PF_S2_STITCH     backtrace_and_stitch of each pair's best endpoint
+ *                    (Parflex.py:1032-1103, 1318-1324).
+ * Arguments as for pf_stage2; buffers that no requested phase uses may be
+ * null.  pf_stage2 is all four phases. */
+#define PF_S2_PROPAGATE 1
+#define PF_S2_SYNC 2
+#define PF_S2_SCAN 4
+#define PF_S2_STITCH 8
+int pf_stage2_phases(int phases, const double *d_edge_C, const int64_t *d_pairs, int64_t npairs,
+                     const int64_t *d_tiles, int64_t ntiles, int64_t L, int64_t max_diag,
+                     int64_t max_tile_diag, int64_t max_tiles_pair, int64_t max_segs,
+                     const int32_t *steps, int K, const uint32_t *d_Bpacked,
+                     const double *d_edge_D, const int32_t *d_edge_P, double *d_D_arr,
+                     double *d_L_arr, double *d_olap, uint32_t *d_bits, double *d_cand_D,
+                     double *d_cand_L, double *d_seg_f, int64_t *d_seg_i, double *d_best_f,
+                     int64_t *d_best_i, int64_t *d_chain, int64_t *d_pts, int64_t *d_path,
+                     int64_t *d_path_len, void *stream);
 
 /* backtrace_and_stitch (Parflex.py:1032-1103) for n requests, each
  * int64[5] = (pair, ci, cj, et, idx) in d_req; stage-1 outputs as for

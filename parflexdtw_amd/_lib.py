@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libparflex_hip.so")
 
 PF_F64, PF_F32, PF_BF16 = 0, 1, 2
+PF_S2_PROPAGATE, PF_S2_SYNC, PF_S2_SCAN, PF_S2_STITCH = 1, 2, 4, 8
+PF_S2_ALL = PF_S2_PROPAGATE | PF_S2_SYNC | PF_S2_SCAN | PF_S2_STITCH
 PF_EINVAL, PF_EHIP, PF_ENOENDPOINT = -1, -2, -3
 
 _lock = threading.Lock()
@@ -36,6 +38,8 @@ _SIGNATURES = {
     "pf_stage1_workspace": (_int, [_int, _i64, _i64, _int, _vp, _vp]),
     "pf_stage1_features": (_int, [_vp, _vp, _int, _vp, _i64, _i64, _vp, _vp, _int, _int, _int,
                                   _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pf_set_wait_limit": (_int, [_i64, _vp]),
+    "pf_device_status": (_int, [_vp]),
     "pf_features_bf16": (_int, [_vp, _i64, _i64, _vp, _vp]),
     "pf_tile_full": (_int, [_vp, _i64, _i64, _i64, _i64, _i64, _vp, _vp, _int, _vp, _vp, _vp,
                             _vp]),
@@ -44,6 +48,9 @@ _SIGNATURES = {
     "pf_stage2": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp,
                          _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                          _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pf_stage2_phases": (_int, [_int, _vp, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _i64, _i64,
+                                _vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "pf_stitch": (_int, [_vp, _i64, _vp, _vp, _i64, _vp, _int, _vp, _vp, _vp, _vp, _i64, _vp,
                          _vp, _vp]),
     "pf_cost": (_int, [_int, _vp, _vp, _i64, _i64, _i64, _vp, _i64, _vp, _vp]),

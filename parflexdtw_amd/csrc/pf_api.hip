@@ -40,7 +40,7 @@ int make_stepset(const int32_t *steps, const double *weights, int K, StepSet *ou
 
 extern "C" const char *pf_last_error(void) { return pf::g_last_error.c_str(); }
 
-extern "C" int pf_version(void) { return 100; }
+extern "C" int pf_version(void) { return 101; }
 
 extern "C" int pf_check_steps(const int32_t *steps, const double *weights, int K) {
   pf::StepSet ss;

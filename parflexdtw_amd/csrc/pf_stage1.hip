@@ -105,6 +105,15 @@ struct Xch {
 static_assert(sizeof(Xch<double>) == 32 && sizeof(Xch<float>) == 16,
               "pf_stage1_workspace reports these entry sizes");
 
+// Set by a stage-1 wave whose wait for a neighbour gave up (see k_stage1);
+// read and cleared by pf_device_status.
+__device__ int g_s1_wait_error;
+// Polls a stage-1 wait makes before it gives up (pf_set_wait_limit), read
+// only when a wait starts: about 4 M polls of 0.1-1 us, seconds, while a
+// wait in a correct run lasts at most the few ms a neighbour needs for its
+// next rows.
+__device__ int g_s1_wait_limit = 1 << 22;
+
 constexpr int kMaxG = 64;   // rows per exchange block, upper bound
 // Rows of C prefetched ahead, and rows per unrolled step of the sweep: 8
 // for the packed DP with the standard step set in FP32 (up to 4 columns per
@@ -434,10 +443,34 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
   int next_pub = G - 1;
   while (next_pub < 1) next_pub += G;
   // Rows 0 .. G-1 of the strip on the left feed rows 1 .. G.
+  // Every wait polls at most g_s1_wait_limit times.  A wave whose wait gives up
+  // records the error (g_s1_wait_error, read by pf_device_status) and waits
+  // no more: it finishes its rows on whatever values it has, so every wave
+  // still exits (neighbours waiting on it see its progress, or give up too).
+  // A wave that gave up has sync_ok = right_ok = kNever (a wait is entered
+  // only at a row past them, so neither is kNever while a wait runs).
+  auto wait_limit = []() -> int {
+    return __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(&g_s1_wait_limit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  };
+  auto give_up = [&]() {
+    sync_ok = kNever;
+    right_ok = kNever;
+    if (lane == 0) __hip_atomic_fetch_or(&g_s1_wait_error, 1, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+  };
   if (consumer && wave == 0 && R > 1) {
     const int need = (G < R) ? G : R;
-    while (__hip_atomic_load(fin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need)
+    for (int n = wait_limit();
+         __builtin_amdgcn_readfirstlane(
+             __hip_atomic_load(fin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) < need;
+         --n) {
+      if (n <= 0) {
+        give_up();
+        break;
+      }
       __builtin_amdgcn_s_sleep(2);
+    }
     for (int q0 = lane; q0 < G; q0 += 64) {
       if (q0 < R) {
         const Xch<DT> v = xin[q0];
@@ -458,20 +491,33 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
       if (r > sync_ok) {
         if (wave > 0) {
           // wait until the wave on the left has handed over row r-1
-          do {
+          for (int n = wait_limit();; --n) {
+            if (n <= 0) {
+              give_up();
+              break;
+            }
             seen_left = __builtin_amdgcn_readfirstlane(__hip_atomic_load(
                 &prog[wave - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
-            if (seen_left < r - 1) __builtin_amdgcn_s_sleep(1);
-          } while (seen_left < r - 1);
-          sync_ok = seen_left + 1;
+            if (seen_left >= r - 1) break;
+            __builtin_amdgcn_s_sleep(1);
+          }
+          if (sync_ok != kNever) sync_ok = seen_left + 1;
         } else {
           // first row of an exchange block: wait for the strip on the left
           // and stage its rows r-1 .. r+G-2 (row r'-1 of every row r' of the
           // block)
           const int last = r + G - 1;
           const int need = (last < R) ? last : R;
-          while (__hip_atomic_load(fin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need)
+          for (int n = wait_limit();
+               __builtin_amdgcn_readfirstlane(
+                   __hip_atomic_load(fin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) < need;
+               --n) {
+            if (n <= 0) {
+              give_up();
+              break;
+            }
             __builtin_amdgcn_s_sleep(2);
+          }
           for (int q0 = lane; q0 < G; q0 += 64) {
             const int q = r - 1 + q0;
             if (q < R) {
@@ -483,7 +529,7 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
           }
           // lane 0 reads the staged rows from this row on
           __threadfence_block();
-          sync_ok = (r + G < R) ? r + G - 1 : kNever;
+          sync_ok = (r + G < R && sync_ok != kNever) ? r + G - 1 : kNever;
         }
       }
       // The source is the same for the whole wave (only lane 0's copy is
@@ -743,12 +789,17 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
       }
       // Hand row r to the wave on the right once it has read row r - kRing.
       if (r > right_ok) {
-        do {
+        for (int n = wait_limit();; --n) {
+          if (n <= 0) {
+            give_up();
+            break;
+          }
           seen_right = __builtin_amdgcn_readfirstlane(__hip_atomic_load(
               &prog[wave + 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
-          if (seen_right < r - kRing + 1) __builtin_amdgcn_s_sleep(1);
-        } while (seen_right < r - kRing + 1);
-        right_ok = seen_right + kRing - 1;
+          if (seen_right >= r - kRing + 1) break;
+          __builtin_amdgcn_s_sleep(1);
+        }
+        if (right_ok != kNever) right_ok = seen_right + kRing - 1;
       }
       if (lane == 63) {
         const int s0 = slot * 16 + wave;
@@ -1006,6 +1057,37 @@ extern "C" int pf_stage1_features(const void *d_F1b, const void *d_F2b, int Kp,
   if (std3) return cpl == 2 ? PF_FEAT(true, 2) : PF_FEAT(true, 4);
   return cpl == 2 ? PF_FEAT(false, 2) : PF_FEAT(false, 4);
 #undef PF_FEAT
+}
+
+extern "C" int pf_set_wait_limit(int64_t polls, int64_t *previous) {
+  PF_REQUIRE(polls >= 0 && polls <= 0x7fffffff, "pf_set_wait_limit: limit outside [0, 2^31)");
+  int prev = 0;
+  PF_CHECK_HIP(hipMemcpyFromSymbol(&prev, HIP_SYMBOL(g_s1_wait_limit), sizeof(int), 0,
+                                   hipMemcpyDeviceToHost));
+  const int v = (int)polls;
+  PF_CHECK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_s1_wait_limit), &v, sizeof(int), 0,
+                                 hipMemcpyHostToDevice));
+  PF_CHECK_HIP(hipDeviceSynchronize());
+  if (previous) *previous = prev;
+  return PF_OK;
+}
+
+extern "C" int pf_device_status(void *stream) {
+  hipStream_t st = as_stream(stream);
+  int word = 0;
+  PF_CHECK_HIP(hipMemcpyFromSymbolAsync(&word, HIP_SYMBOL(g_s1_wait_error), sizeof(int), 0,
+                                        hipMemcpyDeviceToHost, st));
+  PF_CHECK_HIP(hipStreamSynchronize(st));
+  if (word != 0) {
+    const int zero = 0;
+    PF_CHECK_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_s1_wait_error), &zero, sizeof(int), 0,
+                                        hipMemcpyHostToDevice, st));
+    PF_CHECK_HIP(hipStreamSynchronize(st));
+    set_error("a stage-1 kernel gave up waiting for a neighbouring wave or strip "
+              "(spin limit reached); its outputs are not valid");
+    return PF_EHIP;
+  }
+  return PF_OK;
 }
 
 extern "C" int pf_stage1_workspace(int dtype, int64_t ntiles, int64_t Lmax, int strips,

@@ -262,8 +262,11 @@ __global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs,
   }
 }
 
-// sync_tile_overlap_edges (Parflex.py:919-969), pass `pass` (0, 1, 2).
-__global__ void k_s2_sync(int pass, const PairDesc *__restrict__ pairs, int64_t L,
+// sync_tile_overlap_edges (Parflex.py:919-969), pass `pass` (0, 1, 2).  The
+// edge lengths are the tile table's W (edge 0) and R (edge 1), the
+// reference's edge_lens; a zero length skips the tile as there.
+__global__ void k_s2_sync(int pass, const PairDesc *__restrict__ pairs,
+                          const TileDesc *__restrict__ tiles, int64_t L,
                           double *__restrict__ Darr, double *__restrict__ Larr) {
   const PairDesc pd = pairs[blockIdx.y];
   const int64_t n1 = pd.n1, n2 = pd.n2;
@@ -271,19 +274,20 @@ __global__ void k_s2_sync(int pass, const PairDesc *__restrict__ pairs, int64_t 
   if (q >= n1 * n2) return;
   const int64_t i = q / n2, j = q - i * n2;
   const int64_t t = pd.tile_base + q;
-  const Geo g = tile_geo(pd, L, i, j);
+  const int64_t cols = tiles[t].W, rows = tiles[t].R;
   int64_t a, b;  // slot a is the tile's own, b the other one
   if (pass == 0) {
-    if (j + 1 >= n2) return;
-    a = (t * 2 + 0) * L + g.cols - 1;
+    if (j + 1 >= n2 || cols == 0) return;
+    a = (t * 2 + 0) * L + cols - 1;
     b = ((t + 1) * 2 + 0) * L;
   } else if (pass == 1) {
-    if (i + 1 >= n1) return;
-    a = (t * 2 + 1) * L + g.rows - 1;
+    if (i + 1 >= n1 || rows == 0) return;
+    a = (t * 2 + 1) * L + rows - 1;
     b = ((t + n2) * 2 + 1) * L;
   } else {
-    a = (t * 2 + 0) * L + g.cols - 1;  // corner seen by the top edge
-    b = (t * 2 + 1) * L + g.rows - 1;  // corner seen by the right edge
+    if (cols == 0 || rows == 0) return;
+    a = (t * 2 + 0) * L + cols - 1;  // corner seen by the top edge
+    b = (t * 2 + 1) * L + rows - 1;  // corner seen by the right edge
   }
   const double da = Darr[a], db = Darr[b];
   const bool fa = isfinite(da), fb = isfinite(db);
@@ -717,6 +721,81 @@ static int stitch_common(hipStream_t st, int from_best, const int64_t *d_req, in
   return PF_OK;
 }
 
+extern "C" int pf_stage2_phases(int phases, const double *d_edge_C, const int64_t *d_pairs,
+                                int64_t npairs, const int64_t *d_tiles, int64_t ntiles, int64_t L,
+                                int64_t max_diag, int64_t max_tile_diag, int64_t max_tiles_pair,
+                                int64_t max_segs, const int32_t *steps, int K,
+                                const uint32_t *d_Bpacked, const double *d_edge_D,
+                                const int32_t *d_edge_P, double *d_D_arr, double *d_L_arr,
+                                double *d_olap, uint32_t *d_bits, double *d_cand_D,
+                                double *d_cand_L, double *d_seg_f, int64_t *d_seg_i,
+                                double *d_best_f, int64_t *d_best_i, int64_t *d_chain,
+                                int64_t *d_pts, int64_t *d_path, int64_t *d_path_len,
+                                void *stream) {
+  StepSet ss;
+  double w[4] = {1, 1, 1, 1};
+  PF_REQUIRE(phases > 0 && (phases & ~(PF_S2_PROPAGATE | PF_S2_SYNC | PF_S2_SCAN |
+                                       PF_S2_STITCH)) == 0,
+             "pf_stage2_phases: unknown phase bits");
+  if (phases & PF_S2_STITCH) {
+    int rc = make_stepset(steps, w, K, &ss);
+    if (rc) return rc;
+  }
+  PF_REQUIRE(npairs > 0 && ntiles > 0 && L >= 2, "pf_stage2: empty batch or bad tile length");
+  PF_REQUIRE(max_diag >= 1 && max_tile_diag >= 1 && max_tiles_pair >= 1 && max_segs >= 1,
+             "pf_stage2: bad batch dimensions");
+  PF_REQUIRE(L <= 8192, "pf_stage2: tile length above 8192 is not supported");
+  PF_REQUIRE(d_pairs && d_tiles && d_D_arr && d_L_arr, "pf_stage2: null buffer");
+  PF_REQUIRE(!(phases & PF_S2_PROPAGATE) || (d_edge_C && d_edge_D && d_edge_P && d_olap && d_bits),
+             "pf_stage2: null buffer (propagation)");
+  PF_REQUIRE(!(phases & PF_S2_SCAN) ||
+                 (d_cand_D && d_cand_L && d_seg_f && d_seg_i && d_best_f && d_best_i),
+             "pf_stage2: null buffer (scan)");
+  PF_REQUIRE(!(phases & PF_S2_STITCH) ||
+                 (d_Bpacked && d_edge_P && d_best_i && d_chain && d_pts && d_path && d_path_len),
+             "pf_stage2: null buffer (stitch)");
+  PF_REQUIRE(npairs <= 65535 && max_segs <= (1 << 20), "pf_stage2: batch too large");
+  PF_REQUIRE(ntiles < (1ll << 31), "pf_stage2: too many tiles");
+  hipStream_t st = as_stream(stream);
+  const PairDesc *pairs = reinterpret_cast<const PairDesc *>(d_pairs);
+  const TileDesc *tiles = reinterpret_cast<const TileDesc *>(d_tiles);
+  const int64_t bitw = (L + 32) / 32;
+  if (phases & PF_S2_PROPAGATE) {
+    hipLaunchKernelGGL(k_s2_prep, dim3((unsigned)ntiles), dim3(256), 0, st, tiles, L, d_edge_P,
+                       d_edge_C, d_D_arr, d_L_arr, d_olap, d_bits, bitw);
+    PF_CHECK_HIP(hipGetLastError());
+    for (int64_t d = 0; d < max_diag; ++d) {
+      const int64_t ntd = (d + 1) < max_tile_diag ? (d + 1) : max_tile_diag;
+      const int64_t nth = ntd * 2 * L;
+      PF_REQUIRE((nth + 255) / 256 <= (1ll << 31) - 1, "pf_stage2: diagonal too large");
+      hipLaunchKernelGGL(k_s2_diag, dim3((unsigned)((nth + 255) / 256), (unsigned)npairs),
+                         dim3(256), 0, st, d, pairs, tiles, L, d_edge_D, d_edge_P, d_olap, d_bits,
+                         bitw, d_D_arr, d_L_arr);
+      PF_CHECK_HIP(hipGetLastError());
+    }
+  }
+  if (phases & PF_S2_SYNC) {
+    for (int pass = 0; pass < 3; ++pass) {
+      hipLaunchKernelGGL(k_s2_sync,
+                         dim3((unsigned)((max_tiles_pair + 255) / 256), (unsigned)npairs),
+                         dim3(256), 0, st, pass, pairs, tiles, L, d_D_arr, d_L_arr);
+      PF_CHECK_HIP(hipGetLastError());
+    }
+  }
+  if (phases & PF_S2_SCAN) {
+    hipLaunchKernelGGL(k_s2_scan, dim3((unsigned)max_segs, 2, (unsigned)npairs), dim3(256), 0,
+                       st, pairs, L, d_D_arr, d_L_arr, d_cand_D, d_cand_L, d_seg_f, d_seg_i);
+    PF_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_s2_best, dim3((unsigned)npairs), dim3(256), 0, st, pairs, d_seg_f,
+                       d_seg_i, d_best_f, d_best_i);
+    PF_CHECK_HIP(hipGetLastError());
+  }
+  if (phases & PF_S2_STITCH)
+    return stitch_common(st, 1, nullptr, npairs, d_best_i, pairs, tiles, L, ss, d_Bpacked,
+                         d_edge_P, d_chain, d_pts, 1, 0, d_path, d_path_len);
+  return PF_OK;
+}
+
 extern "C" int pf_stage2(const double *d_edge_C, const int64_t *d_pairs, int64_t npairs,
                          const int64_t *d_tiles, int64_t ntiles, int64_t L, int64_t max_diag,
                          int64_t max_tile_diag, int64_t max_tiles_pair, int64_t max_segs,
@@ -726,49 +805,11 @@ extern "C" int pf_stage2(const double *d_edge_C, const int64_t *d_pairs, int64_t
                          double *d_cand_L, double *d_seg_f, int64_t *d_seg_i, double *d_best_f,
                          int64_t *d_best_i, int64_t *d_chain, int64_t *d_pts, int64_t *d_path,
                          int64_t *d_path_len, void *stream) {
-  StepSet ss;
-  double w[4] = {1, 1, 1, 1};
-  int rc = make_stepset(steps, w, K, &ss);
-  if (rc) return rc;
-  PF_REQUIRE(npairs > 0 && ntiles > 0 && L >= 2, "pf_stage2: empty batch or bad tile length");
-  PF_REQUIRE(max_diag >= 1 && max_tile_diag >= 1 && max_tiles_pair >= 1 && max_segs >= 1,
-             "pf_stage2: bad batch dimensions");
-  PF_REQUIRE(L <= 8192, "pf_stage2: tile length above 8192 is not supported");
-  PF_REQUIRE(d_edge_C && d_pairs && d_tiles && d_Bpacked && d_edge_D && d_edge_P && d_D_arr &&
-                 d_L_arr && d_olap && d_bits && d_cand_D && d_cand_L && d_seg_f && d_seg_i &&
-                 d_best_f && d_best_i && d_chain && d_pts && d_path && d_path_len,
-             "pf_stage2: null buffer");
-  PF_REQUIRE(npairs <= 65535 && max_segs <= (1 << 20), "pf_stage2: batch too large");
-  hipStream_t st = as_stream(stream);
-  const PairDesc *pairs = reinterpret_cast<const PairDesc *>(d_pairs);
-  const TileDesc *tiles = reinterpret_cast<const TileDesc *>(d_tiles);
-  const int64_t bitw = (L + 32) / 32;
-  PF_REQUIRE(ntiles < (1ll << 31), "pf_stage2: too many tiles");
-  hipLaunchKernelGGL(k_s2_prep, dim3((unsigned)ntiles), dim3(256), 0, st, tiles, L, d_edge_P,
-                     d_edge_C, d_D_arr, d_L_arr, d_olap, d_bits, bitw);
-  PF_CHECK_HIP(hipGetLastError());
-  for (int64_t d = 0; d < max_diag; ++d) {
-    const int64_t ntd = (d + 1) < max_tile_diag ? (d + 1) : max_tile_diag;
-    const int64_t nth = ntd * 2 * L;
-    PF_REQUIRE((nth + 255) / 256 <= (1ll << 31) - 1, "pf_stage2: diagonal too large");
-    hipLaunchKernelGGL(k_s2_diag, dim3((unsigned)((nth + 255) / 256), (unsigned)npairs),
-                       dim3(256), 0, st, d, pairs, tiles, L, d_edge_D, d_edge_P, d_olap, d_bits, bitw,
-                       d_D_arr, d_L_arr);
-    PF_CHECK_HIP(hipGetLastError());
-  }
-  for (int pass = 0; pass < 3; ++pass) {
-    hipLaunchKernelGGL(k_s2_sync, dim3((unsigned)((max_tiles_pair + 255) / 256), (unsigned)npairs),
-                       dim3(256), 0, st, pass, pairs, L, d_D_arr, d_L_arr);
-    PF_CHECK_HIP(hipGetLastError());
-  }
-  hipLaunchKernelGGL(k_s2_scan, dim3((unsigned)max_segs, 2, (unsigned)npairs), dim3(256), 0, st,
-                     pairs, L, d_D_arr, d_L_arr, d_cand_D, d_cand_L, d_seg_f, d_seg_i);
-  PF_CHECK_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_s2_best, dim3((unsigned)npairs), dim3(256), 0, st, pairs, d_seg_f,
-                     d_seg_i, d_best_f, d_best_i);
-  PF_CHECK_HIP(hipGetLastError());
-  return stitch_common(st, 1, nullptr, npairs, d_best_i, pairs, tiles, L, ss, d_Bpacked,
-                       d_edge_P, d_chain, d_pts, 1, 0, d_path, d_path_len);
+  return pf_stage2_phases(PF_S2_PROPAGATE | PF_S2_SYNC | PF_S2_SCAN | PF_S2_STITCH, d_edge_C,
+                          d_pairs, npairs, d_tiles, ntiles, L, max_diag, max_tile_diag,
+                          max_tiles_pair, max_segs, steps, K, d_Bpacked, d_edge_D, d_edge_P,
+                          d_D_arr, d_L_arr, d_olap, d_bits, d_cand_D, d_cand_L, d_seg_f, d_seg_i,
+                          d_best_f, d_best_i, d_chain, d_pts, d_path, d_path_len, stream);
 }
 
 extern "C" int pf_stitch(const int64_t *d_req, int64_t nreq, const int64_t *d_pairs,

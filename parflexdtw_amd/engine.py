@@ -16,8 +16,9 @@ from . import _lib
 from ._lib import PF_BF16, PF_F32, PF_F64, check, lib
 from ._plan import MAX_STITCH_TILES, BatchPlan
 
-__all__ = ["StepWeights", "stage1", "stage2", "run_batch", "BatchResult", "dtype_code",
-           "NO_ENDPOINT_MSG", "BatchRunner", "autotune_stage1", "stage1_config", "tune_batch"]
+__all__ = ["StepWeights", "stage1", "stage2", "stage2_phases", "run_batch", "BatchResult",
+           "dtype_code", "NO_ENDPOINT_MSG", "BatchRunner", "autotune_stage1", "stage1_config",
+           "tune_batch"]
 
 NO_ENDPOINT_MSG = "Stage 2: No valid endpoint found on global top/right edges."
 
@@ -137,6 +138,13 @@ def stage1(C_dev, plan, sw, ws, stream=None, config=None, feat=None):
     check(rc, "pf_stage1")
 
 
+def device_status(stream=None):
+    """Synchronise the stream and raise ParflexError if a stage-1 wait gave
+    up since the last check (pf_device_status; the results are then not
+    valid)."""
+    check(lib().pf_device_status(_stream(stream)), "stage 1")
+
+
 def stage2(C_dev, plan, sw, ws, stream=None):
     """Propagation, sync, scan and stitch on the device (Parflex.py:862-1356)."""
     rc = lib().pf_stage2(
@@ -150,6 +158,23 @@ def stage2(C_dev, plan, sw, ws, stream=None):
         ws["chain"].data_ptr(), ws["pts"].data_ptr(), ws["path"].data_ptr(),
         ws["path_len"].data_ptr(), _stream(stream))
     check(rc, "pf_stage2")
+
+
+def stage2_phases(plan, sw, ws, phases, stream=None):
+    """Some phases of stage 2 (pf_stage2_phases): _lib.PF_S2_PROPAGATE, _SYNC,
+    _SCAN, _STITCH, run in that order; workspace entries that no requested
+    phase uses may be missing from ws."""
+    def ptr(k):
+        t = ws.get(k)
+        return None if t is None else t.data_ptr()
+    rc = lib().pf_stage2_phases(
+        int(phases), ptr("edge_C"), ws["pairs"].data_ptr(), plan.npairs, ws["tiles"].data_ptr(),
+        plan.ntiles, plan.L, plan.max_diag, plan.max_tile_diag, plan.max_tiles_pair,
+        plan.max_segs, sw.sp, sw.K, ptr("Bp"), ptr("edge_D"), ptr("edge_P"),
+        ws["D_arr"].data_ptr(), ws["L_arr"].data_ptr(), ptr("olap"), ptr("bits"), ptr("cand_D"),
+        ptr("cand_L"), ptr("seg_f"), ptr("seg_i"), ptr("best_f"), ptr("best_i"), ptr("chain"),
+        ptr("pts"), ptr("path"), ptr("path_len"), _stream(stream))
+    check(rc, "pf_stage2_phases")
 
 
 def stitch_requests(plan, sw, ws, requests, stream=None):
@@ -188,6 +213,8 @@ class BatchResult:
 def collect(plan, ws):
     """Copy only best costs, endpoints and stitched paths back to the host
     (ws entries may also be host tensors already holding them)."""
+    if ws["best_f"].is_cuda:
+        device_status()
     best_f = ws["best_f"].cpu().numpy()
     best_i = ws["best_i"].cpu().numpy().reshape(plan.npairs, 8)
     plen = ws["path_len"].cpu().numpy()
@@ -211,6 +238,7 @@ def collect_pinned(plan, ws, host):
             host[k] = torch.empty(ws[k].shape, dtype=ws[k].dtype, pin_memory=True)
         host[k].copy_(ws[k], non_blocking=True)
     torch.cuda.synchronize()
+    device_status()
     return collect(plan, host)
 
 
@@ -234,7 +262,7 @@ def collect_packed(plan, ws, host, stream=None):
             if k not in host or host[k].shape != ws[k].shape:
                 host[k] = torch.empty(ws[k].shape, dtype=ws[k].dtype, pin_memory=True)
             host[k].copy_(ws[k], non_blocking=True)
-    st.synchronize()
+    device_status(st)
     plen = host["path_len"].numpy().copy()
     total = 2 * int(plen.sum())
     buf = torch.empty(max(total, 1), dtype=torch.int64, pin_memory=True)

@@ -323,8 +323,13 @@ class LearnedRunner:
         """One complete pass.  With a dict for `timings`, the device is
         synchronised after every phase and the phase times (s) are added.
         With a pair of torch.cuda.Event for `events`, they are recorded on the
-        launch stream around the stage-1 kernel."""
+        launch stream around the stage-1 kernel.  With a stream other than
+        the current one, the whole pass (fills, uploads, fallback included)
+        runs with that stream made current."""
         torch = _torch()
+        if stream is not None and stream != torch.cuda.current_stream():
+            with torch.cuda.stream(stream):
+                return self.run(None, timings, events)
         st = _stream(stream)
         L = self.L
         clock = [time.perf_counter()]

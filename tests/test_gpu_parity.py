@@ -236,8 +236,17 @@ def test_stage1_packed_outputs_vs_oracle(gpu, oracle_mod, case):
             assert _same(codes.astype(np.int8), ob)
 
 
-FP32_CONFIGS = [(None, 200), ((1, 1, 16), 200), ((2, 1, 16), 200), ((2, 1, 7), 150),
-                ((1, 2, 32), 90)]
+FP32_CONFIGS = [  # (launch shape, L, C shape)
+    (None, 200, (333, 410)), ((1, 1, 16), 200, (333, 410)), ((2, 1, 16), 200, (333, 410)),
+    ((2, 1, 7), 150, (333, 410)), ((1, 2, 32), 90, (333, 410)),
+    # the learned form's launch shape (one strip of 16 waves, 2 columns per
+    # lane, 64-row exchange blocks) on 2048-column tiles of at most 150 rows
+    ((1, 16, 64), 2048, (150, 2100)),
+    # 4 columns per lane (one strip of 4 waves) and 8 (one strip of 2 waves)
+    ((1, 4, 16), 1000, (180, 1100)), ((1, 2, 16), 1000, (180, 1100)),
+    # two strips of 8 waves with exchange blocks of 64 rows
+    ((2, 8, 64), 2048, (190, 2100)),
+]
 
 
 @pytest.mark.parametrize("case", range(len(FP32_CONFIGS)))
@@ -247,13 +256,13 @@ def test_stage1_fp32_integer_costs_vs_oracle(gpu, oracle_mod, case, dtype):
     most 200 rows: every sum and cross product is exact in FP32 and the FP64
     quotients are distinct wherever they differ, so edges and every packed
     back-pointer must equal the FP64 oracle's (ragged last rows and columns,
-    one and two strips)."""
+    one and two strips, 2, 4 and 8 columns per lane, up to 16 waves)."""
     import torch
     from parflexdtw_amd import engine
     from parflexdtw_amd._plan import BatchPlan
-    cfg, L = FP32_CONFIGS[case]
+    cfg, L, shape = FP32_CONFIGS[case]
     rng = np.random.RandomState(77 + case)
-    C = rng.randint(0, 8, size=(333, 410)).astype(np.float64)
+    C = rng.randint(0, 8, size=shape).astype(np.float64)
     plan = BatchPlan([C.shape], L)
     sw = engine.StepWeights(STEPS, [2, 3, 3])
     C_dev = torch.from_numpy(C).to(getattr(torch, dtype)).cuda()
@@ -343,3 +352,26 @@ def test_cost_mfma_bf16_layout_and_error(gpu, K, N, M):
     ref64 = cost_matrix_det(F1, F2)
     assert np.max(np.abs(C16 - ref64)) < 1.6e-2
     assert np.max(np.abs(C32 - ref64)) < 1e-2
+
+
+@pytest.mark.parametrize("L", [340, 341, 20000])
+def test_one_tile_tile_length_at_least_matrix(gpu, oracle_mod, L):
+    """L >= max(L1, L2): one tile covers the matrix (Parflex.py:388-391),
+    also for L above the multi-tile limit; the result, and the stage-2 dict
+    with its (1, 1, 2, L) arrays, equal the oracle's at the caller's L."""
+    _, PG = gpu
+    from parflexdtw_amd.synth import cost_matrix_det, make_pair
+    F1, F2, _ = make_pair(300, 380, seed=17)
+    C = cost_matrix_det(F1, F2)
+    assert max(C.shape) <= L
+    bc, wp = PG.parflex(C, STEPS, [2, 3, 3], 0.1, L, profile_dir=None)
+    bo, wpo = oracle_mod.parflex(C, STEPS, [2, 3, 3], 0.1, L)
+    assert bc == bo and _same(wp, wpo)
+    tiled = {"C_dev": None, "C": C, "L_block": L,
+             "stage1_params": {"steps": STEPS, "weights": [2, 3, 3], "buffer": 1.0}}
+    r = PG.run_stage2_from_tiled(tiled, C, beta=0.1)
+    s1 = oracle_mod.run_stage1(C, L, STEPS, [2, 3, 3])
+    ro = oracle_mod.run_stage2(C, s1, 0.1)
+    assert r["D_arr"].shape == (1, 1, 2, L)
+    assert _same(r["D_arr"], ro["D_arr"]) and _same(r["L_arr"], ro["L_arr"])
+    assert _same(r["stitched_wp"], ro["stitched_wp"])

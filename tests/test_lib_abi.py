@@ -24,7 +24,7 @@ def test_library_exports_header_symbols():
     for n in names:
         assert hasattr(h, n), n
     assert set(names) == set(_lib.EXPORTS)
-    assert h.pf_version() == 100
+    assert h.pf_version() == 101
 
 
 def _check(steps, weights):
