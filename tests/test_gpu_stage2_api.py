@@ -52,14 +52,14 @@ def _check_dict(r, g):
     assert _same(np.array(segs, np.int64).reshape(-1, 8), g["s2_segments"])
 
 
-def _step_by_step(PG, chunks, C, L, n1, n2, g, pre, name):
+def _step_by_step(PG, chunks, C, L, n1, n2, g, pre, name, C_dev=None):
     """The reference's stage-2 sequence (Parflex.py:1520-1541) through the
     drop-in functions."""
     tiled = PG.build_tiled_metadata(chunks, L, n1, n2, C,
                                     stage1_params={"steps": g["steps"], "weights": g["weights"],
                                                    "buffer": 1.0})
     D_arr, L_arr, edge_lens = PG.propagate_tile_edge_costs(chunks, L, n1, n2, buffer_param=1,
-                                                           profile_dir=None)
+                                                           profile_dir=None, C_dev=C_dev)
     assert _same(D_arr, pre[f"{name}__D_arr"])
     assert _same(L_arr, pre[f"{name}__L_arr"])
     assert _same(edge_lens, pre[f"{name}__edge_lens"])
@@ -125,7 +125,7 @@ def test_stage2_step_by_step_on_reference_chunks(PG, presync):
                                                            C_dev=C_dev)
     assert _same(D_arr, presync["tiles_seed2__D_arr"])
     assert _same(L_arr, presync["tiles_seed2__L_arr"])
-    _step_by_step(PG, chunks, C, L, n1, n2, g, presync, "tiles_seed2")
+    _step_by_step(PG, chunks, C, L, n1, n2, g, presync, "tiles_seed2", C_dev=C_dev)
 
 
 @pytest.mark.parametrize("name", ["stage2_chroma300_L50", "stage2_chroma420_L37",
