@@ -80,6 +80,24 @@ int pf_stage1(int dtype, const void *d_C, const int64_t *d_tiles, int64_t ntiles
               void *d_xch, uint32_t *d_Bpacked, double *d_edge_D, int32_t *d_edge_P,
               double *d_edge_C, void *stream);
 
+/* pf_stage1 for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The strip-exchange counters and rows that
+ * pf_stage1 takes as d_flags / d_xch come from a scratch array the library
+ * holds in device memory for the whole process, PF_NOALLOC_SCRATCH_BYTES
+ * per GPU; this call allocates nothing (no hipMalloc, no library
+ * allocation).  Arguments and results as for pf_stage1; a launch whose
+ * exchange needs more than the scratch (align256(4*(ntiles*strips+1)) +
+ * pf_stage1_workspace's xch_bytes) is rejected with PF_EINVAL.  Every call
+ * on one GPU shares that GPU's scratch: calls that overlap in time on one
+ * GPU (other streams, other threads) race and must be serialised by the
+ * caller. */
+#define PF_NOALLOC_SCRATCH_BYTES (32u << 20)
+int pf_stage1_noalloc(int dtype, const void *d_C, const int64_t *d_tiles, int64_t ntiles,
+                      int64_t Lmax, const int32_t *steps, const double *weights, int K,
+                      int strips, int waves, int rows_per_block, uint32_t *d_Bpacked,
+                      double *d_edge_D, int32_t *d_edge_P, double *d_edge_C, void *stream);
+
 /* Stage 1 computing the cost inside the sweep from features (the fused
  * form of tiled_stage1_from_features, Parflex_gpu.py:610-653): C = 1 -
  * F1n^T F2n is never stored; every 32 rows each wave computes its block of C

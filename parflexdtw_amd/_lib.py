@@ -35,6 +35,8 @@ _SIGNATURES = {
     "pf_check_steps": (_int, [_vp, _vp, _int]),
     "pf_stage1": (_int, [_int, _vp, _vp, _i64, _i64, _vp, _vp, _int, _int, _int, _int, _vp, _vp,
                          _vp, _vp, _vp, _vp, _vp]),
+    "pf_stage1_noalloc": (_int, [_int, _vp, _vp, _i64, _i64, _vp, _vp, _int, _int, _int, _int,
+                                 _vp, _vp, _vp, _vp, _vp]),
     "pf_stage1_workspace": (_int, [_int, _i64, _i64, _int, _vp, _vp]),
     "pf_stage1_features": (_int, [_vp, _vp, _int, _vp, _i64, _i64, _vp, _vp, _int, _int, _int,
                                   _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

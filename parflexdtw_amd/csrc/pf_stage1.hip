@@ -44,6 +44,8 @@
 // every cell (the 6-tuple of gpu_flexdtw.flexdtw_chunk_from_global_C).
 #include "pf_common.h"
 
+#include <mutex>
+
 namespace pf {
 
 template <typename T>
@@ -1100,6 +1102,63 @@ extern "C" int pf_stage1_workspace(int dtype, int64_t ntiles, int64_t Lmax, int 
   *flags_count = strips > 1 ? ntiles * strips + 1 : 0;
   *xch_bytes = strips > 1 ? ntiles * strips * Lmax * esz : 0;
   return PF_OK;
+}
+
+// Scratch of pf_stage1_noalloc: the strip-exchange counters and rows of one
+// call, in device memory the library holds for the whole process, one copy
+// per GPU.  Host code never uses the name `scratch` as an address: the
+// device address of each GPU's copy comes from hipGetSymbolAddress
+// (scratch_base) and only that is passed to kernels.
+static __device__ __align__(256) unsigned char scratch[PF_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// Device address of the current GPU's scratch, looked up on the first call
+// on that GPU (with it current) and kept.
+static int scratch_base(void **out) {
+  int dev = 0;
+  PF_CHECK_HIP(hipGetDevice(&dev));
+  PF_REQUIRE(dev >= 0 && dev < kMaxDevices, "pf_stage1_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void *p = nullptr;
+    PF_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PF_REQUIRE(p != nullptr, "pf_stage1_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return PF_OK;
+}
+
+extern "C" int pf_stage1_noalloc(int dtype, const void *d_C, const int64_t *d_tiles,
+                                 int64_t ntiles, int64_t Lmax, const int32_t *steps,
+                                 const double *weights, int K, int strips, int waves,
+                                 int rows_per_block, uint32_t *d_Bpacked, double *d_edge_D,
+                                 int32_t *d_edge_P, double *d_edge_C, void *stream) {
+  int32_t *flags = nullptr;
+  void *xch = nullptr;
+  if (strips > 1 && ntiles > 0) {
+    int64_t nf = 0, nx = 0;
+    int rc = pf_stage1_workspace(dtype, ntiles, Lmax, strips, &nf, &nx);
+    if (rc) return rc;
+    const int64_t fbytes = (nf * (int64_t)sizeof(int32_t) + 255) / 256 * 256;
+    if (fbytes + nx > (int64_t)PF_NOALLOC_SCRATCH_BYTES) {
+      set_error("pf_stage1_noalloc: the strip exchange needs " + std::to_string(fbytes + nx) +
+                " bytes, above the " + std::to_string((int64_t)PF_NOALLOC_SCRATCH_BYTES) +
+                "-byte scratch (use fewer strips or tiles per call, or pf_stage1)");
+      return PF_EINVAL;
+    }
+    void *base = nullptr;
+    rc = scratch_base(&base);
+    if (rc) return rc;
+    flags = static_cast<int32_t *>(base);
+    xch = static_cast<unsigned char *>(base) + fbytes;
+  }
+  // The counters are cleared by pf_stage1's fill kernel (no memset).
+  return pf_stage1(dtype, d_C, d_tiles, ntiles, Lmax, steps, weights, K, strips, waves,
+                   rows_per_block, flags, xch, d_Bpacked, d_edge_D, d_edge_P, d_edge_C, stream);
 }
 
 extern "C" int pf_tile_full(const double *d_C, int64_t ldc, int64_t r0, int64_t r1, int64_t c0,
