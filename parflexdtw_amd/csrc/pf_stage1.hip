@@ -110,11 +110,10 @@ static_assert(sizeof(Xch<double>) == 32 && sizeof(Xch<float>) == 16,
 // Set by a stage-1 wave whose wait for a neighbour gave up (see k_stage1);
 // read and cleared by pf_device_status.
 __device__ int g_s1_wait_error;
-// Polls a stage-1 wait makes before it gives up (pf_set_wait_limit), read
-// only when a wait starts: about 4 M polls of 0.1-1 us, seconds, while a
-// wait in a correct run lasts at most the few ms a neighbour needs for its
-// next rows.
-__device__ int g_s1_wait_limit = 1 << 22;
+// Polls a stage-1 wait makes before it gives up (pf_set_wait_limit; passed
+// to every launch): about 4 M polls of 0.1-1 us, seconds, while a wait in a
+// correct run lasts at most the few ms a neighbour needs for its next rows.
+static int g_wait_limit = 1 << 22;
 
 constexpr int kMaxG = 64;   // rows per exchange block, upper bound
 // Rows of C prefetched ahead, and rows per unrolled step of the sweep: 8
@@ -218,7 +217,7 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
     uint32_t *__restrict__ Bp, double *__restrict__ eD, int32_t *__restrict__ eP,
     double *__restrict__ eC, double *__restrict__ Df, int32_t *__restrict__ Pf,
     int8_t *__restrict__ Bf, const __hip_bfloat16 *__restrict__ Fa,
-    const __hip_bfloat16 *__restrict__ Fb, int Kp) {
+    const __hip_bfloat16 *__restrict__ Fb, int Kp, int spin_limit) {
   static_assert(CPL >= 2, "a lane needs two columns of left context");
   static_assert(!FEAT || (!FULL && sizeof(DT) == 4), "features form: packed FP32 DP only");
   extern __shared__ __attribute__((aligned(16))) __hip_bfloat16 cdyn[];
@@ -445,25 +444,20 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
   int next_pub = G - 1;
   while (next_pub < 1) next_pub += G;
   // Rows 0 .. G-1 of the strip on the left feed rows 1 .. G.
-  // Every wait polls at most g_s1_wait_limit times.  A wave whose wait gives up
-  // records the error (g_s1_wait_error, read by pf_device_status) and waits
-  // no more: it finishes its rows on whatever values it has, so every wave
-  // still exits (neighbours waiting on it see its progress, or give up too).
-  // A wave that gave up has sync_ok = right_ok = kNever (a wait is entered
-  // only at a row past them, so neither is kNever while a wait runs).
-  auto wait_limit = []() -> int {
-    return __builtin_amdgcn_readfirstlane(
-        __hip_atomic_load(&g_s1_wait_limit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-  };
+  // Every wait polls at most spin_limit times.  A wave whose wait gives up
+  // marks sync_ok and right_ok with kGaveUp (no wait is entered again: rows
+  // stay below it) and finishes its rows on whatever values it has, so every
+  // wave still exits (neighbours waiting on it see its progress, or give up
+  // too); at its end it records the error (g_s1_wait_error, read by
+  // pf_device_status).
+  constexpr int kGaveUp = kNever - 1;
   auto give_up = [&]() {
-    sync_ok = kNever;
-    right_ok = kNever;
-    if (lane == 0) __hip_atomic_fetch_or(&g_s1_wait_error, 1, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
+    sync_ok = kGaveUp;
+    right_ok = kGaveUp;
   };
   if (consumer && wave == 0 && R > 1) {
     const int need = (G < R) ? G : R;
-    for (int n = wait_limit();
+    for (int n = spin_limit;
          __builtin_amdgcn_readfirstlane(
              __hip_atomic_load(fin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) < need;
          --n) {
@@ -493,7 +487,7 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
       if (r > sync_ok) {
         if (wave > 0) {
           // wait until the wave on the left has handed over row r-1
-          for (int n = wait_limit();; --n) {
+          for (int n = spin_limit;; --n) {
             if (n <= 0) {
               give_up();
               break;
@@ -503,14 +497,14 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
             if (seen_left >= r - 1) break;
             __builtin_amdgcn_s_sleep(1);
           }
-          if (sync_ok != kNever) sync_ok = seen_left + 1;
+          if (sync_ok != kGaveUp) sync_ok = seen_left + 1;
         } else {
           // first row of an exchange block: wait for the strip on the left
           // and stage its rows r-1 .. r+G-2 (row r'-1 of every row r' of the
           // block)
           const int last = r + G - 1;
           const int need = (last < R) ? last : R;
-          for (int n = wait_limit();
+          for (int n = spin_limit;
                __builtin_amdgcn_readfirstlane(
                    __hip_atomic_load(fin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) < need;
                --n) {
@@ -531,7 +525,7 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
           }
           // lane 0 reads the staged rows from this row on
           __threadfence_block();
-          sync_ok = (r + G < R && sync_ok != kNever) ? r + G - 1 : kNever;
+          if (sync_ok != kGaveUp) sync_ok = (r + G < R) ? r + G - 1 : kNever;
         }
       }
       // The source is the same for the whole wave (only lane 0's copy is
@@ -791,7 +785,7 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
       }
       // Hand row r to the wave on the right once it has read row r - kRing.
       if (r > right_ok) {
-        for (int n = wait_limit();; --n) {
+        for (int n = spin_limit;; --n) {
           if (n <= 0) {
             give_up();
             break;
@@ -801,7 +795,7 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
           if (seen_right >= r - kRing + 1) break;
           __builtin_amdgcn_s_sleep(1);
         }
-        if (right_ok != kNever) right_ok = seen_right + kRing - 1;
+        if (right_ok != kGaveUp) right_ok = seen_right + kRing - 1;
       }
       if (lane == 63) {
         const int s0 = slot * 16 + wave;
@@ -844,6 +838,8 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
     const int r = rb + u;
     if (r < R) row_step(r, r % kRing, Cq[u]);
   }
+  // every lane stores the same word: one vector store
+  if (sync_ok == kGaveUp) g_s1_wait_error = 1;
   if constexpr (!FULL) {
     // Last back-pointer word when it holds fewer than 16 rows.
     if (R > 1 && ((R - 1) & 15) != 15) {
@@ -886,16 +882,18 @@ static void launch_cpl(dim3 grid, dim3 block, hipStream_t st, const CT *C, const
     if (!full && block.x <= 256) {
       hipLaunchKernelGGL((k_stage1<CPL, CT, DT, false, STD, false, 256>), grid, block, 0, st, C,
                          tiles, td0, ss, Lmax, S, G, flags, x, Bp, eD, eP, eC, Df, Pf, Bf,
-                         nullptr, nullptr, 0);
+                         nullptr, nullptr, 0, g_wait_limit);
       return;
     }
   }
   if (full)
     hipLaunchKernelGGL((k_stage1<CPL, CT, DT, true, STD>), grid, block, 0, st, C, tiles, td0, ss,
-                       Lmax, S, G, flags, x, Bp, eD, eP, eC, Df, Pf, Bf, nullptr, nullptr, 0);
+                       Lmax, S, G, flags, x, Bp, eD, eP, eC, Df, Pf, Bf, nullptr, nullptr, 0,
+                       g_wait_limit);
   else
     hipLaunchKernelGGL((k_stage1<CPL, CT, DT, false, STD>), grid, block, 0, st, C, tiles, td0, ss,
-                       Lmax, S, G, flags, x, Bp, eD, eP, eC, Df, Pf, Bf, nullptr, nullptr, 0);
+                       Lmax, S, G, flags, x, Bp, eD, eP, eC, Df, Pf, Bf, nullptr, nullptr, 0,
+                       g_wait_limit);
 }
 
 // Features form: LDS for the per-wave C blocks is dynamic (nw * 32 * 64*CPL bf16).
@@ -911,7 +909,7 @@ static int launch_feat_cpl(dim3 grid, dim3 block, size_t lds, hipStream_t st,
   const TileDesc td0{};
   hipLaunchKernelGGL(kern, grid, block, lds, st, static_cast<const __hip_bfloat16 *>(nullptr),
                      tiles, td0, ss, Lmax, S, G, flags, static_cast<Xch<float> *>(xch), Bp, eD, eP,
-                     eC, nullptr, nullptr, nullptr, Fa, Fb, Kp);
+                     eC, nullptr, nullptr, nullptr, Fa, Fb, Kp, g_wait_limit);
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -1063,14 +1061,8 @@ extern "C" int pf_stage1_features(const void *d_F1b, const void *d_F2b, int Kp,
 
 extern "C" int pf_set_wait_limit(int64_t polls, int64_t *previous) {
   PF_REQUIRE(polls >= 0 && polls <= 0x7fffffff, "pf_set_wait_limit: limit outside [0, 2^31)");
-  int prev = 0;
-  PF_CHECK_HIP(hipMemcpyFromSymbol(&prev, HIP_SYMBOL(g_s1_wait_limit), sizeof(int), 0,
-                                   hipMemcpyDeviceToHost));
-  const int v = (int)polls;
-  PF_CHECK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_s1_wait_limit), &v, sizeof(int), 0,
-                                 hipMemcpyHostToDevice));
-  PF_CHECK_HIP(hipDeviceSynchronize());
-  if (previous) *previous = prev;
+  if (previous) *previous = g_wait_limit;
+  g_wait_limit = (int)polls;
   return PF_OK;
 }
 
