@@ -109,6 +109,61 @@ def _cpu_model():
         return "unknown"
 
 
+def _cpu_timed(oracle, Cs, L, nthreads, trials):
+    """runtime.py:144-173's method: perf_counter around each whole parflex
+    call, one warm-up pass over the sample (dropped, as the reference drops
+    its first trials, runtime_analysis.ipynb#cell2), then `trials` timed
+    passes.  Returns (seconds per timed pass, results of the last pass)."""
+    times, res = [], []
+    for k in range(trials + 1):
+        res = []
+        t0 = time.perf_counter()
+        for C in Cs:
+            res.append(oracle.parflex(C, STEPS, WEIGHTS, BETA, L, nthreads=nthreads))
+        if k > 0:
+            times.append(time.perf_counter() - t0)
+    return times, res
+
+
+def _cpu_baseline(args, Cs, Cs1, L, what, per_unit, unit, expect=None):
+    """The CPU baseline (the FP64 oracle, a port of the reference's
+    parflex): the cpu_parflex.py:392 schedule (stage 1 on a pool of
+    min(tiles, threads) workers, stage 2 on one thread) on the sample Cs, and
+    the single-thread form (Parflex.py:18-22 with one thread, runtime.py:144)
+    on the sample Cs1.  `per_unit` converts cells/s into the metric's unit
+    (pairs/s at the workload's mean pair size, or GCUPS).  `expect` holds the
+    plain GPU results of Cs, which the CPU results must equal."""
+    from oracle import oracle
+    cores = len(os.sched_getaffinity(0))
+    nthreads = max(1, min(args.cpu_threads, cores))
+    trials = max(1, args.cpu_trials)
+    cells = sum(C.shape[0] * C.shape[1] for C in Cs)
+    cells1 = sum(C.shape[0] * C.shape[1] for C in Cs1)
+    t_m, res = _cpu_timed(oracle, Cs, L, nthreads, trials)
+    t_1, _ = _cpu_timed(oracle, Cs1, L, 1, trials)
+    med_m, med_1 = float(np.median(t_m)), float(np.median(t_1))
+    exact = None
+    if expect is not None:
+        exact = all(bo == be and np.array_equal(wpo, wpe)
+                    for (bo, wpo), (be, wpe) in zip(res, expect))
+    out = {
+        "value": round(per_unit(cells / med_m), 4), "unit": unit, "cores": nthreads,
+        "kind": "port",
+        "sample": f"{what}: {len(Cs)} pair(s), {cells / 1e9:.3f} Gcells, FP64 parflex end to "
+                  f"end at L={L} on {nthreads} worker threads (cpu_parflex.py:392 schedule), "
+                  f"median of {trials} timed passes after 1 warm-up pass",
+        "seconds": round(med_m, 4), "trials_s": [round(t, 4) for t in t_m],
+        "gcups": round(cells / med_m / 1e9, 4),
+        "affinity_cores": cores, "cpu_model": _cpu_model(),
+        "single_thread": {
+            "value": round(per_unit(cells1 / med_1), 4), "unit": unit, "cores": 1,
+            "gcups": round(cells1 / med_1 / 1e9, 4), "seconds": round(med_1, 4),
+            "trials_s": [round(t, 4) for t in t_1],
+            "sample": f"{len(Cs1)} pair(s), {cells1 / 1e9:.3f} Gcells, one thread, median of "
+                      f"{trials} timed passes after 1 warm-up pass"}}
+    return out, exact
+
+
 def _roofline(cells, s1_ms, bytes_per_cell, kernel):
     achieved = cells * bytes_per_cell / (s1_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -270,27 +325,20 @@ def bench_learned(args, torch):
             "note": "fraction of the VALU issue rate (the kernel's ceiling; insts per cell "
                     "from a PMC run of this workload)"}
     if not args.no_cpu:
-        from oracle import oracle
-        nthreads = args.cpu_threads
-        # Held-out pairs in order until about 10 s of CPU work (at most 8);
-        # one pair's C (up to ~19 GB) is on the host at a time.
-        t_cpu, n_cpu, exact = 0.0, 0, True
-        while n_cpu < min(8, P) and t_cpu < 10.0:
-            C = cost_matrix_from_features_gpu(*pairs[n_cpu]).cpu().numpy()
-            tc = time.perf_counter()
-            bo, wpo = oracle.parflex(C, STEPS, WEIGHTS, BETA, L, nthreads=nthreads)
-            t_cpu += time.perf_counter() - tc
-            del C
-            exact = exact and bo == plain[n_cpu][0] and np.array_equal(wpo, plain[n_cpu][1])
-            n_cpu += 1
-        cells_cpu = sum(shapes[p][0] * shapes[p][1] for p in range(n_cpu))
-        out["cpu_baseline"] = {
-            "value": round(n_cpu / t_cpu, 4), "unit": "pairs/s", "cores": nthreads, "kind": "port",
-            "sample": f"pairs 0-{n_cpu - 1} of the held-out set ({cells_cpu / 1e9:.2f} Gcells), "
-                      f"FP64 parflex end to end at L={L}, {nthreads} worker threads "
-                      f"(cpu_parflex.py:392 schedule); {_cpu_model()}",
-            "seconds": round(t_cpu, 4)}
-        out["parity"] = {"plain_gpu_vs_cpu_oracle_exact": bool(exact), "pairs": n_cpu}
+        # The three smallest of the first 8 held-out pairs (about 2 Gcells;
+        # multi-thread) and the smallest of them (single thread), each checked
+        # against the plain GPU path; one pair's C is up to ~7 GB on the host.
+        first = list(range(min(8, P)))
+        small = sorted(first, key=lambda p: shapes[p][0] * shapes[p][1])[:3]
+        Cs = [cost_matrix_from_features_gpu(*pairs[p]).cpu().numpy() for p in small]
+        mean_cells = sum(a * b for a, b in shapes) / P
+        out["cpu_baseline"], exact = _cpu_baseline(
+            args, Cs, Cs[:1], L, f"held-out pairs {small}", lambda cps: cps / mean_cells,
+            "pairs/s", expect=[plain[p] for p in small])
+        out["cpu_baseline"]["sample"] += (f"; pairs/s = cells/s / {mean_cells / 1e9:.3f} Gcells, "
+                                          "the mean pair of the workload")
+        del Cs
+        out["parity"] = {"plain_gpu_vs_cpu_oracle_exact": bool(exact), "pairs": small}
     return out
 
 
@@ -404,24 +452,15 @@ def bench_batch(args, torch):
         "plain_fp64_sample": ([wp for _, wp in plain], gts[:ns])}, runner=br)
 
     if not args.no_cpu:
-        from oracle import oracle
-        nthreads = args.cpu_threads
-        t_cpu, n_cpu, exact = 0.0, 0, True
-        oracle.parflex(Cs_host[0], STEPS, WEIGHTS, BETA, L, nthreads=nthreads)  # warm-up
-        for p, C in enumerate(Cs_host):
-            tc = time.perf_counter()
-            bo, wpo = oracle.parflex(C, STEPS, WEIGHTS, BETA, L, nthreads=nthreads)
-            t_cpu += time.perf_counter() - tc
-            n_cpu += 1
-            exact = exact and bo == plain[p][0] and np.array_equal(wpo, plain[p][1])
-        out["cpu_baseline"] = {
-            "value": round(n_cpu / t_cpu, 4), "unit": "pairs/s", "cores": nthreads, "kind": "port",
-            "sample": f"pairs 0-{n_cpu - 1} of the batch ("
-                      f"{sum(shapes[p][0] * shapes[p][1] for p in range(n_cpu)) / 1e9:.2f} Gcells), "
-                      f"FP64 parflex end to end at L={L}, {nthreads} worker threads "
-                      f"(cpu_parflex.py:392 schedule); {_cpu_model()}",
-            "seconds": round(t_cpu, 4)}
-        out["parity"] = {"plain_gpu_vs_cpu_oracle_exact": bool(exact), "pairs": n_cpu}
+        mean_cells = sum(a * b for a, b in shapes) / P
+        small = sorted(range(len(Cs_host)), key=lambda p: Cs_host[p].size)[:1]
+        out["cpu_baseline"], exact = _cpu_baseline(
+            args, Cs_host, [Cs_host[p] for p in small], L,
+            f"pairs 0-{len(Cs_host) - 1} of the batch", lambda cps: cps / mean_cells, "pairs/s",
+            expect=plain)
+        out["cpu_baseline"]["sample"] += (f"; pairs/s = cells/s / {mean_cells / 1e9:.3f} Gcells, "
+                                          "the mean pair of the workload")
+        out["parity"] = {"plain_gpu_vs_cpu_oracle_exact": bool(exact), "pairs": len(Cs_host)}
     return out
 
 
@@ -482,27 +521,12 @@ def bench_plain(args, torch):
     out["stage2_ms"] = round(s2_ms, 4)
     out["roofline"] = _roofline(cells, s1_ms, 8.25, "k_stage1<2,double,double,false,true,false,256>")
     if not args.no_cpu:
-        from oracle import oracle
         C = C_dev.cpu().numpy()
-        nthreads = min(plan.ntiles, args.cpu_threads)
-        times = []
-        bo = wpo = None
-        for _ in range(args.cpu_trials):
-            tc = time.perf_counter()
-            bo, wpo = oracle.parflex(C, STEPS, WEIGHTS, BETA, L, nthreads=nthreads)
-            times.append(time.perf_counter() - tc)
-        t_cpu = min(times[1:]) if len(times) > 1 else times[0]
-        exact = (bo == float(res.best_cost[0])) and np.array_equal(wpo, res.paths[0])
-        out["cpu_baseline"] = {
-            "value": round(cells / t_cpu / 1e9, 4), "unit": "GCUPS", "cores": nthreads,
-            "kind": "port",
-            "sample": f"whole {L1}x{L2} L={L} pair, parflex end to end, "
-                      f"{nthreads} worker threads (cpu_parflex.py:392 schedule), best of "
-                      f"{len(times) - 1 if len(times) > 1 else 1} timed trials after 1 warm-up; "
-                      f"{_cpu_model()}",
-            "seconds": round(t_cpu, 4)}
+        out["cpu_baseline"], exact = _cpu_baseline(
+            args, [C], [C], L, f"the whole {L1}x{L2} pair", lambda cps: cps / 1e9, "GCUPS",
+            expect=[(float(res.best_cost[0]), res.paths[0])])
         out["parity"] = {"path_exact": bool(exact), "best_cost_gpu": float(res.best_cost[0]),
-                         "best_cost_cpu": float(bo), "path_len": int(res.paths[0].shape[1])}
+                         "path_len": int(res.paths[0].shape[1])}
     return out
 
 
@@ -614,19 +638,14 @@ def bench_precision(args, torch):
         out["roofline"] = _roofline(cells, s1_ms, 2.25,
                                     "k_stage1<2,__hip_bfloat16,float,false,true,false,1024>")
     if not args.no_cpu:
-        from oracle import oracle
         C = C64.cpu().numpy()
-        tc = time.perf_counter()
-        bo, wpo = oracle.parflex(C, STEPS, WEIGHTS, BETA, L, nthreads=args.cpu_threads)
-        t_cpu = time.perf_counter() - tc
-        out["cpu_baseline"] = {
-            "value": round(cells / t_cpu / 1e9, 4), "unit": "GCUPS", "cores": args.cpu_threads,
-            "kind": "port",
-            "sample": f"whole {L1}x{L2} L={L} pair, FP64 parflex end to end, "
-                      f"{args.cpu_threads} worker threads; {_cpu_model()}",
-            "seconds": round(t_cpu, 4)}
-        out["parity"] = {"plain_gpu_vs_cpu_oracle_exact":
-                         bool(bo == bc64 and np.array_equal(wpo, wp64))}
+        # single thread: the top-left 16000 x 16000 block (2.6e8 cells, 16 tiles)
+        C1 = np.ascontiguousarray(C[:16000, :16000])
+        out["cpu_baseline"], exact = _cpu_baseline(
+            args, [C], [C1], L, f"the whole {L1}x{L2} pair", lambda cps: cps / 1e9, "GCUPS",
+            expect=[(bc64, wp64)])
+        out["cpu_baseline"]["single_thread"]["sample"] += " (top-left 16000x16000 block)"
+        out["parity"] = {"plain_gpu_vs_cpu_oracle_exact": bool(exact)}
     return out
 
 
