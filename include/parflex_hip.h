@@ -353,6 +353,34 @@ int pf_pack_weights(const float *d_w, void *d_Wt, void *d_Wf, void *stream);
 int pf_tile_scores(const float *d_Z, int64_t H, int64_t W, int pool, int64_t L, int64_t L1,
                    int64_t L2, int ext, float *d_score, void *stream);
 
+/* The four layers of the network (pf_conv3x3_first_pooled, pf_conv3x3 mode 0
+ * twice, pf_conv3x3 mode 1) for every pair of a batch in one launch: the
+ * logit maps equal those of the per-layer calls bit for bit, with no
+ * activation map in HBM.  d_A (K x lda) / d_B (K x ldb): pooled features of
+ * every pair (pf_pool_features_batch); d_st: the statistics of
+ * pf_input_stats_batch.  Pair table int64[8] per pair: {a_off, H, b_off, W,
+ * z_off, st_off, 0, 0}: the pair's pooled columns of d_A and d_B, its map
+ * size, its logit map at d_Z + z_off (f32 H x W) and its statistics at
+ * d_st + st_off.  Job table int64[4] per job: {pair, x0, y0, y1}: output
+ * columns x0 .. x0 + cols - 1 (clipped to W) and rows y0 .. y1 - 1, with
+ * x0 a multiple of cols and y1 - y0 <= rows from pf_tile_net_geometry.
+ * Weights W1..W4 bf16 [32][288] (pf_pack_weights), biases b1..b4 f32 [32],
+ * all 16-byte aligned; K <= 16. */
+int pf_tile_net_geometry(int64_t *cols_per_job, int64_t *rows_per_job);
+int pf_tile_net(const float *d_A, int64_t lda, const float *d_B, int64_t ldb, int64_t K,
+                const float *d_st, const int64_t *d_pairs, int64_t npairs, const int64_t *d_jobs,
+                int64_t njobs, const void *d_W1, const float *d_b1, const void *d_W2,
+                const float *d_b2, const void *d_W3, const float *d_b3, const void *d_W4,
+                const float *d_b4, float *d_Z, void *stream);
+/* pf_tile_scores for every pair of a batch in one launch: tile t (the
+ * batch's tile order, pair by pair, i-major) belongs to pair d_tpair[t]
+ * (int32), whose row of d_pairs (int64[8]) is {H, W, z_off, L1, L2, n2,
+ * tile_base, 0}; d_score[t] as pf_tile_scores gives it for that pair's map
+ * at d_Z + z_off. */
+int pf_tile_scores_batch(const float *d_Z, const int64_t *d_pairs, const int32_t *d_tpair,
+                         int64_t ntiles, int pool, int64_t L, int ext, float *d_score,
+                         void *stream);
+
 /* Selection check of the learned form: d_flag[p] |= 1 (int32, cleared by
  * the caller) when a tile holding a cell within `guard` frames of a point of
  * pair p's stitched path (d_path / d_path_len as written by pf_stage2, pair

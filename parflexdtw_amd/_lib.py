@@ -74,6 +74,10 @@ _SIGNATURES = {
     "pf_adam": (_int, [_vp, _vp, _vp, _vp, _vp, _i64, _flt, _flt, _flt, _flt, _i64, _vp]),
     "pf_pack_weights": (_int, [_vp, _vp, _vp, _vp]),
     "pf_tile_scores": (_int, [_vp, _i64, _i64, _int, _i64, _i64, _i64, _int, _vp, _vp]),
+    "pf_tile_net_geometry": (_int, [_vp, _vp]),
+    "pf_tile_net": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp,
+                           _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pf_tile_scores_batch": (_int, [_vp, _vp, _vp, _i64, _int, _i64, _int, _vp, _vp]),
     "pf_selection_check": (_int, [_vp, _i64, _i64, _vp, _vp, _vp, _i64, _i64, _vp, _vp]),
     "pf_eval_paths": (_int, [_vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _dbl, _vp, _i64,
                              _vp, _vp, _vp]),

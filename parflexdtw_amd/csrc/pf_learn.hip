@@ -767,13 +767,9 @@ __global__ void k_pack_weights(const float *__restrict__ w, __hip_bfloat16 *__re
 // A wave walks rows wave, wave+4, ... of the region with its lanes along
 // the row (coalesced), four rows in flight per step; max is exact in any
 // order.  Element offsets fit 32 bits (the host checks H * W < 2^31).
-__global__ __launch_bounds__(256) void k_tile_scores(const float *__restrict__ Z, int64_t H,
-                                                     int64_t W, int pool, int64_t L, int64_t L1,
-                                                     int64_t L2, int64_t n2, int ext,
-                                                     float *__restrict__ score) {
-  __shared__ float red[4];
-  const int64_t t = blockIdx.x;
-  const int64_t i = t / n2, j = t - i * n2;
+__device__ __forceinline__ float tile_score_of(const float *__restrict__ Z, int64_t H, int64_t W,
+                                               int pool, int64_t L, int64_t L1, int64_t L2,
+                                               int64_t i, int64_t j, int ext, float *red) {
   const int64_t hop = L - 1;
   const int64_t r0 = i * hop, r1 = (r0 + L < L1 ? r0 + L : L1) - 1;
   const int64_t c0 = j * hop, c1 = (c0 + L < L2 ? c0 + L : L2) - 1;
@@ -802,7 +798,282 @@ __global__ __launch_bounds__(256) void k_tile_scores(const float *__restrict__ Z
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
   if (lane == 0) red[wave] = m;
   __syncthreads();
-  if (threadIdx.x == 0) score[t] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  return fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+__global__ __launch_bounds__(256) void k_tile_scores(const float *__restrict__ Z, int64_t H,
+                                                     int64_t W, int pool, int64_t L, int64_t L1,
+                                                     int64_t L2, int64_t n2, int ext,
+                                                     float *__restrict__ score) {
+  __shared__ float red[4];
+  const int64_t t = blockIdx.x;
+  const int64_t i = t / n2, j = t - i * n2;
+  const float m = tile_score_of(Z, H, W, pool, L, L1, L2, i, j, ext, red);
+  if (threadIdx.x == 0) score[t] = m;
+}
+
+// Tile scores of every pair of a batch in one launch: tile t (of the
+// batch's i-major tile order) belongs to pair tpair[t], whose row of
+// `pairs` (int64[8]) is {H, W, z_off, L1, L2, n2, tile_base, 0}: its logit
+// map is Z + z_off (H x W), its tiles tile_base .. tile_base + n1*n2 - 1.
+struct ScorePair {
+  int64_t H, W, z_off, L1, L2, n2, tile_base, r7;
+};
+__global__ __launch_bounds__(256) void k_tile_scores_batch(const float *__restrict__ Z,
+                                                           const ScorePair *__restrict__ pairs,
+                                                           const int32_t *__restrict__ tpair,
+                                                           int64_t ntiles, int pool, int64_t L,
+                                                           int ext, float *__restrict__ score) {
+  __shared__ float red[4];
+  const int64_t t = blockIdx.x + (int64_t)blockIdx.y * 65536;
+  if (t >= ntiles) return;   // the whole workgroup: the last grid row is partial
+  const ScorePair pd = pairs[tpair[t]];
+  const int64_t q = t - pd.tile_base;
+  const int64_t i = q / pd.n2, j = q - i * pd.n2;
+  const float m = tile_score_of(Z + pd.z_off, pd.H, pd.W, pool, L, pd.L1, pd.L2, i, j, ext, red);
+  if (threadIdx.x == 0) score[t] = m;
+}
+
+// The four layers of the tile network in one kernel (inference).  A job is
+// a strip of kNtOut output columns x up to kNtSeg output rows of one pair's
+// logit map.  The kernel streams down the rows: every step each layer
+// computes one row of kNtRow pixels (two MFMA tiles of 32), the first layer
+// from the network input (built from the pooled features as
+// k_conv3x3_first_pooled does), the others from the last three rows of the
+// layer below, which LDS holds as rings of four rows.  Layer l's row runs
+// 2l rows behind the first layer's, so every read is of a row written in one
+// of the three steps before, and each step writes the fourth slot: one
+// barrier per step.  Each layer's columns start one pixel right of those
+// of the layer below (a 3x3 tap reads pixels c .. c+2 of the row below), so
+// of kNtRow pixels 2 per layer are lost and kNtRow - 6 logits remain.
+// Waves 0, 1, 2 run layers 2, 3, 4, each with its layer's weight fragments
+// in registers; wave 3 builds the input row and runs layer 1 (one MFMA per
+// tile), off the path of the others.
+// Pixels and rows outside the map are stored as zeros (the zero padding of
+// the per-layer kernels), and every sum runs in the order of those kernels,
+// so the logits equal those of k_conv3x3_first_pooled, k_conv3x3<0> twice
+// and k_conv3x3<1> bit for bit.
+constexpr int kNtT = 2;                  // MFMA tiles per layer row
+constexpr int kNtRow = 32 * kNtT;        // pixels a layer computes per row
+constexpr int kNtOut = kNtRow - 6;       // logits per job row
+constexpr int kNtRing = kNtRow + 2;      // pixel slots of a ring row (2 zero)
+constexpr int kNtSeg = 128;              // output rows per job
+constexpr int kNtDepth = 4;              // rows per ring: 3 read, 1 written per step
+constexpr int kNtMaxK = 16;
+
+struct NetPair {
+  int64_t a_off, H, b_off, W, z_off, st_off, r6, r7;
+};
+struct NetJob {
+  int64_t pair, x0, y0, y1;
+};
+
+__global__ __launch_bounds__(256) void k_tile_net(
+    const float *__restrict__ A, int64_t lda, const float *__restrict__ B, int64_t ldb, int K,
+    const float *__restrict__ st_all, const NetPair *__restrict__ pairs,
+    const NetJob *__restrict__ jobs, const __hip_bfloat16 *__restrict__ W1,
+    const float *__restrict__ b1, const __hip_bfloat16 *__restrict__ W2,
+    const float *__restrict__ b2, const __hip_bfloat16 *__restrict__ W3,
+    const float *__restrict__ b3, const __hip_bfloat16 *__restrict__ W4,
+    const float *__restrict__ b4, float *__restrict__ Z_all) {
+  // rings of layers 1-3: [layer][slot][pixel][channel], pixel stride kPixPad
+  __shared__ __attribute__((aligned(16))) __hip_bfloat16 ring[3][kNtDepth][kNtRing * kPixPad];
+  __shared__ __hip_bfloat16 xin[kNtDepth][kNtRing];   // network input rows (one channel)
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const NetJob jb = jobs[blockIdx.x];
+  const NetPair pd = pairs[jb.pair];
+  const int64_t H = pd.H, W = pd.W;
+  const int x0 = (int)jb.x0, y0 = (int)jb.y0, y1 = (int)jb.y1;
+  const float *st = st_all + pd.st_off;
+  float *Z = Z_all + pd.z_off;
+  const int px = lane & 31, kh = (lane >> 5) * 8;
+  const int Hi = (int)H, Wi = (int)W;
+
+  // centred row features of the job's input rows y0 - 4 .. y1 + 3 and
+  // column features of its input slots (slot c: column x0 - 4 + c), zero for
+  // k >= K and outside the map: the operands k_conv3x3_first_pooled stages
+  __shared__ __attribute__((aligned(16))) float rowf[(kNtSeg + 8) * kNtMaxK];
+  __shared__ __attribute__((aligned(16))) float colf[kNtRing * kNtMaxK];
+  __shared__ __attribute__((aligned(16))) float sbias[4][kCh];
+  const int nin = (y1 - y0) + 8;
+  for (int q = tid; q < nin * kNtMaxK; q += 256) {
+    const int r = q / kNtMaxK, k = q - r * kNtMaxK;
+    const int y = y0 - 4 + r;
+    rowf[q] = (k < K && y >= 0 && y < Hi) ? A[k * lda + pd.a_off + y] - st[k] : 0.0f;
+  }
+  // colf[k][c]: lanes read consecutive slots (no bank conflicts)
+  for (int q = tid; q < kNtRing * kNtMaxK; q += 256) {
+    const int k = q / kNtRing, c = q - k * kNtRing;
+    const int x = x0 - 4 + c;
+    colf[q] = (k < K && x >= 0 && x < Wi) ? B[k * ldb + pd.b_off + x] - st[K + k] : 0.0f;
+  }
+  if (tid < 4 * kCh) {
+    const float *bl = tid < kCh ? b1 : (tid < 2 * kCh ? b2 : (tid < 3 * kCh ? b3 : b4));
+    sbias[tid / kCh][tid % kCh] = bl[tid % kCh];
+  }
+  // zero rings and input rows (pad slots stay zero)
+  for (int q = tid; q < 3 * kNtDepth * kNtRing * kPixPad / 8; q += 256)
+    reinterpret_cast<lu32x4_t *>(&ring[0][0][0])[q] = lu32x4_t{0u, 0u, 0u, 0u};
+  for (int q = tid; q < kNtDepth * kNtRing; q += 256) (&xin[0][0])[q] = __float2bfloat16(0.0f);
+
+  // the layer this wave runs (waves 0, 1, 2: layers 2, 3, 4; wave 3 layer 1)
+  const __hip_bfloat16 *Wl = wave == 0 ? W2 : (wave == 1 ? W3 : W4);
+  lbf16x8_t wb[18];
+#pragma unroll
+  for (int q = 0; q < 18; ++q)
+    wb[q] = *reinterpret_cast<const lbf16x8_t *>(&Wl[px * kKdim + q * 16 + kh]);
+  // first-layer taps (wave 3): k = dy*3 + dx of input channel 0
+  lbf16x8_t w1;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int k = kh + i;
+    w1[i] = k < 9 ? (__bf16)bf2f(W1[px * kKdim + k * kCh]) : (__bf16)0.0f;
+  }
+  const float inv = st[2 * K];
+  // input row y (staged row r = y - y0 + 4) into slot `slot` (wave 3):
+  // bf16(-s * inv), s summed over 16 terms in order, zero outside the map;
+  // lane l writes slots l and l + 64
+  auto input_row = [&](const int y, const int slot) {
+    const bool yok = y >= 0 && y < Hi;
+    const float *ca = &rowf[(y - y0 + 4) * kNtMaxK];
+    const int l1 = lane + 64 < kNtRing ? lane + 64 : 0;
+    float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kNtMaxK; ++k) {
+      s0 += ca[k] * colf[k * kNtRing + lane];
+      s1 += ca[k] * colf[k * kNtRing + l1];
+    }
+    const int xa = x0 - 4 + lane, xb = xa + 64;
+    xin[slot][lane] = (yok && xa >= 0 && xa < Wi) ? __float2bfloat16(-s0 * inv)
+                                                  : __float2bfloat16(0.0f);
+    if (lane + 64 < kNtRing)
+      xin[slot][lane + 64] = (yok && xb >= 0 && xb < Wi) ? __float2bfloat16(-s1 * inv)
+                                                         : __float2bfloat16(0.0f);
+  };
+  __syncthreads();
+  lfloatx4_t bb[4];
+  load_bias4<0>(&sbias[wave < 3 ? wave + 1 : 0][0], lane, bb);
+  if (wave == 3) {
+    input_row(y0 - 4, 0);
+    input_row(y0 - 3, 1);
+  }
+  // one 3x3 layer row from ring `src`: two tiles, k-steps in the order
+  // (channel half, dy, dx) of k_conv3x3
+  auto layer_row = [&](const __hip_bfloat16 *src, const int s, lfloatx16_t (&acc)[kNtT]) {
+#pragma unroll
+    for (int t = 0; t < kNtT; ++t) acc[t] = lfloatx16_t{};
+#pragma unroll
+    for (int ch = 0; ch < 2; ++ch)
+#pragma unroll
+      for (int dy = 0; dy < 3; ++dy) {
+        const int slot = (s + 1 + dy) % kNtDepth;
+        const __hip_bfloat16 *rowp = src + slot * kNtRing * kPixPad;
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+          for (int t = 0; t < kNtT; ++t) {
+            const lbf16x8_t a = *reinterpret_cast<const lbf16x8_t *>(
+                &rowp[(t * 32 + px + dx) * kPixPad + ch * 16 + kh]);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wb[(dy * 3 + dx) * 2 + ch], a,
+                                                             acc[t], 0, 0, 0);
+          }
+      }
+  };
+  // relu(acc + bias) of row y (columns xs0 + slot) into ring row `dst`,
+  // zero outside the map
+  auto store_row = [&](const lfloatx16_t (&acc)[kNtT], const lfloatx4_t (&bias)[4], const int y,
+                       const int xs0, __hip_bfloat16 *dst) {
+    const bool yok = y >= 0 && y < Hi;
+    const int cq = (lane >> 5) * 4;
+#pragma unroll
+    for (int t = 0; t < kNtT; ++t) {
+      const int x = xs0 + t * 32 + px;
+      const bool ok = yok && x >= 0 && x < Wi;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        float r[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          r[i] = acc[t][g * 4 + i] + bias[g][i];
+          r[i] = r[i] > 0.0f ? r[i] : 0.0f;
+        }
+        *reinterpret_cast<lu32x2_t *>(&dst[(t * 32 + px) * kPixPad + g * 8 + cq]) =
+            ok ? lu32x2_t{pack_bf16x2(r[0], r[1]), pack_bf16x2(r[2], r[3])}
+               : lu32x2_t{0u, 0u};
+      }
+    }
+  };
+  auto zero_row = [&](__hip_bfloat16 *dst) {
+    for (int q = lane; q < kNtRow * kPixPad / 8; q += 64)
+      reinterpret_cast<lu32x4_t *>(dst)[q] = lu32x4_t{0u, 0u, 0u, 0u};
+  };
+  const int nsteps = (y1 - y0) + 9;
+  for (int s = 0; s < nsteps; ++s) {
+    const int a = y0 - 3 + s;          // first-layer row of this step
+    // layer l (1..4) row a - 2(l-1); needed rows of layer l: [y0 - 4 + l, y1 + 4 - l).
+    // Every layer reads rows written in the three previous steps (slots
+    // (s+1 .. s+3) % 4) and writes its row into slot s % 4, which nobody
+    // reads in this step: one barrier per step.
+    const int slot = s % kNtDepth;
+    if (wave == 3) {
+      // first layer, row a, from input rows a-1 .. a+1 (input row a+1 now,
+      // into slot (s+2) % 4, whose row a-3 the first layer no longer reads)
+      if (a + 1 < y1 + 4) input_row(a + 1, (s + 2) % kNtDepth);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (a >= y0 - 3 && a < y1 + 3) {
+        lfloatx16_t acc1[kNtT];
+#pragma unroll
+        for (int t = 0; t < kNtT; ++t) {
+          lbf16x8_t in;
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            const int k = kh + i;
+            const int dy = k / 3, dx = k - (k / 3) * 3;
+            const __bf16 xv = reinterpret_cast<const __bf16 *>(&xin[0][0])
+                [k < 9 ? ((s + dy) % kNtDepth) * kNtRing + t * 32 + px + dx : 0];
+            in[i] = k < 9 ? xv : (__bf16)0.0f;
+          }
+          acc1[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, in, lfloatx16_t{}, 0, 0, 0);
+        }
+        store_row(acc1, bb, a, x0 - 3, &ring[0][slot][0]);
+      }
+    } else {
+      lfloatx16_t acc[kNtT];
+      int row;
+      bool run;
+      if (wave == 0) {
+        row = a - 2;
+        run = row >= y0 - 2 && row < y1 + 2;
+      } else if (wave == 1) {
+        row = a - 4;
+        run = row >= y0 - 1 && row < y1 + 1;
+      } else {
+        row = a - 6;
+        run = row >= y0 && row < y1;
+      }
+      const bool inmap = row >= 0 && row < Hi;
+      if (run && inmap) layer_row(&ring[wave][0][0], s, acc);
+      if (wave < 2) {
+        __hip_bfloat16 *dst = &ring[wave + 1][slot][0];
+        if (run) {
+          if (inmap) store_row(acc, bb, row, x0 - 2 + wave, dst);
+          else zero_row(dst);
+        }
+      } else if (run && inmap && lane < 32) {
+        // logits of row `row`, columns x0 .. x0 + kNtOut - 1
+#pragma unroll
+        for (int t = 0; t < kNtT; ++t) {
+          const int c = t * 32 + px, x = x0 + c;
+          if (c < kNtOut && x < Wi) Z[(int64_t)row * W + x] = acc[t][0] + bb[0][0];
+        }
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // Selection check of the learned form: flag[p] = 1 when a tile that holds a
@@ -1033,6 +1304,59 @@ extern "C" int pf_tile_scores(const float *d_Z, int64_t H, int64_t W, int pool, 
   const int64_t n1 = (L1 - 1 + hop - 1) / hop, n2 = (L2 - 1 + hop - 1) / hop;
   hipLaunchKernelGGL(k_tile_scores, dim3((unsigned)(n1 * n2)), dim3(256), 0, as_stream(stream), d_Z,
                      H, W, pool, L, L1, L2, n2, ext, d_score);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_tile_net_geometry(int64_t *cols_per_job, int64_t *rows_per_job) {
+  PF_REQUIRE(cols_per_job && rows_per_job, "pf_tile_net_geometry: null output");
+  *cols_per_job = kNtOut;
+  *rows_per_job = kNtSeg;
+  return PF_OK;
+}
+
+extern "C" int pf_tile_net(const float *d_A, int64_t lda, const float *d_B, int64_t ldb, int64_t K,
+                           const float *d_st, const int64_t *d_pairs, int64_t npairs,
+                           const int64_t *d_jobs, int64_t njobs, const void *d_W1,
+                           const float *d_b1, const void *d_W2, const float *d_b2,
+                           const void *d_W3, const float *d_b3, const void *d_W4,
+                           const float *d_b4, float *d_Z, void *stream) {
+  PF_REQUIRE(d_A && d_B && d_st && d_pairs && d_jobs && d_W1 && d_b1 && d_W2 && d_b2 && d_W3 &&
+                 d_b3 && d_W4 && d_b4 && d_Z && npairs >= 1 && njobs >= 0 && K >= 1,
+             "pf_tile_net: bad arguments");
+  PF_REQUIRE(K <= kNtMaxK, "pf_tile_net: at most 16 features per frame");
+  PF_REQUIRE(njobs < (1ll << 31), "pf_tile_net: too many jobs");
+  const uintptr_t al = reinterpret_cast<uintptr_t>(d_W1) | reinterpret_cast<uintptr_t>(d_W2) |
+                       reinterpret_cast<uintptr_t>(d_W3) | reinterpret_cast<uintptr_t>(d_W4) |
+                       reinterpret_cast<uintptr_t>(d_b1) | reinterpret_cast<uintptr_t>(d_b2) |
+                       reinterpret_cast<uintptr_t>(d_b3) | reinterpret_cast<uintptr_t>(d_b4);
+  PF_REQUIRE(al % 16 == 0, "pf_tile_net: weights and biases must be 16-byte aligned");
+  if (njobs == 0) return PF_OK;
+  hipLaunchKernelGGL(k_tile_net, dim3((unsigned)njobs), dim3(256), 0, as_stream(stream), d_A, lda,
+                     d_B, ldb, (int)K, d_st, reinterpret_cast<const NetPair *>(d_pairs),
+                     reinterpret_cast<const NetJob *>(d_jobs),
+                     static_cast<const __hip_bfloat16 *>(d_W1), d_b1,
+                     static_cast<const __hip_bfloat16 *>(d_W2), d_b2,
+                     static_cast<const __hip_bfloat16 *>(d_W3), d_b3,
+                     static_cast<const __hip_bfloat16 *>(d_W4), d_b4, d_Z);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_tile_scores_batch(const float *d_Z, const int64_t *d_pairs,
+                                    const int32_t *d_tpair, int64_t ntiles, int pool, int64_t L,
+                                    int ext, float *d_score, void *stream) {
+  PF_REQUIRE(d_Z && d_pairs && d_tpair && d_score && ntiles >= 0 && pool >= 1 && L >= 2 &&
+                 ext >= 0,
+             "pf_tile_scores_batch: bad arguments");
+  if (ntiles == 0) return PF_OK;
+  const int64_t gy = (ntiles + 65535) / 65536;
+  PF_REQUIRE(gy < 65536, "pf_tile_scores_batch: too many tiles");
+  hipLaunchKernelGGL(k_tile_scores_batch, dim3((unsigned)(ntiles < 65536 ? ntiles : 65536),
+                                               (unsigned)gy),
+                     dim3(256), 0, as_stream(stream), d_Z,
+                     reinterpret_cast<const ScorePair *>(d_pairs), d_tpair, ntiles, pool, L, ext,
+                     d_score);
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
 }

@@ -123,7 +123,8 @@ class LearnedRunner:
     stage 1 + stage 2, fallback) and returns (BatchResult, stats)."""
 
     def __init__(self, pairs, L, steps, weights, beta, net: TileNet, threshold, ext=8,
-                 guard=128, cost_bound=math.inf, device=None, graph=True, fused=True):
+                 guard=128, cost_bound=math.inf, device=None, graph=True, fused=True,
+                 fused_net=True):
         torch = _torch()
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.L, self.beta = int(L), float(beta)
@@ -176,6 +177,13 @@ class LearnedRunner:
         self.Ap = torch.empty(K * self.nbA, dtype=torch.float32, device=self.device)
         self.Bp = torch.empty(K * self.nbB, dtype=torch.float32, device=self.device)
         self.stats = torch.empty(n * (2 * K + 1), dtype=torch.float32, device=self.device)
+        # fused_net: the four layers of every pair in one launch (pf_tile_net)
+        # and the tile maxima of every pair in one more (pf_tile_scores_batch);
+        # otherwise one launch per layer and pair (the per-layer kernels, which
+        # give the same logits bit for bit)
+        self.fused_net = bool(fused_net)
+        if self.fused_net:
+            self._net_tables()
         # fused: the selected tiles' cost is computed inside stage 1 from the
         # BF16 features (pf_stage1_features) instead of per-tile cost slots
         self.fused = bool(fused)
@@ -186,8 +194,48 @@ class LearnedRunner:
                          torch.empty(int(self.F2.shape[1]) * Kp, dtype=torch.bfloat16,
                                      device=self.device), Kp)
 
+    def _net_tables(self):
+        """Pair and job tables of pf_tile_net and pf_tile_scores_batch (the
+        layouts of include/parflex_hip.h) and the logit maps of every pair."""
+        torch = _torch()
+        cj, rj = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().pf_tile_net_geometry(ctypes.byref(cj), ctypes.byref(rj)),
+              "pf_tile_net_geometry")
+        n = len(self.pairs)
+        K = self.K
+        npair = np.zeros((n, 8), np.int64)
+        spair = np.zeros((n, 8), np.int64)
+        jobs, tpair = [], []
+        z_off = 0
+        for p, ((H, W), (L1, L2)) in enumerate(zip(self.HW, self.shapes)):
+            npair[p] = [self.hb[p], H, self.wb[p], W, z_off, p * (2 * K + 1), 0, 0]
+            n1, n2 = self.plan.grids[p]
+            spair[p] = [H, W, z_off, L1, L2, n2, self.plan.pairs[p, 4], 0]
+            for x0 in range(0, W, cj.value):
+                for y0 in range(0, H, rj.value):
+                    jobs.append((p, x0, y0, min(y0 + rj.value, H)))
+            tpair.append(np.full(n1 * n2, p, np.int32))
+            z_off += H * W
+        dev = self.device
+        self._npairs_t = torch.from_numpy(npair).to(dev)
+        self._spairs_t = torch.from_numpy(spair).to(dev)
+        self._jobs_t = torch.from_numpy(np.array(jobs, np.int64).reshape(-1, 4)).to(dev)
+        self._njobs = len(jobs)
+        self._tpair_t = torch.from_numpy(np.concatenate(tpair)).to(dev)
+        self._Z = torch.empty(max(z_off, 1), dtype=torch.float32, device=dev)
+
+    def logit_maps(self, stream=None):
+        """Logit map (H, W) of every pair as host arrays (scoring pass
+        without the tile maxima), from the fused kernel or the per-layer
+        kernels as the runner is built."""
+        torch = _torch()
+        out = torch.empty(int(self.plan.ntiles), dtype=torch.float32, device=self.device)
+        maps = self._score_into(out, stream, keep_maps=True)
+        torch.cuda.synchronize()
+        return [m.cpu().numpy().reshape(h, w) for m, (h, w) in zip(maps, self.HW)]
+
     # ------------------------------------------------------------------
-    def _score_into(self, out, stream=None):
+    def _score_into(self, out, stream=None, keep_maps=False):
         """Tile logits of every pair into the device buffer `out` (the
         pairs' grids concatenated in i-major order).  Pooled features and
         input statistics of all pairs take three launches on `stream`; the
@@ -208,6 +256,25 @@ class LearnedRunner:
         check(lib().pf_input_stats_batch(self.Ap.data_ptr(), self.nbA, self.Bp.data_ptr(),
                                          self.nbB, K, stp.data_ptr(), n, self.stats.data_ptr(),
                                          ms), "pf_input_stats_batch")
+        if self.fused_net:
+            net = self.net
+            bias = [net.layer(i)[32 * 288:] for i in range(4)]
+            check(lib().pf_tile_net(self.Ap.data_ptr(), self.nbA, self.Bp.data_ptr(), self.nbB, K,
+                                    self.stats.data_ptr(), self._npairs_t.data_ptr(), n,
+                                    self._jobs_t.data_ptr(), self._njobs,
+                                    net.Wt[0].data_ptr(), bias[0].data_ptr(),
+                                    net.Wt[1].data_ptr(), bias[1].data_ptr(),
+                                    net.Wt[2].data_ptr(), bias[2].data_ptr(),
+                                    net.Wt[3].data_ptr(), bias[3].data_ptr(),
+                                    self._Z.data_ptr(), ms), "pf_tile_net")
+            check(lib().pf_tile_scores_batch(self._Z.data_ptr(), self._spairs_t.data_ptr(),
+                                             self._tpair_t.data_ptr(), int(self.plan.ntiles),
+                                             POOL, self.L, self.ext, out.data_ptr(), ms),
+                  "pf_tile_scores_batch")
+            if keep_maps:
+                zo = self._spairs_t.cpu().numpy()[:, 2]
+                return [self._Z[int(o):int(o) + h * w] for o, (h, w) in zip(zo, self.HW)]
+            return None
         ns = self.nstreams
         if self._side is None:
             self._side = [torch.cuda.Stream() for _ in range(ns)]
@@ -223,13 +290,21 @@ class LearnedRunner:
             sd.wait_stream(main)
         st0 = self.stats.data_ptr()
         o = 0
+        maps = []
         for p in range(n):
             sd = self._side[p % ns]
             H, W = self.HW[p]
             L1, L2 = self.shapes[p]
+            if keep_maps:
+                bufs = (self._bufs[p % ns][0], self._bufs[p % ns][1],
+                        torch.empty(H * W, dtype=torch.float32, device=self.device))
+            else:
+                bufs = self._bufs[p % ns]
             Z = self.net.forward_pooled(self.Ap, self.nbA, int(self.hb[p]), H, self.Bp, self.nbB,
                                         int(self.wb[p]), W, K, st0 + 4 * p * (2 * K + 1),
-                                        self._bufs[p % ns], sd)
+                                        bufs, sd)
+            if keep_maps:
+                maps.append(Z)
             n1, n2 = -(-(L1 - 1) // (self.L - 1)), -(-(L2 - 1) // (self.L - 1))
             check(lib().pf_tile_scores(Z.data_ptr(), H, W, POOL, self.L, L1, L2, self.ext,
                                        out[o:o + n1 * n2].data_ptr(), sd.cuda_stream),
@@ -237,6 +312,7 @@ class LearnedRunner:
             o += n1 * n2
         for sd in self._side:
             main.wait_stream(sd)
+        return maps if keep_maps else None
 
     def score(self, stream=None):
         """Tile logits of every pair, as one host array per pair (n1, n2).

@@ -102,6 +102,29 @@ def test_batched_scoring_equals_per_pair_scoring(torch_gpu):
         assert np.array_equal(got[p], s.cpu().numpy().reshape(n1, n2)), p
 
 
+def test_fused_network_equals_per_layer_kernels(torch_gpu):
+    """pf_tile_net (the four layers in one launch over every pair) gives the
+    logit maps of the per-layer kernels bit for bit, for maps from 2 x 3
+    pooled cells up to several job rows (128) and job columns (58), and
+    pf_tile_scores_batch the per-pair tile scores."""
+    from parflexdtw_amd.learned import LearnedRunner
+    from parflexdtw_amd.synth import make_pair
+    from parflexdtw_amd.tile_net import TileNet
+    net, meta = TileNet.load(MODEL)
+    sizes = [(60, 75), (100, 40), (1900, 1850), (4130, 4400), (9000, 7777), (33, 33),
+             (5000, 1200)]
+    pairs = [make_pair(n, m, seed=4242 + q)[:2] for q, (n, m) in enumerate(sizes)]
+    kw = dict(ext=3, guard=10, graph=False)
+    fused = LearnedRunner(pairs, 300, STEPS, WEIGHTS, 0.1, net, 0.0, fused_net=True, **kw)
+    layer = LearnedRunner(pairs, 300, STEPS, WEIGHTS, 0.1, net, 0.0, fused_net=False, **kw)
+    a, b = fused.logit_maps(), layer.logit_maps()
+    for p in range(len(pairs)):
+        assert a[p].shape == b[p].shape
+        assert np.array_equal(a[p], b[p]), (p, float(np.abs(a[p] - b[p]).max()))
+    for p, (x, y) in enumerate(zip(fused.score(), layer.score())):
+        assert np.array_equal(x, y), p
+
+
 def test_selection_check_kernel_matches_host_rule(torch_gpu):
     """pf_selection_check flags a pair exactly when a tile within `guard`
     frames of its stitched path is unselected (host rule: learned.path_tiles)."""
