@@ -235,6 +235,23 @@ int pf_stage2_phases(int phases, const double *d_edge_C, const int64_t *d_pairs,
                      int64_t *d_best_i, int64_t *d_chain, int64_t *d_pts, int64_t *d_path,
                      int64_t *d_path_len, void *stream);
 
+/* The PF_S2_PROPAGATE phase with each tile anti-diagonal launched over a
+ * list of its tiles, for a batch where most tiles had no stage 1 (the
+ * learned form): the tiles with stage 1 get a thread per slot, the others
+ * one per edge (only their corner copies take a value), instead of a thread
+ * for every slot of every tile on the diagonal.  d_list: int32 tile-table
+ * indices, every tile once, diagonal by diagonal (the diagonal of tile (i,
+ * j) is i + j); h_diag (host): int64[3] per diagonal d = 0 .. ndiag-1:
+ * {offset of its list in d_list, tiles with stage 1 (listed first), tiles
+ * without}.  Results equal those of PF_S2_PROPAGATE; the caller then runs
+ * pf_stage2_phases(PF_S2_SYNC | PF_S2_SCAN | PF_S2_STITCH).  No reference
+ * counterpart (the reference computes every tile). */
+int pf_stage2_propagate_listed(const double *d_edge_C, const int64_t *d_pairs, int64_t npairs,
+                               const int64_t *d_tiles, int64_t ntiles, int64_t L,
+                               const int32_t *d_list, const int64_t *h_diag, int64_t ndiag,
+                               const double *d_edge_D, const int32_t *d_edge_P, double *d_D_arr,
+                               double *d_L_arr, double *d_olap, uint32_t *d_bits, void *stream);
+
 /* backtrace_and_stitch (Parflex.py:1032-1103) for n requests, each
  * int64[5] = (pair, ci, cj, et, idx) in d_req; stage-1 outputs as for
  * pf_stage2.  Request q writes its path at d_path + q*path_cap*2 and its

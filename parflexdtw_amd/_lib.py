@@ -53,6 +53,8 @@ _SIGNATURES = {
     "pf_stage2_phases": (_int, [_int, _vp, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _i64, _i64,
                                 _vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pf_stage2_propagate_listed": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp, _vp, _i64, _vp,
+                                          _vp, _vp, _vp, _vp, _vp, _vp]),
     "pf_stitch": (_int, [_vp, _i64, _vp, _vp, _i64, _vp, _int, _vp, _vp, _vp, _vp, _i64, _vp,
                          _vp, _vp]),
     "pf_cost": (_int, [_int, _vp, _vp, _i64, _i64, _i64, _vp, _i64, _vp, _vp]),

@@ -117,26 +117,18 @@ __device__ __forceinline__ bool bit_at(const uint32_t *bits, int64_t bitw, int64
   return (bits[(t * 2 + map) * bitw + (pos >> 5)] >> (pos & 31)) & 1u;
 }
 
-// One tile anti-diagonal d of _nb_initialize_chunks_sparse (Parflex.py:631-760)
-// and _nb_dp_fill_chunks_sparse (Parflex.py:763-855), with the valid mask of
+// Slot (e, pos) of tile (i, j) of pair pd in the tile anti-diagonal sweep
+// of _nb_initialize_chunks_sparse (Parflex.py:631-760) and
+// _nb_dp_fill_chunks_sparse (Parflex.py:763-855), with the valid mask of
 // _build_valid_mask (Parflex.py:582-624) evaluated per slot.
-__global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs,
-                          const TileDesc *__restrict__ tiles, int64_t L,
-                          const double *__restrict__ eD, const int32_t *__restrict__ eP,
-                          const double *__restrict__ olap, const uint32_t *__restrict__ bits,
-                          int64_t bitw, double *__restrict__ Darr, double *__restrict__ Larr) {
-  const PairDesc pd = pairs[blockIdx.y];
+__device__ __forceinline__ void diag_slot(const PairDesc &pd, const TileDesc *__restrict__ tiles,
+                                          int64_t L, int64_t i, int64_t j, int e, int64_t pos,
+                                          const double *__restrict__ eD,
+                                          const int32_t *__restrict__ eP,
+                                          const double *__restrict__ olap,
+                                          const uint32_t *__restrict__ bits, int64_t bitw,
+                                          double *__restrict__ Darr, double *__restrict__ Larr) {
   const int64_t n1 = pd.n1, n2 = pd.n2;
-  if (d > n1 + n2 - 2) return;
-  const int64_t i0 = d - (n2 - 1) > 0 ? d - (n2 - 1) : 0;
-  const int64_t i1 = d < n1 - 1 ? d : n1 - 1;
-  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t tq = q / (2 * L);
-  if (i0 + tq > i1) return;
-  const int64_t rem = q - tq * 2 * L;
-  const int e = (int)(rem / L);
-  const int64_t pos = rem - (int64_t)e * L;
-  const int64_t i = i0 + tq, j = d - i;
   const Geo g = tile_geo(pd, L, i, j);
   const int64_t elen = e == 0 ? g.cols : g.rows;
   if (pos >= elen) return;
@@ -260,6 +252,59 @@ __global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs,
     Darr[slot] = eDv;
     Larr[slot] = seg_len;
   }
+}
+
+// One tile anti-diagonal d of every pair: a thread per slot.
+__global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs,
+                          const TileDesc *__restrict__ tiles, int64_t L,
+                          const double *__restrict__ eD, const int32_t *__restrict__ eP,
+                          const double *__restrict__ olap, const uint32_t *__restrict__ bits,
+                          int64_t bitw, double *__restrict__ Darr, double *__restrict__ Larr) {
+  const PairDesc pd = pairs[blockIdx.y];
+  const int64_t n1 = pd.n1, n2 = pd.n2;
+  if (d > n1 + n2 - 2) return;
+  const int64_t i0 = d - (n2 - 1) > 0 ? d - (n2 - 1) : 0;
+  const int64_t i1 = d < n1 - 1 ? d : n1 - 1;
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t tq = q / (2 * L);
+  if (i0 + tq > i1) return;
+  const int64_t rem = q - tq * 2 * L;
+  const int e = (int)(rem / L);
+  const int64_t pos = rem - (int64_t)e * L;
+  const int64_t i = i0 + tq;
+  diag_slot(pd, tiles, L, i, d - i, e, pos, eD, eP, olap, bits, bitw, Darr, Larr);
+}
+
+// The same diagonal from a list of its tiles: `nlive` tiles with stage 1
+// (every slot, a thread each) then tiles without (only their slots at
+// position 0 take a value, the corner copies: a thread per edge).  list
+// holds tile-table indices; the pair and (i, j) come from the table.
+__global__ void k_s2_diag_list(const int32_t *__restrict__ list, int64_t nlive, int64_t npruned,
+                               const PairDesc *__restrict__ pairs,
+                               const TileDesc *__restrict__ tiles, int64_t L,
+                               const double *__restrict__ eD, const int32_t *__restrict__ eP,
+                               const double *__restrict__ olap, const uint32_t *__restrict__ bits,
+                               int64_t bitw, double *__restrict__ Darr, double *__restrict__ Larr) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t k, pos;
+  int e;
+  if (q < nlive * 2 * L) {
+    k = q / (2 * L);
+    const int64_t rem = q - k * 2 * L;
+    e = (int)(rem / L);
+    pos = rem - (int64_t)e * L;
+  } else {
+    const int64_t u = q - nlive * 2 * L;
+    if (u >= npruned * 2) return;
+    k = nlive + (u >> 1);
+    e = (int)(u & 1);
+    pos = 0;
+  }
+  const int64_t t = list[k];
+  const TileDesc td = tiles[t];
+  const PairDesc pd = pairs[td.pair];
+  diag_slot(pd, tiles, L, td.ij >> 32, td.ij & 0xffffffffll, e, pos, eD, eP, olap, bits, bitw,
+            Darr, Larr);
 }
 
 // sync_tile_overlap_edges (Parflex.py:919-969), pass `pass` (0, 1, 2).  The
@@ -793,6 +838,46 @@ extern "C" int pf_stage2_phases(int phases, const double *d_edge_C, const int64_
   if (phases & PF_S2_STITCH)
     return stitch_common(st, 1, nullptr, npairs, d_best_i, pairs, tiles, L, ss, d_Bpacked,
                          d_edge_P, d_chain, d_pts, 1, 0, d_path, d_path_len);
+  return PF_OK;
+}
+
+extern "C" int pf_stage2_propagate_listed(const double *d_edge_C, const int64_t *d_pairs,
+                                          int64_t npairs, const int64_t *d_tiles, int64_t ntiles,
+                                          int64_t L, const int32_t *d_list, const int64_t *h_diag,
+                                          int64_t ndiag, const double *d_edge_D,
+                                          const int32_t *d_edge_P, double *d_D_arr,
+                                          double *d_L_arr, double *d_olap, uint32_t *d_bits,
+                                          void *stream) {
+  PF_REQUIRE(npairs > 0 && ntiles > 0 && L >= 2 && L <= 8192 && ndiag >= 1 && h_diag,
+             "pf_stage2_propagate_listed: bad arguments");
+  PF_REQUIRE(d_edge_C && d_pairs && d_tiles && d_list && d_edge_D && d_edge_P && d_D_arr &&
+                 d_L_arr && d_olap && d_bits,
+             "pf_stage2_propagate_listed: null buffer");
+  PF_REQUIRE(ntiles < (1ll << 31), "pf_stage2_propagate_listed: too many tiles");
+  int64_t listed = 0;
+  for (int64_t d = 0; d < ndiag; ++d) {
+    PF_REQUIRE(h_diag[3 * d] == listed && h_diag[3 * d + 1] >= 0 && h_diag[3 * d + 2] >= 0,
+               "pf_stage2_propagate_listed: diagonal lists must be contiguous and in order");
+    listed += h_diag[3 * d + 1] + h_diag[3 * d + 2];
+  }
+  PF_REQUIRE(listed == ntiles, "pf_stage2_propagate_listed: every tile must be listed once");
+  hipStream_t st = as_stream(stream);
+  const PairDesc *pairs = reinterpret_cast<const PairDesc *>(d_pairs);
+  const TileDesc *tiles = reinterpret_cast<const TileDesc *>(d_tiles);
+  const int64_t bitw = (L + 32) / 32;
+  hipLaunchKernelGGL(k_s2_prep, dim3((unsigned)ntiles), dim3(256), 0, st, tiles, L, d_edge_P,
+                     d_edge_C, d_D_arr, d_L_arr, d_olap, d_bits, bitw);
+  PF_CHECK_HIP(hipGetLastError());
+  for (int64_t d = 0; d < ndiag; ++d) {
+    const int64_t nl = h_diag[3 * d + 1], np = h_diag[3 * d + 2];
+    const int64_t nth = nl * 2 * L + np * 2;
+    if (nth == 0) continue;
+    PF_REQUIRE((nth + 255) / 256 <= (1ll << 31) - 1, "pf_stage2_propagate_listed: diagonal too large");
+    hipLaunchKernelGGL(k_s2_diag_list, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, st,
+                       d_list + h_diag[3 * d], nl, np, pairs, tiles, L, d_edge_D, d_edge_P,
+                       d_olap, d_bits, bitw, d_D_arr, d_L_arr);
+    PF_CHECK_HIP(hipGetLastError());
+  }
   return PF_OK;
 }
 

@@ -177,6 +177,58 @@ def stage2_phases(plan, sw, ws, phases, stream=None):
     check(rc, "pf_stage2_phases")
 
 
+def diagonal_lists(plan, live, cache=None):
+    """Tile lists of the tile anti-diagonals (diagonal of tile (i, j) = i + j)
+    for pf_stage2_propagate_listed: every tile of the plan once, diagonal by
+    diagonal, the tiles with live[t] first.  Returns (list int32, h_diag
+    int64 (max_diag, 3): offset, live count, other count).  `cache` (a dict)
+    keeps the per-plan diagonal order between calls."""
+    if cache is None or "order" not in cache:
+        ij = plan.tiles[:, 7]
+        d = ((ij >> 32) + (ij & 0xFFFFFFFF)).astype(np.int64)
+        c = {"d": d, "order": np.argsort(d, kind="stable"),
+             "count": np.bincount(d, minlength=plan.max_diag)}
+        if cache is not None:
+            cache.update(c)
+        cache = c
+    d, order, nt = cache["d"], cache["order"], cache["count"]
+    live = np.asarray(live, bool)
+    nd = plan.max_diag
+    nl = np.bincount(d[live], minlength=nd)
+    npr = nt - nl
+    off = np.concatenate([[0], np.cumsum(nt)[:-1]])
+    s_ord = live[order]
+    dd = d[order]
+    live_start = np.concatenate([[0], np.cumsum(nl)[:-1]])
+    prun_start = np.concatenate([[0], np.cumsum(npr)[:-1]])
+    live_rank = np.cumsum(s_ord) - 1
+    prun_rank = np.cumsum(~s_ord) - 1
+    pos = np.where(s_ord, off[dd] + live_rank - live_start[dd],
+                   off[dd] + nl[dd] + prun_rank - prun_start[dd])
+    lst = np.empty(len(order), np.int32)
+    lst[pos] = order
+    return lst, np.ascontiguousarray(np.stack([off, nl, npr], axis=1).astype(np.int64))
+
+
+def stage2_listed(plan, sw, ws, live, stream=None, cache=None):
+    """stage2 for a batch whose tiles with live[t] False had no stage 1
+    (tile table column 0 < 0): the propagation launches each tile diagonal
+    over the list of its tiles (pf_stage2_propagate_listed), then the sync,
+    scan and stitch phases.  Returns the device list (keep it alive until
+    the stream has run)."""
+    torch = _torch()
+    lst, h_diag = diagonal_lists(plan, live, cache)
+    d_list = torch.from_numpy(lst).to(ws["tiles"].device)
+    check(lib().pf_stage2_propagate_listed(
+        ws["edge_C"].data_ptr(), ws["pairs"].data_ptr(), plan.npairs, ws["tiles"].data_ptr(),
+        plan.ntiles, plan.L, d_list.data_ptr(), h_diag.ctypes.data, plan.max_diag,
+        ws["edge_D"].data_ptr(), ws["edge_P"].data_ptr(), ws["D_arr"].data_ptr(),
+        ws["L_arr"].data_ptr(), ws["olap"].data_ptr(), ws["bits"].data_ptr(), _stream(stream)),
+        "pf_stage2_propagate_listed")
+    stage2_phases(plan, sw, ws, _lib.PF_S2_SYNC | _lib.PF_S2_SCAN | _lib.PF_S2_STITCH, stream)
+    return d_list
+
+
 def stitch_requests(plan, sw, ws, requests, stream=None):
     """backtrace_and_stitch (Parflex.py:1032-1103) from arbitrary edge slots.
     requests: int64 (n, 5) = (pair, ci, cj, et, idx).  Returns list of (n_i, 2)."""

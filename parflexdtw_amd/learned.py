@@ -478,7 +478,7 @@ class LearnedRunner:
             if events is not None:
                 events[1].record(stream or torch.cuda.current_stream())
             mark("stage1")
-        engine.stage2(None, plan, self.sw, ws, stream)
+        self._stage2_listed(sel, ws, st)
         mark("stage2")
         # selection check on the device: a pruned tile near a path
         d_sel = torch.from_numpy(sel.astype(np.uint8)).to(dev)
@@ -517,6 +517,13 @@ class LearnedRunner:
                  "cells_full": int(plan.cells), "cells_computed": cells,
                  "fallback_pairs": fb, "fallback_reasons": why}
         return res, stats
+
+    def _stage2_listed(self, sel, ws, st):
+        """Stage 2 with the tile diagonals launched over lists of their
+        tiles, selected tiles first; the same results as engine.stage2."""
+        if getattr(self, "_diag_cache", None) is None:
+            self._diag_cache = {}
+        self._d_list = engine.stage2_listed(self.plan, self.sw, ws, sel, cache=self._diag_cache)
 
     def _s1_shape(self, ntiles):
         """Stage-1 launch shape (strips, waves, G) for a large selection: 2

@@ -231,3 +231,48 @@ def test_stage2_reads_no_edges_of_tiles_marked_without_stage1(torch_gpu):
     ra, rb = engine.collect(plan, a), engine.collect(plan, b)
     for p in range(len(pairs)):
         assert np.array_equal(ra.paths[p], rb.paths[p]), p
+
+
+def test_listed_stage2_equals_full_stage2(torch_gpu):
+    """Stage 2 with each tile diagonal launched over a list of its tiles
+    (live tiles first, the others only for their corner copies) gives the
+    outputs of the full stage 2, bit for bit, on a batch with 40% of the
+    tiles marked as having had no stage 1, and with none marked."""
+    from parflexdtw_amd import engine
+    from parflexdtw_amd.engine import BatchRunner
+    torch = torch_gpu
+    pairs = _pairs([(1500, 1700), (900, 1250), (2100, 1900), (300, 2500)], 777)
+    L = 300
+    br = BatchRunner(pairs, L, STEPS, WEIGHTS, beta=0.1, dtype=torch.bfloat16)
+    br.launch()
+    torch.cuda.synchronize()
+    plan, ws = br.plan, br.ws
+    rng = np.random.default_rng(11)
+    for frac in (0.4, 0.0):
+        pruned = rng.random(plan.ntiles) < frac
+        tiles = plan.tiles.copy()
+        tiles[pruned, 0] = -1
+        outs = []
+        for listed in (False, True):
+            w = dict(ws)
+            w["tiles"] = torch.from_numpy(tiles).cuda()
+            for k in ("D_arr", "L_arr", "olap", "bits", "path", "path_len", "best_f", "best_i",
+                      "cand_D", "cand_L"):
+                w[k] = torch.empty_like(ws[k])
+            if listed:
+                engine.stage2_listed(plan, br.sw, w, ~pruned)
+            else:
+                engine.stage2(None, plan, br.sw, w, None)
+            torch.cuda.synchronize()
+            outs.append(w)
+        a, b = outs
+        for k in ("D_arr", "bits", "best_f", "best_i", "path_len", "cand_D", "cand_L"):
+            x, y = a[k], b[k]
+            if x.dtype == torch.float64:
+                x, y = x.view(torch.int64), y.view(torch.int64)
+            assert torch.equal(x, y), (frac, k)
+        fin = torch.isfinite(a["D_arr"])
+        assert torch.equal(a["L_arr"][fin].view(torch.int64), b["L_arr"][fin].view(torch.int64))
+        ra, rb = engine.collect(plan, a), engine.collect(plan, b)
+        for p in range(len(pairs)):
+            assert np.array_equal(ra.paths[p], rb.paths[p]), (frac, p)
