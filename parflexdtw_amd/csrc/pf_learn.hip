@@ -856,7 +856,7 @@ __global__ __launch_bounds__(256) void k_tile_scores_batch(const float *__restri
 constexpr int kNtT = 2;                  // MFMA tiles per layer row
 constexpr int kNtRow = 32 * kNtT;        // pixels a layer computes per row
 constexpr int kNtOut = kNtRow - 6;       // logits per job row
-constexpr int kNtRing = kNtRow + 2;      // pixel slots of a ring row (2 zero)
+constexpr int kNtRing = kNtRow + 2;      // input slots of a row
 constexpr int kNtSeg = 128;              // output rows per job
 constexpr int kNtDepth = 4;              // rows per ring: 3 read, 1 written per step
 constexpr int kNtMaxK = 16;
@@ -868,7 +868,7 @@ struct NetJob {
   int64_t pair, x0, y0, y1;
 };
 
-__global__ __launch_bounds__(256) void k_tile_net(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_tile_net(
     const float *__restrict__ A, int64_t lda, const float *__restrict__ B, int64_t ldb, int K,
     const float *__restrict__ st_all, const NetPair *__restrict__ pairs,
     const NetJob *__restrict__ jobs, const __hip_bfloat16 *__restrict__ W1,
@@ -876,9 +876,21 @@ __global__ __launch_bounds__(256) void k_tile_net(
     const float *__restrict__ b2, const __hip_bfloat16 *__restrict__ W3,
     const float *__restrict__ b3, const __hip_bfloat16 *__restrict__ W4,
     const float *__restrict__ b4, float *__restrict__ Z_all) {
-  // rings of layers 1-3: [layer][slot][pixel][channel], pixel stride kPixPad
-  __shared__ __attribute__((aligned(16))) __hip_bfloat16 ring[3][kNtDepth][kNtRing * kPixPad];
+  // rings of layers 1-3: [layer][slot][pixel][32 channels], kNtRow pixels
+  // a row. The four 16-byte chunks (8 channels) of pixel p are stored in
+  // the order c ^ ((p >> 2) & 3): 16 consecutive pixels of one chunk (one
+  // 16-lane group of a 16-byte read) then fill all 64 banks, from any
+  // starting pixel. Reads of pixels kNtRow and kNtRow + 1 wrap to pixels 0
+  // and 1 of the same row: they feed only the last two pixels of a layer
+  // row (p >= kNtRow - 2), and those reach only layer-4 pixels at or beyond
+  // kNtOut, which are not written (each pixel is a column of its own in the
+  // MFMA). 48 KB of rings + 4.2 KB of column features: three workgroups a
+  // CU.
+  __shared__ __attribute__((aligned(16))) __hip_bfloat16 ring[3][kNtDepth][kNtRow * kCh];
   __shared__ __hip_bfloat16 xin[kNtDepth][kNtRing];   // network input rows (one channel)
+  // colf[k][c]: centred column features of input slot c (column x0 - 4 + c),
+  // zero for k >= K and outside the map; lanes read consecutive slots
+  __shared__ __attribute__((aligned(16))) float colf[kNtMaxK * kNtRing];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -890,31 +902,19 @@ __global__ __launch_bounds__(256) void k_tile_net(
   float *Z = Z_all + pd.z_off;
   const int px = lane & 31, kh = (lane >> 5) * 8;
   const int Hi = (int)H, Wi = (int)W;
+  // element offset of chunk c (channels 8c .. 8c+7) of ring pixel p
+  auto chunk_at = [](const int p, const int c) {
+    const int q = p & (kNtRow - 1);
+    return q * kCh + ((c ^ ((q >> 2) & 3)) << 3);
+  };
 
-  // centred row features of the job's input rows y0 - 4 .. y1 + 3 and
-  // column features of its input slots (slot c: column x0 - 4 + c), zero for
-  // k >= K and outside the map: the operands k_conv3x3_first_pooled stages
-  __shared__ __attribute__((aligned(16))) float rowf[(kNtSeg + 8) * kNtMaxK];
-  __shared__ __attribute__((aligned(16))) float colf[kNtRing * kNtMaxK];
-  __shared__ __attribute__((aligned(16))) float sbias[4][kCh];
-  const int nin = (y1 - y0) + 8;
-  for (int q = tid; q < nin * kNtMaxK; q += 256) {
-    const int r = q / kNtMaxK, k = q - r * kNtMaxK;
-    const int y = y0 - 4 + r;
-    rowf[q] = (k < K && y >= 0 && y < Hi) ? A[k * lda + pd.a_off + y] - st[k] : 0.0f;
-  }
-  // colf[k][c]: lanes read consecutive slots (no bank conflicts)
   for (int q = tid; q < kNtRing * kNtMaxK; q += 256) {
     const int k = q / kNtRing, c = q - k * kNtRing;
     const int x = x0 - 4 + c;
     colf[q] = (k < K && x >= 0 && x < Wi) ? B[k * ldb + pd.b_off + x] - st[K + k] : 0.0f;
   }
-  if (tid < 4 * kCh) {
-    const float *bl = tid < kCh ? b1 : (tid < 2 * kCh ? b2 : (tid < 3 * kCh ? b3 : b4));
-    sbias[tid / kCh][tid % kCh] = bl[tid % kCh];
-  }
-  // zero rings and input rows (pad slots stay zero)
-  for (int q = tid; q < 3 * kNtDepth * kNtRing * kPixPad / 8; q += 256)
+  // zero rings and input rows
+  for (int q = tid; q < 3 * kNtDepth * kNtRow * kCh / 8; q += 256)
     reinterpret_cast<lu32x4_t *>(&ring[0][0][0])[q] = lu32x4_t{0u, 0u, 0u, 0u};
   for (int q = tid; q < kNtDepth * kNtRing; q += 256) (&xin[0][0])[q] = __float2bfloat16(0.0f);
 
@@ -931,19 +931,28 @@ __global__ __launch_bounds__(256) void k_tile_net(
     const int k = kh + i;
     w1[i] = k < 9 ? (__bf16)bf2f(W1[px * kKdim + k * kCh]) : (__bf16)0.0f;
   }
+  lfloatx4_t bb[4];
+  load_bias4<0>(wave == 0 ? b2 : (wave == 1 ? b3 : (wave == 2 ? b4 : b1)), lane, bb);
   const float inv = st[2 * K];
-  // input row y (staged row r = y - y0 + 4) into slot `slot` (wave 3):
-  // bf16(-s * inv), s summed over 16 terms in order, zero outside the map;
-  // lane l writes slots l and l + 64
-  auto input_row = [&](const int y, const int slot) {
+  // lane k (k < 16) of feat_of(y): centred row feature k of input row y,
+  // zero for k >= K and outside the map
+  auto feat_of = [&](const int y) {
+    const int k = lane & (kNtMaxK - 1);
+    return (k < K && y >= 0 && y < Hi) ? A[k * lda + pd.a_off + y] - st[k] : 0.0f;
+  };
+  // input row y into slot `slot` (wave 3) from its features f (lane k
+  // holds feature k): bf16(-s * inv), s summed over 16 terms in order, zero
+  // outside the map; lane l writes slots l and l + 64
+  auto input_row = [&](const int y, const int slot, const float f) {
     const bool yok = y >= 0 && y < Hi;
-    const float *ca = &rowf[(y - y0 + 4) * kNtMaxK];
     const int l1 = lane + 64 < kNtRing ? lane + 64 : 0;
     float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
     for (int k = 0; k < kNtMaxK; ++k) {
-      s0 += ca[k] * colf[k * kNtRing + lane];
-      s1 += ca[k] * colf[k * kNtRing + l1];
+      const float ck = __builtin_bit_cast(
+          float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, f), k));
+      s0 += ck * colf[k * kNtRing + lane];
+      s1 += ck * colf[k * kNtRing + l1];
     }
     const int xa = x0 - 4 + lane, xb = xa + 64;
     xin[slot][lane] = (yok && xa >= 0 && xa < Wi) ? __float2bfloat16(-s0 * inv)
@@ -953,11 +962,12 @@ __global__ __launch_bounds__(256) void k_tile_net(
                                                          : __float2bfloat16(0.0f);
   };
   __syncthreads();
-  lfloatx4_t bb[4];
-  load_bias4<0>(&sbias[wave < 3 ? wave + 1 : 0][0], lane, bb);
+  // features of the next input row (wave 3), loaded a step ahead
+  float f_next = 0.0f;
   if (wave == 3) {
-    input_row(y0 - 4, 0);
-    input_row(y0 - 3, 1);
+    input_row(y0 - 4, 0, feat_of(y0 - 4));
+    input_row(y0 - 3, 1, feat_of(y0 - 3));
+    f_next = feat_of(y0 - 2);
   }
   // one 3x3 layer row from ring `src`: two tiles, k-steps in the order
   // (channel half, dy, dx) of k_conv3x3
@@ -968,14 +978,13 @@ __global__ __launch_bounds__(256) void k_tile_net(
     for (int ch = 0; ch < 2; ++ch)
 #pragma unroll
       for (int dy = 0; dy < 3; ++dy) {
-        const int slot = (s + 1 + dy) % kNtDepth;
-        const __hip_bfloat16 *rowp = src + slot * kNtRing * kPixPad;
+        const __hip_bfloat16 *rowp = src + ((s + 1 + dy) % kNtDepth) * kNtRow * kCh;
 #pragma unroll
         for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
           for (int t = 0; t < kNtT; ++t) {
             const lbf16x8_t a = *reinterpret_cast<const lbf16x8_t *>(
-                &rowp[(t * 32 + px + dx) * kPixPad + ch * 16 + kh]);
+                &rowp[chunk_at(t * 32 + px + dx, ch * 2 + (kh >> 3))]);
             acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wb[(dy * 3 + dx) * 2 + ch], a,
                                                              acc[t], 0, 0, 0);
           }
@@ -999,28 +1008,33 @@ __global__ __launch_bounds__(256) void k_tile_net(
           r[i] = acc[t][g * 4 + i] + bias[g][i];
           r[i] = r[i] > 0.0f ? r[i] : 0.0f;
         }
-        *reinterpret_cast<lu32x2_t *>(&dst[(t * 32 + px) * kPixPad + g * 8 + cq]) =
+        *reinterpret_cast<lu32x2_t *>(&dst[chunk_at(t * 32 + px, g) + cq]) =
             ok ? lu32x2_t{pack_bf16x2(r[0], r[1]), pack_bf16x2(r[2], r[3])}
                : lu32x2_t{0u, 0u};
       }
     }
   };
   auto zero_row = [&](__hip_bfloat16 *dst) {
-    for (int q = lane; q < kNtRow * kPixPad / 8; q += 64)
+    for (int q = lane; q < kNtRow * kCh / 8; q += 64)
       reinterpret_cast<lu32x4_t *>(dst)[q] = lu32x4_t{0u, 0u, 0u, 0u};
   };
+  // Step s: layer l (1..4) computes its row a - 2(l-1), a = y0 - 3 + s;
+  // the rows layer l needs are [y0 - 4 + l, y1 + 4 - l). Every layer reads
+  // rows written in the three previous steps (slots (s+1 .. s+3) % 4) and
+  // writes its row into slot s % 4, which nobody reads in this step: one
+  // barrier per step. Each role runs its own copy of the step loop (wave
+  // uniform, the same barrier count), so each holds only its own registers.
   const int nsteps = (y1 - y0) + 9;
-  for (int s = 0; s < nsteps; ++s) {
-    const int a = y0 - 3 + s;          // first-layer row of this step
-    // layer l (1..4) row a - 2(l-1); needed rows of layer l: [y0 - 4 + l, y1 + 4 - l).
-    // Every layer reads rows written in the three previous steps (slots
-    // (s+1 .. s+3) % 4) and writes its row into slot s % 4, which nobody
-    // reads in this step: one barrier per step.
-    const int slot = s % kNtDepth;
-    if (wave == 3) {
+  if (wave == 3) {
+    for (int s = 0; s < nsteps; ++s) {
+      const int a = y0 - 3 + s;          // first-layer row of this step
       // first layer, row a, from input rows a-1 .. a+1 (input row a+1 now,
       // into slot (s+2) % 4, whose row a-3 the first layer no longer reads)
-      if (a + 1 < y1 + 4) input_row(a + 1, (s + 2) % kNtDepth);
+      if (a + 1 < y1 + 4) {
+        const float f = f_next;
+        f_next = feat_of(a + 2);   // in flight until the next step
+        input_row(a + 1, (s + 2) % kNtDepth, f);
+      }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1039,26 +1053,20 @@ __global__ __launch_bounds__(256) void k_tile_net(
           }
           acc1[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, in, lfloatx16_t{}, 0, 0, 0);
         }
-        store_row(acc1, bb, a, x0 - 3, &ring[0][slot][0]);
+        store_row(acc1, bb, a, x0 - 3, &ring[0][s % kNtDepth][0]);
       }
-    } else {
-      lfloatx16_t acc[kNtT];
-      int row;
-      bool run;
-      if (wave == 0) {
-        row = a - 2;
-        run = row >= y0 - 2 && row < y1 + 2;
-      } else if (wave == 1) {
-        row = a - 4;
-        run = row >= y0 - 1 && row < y1 + 1;
-      } else {
-        row = a - 6;
-        run = row >= y0 && row < y1;
-      }
+      __syncthreads();
+    }
+  } else {
+    // wave w runs layer w + 2: row a - 2(w + 1), rows [y0 - 2 + w, y1 + 2 - w)
+    for (int s = 0; s < nsteps; ++s) {
+      const int row = y0 - 5 - 2 * wave + s;
+      const bool run = row >= y0 - 2 + wave && row < y1 + 2 - wave;
       const bool inmap = row >= 0 && row < Hi;
+      lfloatx16_t acc[kNtT];
       if (run && inmap) layer_row(&ring[wave][0][0], s, acc);
       if (wave < 2) {
-        __hip_bfloat16 *dst = &ring[wave + 1][slot][0];
+        __hip_bfloat16 *dst = &ring[wave + 1][s % kNtDepth][0];
         if (run) {
           if (inmap) store_row(acc, bb, row, x0 - 2 + wave, dst);
           else zero_row(dst);
@@ -1071,8 +1079,8 @@ __global__ __launch_bounds__(256) void k_tile_net(
           if (c < kNtOut && x < Wi) Z[(int64_t)row * W + x] = acc[t][0] + bb[0][0];
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
