@@ -19,6 +19,8 @@
 // (v/4)*8 + (l/32)*4 + v%4 for accumulator v.
 #include "pf_common.h"
 
+#include <type_traits>
+
 namespace pf {
 
 typedef __bf16 lbf16x8_t __attribute__((ext_vector_type(8)));
@@ -1026,14 +1028,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   // uniform, the same barrier count), so each holds only its own registers.
   const int nsteps = (y1 - y0) + 9;
   if (wave == 3) {
-    for (int s = 0; s < nsteps; ++s) {
+    auto step = [&](const int s, auto J) {
+      constexpr int j = decltype(J)::value;
       const int a = y0 - 3 + s;          // first-layer row of this step
       // first layer, row a, from input rows a-1 .. a+1 (input row a+1 now,
       // into slot (s+2) % 4, whose row a-3 the first layer no longer reads)
       if (a + 1 < y1 + 4) {
         const float f = f_next;
         f_next = feat_of(a + 2);   // in flight until the next step
-        input_row(a + 1, (s + 2) % kNtDepth, f);
+        input_row(a + 1, (j + 2) % kNtDepth, f);
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -1048,25 +1051,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             const int k = kh + i;
             const int dy = k / 3, dx = k - (k / 3) * 3;
             const __bf16 xv = reinterpret_cast<const __bf16 *>(&xin[0][0])
-                [k < 9 ? ((s + dy) % kNtDepth) * kNtRing + t * 32 + px + dx : 0];
+                [k < 9 ? ((j + dy) % kNtDepth) * kNtRing + t * 32 + px + dx : 0];
             in[i] = k < 9 ? xv : (__bf16)0.0f;
           }
           acc1[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, in, lfloatx16_t{}, 0, 0, 0);
         }
-        store_row(acc1, bb, a, x0 - 3, &ring[0][s % kNtDepth][0]);
+        store_row(acc1, bb, a, x0 - 3, &ring[0][j][0]);
       }
       __syncthreads();
+    };
+    // steps four at a time: every slot a constant
+    for (int s = 0; s < nsteps; s += 4) {
+      step(s, std::integral_constant<int, 0>{});
+      if (s + 1 < nsteps) step(s + 1, std::integral_constant<int, 1>{});
+      if (s + 2 < nsteps) step(s + 2, std::integral_constant<int, 2>{});
+      if (s + 3 < nsteps) step(s + 3, std::integral_constant<int, 3>{});
     }
   } else {
-    // wave w runs layer w + 2: row a - 2(w + 1), rows [y0 - 2 + w, y1 + 2 - w)
-    for (int s = 0; s < nsteps; ++s) {
+    // wave w runs layer w + 2: row a - 2(w + 1), rows [y0 - 2 + w, y1 + 2 - w).
+    // Steps run four at a time, so every ring slot is a constant offset.
+    const __hip_bfloat16 *src = &ring[wave][0][0];
+    __hip_bfloat16 *dst0 = &ring[wave < 2 ? wave + 1 : 0][0][0];
+    auto step = [&](const int s, auto J) {
+      constexpr int j = decltype(J)::value;
       const int row = y0 - 5 - 2 * wave + s;
       const bool run = row >= y0 - 2 + wave && row < y1 + 2 - wave;
       const bool inmap = row >= 0 && row < Hi;
       lfloatx16_t acc[kNtT];
-      if (run && inmap) layer_row(&ring[wave][0][0], s, acc);
+      if (run && inmap) layer_row(src, j, acc);
       if (wave < 2) {
-        __hip_bfloat16 *dst = &ring[wave + 1][s % kNtDepth][0];
+        __hip_bfloat16 *dst = dst0 + j * kNtRow * kCh;
         if (run) {
           if (inmap) store_row(acc, bb, row, x0 - 2 + wave, dst);
           else zero_row(dst);
@@ -1080,6 +1094,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
       }
       __syncthreads();
+    };
+    for (int s = 0; s < nsteps; s += 4) {
+      step(s, std::integral_constant<int, 0>{});
+      if (s + 1 < nsteps) step(s + 1, std::integral_constant<int, 1>{});
+      if (s + 2 < nsteps) step(s + 2, std::integral_constant<int, 2>{});
+      if (s + 3 < nsteps) step(s + 3, std::integral_constant<int, 3>{});
     }
   }
 }
