@@ -398,6 +398,28 @@ int pf_tile_scores_batch(const float *d_Z, const int64_t *d_pairs, const int32_t
                          int64_t ntiles, int pool, int64_t L, int ext, float *d_score,
                          void *stream);
 
+/* Tile selection of the learned form (this library's tile proposal) on the
+ * device: the rule of tile_net.select_tiles with margin 0, tile t selected
+ * when d_score[t] >= threshold in FP64, over the
+ * plan's tile table d_plan (ntiles x int64[8], as for pf_stage1), one
+ * workgroup.  d_frames (npairs x int64[2]): each pair's first frame in the
+ * concatenated feature arrays; d_diag_off (max_diag + 1): first list slot
+ * of every tile diagonal d = i + j (the plan's tiles per diagonal, summed).
+ * Writes d_sel (uint8 per tile), d_full (the tile table with b_off packed
+ * for the selected tiles after the zero region of ceil(L/16)*L words, 0
+ * and c_off = -1 for the pruned), d_work (the selected tiles in plan order,
+ * c_off / ldc replaced by the frames of the tile's first row and column, as
+ * pf_stage1_features takes them), d_list (every tile once, diagonal by
+ * diagonal, selected first, as pf_stage2_propagate_listed takes it) and
+ * d_counts (int64[3 + 3 max_diag]: selected tiles, packed back-pointer words
+ * with the zero region, selected cells, then h_diag of
+ * pf_stage2_propagate_listed).  d_tmp: workspace of
+ * 3 * ceil(ntiles / 1024) + 2 * max_diag int64.  max_diag <= 4096. */
+int pf_select_tiles(const float *d_score, double threshold, const int64_t *d_plan, int64_t ntiles,
+                    const int64_t *d_frames, int64_t L, const int64_t *d_diag_off,
+                    int64_t max_diag, uint8_t *d_sel, int64_t *d_full, int64_t *d_work,
+                    int32_t *d_list, int64_t *d_counts, int64_t *d_tmp, void *stream);
+
 /* Selection check of the learned form: d_flag[p] |= 1 (int32, cleared by
  * the caller) when a tile holding a cell within `guard` frames of a point of
  * pair p's stitched path (d_path / d_path_len as written by pf_stage2, pair

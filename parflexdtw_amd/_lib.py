@@ -81,6 +81,8 @@ _SIGNATURES = {
                            _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "pf_tile_scores_batch": (_int, [_vp, _vp, _vp, _i64, _int, _i64, _int, _vp, _vp]),
     "pf_selection_check": (_int, [_vp, _i64, _i64, _vp, _vp, _vp, _i64, _i64, _vp, _vp]),
+    "pf_select_tiles": (_int, [_vp, _dbl, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp,
+                               _vp, _vp, _vp, _vp, _vp]),
     "pf_eval_paths": (_int, [_vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _dbl, _vp, _i64,
                              _vp, _vp, _vp]),
     "pf_fill_f64": (_int, [_vp, _dbl, _i64, _vp]),

@@ -124,16 +124,18 @@ class BatchPlan:
     def tile_index(self, p, i, j):
         return int(self.pairs[p, 4]) + i * int(self.pairs[p, 3]) + j
 
-    def allocate(self, device, b_words=None):
+    def allocate(self, device, b_words=None, tiles=None, pairs=None):
         """Device workspace for stage 1 and stage 2 (torch tensors); b_words
-        overrides the size of the packed back-pointer buffer."""
+        overrides the size of the packed back-pointer buffer, and device
+        tables given as tiles / pairs are used instead of uploads of the
+        plan's."""
         import torch
         L, T, P = self.L, self.ntiles, self.npairs
         bitw = (L + 32) // 32
         f64, i64, i32 = torch.float64, torch.int64, torch.int32
         ws = {
-            "tiles": torch.from_numpy(self.tiles).to(device),
-            "pairs": torch.from_numpy(self.pairs).to(device),
+            "tiles": torch.from_numpy(self.tiles).to(device) if tiles is None else tiles,
+            "pairs": torch.from_numpy(self.pairs).to(device) if pairs is None else pairs,
             "Bp": torch.empty(max(self.b_words if b_words is None else b_words, 1), dtype=i32,
                               device=device),
             "edge_D": torch.empty(T * 2 * L, dtype=f64, device=device),

@@ -1104,6 +1104,154 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   }
 }
 
+// Tile selection of the learned form on the device (the rule of
+// tile_net.select_tiles with margin 0): tile t is selected when
+// score[t] >= threshold (compared in FP64). Outputs:
+//  * sel[t] (0/1);
+//  * full: the plan's tile table with b_off remapped (selected tiles packed
+//    back to back in plan order after the zero region of zero_words words,
+//    pruned tiles at offset 0, the zero region) and c_off = -1 for pruned
+//    tiles (no stage 1; stage 2 takes their edges as +inf);
+//  * work: the selected tiles in plan order, c_off / ldc replaced by the
+//    feature frames of the tile's first row and column;
+//  * list: every tile once, diagonal by diagonal (diagonal i + j, first slot
+//    diag_off[d]), the selected tiles of a diagonal first (in any order:
+//    tiles of one diagonal are independent);
+//  * counts: nsel, b_words (zero region included), selected cells, then
+//    (first slot, selected, pruned) of every diagonal.
+// Three launches over blocks of 1024 tiles: counters cleared; per block
+// the selected count, words and cells, and the selected tiles of every
+// diagonal (atomics); then every block its prefix over the blocks before
+// it, a block scan and the writes.
+constexpr int kSelThreads = 1024;
+constexpr int kSelMaxDiag = 4096;
+
+__device__ __forceinline__ int sel_diag(const TileDesc &td) {
+  return (int)((td.ij >> 32) + (td.ij & 0xffffffffll));
+}
+__device__ __forceinline__ int64_t sel_words(const TileDesc &td) {
+  return ((td.R + 15) / 16) * td.W;
+}
+
+__global__ void k_select_clear(int32_t *__restrict__ ctr, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) ctr[i] = 0;
+}
+
+// inclusive scan of v over the 1024 threads of the block (16 waves)
+__device__ __forceinline__ int64_t sel_block_scan(int64_t v, int64_t *wsum, int64_t &total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int64_t u = __shfl_up(v, d);
+    if (lane >= d) v += u;
+  }
+  if (lane == 63) wsum[wv] = v;
+  __syncthreads();
+  int64_t before = 0;
+  total = 0;
+  for (int k = 0; k < kSelThreads / 64; ++k) {
+    const int64_t x = wsum[k];
+    before += k < wv ? x : 0;
+    total += x;
+  }
+  __syncthreads();
+  return v + before;
+}
+
+__global__ __launch_bounds__(kSelThreads) void k_select_count(
+    const float *__restrict__ score, double threshold, const TileDesc *__restrict__ plan,
+    int64_t ntiles, int64_t *__restrict__ bsum, int32_t *__restrict__ nlive) {
+  __shared__ int64_t wsum[kSelThreads / 64];
+  const int64_t t = blockIdx.x * (int64_t)kSelThreads + threadIdx.x;
+  int64_t n = 0, w = 0, c = 0;
+  if (t < ntiles && (double)score[t] >= threshold) {
+    const TileDesc td = plan[t];
+    n = 1;
+    w = sel_words(td);
+    c = td.R * td.W;
+    atomicAdd(&nlive[sel_diag(td)], 1);
+  }
+  int64_t tn, tw, tc;
+  sel_block_scan(n, wsum, tn);
+  sel_block_scan(w, wsum, tw);
+  sel_block_scan(c, wsum, tc);
+  if (threadIdx.x == 0) {
+    bsum[blockIdx.x * 3] = tn;
+    bsum[blockIdx.x * 3 + 1] = tw;
+    bsum[blockIdx.x * 3 + 2] = tc;
+  }
+}
+
+__global__ __launch_bounds__(kSelThreads) void k_select_write(
+    const float *__restrict__ score, double threshold, const TileDesc *__restrict__ plan,
+    int64_t ntiles, const int64_t *__restrict__ frames, int64_t L,
+    const int64_t *__restrict__ diag_off, int64_t max_diag, const int64_t *__restrict__ bsum,
+    const int32_t *__restrict__ nlive, int32_t *__restrict__ lpos, int32_t *__restrict__ ppos,
+    uint8_t *__restrict__ sel, TileDesc *__restrict__ full, TileDesc *__restrict__ work,
+    int32_t *__restrict__ list, int64_t *__restrict__ counts) {
+  __shared__ int64_t wsum[kSelThreads / 64];
+  const int64_t nb = gridDim.x;
+  // selected tiles and words of the blocks before this one, and the totals
+  int64_t bn = 0, bw = 0, an = 0, aw = 0, ac = 0;
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t x = bsum[b * 3], y = bsum[b * 3 + 1];
+    if (b < blockIdx.x) {
+      bn += x;
+      bw += y;
+    }
+    an += x;
+    aw += y;
+    ac += bsum[b * 3 + 2];
+  }
+  const int64_t t = blockIdx.x * (int64_t)kSelThreads + threadIdx.x;
+  const bool in = t < ntiles;
+  TileDesc td{};
+  bool on = false;
+  if (in) {
+    td = plan[t];
+    on = (double)score[t] >= threshold;
+  }
+  const int64_t w = on ? sel_words(td) : 0;
+  int64_t tn, tw;
+  const int64_t pn = sel_block_scan(on ? 1 : 0, wsum, tn) - (on ? 1 : 0);
+  const int64_t pw = sel_block_scan(w, wsum, tw) - w;
+  const int64_t zero_words = ((L + 15) / 16) * L;
+  if (in) {
+    const int d = sel_diag(td);
+    sel[t] = on ? 1 : 0;
+    int32_t slot;
+    if (on) {
+      td.b_off = zero_words + bw + pw;
+      full[t] = td;
+      TileDesc wk = td;
+      const int64_t hop = L - 1, pr = td.pair;
+      wk.c_off = frames[pr * 2] + (td.ij >> 32) * hop;
+      wk.ldc = frames[pr * 2 + 1] + (td.ij & 0xffffffffll) * hop;
+      work[bn + pn] = wk;
+      slot = (int32_t)diag_off[d] + atomicAdd(&lpos[d], 1);
+    } else {
+      td.b_off = 0;
+      td.c_off = -1;
+      full[t] = td;
+      slot = (int32_t)diag_off[d] + nlive[d] + atomicAdd(&ppos[d], 1);
+    }
+    list[slot] = (int32_t)t;
+  }
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) {
+      counts[0] = an;
+      counts[1] = zero_words + aw;
+      counts[2] = ac;
+    }
+    for (int64_t d = threadIdx.x; d < max_diag; d += kSelThreads) {
+      counts[3 + 3 * d] = diag_off[d];
+      counts[3 + 3 * d + 1] = nlive[d];
+      counts[3 + 3 * d + 2] = (diag_off[d + 1] - diag_off[d]) - nlive[d];
+    }
+  }
+}
+
 // Selection check of the learned form: flag[p] = 1 when a tile that holds a
 // cell within `guard` frames (in row and column) of a point of pair p's
 // stitched path is not selected.  The tiles holding frames [x0, x1] are
@@ -1385,6 +1533,36 @@ extern "C" int pf_tile_scores_batch(const float *d_Z, const int64_t *d_pairs,
                      dim3(256), 0, as_stream(stream), d_Z,
                      reinterpret_cast<const ScorePair *>(d_pairs), d_tpair, ntiles, pool, L, ext,
                      d_score);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_select_tiles(const float *d_score, double threshold, const int64_t *d_plan,
+                               int64_t ntiles, const int64_t *d_frames, int64_t L,
+                               const int64_t *d_diag_off, int64_t max_diag, uint8_t *d_sel,
+                               int64_t *d_full, int64_t *d_work, int32_t *d_list,
+                               int64_t *d_counts, int64_t *d_tmp, void *stream) {
+  PF_REQUIRE(d_score && d_plan && d_frames && d_diag_off && d_sel && d_full && d_work &&
+                 d_list && d_counts && d_tmp && ntiles >= 1 && L >= 2 && max_diag >= 1,
+             "pf_select_tiles: bad arguments");
+  PF_REQUIRE(max_diag <= kSelMaxDiag, "pf_select_tiles: more than 4096 tile diagonals");
+  PF_REQUIRE(ntiles < (1ll << 31), "pf_select_tiles: too many tiles");
+  const int64_t nb = (ntiles + kSelThreads - 1) / kSelThreads;
+  PF_REQUIRE(nb <= 65535, "pf_select_tiles: too many tiles for one launch");
+  const hipStream_t st = as_stream(stream);
+  int64_t *bsum = d_tmp;
+  int32_t *ctr = reinterpret_cast<int32_t *>(d_tmp + 3 * nb);   // nlive, lpos, ppos
+  hipLaunchKernelGGL(k_select_clear, dim3((unsigned)((3 * max_diag + 255) / 256)), dim3(256), 0,
+                     st, ctr, 3 * max_diag);
+  PF_CHECK_HIP(hipGetLastError());
+  const TileDesc *plan = reinterpret_cast<const TileDesc *>(d_plan);
+  hipLaunchKernelGGL(k_select_count, dim3((unsigned)nb), dim3(kSelThreads), 0, st, d_score,
+                     threshold, plan, ntiles, bsum, ctr);
+  PF_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_select_write, dim3((unsigned)nb), dim3(kSelThreads), 0, st, d_score,
+                     threshold, plan, ntiles, d_frames, L, d_diag_off, max_diag, bsum, ctr,
+                     ctr + max_diag, ctr + 2 * max_diag, d_sel, reinterpret_cast<TileDesc *>(d_full),
+                     reinterpret_cast<TileDesc *>(d_work), d_list, d_counts);
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
 }

@@ -229,6 +229,21 @@ def stage2_listed(plan, sw, ws, live, stream=None, cache=None):
     return d_list
 
 
+def stage2_listed_device(plan, sw, ws, d_list, h_diag, stream=None):
+    """stage2_listed with the diagonal lists already on the device (d_list,
+    int32, one slot per tile) and their layout on the host (h_diag int64
+    (max_diag, 3): first slot, live count, other count), as pf_select_tiles
+    writes them."""
+    h_diag = np.ascontiguousarray(np.asarray(h_diag, np.int64).reshape(plan.max_diag, 3))
+    check(lib().pf_stage2_propagate_listed(
+        ws["edge_C"].data_ptr(), ws["pairs"].data_ptr(), plan.npairs, ws["tiles"].data_ptr(),
+        plan.ntiles, plan.L, d_list.data_ptr(), h_diag.ctypes.data, plan.max_diag,
+        ws["edge_D"].data_ptr(), ws["edge_P"].data_ptr(), ws["D_arr"].data_ptr(),
+        ws["L_arr"].data_ptr(), ws["olap"].data_ptr(), ws["bits"].data_ptr(), _stream(stream)),
+        "pf_stage2_propagate_listed")
+    stage2_phases(plan, sw, ws, _lib.PF_S2_SYNC | _lib.PF_S2_SCAN | _lib.PF_S2_STITCH, stream)
+
+
 def stitch_requests(plan, sw, ws, requests, stream=None):
     """backtrace_and_stitch (Parflex.py:1032-1103) from arbitrary edge slots.
     requests: int64 (n, 5) = (pair, ci, cj, et, idx).  Returns list of (n_i, 2)."""

@@ -329,6 +329,11 @@ class LearnedRunner:
     def _score_flat(self, stream=None):
         """Tile logits of every pair as one host array, the pairs' grids
         concatenated in i-major order (the plan's tile order)."""
+        return self._score_dev(stream).cpu().numpy()
+
+    def _score_dev(self, stream=None):
+        """Tile logits of every pair on the device (the tensor _score_flat
+        copies to the host)."""
         torch = _torch()
         ntot = int(self.plan.ntiles)
         if self._scores is None:
@@ -348,7 +353,59 @@ class LearnedRunner:
                 torch.cuda.current_stream().wait_stream(s)
                 self._graph = g
             self._graph.replay()
-        return self._scores.cpu().numpy()
+        return self._scores
+
+    def _sel_tables(self):
+        """Device tables and buffers of pf_select_tiles, made once: the plan's
+        tile and pair tables, the pairs' first feature frames, the first list
+        slot of every tile diagonal, and the outputs (selection, tile tables,
+        diagonal lists, counts with their pinned host copy)."""
+        if getattr(self, "_sel", None) is not None:
+            return self._sel
+        torch = _torch()
+        plan, dev = self.plan, self.device
+        ij = plan.tiles[:, 7]
+        dg = (ij >> 32) + (ij & 0xFFFFFFFF)
+        per = np.bincount(dg, minlength=plan.max_diag)
+        diag_off = np.concatenate([[0], np.cumsum(per)]).astype(np.int64)
+        frames = np.stack([self.f1_off, self.f2_off], axis=1).astype(np.int64)
+        n = int(plan.ntiles)
+        ncount = 3 + 3 * plan.max_diag
+        self._sel = {
+            "plan": torch.from_numpy(plan.tiles).to(dev),
+            "pairs": torch.from_numpy(plan.pairs).to(dev),
+            "frames": torch.from_numpy(np.ascontiguousarray(frames)).to(dev),
+            "diag_off": torch.from_numpy(diag_off).to(dev),
+            "sel": torch.empty(n, dtype=torch.uint8, device=dev),
+            "full": torch.empty((n, 8), dtype=torch.int64, device=dev),
+            "work": torch.empty((n, 8), dtype=torch.int64, device=dev),
+            "list": torch.empty(n, dtype=torch.int32, device=dev),
+            "counts": torch.empty(ncount, dtype=torch.int64, device=dev),
+            "tmp": torch.empty(3 * (-(-n // 1024)) + 2 * plan.max_diag, dtype=torch.int64,
+                               device=dev),
+            "h_counts": torch.empty(ncount, dtype=torch.int64, pin_memory=True),
+        }
+        return self._sel
+
+    def _select_device(self, d_scores, stream=None):
+        """select_tiles on the device (pf_select_tiles) for the fused form:
+        the tables stay on the device and only the counts come back.
+        Returns (nsel, b_words, cells, h_diag)."""
+        torch = _torch()
+        t = self._sel_tables()
+        plan = self.plan
+        sobj = stream or torch.cuda.current_stream()
+        check(lib().pf_select_tiles(d_scores.data_ptr(), self.threshold, t["plan"].data_ptr(),
+                                    plan.ntiles, t["frames"].data_ptr(), self.L,
+                                    t["diag_off"].data_ptr(), plan.max_diag, t["sel"].data_ptr(),
+                                    t["full"].data_ptr(), t["work"].data_ptr(),
+                                    t["list"].data_ptr(), t["counts"].data_ptr(),
+                                    t["tmp"].data_ptr(), sobj.cuda_stream), "pf_select_tiles")
+        with torch.cuda.stream(sobj):
+            t["h_counts"].copy_(t["counts"], non_blocking=True)
+        sobj.synchronize()
+        c = t["h_counts"].numpy()
+        return int(c[0]), int(c[1]), int(c[2]), c[3:].reshape(-1, 3).copy()
 
     def _pruned_tables(self, sels):
         """Work table, full table with remapped back-pointer offsets, cost
@@ -417,24 +474,28 @@ class LearnedRunner:
                 timings[name] = timings.get(name, 0.0) + now - clock[0]
                 clock[0] = now
 
-        flat = self._score_flat(stream)
-        mark("score")
-        # select_tiles with no margin on every pair at once: score >= threshold
-        sel, full, work, jobs, nblk, b_words, c_elems, cells = self._pruned_tables(
-            flat >= self.threshold)
-        mark("select")
-        nsel = len(work)
         dev = self.device
         plan = self.plan
-        ws = {
-            "tiles": torch.from_numpy(full).to(dev),
-            "pairs": torch.from_numpy(plan.pairs).to(dev),
-        }
-        # Bp holds the zero region (read by pruned tiles) + the selected tiles
-        base = plan.allocate(dev, b_words=b_words)
-        base["tiles"], base["pairs"] = ws["tiles"], ws["pairs"]
-        base["Bp"][:(L + 15) // 16 * L].zero_()
-        ws = base   # edge arrays need no fill: stage 2 does not read pruned tiles' slots
+        if self.fused:
+            # select_tiles with no margin on every pair at once (score >=
+            # threshold), on the device: tables, lists and selection stay there
+            d_scores = self._score_dev(stream)
+            mark("score")
+            nsel, b_words, cells, h_diag = self._select_device(d_scores, stream)
+            t = self._sel_tables()
+            mark("select")
+            ws = plan.allocate(dev, b_words=b_words, tiles=t["full"], pairs=t["pairs"])
+        else:
+            flat = self._score_flat(stream)
+            mark("score")
+            sel, full, work, jobs, nblk, b_words, c_elems, cells = self._pruned_tables(
+                flat >= self.threshold)
+            mark("select")
+            nsel = len(work)
+            ws = plan.allocate(dev, b_words=b_words, tiles=torch.from_numpy(full).to(dev))
+        # Bp holds the zero region (read by pruned tiles) + the selected tiles;
+        # edge arrays need no fill: stage 2 does not read pruned tiles' slots
+        ws["Bp"][:(L + 15) // 16 * L].zero_()
         mark("workspace")
         if nsel:
             if self.fused:
@@ -451,7 +512,7 @@ class LearnedRunner:
                                           d_jobs.data_ptr(), nsel, nblk, C.data_ptr(),
                                           self.work.data_ptr(), st), "pf_cost_batch")
             mark("cost")
-            d_work = torch.from_numpy(work).to(dev)
+            d_work = self._sel["work"] if self.fused else torch.from_numpy(work).to(dev)
             S, nw, G = self._s1_shape(nsel)
             fl = xc = None
             if S > 1:
@@ -478,10 +539,14 @@ class LearnedRunner:
             if events is not None:
                 events[1].record(stream or torch.cuda.current_stream())
             mark("stage1")
-        self._stage2_listed(sel, ws, st)
+        if self.fused:
+            engine.stage2_listed_device(plan, self.sw, ws, self._sel["list"], h_diag, stream)
+            d_sel = self._sel["sel"]
+        else:
+            self._stage2_listed(sel, ws, st)
+            d_sel = torch.from_numpy(sel.astype(np.uint8)).to(dev)
         mark("stage2")
         # selection check on the device: a pruned tile near a path
-        d_sel = torch.from_numpy(sel.astype(np.uint8)).to(dev)
         flag = torch.zeros(len(self.pairs), dtype=torch.int32, device=dev)
         max_path = max(a + b for a, b in self.shapes) + 8
         check(lib().pf_selection_check(ws["pairs"].data_ptr(), len(self.pairs), max_path,

@@ -125,6 +125,45 @@ def test_fused_network_equals_per_layer_kernels(torch_gpu):
         assert np.array_equal(x, y), p
 
 
+def test_device_selection_equals_host_selection(torch_gpu):
+    """pf_select_tiles (the selection, tile tables and diagonal lists on the
+    device) gives the host selection's tables exactly: the same selection,
+    full and work tables, counts, and per tile diagonal the same selected
+    and pruned tiles (their order inside a diagonal is free), at several
+    thresholds including none and every tile selected, and with the
+    scores' exact threshold ties."""
+    torch = torch_gpu
+    from parflexdtw_amd import engine
+    from parflexdtw_amd.learned import LearnedRunner
+    from parflexdtw_amd.synth import make_pair
+    from parflexdtw_amd.tile_net import TileNet
+    net, _ = TileNet.load(MODEL)
+    sizes = [(3000, 2800), (700, 900), (4100, 2500), (60, 75), (2600, 4000)]
+    pairs = [make_pair(n, m, seed=5151 + q)[:2] for q, (n, m) in enumerate(sizes)]
+    lr = LearnedRunner(pairs, 300, STEPS, WEIGHTS, 0.1, net, 0.0, ext=3, guard=10, graph=False)
+    plan = lr.plan
+    d_scores = lr._score_dev()
+    flat = d_scores.cpu().numpy()
+    ties = float(np.sort(flat)[len(flat) // 3])
+    for thr in (-np.inf, float(np.median(flat)), ties, 1e9, float(flat.min())):
+        lr.threshold = thr
+        nsel, b_words, cells, h_diag = lr._select_device(d_scores)
+        sel, full, work, _, _, bw_h, _, cells_h = lr._pruned_tables(flat >= thr)
+        t = lr._sel
+        assert nsel == len(work) == int(sel.sum()), thr
+        assert b_words == bw_h and cells == cells_h
+        assert np.array_equal(t["sel"].cpu().numpy().astype(bool), sel)
+        assert np.array_equal(t["full"].cpu().numpy(), full)
+        assert np.array_equal(t["work"].cpu().numpy()[:nsel], work)
+        lst_h, hd_h = engine.diagonal_lists(plan, sel)
+        assert np.array_equal(h_diag, hd_h)
+        lst = t["list"].cpu().numpy()
+        for d in range(plan.max_diag):
+            o, nl, npr = (int(v) for v in hd_h[d])
+            assert sorted(lst[o:o + nl]) == sorted(lst_h[o:o + nl])
+            assert sorted(lst[o + nl:o + nl + npr]) == sorted(lst_h[o + nl:o + nl + npr])
+
+
 def test_selection_check_kernel_matches_host_rule(torch_gpu):
     """pf_selection_check flags a pair exactly when a tile within `guard`
     frames of its stitched path is unselected (host rule: learned.path_tiles)."""
