@@ -159,6 +159,13 @@ __device__ __forceinline__ double shr1(double v, double old) {
 
 __device__ __forceinline__ double dabs(double v) { return fabs(v); }
 __device__ __forceinline__ float dabs(float v) { return fabsf(v); }
+// two floats as bf16 (round to nearest even, as __float2bfloat16) in one
+// packed conversion, lo in the low half
+typedef __bf16 s1bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float s1floatx2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t bf16_pair(float lo, float hi) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(s1floatx2_t{lo, hi}, s1bf16x2_t));
+}
 __device__ __forceinline__ double copy_sign(double mag, double sgn) { return copysign(mag, sgn); }
 __device__ __forceinline__ float copy_sign(float mag, float sgn) { return copysignf(mag, sgn); }
 
@@ -292,51 +299,74 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
 #pragma unroll
   for (int j = 0; j < CPL; ++j) cl[j] = (c0 + j < W) ? c0 + j : W - 1;
 
-  // features form: this wave's block of C, 32 rows x (64*CPL) columns
+  // features form: this wave's block of C, 32 rows x (64*CPL) columns at a
+  // row stride of kCS = 64*CPL + 4 elements (an odd multiple of 8 bytes:
+  // 32 lanes writing 8 bytes each to 32 rows hit distinct banks)
   constexpr int kWC = 64 * CPL;
+  constexpr int kCS = kWC + 4;
   const bool has_col0 = (cbeg == 0) && (wave == 0);  // wave-uniform
-  __hip_bfloat16 *cblk = cdyn + (FEAT ? (int64_t)(tid >> 6) * 32 * kWC : 0);
+  __hip_bfloat16 *cblk = cdyn + (FEAT ? (int64_t)(tid >> 6) * 32 * kCS : 0);
+  // Kp == 16: one MFMA per 32 x 32 sub-block, with the column features as
+  // the A operand (the same for every block of the tile: loaded once) and
+  // the row features as B, so that a lane holds 4 adjacent columns of one
+  // row; the row features of the next block are loaded a block ahead.
+  const int f_lr = lane & 31, f_kh = (lane >> 5) * 8;
+  s1bf16x8_t fcol[FEAT ? kWC / 32 : 1], frow_next;
+  auto load_frow = [&](const int b) {
+    const int ar = (32 * b + f_lr < R) ? 32 * b + f_lr : R - 1;
+    return *reinterpret_cast<const s1bf16x8_t *>(Fat + ar * (int64_t)Kp + f_kh);
+  };
+  if constexpr (FEAT) {
+    if (Kp == 16) {
+      const int wc0 = cbeg + (tid >> 6) * kWC;
+#pragma unroll
+      for (int cb = 0; cb < kWC / 32; ++cb) {
+        const int gc = (wc0 + cb * 32 + f_lr < W) ? wc0 + cb * 32 + f_lr : W - 1;
+        fcol[cb] = *reinterpret_cast<const s1bf16x8_t *>(Fbt + gc * (int64_t)16 + f_kh);
+      }
+      frow_next = load_frow(0);
+    }
+  }
   auto fill_block = [&](const int b) {
     if constexpr (FEAT) {
-      const int lr = lane & 31, kh = (lane >> 5) * 8;
-      const int ar = (32 * b + lr < R) ? 32 * b + lr : R - 1;
-      const __hip_bfloat16 *ap = Fat + ar * (int64_t)Kp + kh;
-      const int wc0 = cbeg + (tid >> 6) * kWC;
-      auto store_block = [&](const int cb, const s1floatx16_t &acc) {
-#pragma unroll
-        for (int v = 0; v < 16; ++v)
-          cblk[((v >> 2) * 8 + (lane >> 5) * 4 + (v & 3)) * kWC + cb * 32 + lr] =
-              __float2bfloat16(1.0f - acc[v]);
-      };
+      const int lr = f_lr, kh = f_kh;
       if (Kp == 16) {
-        // one MFMA per 32 x 32 sub-block: every feature load is issued
-        // before the first MFMA, so the block costs one load latency
-        const s1bf16x8_t a = *reinterpret_cast<const s1bf16x8_t *>(ap);
-        s1bf16x8_t bb[kWC / 32];
+        const s1bf16x8_t a = frow_next;
+        if (32 * (b + 1) < R) frow_next = load_frow(b + 1);
 #pragma unroll
         for (int cb = 0; cb < kWC / 32; ++cb) {
-          const int gc = (wc0 + cb * 32 + lr < W) ? wc0 + cb * 32 + lr : W - 1;
-          bb[cb] = *reinterpret_cast<const s1bf16x8_t *>(Fbt + gc * (int64_t)16 + kh);
-        }
+          // D[column][row]: lane holds columns 8g + 4(lane>>5) + 0..3 of
+          // sub-block row lr, g = 0..3
+          const s1floatx16_t acc =
+              __builtin_amdgcn_mfma_f32_32x32x16_bf16(fcol[cb], a, s1floatx16_t{}, 0, 0, 0);
 #pragma unroll
-        for (int cb = 0; cb < kWC / 32; ++cb) {
-          store_block(cb, __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb[cb], s1floatx16_t{}, 0,
-                                                                  0, 0));
+          for (int g = 0; g < 4; ++g) {
+            const uint32_t lo = bf16_pair(1.0f - acc[4 * g], 1.0f - acc[4 * g + 1]);
+            const uint32_t hi = bf16_pair(1.0f - acc[4 * g + 2], 1.0f - acc[4 * g + 3]);
+            *reinterpret_cast<uint2 *>(&cblk[lr * kCS + cb * 32 + g * 8 + (lane >> 5) * 4]) =
+                uint2{lo, hi};
+          }
           // one accumulator live at a time (register budget of the sweep)
           __builtin_amdgcn_sched_barrier(0);
         }
       } else {
+        const int ar = (32 * b + lr < R) ? 32 * b + lr : R - 1;
+        const __hip_bfloat16 *ap = Fat + ar * (int64_t)Kp + kh;
+        const int wc0 = cbeg + (tid >> 6) * kWC;
 #pragma unroll
         for (int cb = 0; cb < kWC / 32; ++cb) {
           const int gc = (wc0 + cb * 32 + lr < W) ? wc0 + cb * 32 + lr : W - 1;
           const __hip_bfloat16 *bp = Fbt + gc * (int64_t)Kp + kh;
           s1floatx16_t acc = s1floatx16_t{};
           for (int kk = 0; kk < Kp; kk += 16) {
-            const s1bf16x8_t a = *reinterpret_cast<const s1bf16x8_t *>(ap + kk);
+            const s1bf16x8_t av = *reinterpret_cast<const s1bf16x8_t *>(ap + kk);
             const s1bf16x8_t bv = *reinterpret_cast<const s1bf16x8_t *>(bp + kk);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
           }
-          store_block(cb, acc);
+#pragma unroll
+          for (int v = 0; v < 16; ++v)
+            cblk[((v >> 2) * 8 + (lane >> 5) * 4 + (v & 3)) * kCS + cb * 32 + lr] =
+                __float2bfloat16(1.0f - acc[v]);
         }
       }
       // the block is read by other lanes of this wave
@@ -346,13 +376,13 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
       // column 0 of the tile (stage 2's start cells): one 32-lane store per
       // block instead of one single-lane store per row
       if (has_col0 && lane < 32 && 32 * b + lane < R)
-        eCc[32 * b + lane] = (double)__bfloat162float(cblk[lane * kWC]);
+        eCc[32 * b + lane] = (double)__bfloat162float(cblk[lane * kCS]);
     }
   };
   // C of row `row`, lane slot j (a column clamped into the tile)
   auto c_at = [&](const int row, const int j) -> DT {
     if constexpr (FEAT) {
-      return (DT)__bfloat162float(cblk[(row & 31) * kWC + lane * CPL + j]);
+      return (DT)__bfloat162float(cblk[(row & 31) * kCS + lane * CPL + j]);
     } else {
       return load_c<DT>(C + (int64_t)row * ldc + cl[j]);
     }
@@ -896,7 +926,7 @@ static void launch_cpl(dim3 grid, dim3 block, hipStream_t st, const CT *C, const
                        g_wait_limit);
 }
 
-// Features form: LDS for the per-wave C blocks is dynamic (nw * 32 * 64*CPL bf16).
+// Features form: LDS for the per-wave C blocks is dynamic (nw * 32 * (64*CPL + 4) bf16).
 template <bool STD, int CPL>
 static int launch_feat_cpl(dim3 grid, dim3 block, size_t lds, hipStream_t st,
                            const TileDesc *tiles, const StepSet &ss, int64_t Lmax, int S, int G,
@@ -1026,7 +1056,7 @@ extern "C" int pf_stage1_features(const void *d_F1b, const void *d_F2b, int Kp,
   const int cpl = pick_cpl(Lmax, strips, waves);
   PF_REQUIRE(cpl == 2 || cpl == 4, "pf_stage1_features: strips*waves*64*4 must cover the tile");
   PF_REQUIRE(waves * 64 <= (cpl == 2 ? 1024 : 512), "pf_stage1_features: too many waves");
-  const size_t lds = (size_t)waves * 32 * 64 * cpl * sizeof(__hip_bfloat16);
+  const size_t lds = (size_t)waves * 32 * (64 * cpl + 4) * sizeof(__hip_bfloat16);
   PF_REQUIRE(lds <= 160 * 1024, "pf_stage1_features: C blocks exceed the LDS");
   if (ntiles == 0) return PF_OK;
   PF_REQUIRE(d_F1b && d_F2b && d_tiles && d_Bpacked && d_edge_D && d_edge_P && d_edge_C,
