@@ -142,7 +142,8 @@ typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 // reads) or, when `blocked`, to Fb[((k/16)*n + i)*16 + k%16]: chunks of 16
 // features, all frames of a chunk together, so that an MFMA fragment of 32
 // frames x 8 features is one contiguous 512-byte run per half wave.  For
-// Kp = 16 the two layouts are the same.
+// Kp = 16 the two layouts are the same.  A thread writes each chunk of 16
+// features of its frame as two 16-byte stores (Fb is 16-byte aligned).
 __global__ void k_cost_prep_bf16(const double *__restrict__ F, int64_t K, int64_t n, int64_t Kp,
                                  __hip_bfloat16 *__restrict__ Fb, int blocked) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -153,9 +154,21 @@ __global__ void k_cost_prep_bf16(const double *__restrict__ F, int64_t K, int64_
     s = s + v * v;
   }
   const double nrm = sqrt(s) + 1e-9;
-  for (int64_t k = 0; k < Kp; ++k) {
-    const int64_t at = blocked ? ((k >> 4) * n + i) * 16 + (k & 15) : i * Kp + k;
-    Fb[at] = __float2bfloat16(k < K ? (float)(F[k * n + i] / nrm) : 0.0f);
+  for (int64_t kc = 0; kc < Kp; kc += 16) {
+    uint32_t w[8];
+#pragma unroll
+    for (int q = 0; q < 16; q += 2) {
+      const int64_t k = kc + q;
+      const __hip_bfloat16 lo = __float2bfloat16(k < K ? (float)(F[k * n + i] / nrm) : 0.0f);
+      const __hip_bfloat16 hi =
+          __float2bfloat16(k + 1 < K ? (float)(F[(k + 1) * n + i] / nrm) : 0.0f);
+      w[q >> 1] = (uint32_t)__builtin_bit_cast(uint16_t, lo) |
+                  ((uint32_t)__builtin_bit_cast(uint16_t, hi) << 16);
+    }
+    const int64_t at = blocked ? ((kc >> 4) * n + i) * 16 : i * Kp + kc;
+    u32x4_t *dst = reinterpret_cast<u32x4_t *>(Fb + at);
+    dst[0] = u32x4_t{w[0], w[1], w[2], w[3]};
+    dst[1] = u32x4_t{w[4], w[5], w[6], w[7]};
   }
 }
 

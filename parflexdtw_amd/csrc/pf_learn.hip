@@ -39,18 +39,26 @@ struct PoolSeg {
 
 // Fp[k*ldp + g] = mean over the frames i of pooled block g (32 frames, the
 // last block of a segment ragged) of F[k*ldf + i] / (sqrt(sum_k F[k*ldf+i]^2)
-// + 1e-9).  One thread per frame (coalesced loads); the 32 frames of a block
-// are 32 consecutive lanes, reduced with shuffles.  The segment of block g is
-// the last one with b_off <= g (binary search over `segs`, or `one` when segs
-// is null), so one launch pools every pair of a batch.
+// + 1e-9).  A workgroup pools 8 consecutive blocks: thread t takes frame t % 32
+// of block t / 32 (coalesced loads), normalises it and writes its scaled
+// features to LDS, 16 at a time; then thread k*8 + b sums the 32 values of
+// feature k of block b in the tree of a 32-lane xor butterfly (offsets 16, 8,
+// 4, 2, 1), so the means equal a shuffle reduction's bit for bit.  The
+// segment of block g is the last one with b_off <= g (binary search over
+// `segs`, or `one` when segs is null), so one launch pools every pair of a
+// batch.
+constexpr int kPoolBlocks = 8;                 // pooled blocks per workgroup
+constexpr int kPoolStride = 36;                // floats per block row in LDS (32 + pad)
 __global__ __launch_bounds__(256) void k_pool_features(const double *__restrict__ F, int64_t K,
                                                        int64_t ldf,
                                                        const PoolSeg *__restrict__ segs,
                                                        int64_t nseg, PoolSeg one,
                                                        float *__restrict__ Fp, int64_t ldp,
                                                        int64_t nb) {
-  const int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t g = gt >> 5;
+  __shared__ __attribute__((aligned(16))) float v[16][kPoolBlocks * kPoolStride];
+  const int t = threadIdx.x;
+  const int64_t g0 = blockIdx.x * (int64_t)kPoolBlocks;
+  const int64_t g = g0 + (t >> 5);
   PoolSeg sg = one;
   if (segs) {
     int64_t lo = 0, hi = nseg - 1;
@@ -61,21 +69,41 @@ __global__ __launch_bounds__(256) void k_pool_features(const double *__restrict_
     sg = segs[lo];
   }
   const int64_t b = g - sg.b_off;
-  const int64_t fi = b * 32 + (gt & 31);
+  const int64_t fi = b * 32 + (t & 31);
   const bool in = g < nb && fi < sg.n;
   const int64_t i = sg.f_off + fi;
   const int64_t cnt = (b * 32 + 32 < sg.n) ? 32 : sg.n - b * 32;
   double s = 0.0;
   for (int64_t k = 0; k < K; ++k) {
-    const double v = in ? F[k * ldf + i] : 0.0;
-    s = s + v * v;
+    const double x = in ? F[k * ldf + i] : 0.0;
+    s = s + x * x;
   }
   const double inv = 1.0 / ((sqrt(s) + 1e-9) * (double)(cnt > 0 ? cnt : 1));
-  for (int64_t k = 0; k < K; ++k) {
-    float v = in ? (float)(F[k * ldf + i] * inv) : 0.0f;
+  const int slot = (t >> 5) * kPoolStride + (t & 31);
+  const int rk = t >> 3, rb = t & 7;   // reducing thread: feature rk of block rb
+  for (int64_t k0 = 0; k0 < K; k0 += 16) {
+    const int kn = (int)((K - k0) < 16 ? (K - k0) : 16);
+    for (int q = 0; q < kn; ++q)
+      v[q][slot] = in ? (float)(F[(k0 + q) * ldf + i] * inv) : 0.0f;
+    __syncthreads();
+    if (rk < kn) {
+      float a[32];
+      const float4 *src = reinterpret_cast<const float4 *>(&v[rk][rb * kPoolStride]);
 #pragma unroll
-    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if ((threadIdx.x & 31) == 0 && g < nb) Fp[k * ldp + g] = v;
+      for (int q = 0; q < 8; ++q) {
+        const float4 w = src[q];
+        a[4 * q] = w.x;
+        a[4 * q + 1] = w.y;
+        a[4 * q + 2] = w.z;
+        a[4 * q + 3] = w.w;
+      }
+#pragma unroll
+      for (int o = 16; o > 0; o >>= 1)
+#pragma unroll
+        for (int j = 0; j < o; ++j) a[j] = a[j] + a[j + o];
+      if (g0 + rb < nb) Fp[(k0 + rk) * ldp + g0 + rb] = a[0];
+    }
+    __syncthreads();
   }
 }
 
@@ -1294,7 +1322,7 @@ extern "C" int pf_pool_features(const double *d_F, int64_t K, int64_t n, int poo
   PF_REQUIRE(pool == 32, "pf_pool_features: the pooling factor is 32");
   const int64_t nb = (n + pool - 1) / pool;
   const PoolSeg one{0, n, 0, 0};
-  hipLaunchKernelGGL(k_pool_features, dim3(nb256(nb * 32)), dim3(256), 0, as_stream(stream), d_F,
+  hipLaunchKernelGGL(k_pool_features, dim3((unsigned)((nb + kPoolBlocks - 1) / kPoolBlocks)), dim3(256), 0, as_stream(stream), d_F,
                      K, n, (const PoolSeg *)nullptr, (int64_t)1, one, d_Fp, nb, nb);
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
@@ -1306,7 +1334,7 @@ extern "C" int pf_pool_features_batch(const double *d_F, int64_t K, int64_t n,
   PF_REQUIRE(d_F && d_seg && d_Fp && K >= 1 && n >= 1 && nseg >= 1 && nb >= 1,
              "pf_pool_features_batch: bad arguments");
   PF_REQUIRE(pool == 32, "pf_pool_features_batch: the pooling factor is 32");
-  hipLaunchKernelGGL(k_pool_features, dim3(nb256(nb * 32)), dim3(256), 0, as_stream(stream), d_F,
+  hipLaunchKernelGGL(k_pool_features, dim3((unsigned)((nb + kPoolBlocks - 1) / kPoolBlocks)), dim3(256), 0, as_stream(stream), d_F,
                      K, n, reinterpret_cast<const PoolSeg *>(d_seg), nseg, PoolSeg{0, 0, 0, 0},
                      d_Fp, nb, nb);
   PF_CHECK_HIP(hipGetLastError());
