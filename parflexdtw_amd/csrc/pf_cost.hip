@@ -121,7 +121,9 @@ __global__ __launch_bounds__(256) void k_cost_f64(const double *__restrict__ F1n
 // ---------------------------------------------------------------------------
 // BF16 MFMA form (precision form): features normalised in FP64, rounded to
 // BF16 and stored frame-major with K padded to a multiple of 16; the product
-// runs on v_mfma_f32_32x32x16_bf16 with FP32 accumulation.  A workgroup of
+// runs on v_mfma_f32_32x32x16_bf16 with FP32 accumulation, as C = 1 + sum_k
+// (-a_k) b_k (accumulators started at 1, row features negated), so the MFMA
+// chain yields C and the epilogue only converts it.  A workgroup of
 // 4 waves computes a 128 x 256 block of C, the waves in a 2 x 2 arrangement,
 // each 64 x 128 as 2 x 4 MFMA blocks of 32 x 32: every A fragment feeds four
 // MFMAs and every B fragment two, and the workgroup reads (128 + 256) rows
@@ -136,6 +138,12 @@ __global__ __launch_bounds__(256) void k_cost_f64(const double *__restrict__ F1n
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx16_t __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+// -x of 8 bf16 values (sign bits flipped; exact)
+__device__ __forceinline__ bf16x8_t neg8(bf16x8_t v) {
+  const u32x4_t u = __builtin_bit_cast(u32x4_t, v) ^ 0x80008000u;
+  return __builtin_bit_cast(bf16x8_t, u);
+}
 
 // Frame i's feature k, bf16(F[k*n + i] / (sqrt(sum_k F[k*n+i]^2) + 1e-9)),
 // zero padded to Kp, goes to Fb[i*Kp + k] (frame-major, the layout stage 1
@@ -238,15 +246,19 @@ __global__ __launch_bounds__(256) void k_cost_mfma_bf16(const __hip_bfloat16 *__
     cb[j] = (uint32_t)((jb.f2_off + (c < M ? c : M - 1)) * 16 + kh);
   }
   const uint32_t sa = (uint32_t)(n1 * 16), sb = (uint32_t)(n2 * 16);
+  // C = 1 - a.b as 1 + sum_k (-a_k) b_k: accumulators start at 1 and the row
+  // features are negated, so the MFMA chain yields C itself
   floatx16_t acc[2][4];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = floatx16_t{};
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 1.0f;
   // Operands of step k+16 are loaded before the MFMAs of step k are issued.
   bf16x8_t a[2], b[4];
 #pragma unroll
-  for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8_t *>(A0 + ra[i]);
+  for (int i = 0; i < 2; ++i) a[i] = neg8(*reinterpret_cast<const bf16x8_t *>(A0 + ra[i]));
 #pragma unroll
   for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8_t *>(B0 + cb[j]);
   const uint32_t nk = (uint32_t)(Kp >> 4);
@@ -257,7 +269,7 @@ __global__ __launch_bounds__(256) void k_cost_mfma_bf16(const __hip_bfloat16 *__
       const __hip_bfloat16 *An = A0 + (kc + 1) * sa;
       const __hip_bfloat16 *Bn = B0 + (kc + 1) * sb;
 #pragma unroll
-      for (int i = 0; i < 2; ++i) an[i] = *reinterpret_cast<const bf16x8_t *>(An + ra[i]);
+      for (int i = 0; i < 2; ++i) an[i] = neg8(*reinterpret_cast<const bf16x8_t *>(An + ra[i]));
 #pragma unroll
       for (int j = 0; j < 4; ++j) bn[j] = *reinterpret_cast<const bf16x8_t *>(Bn + cb[j]);
     }
@@ -286,7 +298,7 @@ __global__ __launch_bounds__(256) void k_cost_mfma_bf16(const __hip_bfloat16 *__
 #pragma unroll
           for (int v = 0; v < 16; ++v)
             tile[i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3)][wc + j * 32 + lr] =
-                cvt_out(1.0f - acc[i][j][v], (OT *)nullptr);
+                cvt_out(acc[i][j][v], (OT *)nullptr);
     }
     __syncthreads();
     // One row per wave per step: 16-byte chunks of E elements when the row's

@@ -322,7 +322,11 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
 #pragma unroll
       for (int cb = 0; cb < kWC / 32; ++cb) {
         const int gc = (wc0 + cb * 32 + f_lr < W) ? wc0 + cb * 32 + f_lr : W - 1;
-        fcol[cb] = *reinterpret_cast<const s1bf16x8_t *>(Fbt + gc * (int64_t)16 + f_kh);
+        // negated: the MFMA started at 1 then yields 1 - a.b (see fill_block)
+        const uint4 u = *reinterpret_cast<const uint4 *>(Fbt + gc * (int64_t)16 + f_kh);
+        fcol[cb] = __builtin_bit_cast(
+            s1bf16x8_t, uint4{u.x ^ 0x80008000u, u.y ^ 0x80008000u, u.z ^ 0x80008000u,
+                              u.w ^ 0x80008000u});
       }
       frow_next = load_frow(0);
     }
@@ -337,12 +341,14 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
         for (int cb = 0; cb < kWC / 32; ++cb) {
           // D[column][row]: lane holds columns 8g + 4(lane>>5) + 0..3 of
           // sub-block row lr, g = 0..3
-          const s1floatx16_t acc =
-              __builtin_amdgcn_mfma_f32_32x32x16_bf16(fcol[cb], a, s1floatx16_t{}, 0, 0, 0);
+          s1floatx16_t one;
+#pragma unroll
+          for (int v = 0; v < 16; ++v) one[v] = 1.0f;
+          const s1floatx16_t acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fcol[cb], a, one, 0, 0, 0);
 #pragma unroll
           for (int g = 0; g < 4; ++g) {
-            const uint32_t lo = bf16_pair(1.0f - acc[4 * g], 1.0f - acc[4 * g + 1]);
-            const uint32_t hi = bf16_pair(1.0f - acc[4 * g + 2], 1.0f - acc[4 * g + 3]);
+            const uint32_t lo = bf16_pair(acc[4 * g], acc[4 * g + 1]);
+            const uint32_t hi = bf16_pair(acc[4 * g + 2], acc[4 * g + 3]);
             *reinterpret_cast<uint2 *>(&cblk[lr * kCS + cb * 32 + g * 8 + (lane >> 5) * 4]) =
                 uint2{lo, hi};
           }
@@ -357,16 +363,21 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
         for (int cb = 0; cb < kWC / 32; ++cb) {
           const int gc = (wc0 + cb * 32 + lr < W) ? wc0 + cb * 32 + lr : W - 1;
           const __hip_bfloat16 *bp = Fbt + gc * (int64_t)Kp + kh;
-          s1floatx16_t acc = s1floatx16_t{};
+          s1floatx16_t acc;
+#pragma unroll
+          for (int v = 0; v < 16; ++v) acc[v] = 1.0f;
           for (int kk = 0; kk < Kp; kk += 16) {
-            const s1bf16x8_t av = *reinterpret_cast<const s1bf16x8_t *>(ap + kk);
+            const uint4 u = *reinterpret_cast<const uint4 *>(ap + kk);
+            const s1bf16x8_t av = __builtin_bit_cast(
+                s1bf16x8_t, uint4{u.x ^ 0x80008000u, u.y ^ 0x80008000u, u.z ^ 0x80008000u,
+                                  u.w ^ 0x80008000u});
             const s1bf16x8_t bv = *reinterpret_cast<const s1bf16x8_t *>(bp + kk);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
           }
 #pragma unroll
           for (int v = 0; v < 16; ++v)
             cblk[((v >> 2) * 8 + (lane >> 5) * 4 + (v & 3)) * kCS + cb * 32 + lr] =
-                __float2bfloat16(1.0f - acc[v]);
+                __float2bfloat16(acc[v]);
         }
       }
       // the block is read by other lanes of this wave
@@ -900,6 +911,387 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Features-form sweep with four columns per lane (k_sweep4): the standard
+// step set, FP32 DP, packed outputs, one strip per tile (W <= 256 * waves),
+// K <= 16.  It computes what k_stage1<2 or 4, bf16, float, false, true, true>
+// computes, bit for bit (the same cell arithmetic and the same C values),
+// with less work per row:
+//  * four columns per lane: the four DPP moves and the per-row bookkeeping
+//    serve four cells instead of two;
+//  * C in 16-row blocks: one v_mfma_f32_16x16x16_bf16 per 16 columns (the
+//    same dot products as the 32x32x16 form, bit for bit), accumulator
+//    started at 1 and the column features negated, so the MFMA yields
+//    1 - a.b and the epilogue only rounds to BF16; a wave's block is 8.25 KB
+//    (two 8-wave workgroups per CU);
+//  * every LDS access is a per-lane base plus an immediate offset: rows run
+//    16 to an unrolled step group, r % 16 known in each step;
+//  * the left wave's last two columns of a row come as one 16-byte record
+//    {D, D, X, X} of an 8-slot ring (column 0 of the ring holds the tile's
+//    missing columns -2, -1 for wave 0); progress is published every 2 rows;
+//  * the tile's last column (D, X) is staged in LDS and written 16 rows at a
+//    time; the first column of C is written from the block fill.
+// Rows 1 and 2 take the boundary rules of k_stage1 in full; from row 3 on
+// only columns 0..2 (lane 0 of wave 0) can have a predecessor on column 0,
+// and their start encodings are constants of the chosen step.
+typedef __bf16 s4bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float s4floatx4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kS4Rows = 16;                // rows per block of C
+constexpr int kS4Cols = 256;               // columns per wave (64 lanes x 4)
+constexpr int kS4Stride = kS4Cols + 8;     // bf16 per block row in LDS (528 B = 16 mod 256)
+constexpr int kS4Ring = 8;                 // rows of wave-to-wave hand-over
+constexpr int kS4MaxWaves = 16;
+constexpr int kS4RingW = kS4MaxWaves + 1;  // ring columns; column w + 1 is written by wave w
+
+struct alignas(16) S4Rec {
+  float da, db, xa, xb;
+};
+
+__device__ __forceinline__ s4bf16x4_t s4_load4(const __hip_bfloat16 *p, uint32_t flip) {
+  uint2 v = *reinterpret_cast<const uint2 *>(p);
+  v.x ^= flip;
+  v.y ^= flip;
+  return __builtin_bit_cast(s4bf16x4_t, v);
+}
+
+template <int MAXT>
+__global__ __launch_bounds__(MAXT) void k_sweep4(const TileDesc *__restrict__ tiles, StepSet ss,
+                                                 int64_t Lmax, uint32_t *__restrict__ Bp,
+                                                 double *__restrict__ eD,
+                                                 int32_t *__restrict__ eP,
+                                                 double *__restrict__ eC,
+                                                 const __hip_bfloat16 *__restrict__ Fa,
+                                                 const __hip_bfloat16 *__restrict__ Fb,
+                                                 int spin_limit) {
+  extern __shared__ __attribute__((aligned(16))) __hip_bfloat16 cdyn[];
+  __shared__ S4Rec ring[kS4Ring][kS4RingW];
+  __shared__ int prog[kS4MaxWaves];
+  __shared__ float2 edge_st[kS4Rows];
+
+  const TileDesc td = wave_uniform(tiles[blockIdx.x]);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // the wave index as a lane value: LDS bases built from it stay in VGPRs,
+  // and per-row offsets fold into the instructions' immediates
+  const int vwave = tid >> 6;
+  const int nwaves = (int)blockDim.x >> 6;
+  const int R = (int)td.R, W = (int)td.W;
+  const int c0 = tid * 4;
+  const int wcol0 = wave * kS4Cols;
+  const float INF = __builtin_huge_valf();
+  const float w0 = (float)ss.w[0], w1 = (float)ss.w[1], w2 = (float)ss.w[2];
+
+  double *const eDc = eD + td.e_off + Lmax;
+  int32_t *const ePc = eP + td.e_off + Lmax;
+  double *const eCc = eC + td.e_off + Lmax;
+  uint32_t *const Bt = Bp + td.b_off;
+  const __hip_bfloat16 *const Fat = Fa + td.c_off * 16;
+  const __hip_bfloat16 *const Fbt = Fb + td.ldc * 16;
+
+  // this wave's block of C and the lane's read / fill positions in it
+  __hip_bfloat16 *const cblk = cdyn + vwave * (kS4Rows * kS4Stride);
+  const __hip_bfloat16 *const crd = cblk + lane * 4;
+  const int fr = lane & 15, fk = (lane >> 4) * 4;
+  __hip_bfloat16 *const cwr = cblk + fr * kS4Stride + fk;
+
+  const bool active = wcol0 < W;  // wave-uniform
+  // column features (negated) of the wave's 16 column blocks, row features
+  // of the next block
+  s4bf16x4_t fcol[kS4Cols / 16], frow_next;
+  auto load_frow = [&](const int b) {
+    const int ar = (16 * b + fr < R) ? 16 * b + fr : R - 1;
+    return s4_load4(Fat + ar * 16 + fk, 0u);
+  };
+  if (active) {
+#pragma unroll
+    for (int cb = 0; cb < kS4Cols / 16; ++cb) {
+      const int gc = (wcol0 + cb * 16 + fr < W) ? wcol0 + cb * 16 + fr : W - 1;
+      fcol[cb] = s4_load4(Fbt + gc * 16 + fk, 0x80008000u);
+    }
+    frow_next = load_frow(0);
+  }
+  // C of rows 16b .. 16b + 15 into the wave's block
+  auto fill = [&](const int b) {
+    const s4bf16x4_t a = frow_next;
+    if (16 * (b + 1) < R) frow_next = load_frow(b + 1);
+    const s4floatx4_t one = {1.0f, 1.0f, 1.0f, 1.0f};
+#pragma unroll
+    for (int cb = 0; cb < kS4Cols / 16; cb += 2) {
+      const s4floatx4_t x = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fcol[cb], a, one, 0, 0, 0);
+      const s4floatx4_t y =
+          __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fcol[cb + 1], a, one, 0, 0, 0);
+      *reinterpret_cast<uint2 *>(cwr + cb * 16) = uint2{bf16_pair(x[0], x[1]), bf16_pair(x[2], x[3])};
+      *reinterpret_cast<uint2 *>(cwr + cb * 16 + 16) =
+          uint2{bf16_pair(y[0], y[1]), bf16_pair(y[2], y[3])};
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // column 0 of the tile (stage 2's start cells), 16 rows at once
+    if (wave == 0 && lane < 16 && 16 * b + lane < R)
+      eCc[16 * b + lane] = (double)__bfloat162float(cdyn[lane * kS4Stride]);
+  };
+  auto unpack = [](const uint2 q, float (&cv)[4]) {
+    cv[0] = __uint_as_float(q.x << 16);
+    cv[1] = __uint_as_float(q.x & 0xffff0000u);
+    cv[2] = __uint_as_float(q.y << 16);
+    cv[3] = __uint_as_float(q.y & 0xffff0000u);
+  };
+
+  // the lane and slot holding the tile's last column (wave-uniform wave)
+  const bool own_last_wave = (W - 1 >= wcol0) && (W - 1 < wcol0 + kS4Cols);
+  const int jl = (W - 1) & 3;
+  const bool own_last = own_last_wave && (lane == ((W - 1 - wcol0) >> 2));
+  // back-pointer words as 16-byte stores when every row of them is aligned
+  const bool vec_ok = ((td.b_off | (int64_t)W) & 3) == 0;
+
+  float D1[4], X1[4], D2[4], X2[4];
+  uint32_t bacc[4];
+  // row r - 2 of column c0 - 1 (the left context of step (2, 1))
+  float LD2b = INF, LX2b = 0.0f;
+
+  // Row 0: boundary cells, P = 0, X = +col.
+  uint2 cq = uint2{0u, 0u};
+  if (active) {
+    fill(0);
+    cq = *reinterpret_cast<const uint2 *>(crd);
+  }
+  {
+    float cv[4];
+    unpack(cq, cv);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = c0 + j;
+      D1[j] = (c < W) ? cv[j] : INF;
+      X1[j] = (float)c;
+      D2[j] = INF;
+      X2[j] = 0.0f;
+      bacc[j] = 0u;
+      if (c < W) {
+        eCc[c - Lmax] = (double)cv[j];
+        if (R == 1) {
+          eDc[c - Lmax] = (double)D1[j];
+          ePc[c - Lmax] = 0;
+          Bt[c] = 0u;
+        }
+      }
+    }
+    if (own_last) edge_st[0] = float2{D1[jl], X1[jl]};
+  }
+  for (int q = tid; q < kS4Ring; q += (int)blockDim.x) ring[q][0] = S4Rec{INF, INF, 0.0f, 0.0f};
+  if (lane == 63) {
+    ring[0][vwave + 1] = S4Rec{D1[2], D1[3], X1[2], X1[3]};
+    prog[wave] = 0;
+  }
+  __syncthreads();
+  if (!active) return;
+
+  const bool has_right = (wave + 1 < nwaves) && (wcol0 + kS4Cols < W);
+  constexpr int kNever = 0x7fffffff;
+  constexpr int kGaveUp = kNever - 1;
+  int seen_left = 0, seen_right = 0;
+  int sync_ok = (wave > 0) ? 1 : kNever;
+  int right_ok = has_right ? kS4Ring - 1 : kNever;
+  auto give_up = [&]() {
+    sync_ok = kGaveUp;
+    right_ok = kGaveUp;
+  };
+  // (D, X) of the last column of rows rb .. rb + n - 1, staged in edge_st
+  auto flush_edges = [&](const int rb, const int n) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < n) {
+      const float2 v = edge_st[lane];
+      eDc[rb + lane] = (double)v.x;
+      ePc[rb + lane] = p_of(v.y, rb + lane + W - 1);
+    }
+  };
+  if (R > 1) cq = *reinterpret_cast<const uint2 *>(crd + kS4Stride);
+
+  // One row; U = r % 16, BORDER: rows 1 and 2 (the full boundary rules).
+  auto step = [&](const int r, auto Uc, auto Bc) {
+    constexpr int U = decltype(Uc)::value;
+    constexpr bool BORDER = decltype(Bc)::value;
+    if (r > sync_ok) {
+      // wait until the wave on the left has handed over row r - 1
+      for (int n = spin_limit;; --n) {
+        if (n <= 0) {
+          give_up();
+          break;
+        }
+        seen_left = __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+            &prog[wave - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+        if (seen_left >= r - 1) break;
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (sync_ok != kGaveUp) sync_ok = seen_left + 1;
+    }
+    const S4Rec rec = ring[(U + kS4Ring - 1) % kS4Ring][vwave];
+    const float LD1a = shr1(D1[2], rec.da), LD1b = shr1(D1[3], rec.db);
+    const float LX1a = shr1(X1[2], rec.xa), LX1b = shr1(X1[3], rec.xb);
+    float cv[4];
+    unpack(cq, cv);
+    float ND[4], NX[4];
+    uint64_t mb1[4], mb2[4];
+    bool T1[4], T2[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float pD0 = j >= 1 ? D1[j - 1] : LD1b, pX0 = j >= 1 ? X1[j - 1] : LX1b;
+      const float pD1 = j >= 2 ? D1[j - 2] : (j == 1 ? LD1b : LD1a);
+      const float pX1 = j >= 2 ? X1[j - 2] : (j == 1 ? LX1b : LX1a);
+      const float pD2 = j >= 1 ? D2[j - 1] : LD2b, pX2 = j >= 1 ? X2[j - 1] : LX2b;
+      const float a0 = __builtin_fmaf(cv[j], w0, pD0), m0 = fabsf(pX0) + 2.0f;
+      const float a1 = __builtin_fmaf(cv[j], w1, pD1), m1 = fabsf(pX1) + 3.0f;
+      const float a2 = __builtin_fmaf(cv[j], w2, pD2), m2 = fabsf(pX2) + 3.0f;
+      const bool t1 = a1 * m0 < a0 * m1;
+      float ba = t1 ? a1 : a0, bm = t1 ? m1 : m0, bx = t1 ? pX1 : pX0;
+      const bool t2 = a2 * bm < ba * m2;
+      mb2[j] = __builtin_amdgcn_ballot_w64(t2);
+      mb1[j] = __builtin_amdgcn_ballot_w64(t1) & ~mb2[j];
+      T1[j] = t1;
+      T2[j] = t2;
+      ND[j] = t2 ? a2 : ba;
+      bm = t2 ? m2 : bm;
+      bx = t2 ? pX2 : bx;
+      NX[j] = copysignf(bm, bx);
+    }
+    if constexpr (BORDER) {
+      // Cells whose predecessor is on row 0 or column 0, and column 0 itself
+      // (FlexDTW.py:1326-1327, :1359-1364).
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = c0 + j;
+        const int code = T2[j] ? 2 : (T1[j] ? 1 : 0);
+        const int bpr = r - ((code == 2) ? 2 : 1), bpc = c - ((code == 1) ? 2 : 1);
+        if (c == 0) {
+          ND[j] = cv[j];
+          NX[j] = (float)r;
+        } else if (bpr == 0) {
+          NX[j] = (float)(r + c - bpc);
+        } else if (bpc == 0) {
+          NX[j] = -(float)(r + c - bpr);
+        }
+      }
+    } else {
+      if (wave == 0 && lane == 0) {
+        // r >= 3: column 0 is C with X = r; column 1 always steps from
+        // column 0 (P = -(r - dr), X = -(1 + dr)); column 2 does with step
+        // (1, 2) (P = -(r - 1), X = -3)
+        ND[0] = cv[0];
+        NX[0] = (float)r;
+        NX[1] = T2[1] ? -3.0f : -2.0f;
+        if (T1[2] && !T2[2]) NX[2] = -3.0f;
+      }
+    }
+    LD2b = LD1b;
+    LX2b = LX1b;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      D2[j] = D1[j];
+      X2[j] = X1[j];
+      D1[j] = ND[j];
+      X1[j] = NX[j];
+      bacc[j] = shift_in(shift_in(bacc[j], mb1[j]), mb2[j]);
+    }
+    if (own_last_wave) {
+      if (own_last) {
+        float2 v;
+        if (jl == 0) v = float2{ND[0], NX[0]};
+        else if (jl == 1) v = float2{ND[1], NX[1]};
+        else if (jl == 2) v = float2{ND[2], NX[2]};
+        else v = float2{ND[3], NX[3]};
+        edge_st[U] = v;
+      }
+    }
+    if constexpr (U == 15) {
+      // back-pointer words of rows r - 15 .. r (column 0: code 0)
+      uint32_t *dst = Bt + (int64_t)(r >> 4) * W;
+      uint32_t wv[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        wv[j] = (c0 + j == 0) ? 0u : __builtin_bitreverse32(bacc[j]);
+        bacc[j] = 0u;
+      }
+      if (vec_ok) {
+        if (c0 < W) *reinterpret_cast<uint4 *>(dst + c0) = uint4{wv[0], wv[1], wv[2], wv[3]};
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (c0 + j < W) dst[c0 + j] = wv[j];
+      }
+      if (own_last_wave) flush_edges(r - 15, 16);
+      if (r + 1 < R) fill((r + 1) >> 4);
+    }
+    cq = *reinterpret_cast<const uint2 *>(crd + ((U + 1) % kS4Rows) * kS4Stride);
+    // Hand row r to the wave on the right once it has read row r - kS4Ring.
+    if (r > right_ok) {
+      for (int n = spin_limit;; --n) {
+        if (n <= 0) {
+          give_up();
+          break;
+        }
+        seen_right = __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+            &prog[wave + 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+        if (seen_right >= r - kS4Ring + 1) break;
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (right_ok != kGaveUp) right_ok = seen_right + kS4Ring - 1;
+    }
+    if (lane == 63) {
+      ring[U % kS4Ring][vwave + 1] = S4Rec{D1[2], D1[3], X1[2], X1[3]};
+      if constexpr (U % 2 == 1)
+        __hip_atomic_store(&prog[vwave], r, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  };
+
+  using T = std::true_type;
+  using F = std::false_type;
+  if (R > 1) step(1, std::integral_constant<int, 1>{}, T{});
+  if (R > 2) step(2, std::integral_constant<int, 2>{}, T{});
+  for (int rb = 3; rb < R; rb += 16) {
+    // rb = 3 (mod 16): row rb + u has r % 16 = (3 + u) % 16
+    __builtin_assume((rb & 15) == 3);
+    step(rb, std::integral_constant<int, 3>{}, F{});
+#define PF_S4_STEP(u)                                                           \
+  if (rb + u < R) step(rb + u, std::integral_constant<int, (3 + u) % 16>{}, F{});
+    PF_S4_STEP(1) PF_S4_STEP(2) PF_S4_STEP(3) PF_S4_STEP(4) PF_S4_STEP(5)
+    PF_S4_STEP(6) PF_S4_STEP(7) PF_S4_STEP(8) PF_S4_STEP(9) PF_S4_STEP(10)
+    PF_S4_STEP(11) PF_S4_STEP(12) PF_S4_STEP(13) PF_S4_STEP(14) PF_S4_STEP(15)
+#undef PF_S4_STEP
+  }
+  // the last row for the wave on the right
+  if (lane == 63) __hip_atomic_store(&prog[wave], R - 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  if (sync_ok == kGaveUp) g_s1_wait_error = 1;
+  // the last-column edges of the last rows not yet written
+  if (((R - 1) & 15) != 15 && own_last_wave) flush_edges((R - 1) & ~15, ((R - 1) & 15) + 1);
+  if (R > 1) {
+    // last back-pointer word when it holds fewer than 16 rows
+    if (((R - 1) & 15) != 15) {
+      uint32_t *dst = Bt + (int64_t)((R - 1) >> 4) * W;
+      const int n = ((R - 1) & 15) + 1;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = c0 + j;
+        const uint32_t wv = c == 0 ? 0u : __builtin_bitreverse32(bacc[j]) >> (32 - 2 * n);
+        if (c < W) dst[c] = wv;
+      }
+    }
+    // last row of the tile (edge 0)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = c0 + j;
+      if (c < W) {
+        eDc[c - Lmax] = (double)D1[j];
+        ePc[c - Lmax] = p_of(X1[j], R - 1 + c);
+      }
+    }
+  }
+}
+
 template <typename CT, typename DT, bool STD, int CPL>
 static void launch_cpl(dim3 grid, dim3 block, hipStream_t st, const CT *C, const TileDesc *tiles,
                        const TileDesc &td0, const StepSet &ss, int64_t Lmax, int S, int G,
@@ -1055,9 +1447,14 @@ extern "C" int pf_stage1_features(const void *d_F1b, const void *d_F2b, int Kp,
   PF_REQUIRE(rows_per_block >= 1 && rows_per_block <= kMaxG, "pf_stage1_features: 1 <= G <= 64");
   const int cpl = pick_cpl(Lmax, strips, waves);
   PF_REQUIRE(cpl == 2 || cpl == 4, "pf_stage1_features: strips*waves*64*4 must cover the tile");
-  PF_REQUIRE(waves * 64 <= (cpl == 2 ? 1024 : 512), "pf_stage1_features: too many waves");
-  const size_t lds = (size_t)waves * 32 * (64 * cpl + 4) * sizeof(__hip_bfloat16);
-  PF_REQUIRE(lds <= 160 * 1024, "pf_stage1_features: C blocks exceed the LDS");
+  const bool std3 = ss.K == 3 && ss.dr[0] == 1 && ss.dc[0] == 1 && ss.dr[1] == 1 &&
+                    ss.dc[1] == 2 && ss.dr[2] == 2 && ss.dc[2] == 1;
+  // one strip of 4 columns per lane, standard steps, K <= 16: k_sweep4
+  const bool sweep4 = std3 && Kp == 16 && strips == 1 && cpl == 4;
+  PF_REQUIRE(sweep4 || waves * 64 <= (cpl == 2 ? 1024 : 512), "pf_stage1_features: too many waves");
+  const size_t lds = sweep4 ? (size_t)waves * kS4Rows * kS4Stride * sizeof(__hip_bfloat16)
+                            : (size_t)waves * 32 * (64 * cpl + 4) * sizeof(__hip_bfloat16);
+  PF_REQUIRE(lds <= 150 * 1024, "pf_stage1_features: C blocks exceed the LDS");
   if (ntiles == 0) return PF_OK;
   PF_REQUIRE(d_F1b && d_F2b && d_tiles && d_Bpacked && d_edge_D && d_edge_P && d_edge_C,
              "pf_stage1_features: null buffer");
@@ -1075,12 +1472,20 @@ extern "C" int pf_stage1_features(const void *d_F1b, const void *d_F2b, int Kp,
     PF_CHECK_HIP(hipGetLastError());
     flags = d_flags + 1;
   }
-  const bool std3 = ss.K == 3 && ss.dr[0] == 1 && ss.dc[0] == 1 && ss.dr[1] == 1 &&
-                    ss.dc[1] == 2 && ss.dr[2] == 2 && ss.dc[2] == 1;
   const TileDesc *tiles = reinterpret_cast<const TileDesc *>(d_tiles);
   const auto *Fa = static_cast<const __hip_bfloat16 *>(d_F1b);
   const auto *Fb = static_cast<const __hip_bfloat16 *>(d_F2b);
   const dim3 grid((unsigned)n_items), block((unsigned)(waves * 64));
+  if (sweep4) {
+    auto kern = k_sweep4<1024>;
+    if (lds > 64 * 1024)
+      PF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, grid, block, lds, st, tiles, ss, Lmax, d_Bpacked, d_edge_D,
+                       d_edge_P, d_edge_C, Fa, Fb, g_wait_limit);
+    PF_CHECK_HIP(hipGetLastError());
+    return PF_OK;
+  }
 #define PF_FEAT(STDV, CPLV)                                                                  \
   launch_feat_cpl<STDV, CPLV>(grid, block, lds, st, tiles, ss, Lmax, strips, rows_per_block, \
                               flags, d_xch, d_Bpacked, d_edge_D, d_edge_P, d_edge_C, Fa, Fb, Kp)

@@ -591,12 +591,14 @@ class LearnedRunner:
         self._d_list = engine.stage2_listed(self.plan, self.sw, ws, sel, cache=self._diag_cache)
 
     def _s1_shape(self, ntiles):
-        """Stage-1 launch shape (strips, waves, G) for a large selection: 2
-        columns per lane, one strip of up to 16 waves per tile (at L = 2000
-        22.7 ms against 25.3 ms for two strips of 8 waves, which exchange
-        their boundary columns every G rows), more strips only for tiles wider
-        than 2048 columns, then with 64-row exchange blocks."""
+        """Stage-1 launch shape (strips, waves, G) for a large selection: one
+        strip of 4 columns per lane per tile up to 4096 columns (the fused
+        form then runs k_sweep4: at L = 2000, 8 waves, 14.9 ms against 22.0
+        ms for one strip of 16 waves with 2 columns per lane), else 2 columns
+        per lane in strips of up to 16 waves with 64-row exchange blocks."""
         L = self.L
+        if ntiles >= 512 and self.fused and L <= 4096:
+            return 1, -(-L // 256), 64
         if ntiles >= 512:
             strips = -(-L // 2048)
             nw = max(1, min(16, -(-L // (128 * strips))))

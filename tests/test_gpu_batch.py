@@ -143,7 +143,14 @@ def test_batched_cost_equals_single(torch_gpu, dtype):
 
 @pytest.mark.parametrize("K,kind,L,config", [(12, "chroma", 300, None), (12, "chroma", 700, (3, 2, 16)),
                                              (20, "relu", 257, (2, 1, 32)), (12, "chroma", 1000, (1, 8, 32)),
-                                             (128, "relu", 520, (2, 2, 7))])
+                                             (128, "relu", 520, (2, 2, 7)),
+                                             # one strip of 4 columns per lane (k_sweep4): 1, 4,
+                                             # 8 and 16 waves (idle waves past narrow tiles)
+                                             (12, "chroma", 232, (1, 1, 8)),
+                                             (12, "chroma", 1000, (1, 4, 32)),
+                                             (12, "chroma", 2000, (1, 8, 64)),
+                                             (12, "chroma", 4000, (1, 16, 64)),
+                                             (16, "relu", 700, (1, 4, 16))])
 def test_fused_stage1_equals_materialised_bf16(torch_gpu, K, kind, L, config):
     """Stage 1 computing C inside the sweep (pf_stage1_features) gives the
     same back-pointers, edges and paths, bit for bit, as stage 1 reading the
@@ -155,7 +162,9 @@ def test_fused_stage1_equals_materialised_bf16(torch_gpu, K, kind, L, config):
     ref = BatchRunner(pairs, L, STEPS, WEIGHTS, beta=0.1, dtype=torch.bfloat16)
     fus = BatchRunner(pairs, L, STEPS, WEIGHTS, beta=0.1, dtype=torch.bfloat16, fused=True)
     for br in (ref, fus):
-        if config is not None:
+        # the materialised form has no 16-wave strips of 4 columns per lane:
+        # it runs its default shape there (the results do not depend on it)
+        if config is not None and (br is fus or config[1] * 64 * 4 <= 2048):
             br.ws["s1_config"] = config
         br.launch()
     torch.cuda.synchronize()
@@ -174,6 +183,57 @@ def test_fused_stage1_equals_materialised_bf16(torch_gpu, K, kind, L, config):
     for p in range(len(pairs)):
         assert np.array_equal(a.paths[p], b.paths[p]), p
     assert fus.C.numel() == 1   # no cost matrix is stored
+
+
+ONEHOT_CONFIGS = [  # (N, M, L, launch shape)
+    (150, 2100, 2048, (1, 8, 64)), (140, 1100, 1000, (1, 4, 16)), (150, 600, 232, (1, 1, 8)),
+    (3, 700, 700, (1, 4, 16)), (18, 300, 300, (1, 2, 16)), (131, 3000, 2100, (1, 16, 64)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(ONEHOT_CONFIGS)))
+def test_fused_stage1_one_hot_features_vs_oracle(torch_gpu, oracle_mod, case):
+    """The features form on one-hot frames: C = 1 - a.b is exactly 0 or 1, and
+    with tiles of at most 150 rows every FP32 sum and cross product of the DP
+    is exact, so the edges and every packed back-pointer equal the FP64
+    oracle's on that C (one strip of 4 columns per lane with 1 to 16 waves;
+    tiles of 3 and 18 rows; ragged last tiles)."""
+    from parflexdtw_amd.engine import BatchRunner
+    torch = torch_gpu
+    N, M, L, cfg = ONEHOT_CONFIGS[case]
+    rng = np.random.default_rng(500 + case)
+    a, b = rng.integers(0, 12, N), rng.integers(0, 12, M)
+    F1 = np.zeros((12, N))
+    F2 = np.zeros((12, M))
+    F1[a, np.arange(N)] = 1.0
+    F2[b, np.arange(M)] = 1.0
+    C = (a[:, None] != b[None, :]).astype(np.float64)
+    br = BatchRunner([(F1, F2)], L, STEPS, WEIGHTS, beta=0.1, dtype=torch.bfloat16, fused=True)
+    br.ws["s1_config"] = cfg
+    br.launch()
+    torch.cuda.synchronize()
+    s1 = oracle_mod.run_stage1(C, L, STEPS, WEIGHTS)
+    n1, n2 = br.plan.grids[0]
+    eD = br.ws["edge_D"].cpu().numpy().reshape(n1, n2, 2, L)
+    eP = br.ws["edge_P"].cpu().numpy().reshape(n1, n2, 2, L)
+    eC = br.ws["edge_C"].cpu().numpy().reshape(n1, n2, 2, L)
+    Bp = br.ws["Bp"].cpu().numpy().view(np.uint32)
+    for i in range(n1):
+        for j in range(n2):
+            t = br.plan.tiles[i * n2 + j]
+            R, W, boff = int(t[2]), int(t[3]), int(t[4])
+            r0, c0 = i * (L - 1), j * (L - 1)
+            assert np.array_equal(eC[i, j, 0, :W], C[r0, c0:c0 + W]), (i, j)
+            assert np.array_equal(eC[i, j, 1, :R], C[r0:r0 + R, c0]), (i, j)
+            assert np.array_equal(eD[i, j, 0, :W], s1["edge_D"][i, j, 0, :W]), (i, j)
+            assert np.array_equal(eD[i, j, 1, :R], s1["edge_D"][i, j, 1, :R]), (i, j)
+            assert np.array_equal(eP[i, j, 0, :W], s1["edge_P"][i, j, 0, :W]), (i, j)
+            assert np.array_equal(eP[i, j, 1, :R], s1["edge_P"][i, j, 1, :R]), (i, j)
+            words = Bp[boff:boff + ((R + 15) // 16) * W].reshape(-1, W)
+            r = np.arange(R)
+            codes = (words[r // 16, :] >> (2 * (r % 16))[:, None].astype(np.uint32)) & 3
+            ob = s1["B"][i, j].reshape(-1)[:R * W].reshape(R, W)
+            assert np.array_equal(codes.astype(np.int8), ob), (i, j)
 
 
 def test_stage2_reads_no_edges_of_tiles_marked_without_stage1(torch_gpu):
