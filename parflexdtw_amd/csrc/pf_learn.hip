@@ -288,7 +288,8 @@ __global__ void k_write_input(const float *__restrict__ A, int64_t H, const floa
 // 8-byte write (the two half-waves together write 16 contiguous bytes of
 // every pixel), without staging the tile through LDS.
 // MODE 0: Y = relu(acc + bias) as bf16 (hidden layer).
-// MODE 1: channel 0 of acc + bias as f32, H*W (the logit map).
+// MODE 1: the logit map (output channel 0 + bias, f32, H*W) from per-row
+//         partial sums (logit_frags).
 // MODE 2: Y = (M > 0) ? acc : 0 as bf16 (gradient through the ReLU of the
 //         layer input M, which is that layer's output activation).
 constexpr int kRowsPerWave = 4;
@@ -378,6 +379,26 @@ __device__ __forceinline__ void flush_block(const __hip_bfloat16 *ys, __hip_bflo
 constexpr int kHaloChunks = (kCvRows + 2) * (kCvCols + 2) * 4;   // 16-byte chunks of a halo
 constexpr int kHaloPerThread = (kHaloChunks + 255) / 256;
 
+// The logit layer (one output channel) as per-row partial sums: for input
+// row y' the MFMA with A[m][k] = W[0][(m*3 + dx)*32 + k] (rows m = dy < 3,
+// zero rows above) gives, in accumulator v = dy of lanes 0..31,
+//   P_y'[dy][x] = sum over (channel half, dx) of W[0][dy, dx, c] X[y'][x + dx - 1][c]
+// (k-steps in the order channel half, dx), and the logit of row y is
+//   ((bias + P_{y-1}[0][x]) + P_y[1][x]) + P_{y+1}[2][x].
+// Each input row is read once per (dx, channel half): 6 MFMAs a row instead
+// of 18 per output row.  Fragment q = ch*3 + dx of lane l: k = ch*16 + kh.
+__device__ __forceinline__ void logit_frags(const __hip_bfloat16 *__restrict__ Wt, int lane,
+                                            lbf16x8_t (&wl)[6]) {
+  const int m = lane & 31, kh = (lane >> 5) * 8;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const int ch = q / 3, dx = q - ch * 3;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      wl[q][i] = m < 3 ? (__bf16)bf2f(Wt[(m * 3 + dx) * kCh + ch * 16 + kh + i]) : (__bf16)0.0f;
+  }
+}
+
 // A workgroup computes `seg` consecutive blocks of one 32-column strip,
 // top to bottom.  The next block's input halo is loaded into registers
 // (clamped addresses, no branch around the loads) while the current block
@@ -461,10 +482,37 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __hip_bfloat16 *__restric
   halo_store((int)ya);
   __syncthreads();
   const int r0 = kRowsPerWave * wave;   // first output row of the wave, halo row r0 + dy
+  lbf16x8_t wl[6];
+  if constexpr (MODE == 1) logit_frags(Wt, lane, wl);
   for (int b = 0; b < nb; ++b) {
     const int64_t y0 = ya + (int64_t)b * kCvRows;
     const bool more = b + 1 < nb;
     if (more) halo_load((int)y0 + kCvRows);
+    if constexpr (MODE == 1) {
+      // partial sums of the wave's 6 halo rows, then its 4 logit rows
+      float p0[kRowsPerWave + 2], p1[kRowsPerWave + 2], p2[kRowsPerWave + 2];
+#pragma unroll
+      for (int xr = 0; xr < kRowsPerWave + 2; ++xr) {
+        lfloatx16_t d = lfloatx16_t{};
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          const int ch = q / 3, dx = q - ch * 3;
+          const lbf16x8_t a = *reinterpret_cast<const lbf16x8_t *>(
+              &xs[((r0 + xr) * HC + px + dx) * kPixPad + ch * 16 + kh]);
+          d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wl[q], a, d, 0, 0, 0);
+        }
+        p0[xr] = d[0];
+        p1[xr] = d[1];
+        p2[xr] = d[2];
+      }
+#pragma unroll
+      for (int t = 0; t < kRowsPerWave; ++t) {
+        const int64_t y = y0 + r0 + t, x = x0 + px;
+        if (lane < 32 && y < H && x < W)
+          static_cast<float *>(Yv)[(uint32_t)(y * W + x)] =
+              ((sbias[0] + p0[t]) + p1[t + 1]) + p2[t + 2];
+      }
+    } else {
     lfloatx16_t acc[kRowsPerWave];
 #pragma unroll
     for (int t = 0; t < kRowsPerWave; ++t) acc[t] = lfloatx16_t{};
@@ -494,6 +542,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(const __hip_bfloat16 *__restric
     if constexpr (MODE != 1) {
       __syncthreads();
       flush_block(xs, static_cast<__hip_bfloat16 *>(Yv), y0, x0, H, W, kCvRows, tid);
+    }
     }
     __syncthreads();   // the halo (or the staged block) has been read
     if (more) {
@@ -876,9 +925,11 @@ __global__ __launch_bounds__(256) void k_tile_scores_batch(const float *__restri
 // barrier per step.  Each layer's columns start one pixel right of those
 // of the layer below (a 3x3 tap reads pixels c .. c+2 of the row below), so
 // of kNtRow pixels 2 per layer are lost and kNtRow - 6 logits remain.
-// Waves 0, 1, 2 run layers 2, 3, 4, each with its layer's weight fragments
-// in registers; wave 3 builds the input row and runs layer 1 (one MFMA per
-// tile), off the path of the others.
+// Waves 0, 1 run layers 2, 3, each with its layer's weight fragments in
+// registers; wave 2 runs layer 4 from per-row partial sums (logit_frags: the
+// newest layer-3 row once, 6 MFMAs a tile instead of 18); wave 3 builds the
+// input row and runs layer 1 (one MFMA per tile), off the path of the others.
+// The roles rotate with the workgroup index.
 // Pixels and rows outside the map are stored as zeros (the zero padding of
 // the per-layer kernels), and every sum runs in the order of those kernels,
 // so the logits equal those of k_conv3x3_first_pooled, k_conv3x3<0> twice
@@ -923,7 +974,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   __shared__ __attribute__((aligned(16))) float colf[kNtMaxK * kNtRing];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // the role of this wave (0, 1, 2: layers 2, 3, 4; 3: input and layer 1),
+  // rotated with the workgroup index so that the heavy roles of the
+  // workgroups sharing a CU fall on different SIMDs
+  const int wave = __builtin_amdgcn_readfirstlane(((tid >> 6) + (int)blockIdx.x) & 3);
   const NetJob jb = jobs[blockIdx.x];
   const NetPair pd = pairs[jb.pair];
   const int64_t H = pd.H, W = pd.W;
@@ -949,7 +1003,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   for (int q = tid; q < kNtDepth * kNtRing; q += 256) (&xin[0][0])[q] = __float2bfloat16(0.0f);
 
   // the layer this wave runs (waves 0, 1, 2: layers 2, 3, 4; wave 3 layer 1)
-  const __hip_bfloat16 *Wl = wave == 0 ? W2 : (wave == 1 ? W3 : W4);
+  const __hip_bfloat16 *Wl = wave == 0 ? W2 : W3;   // layers 2, 3 (layer 4: logit_frags)
   lbf16x8_t wb[18];
 #pragma unroll
   for (int q = 0; q < 18; ++q)
@@ -1095,11 +1149,68 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       if (s + 2 < nsteps) step(s + 2, std::integral_constant<int, 2>{});
       if (s + 3 < nsteps) step(s + 3, std::integral_constant<int, 3>{});
     }
+  } else if (wave == 2) {
+    // layer 4, the logits, from per-row partial sums (logit_frags): each
+    // step takes the layer-3 row written in the step before, row yp = a - 5,
+    // and completes logit row yp - 1 = a - 6 from the partials of rows
+    // yp - 2, yp - 1 and yp (kept per tile in registers)
+    lbf16x8_t wl[6];
+    logit_frags(W4, lane, wl);
+    const float bias0 = bb[0][0];
+    float q02[kNtT], q01[kNtT], q11[kNtT];
+#pragma unroll
+    for (int t = 0; t < kNtT; ++t) q02[t] = q01[t] = q11[t] = 0.0f;
+    const __hip_bfloat16 *src = &ring[2][0][0];
+    auto step = [&](const int s, auto J) {
+      constexpr int j = decltype(J)::value;
+      const int yp = y0 - 8 + s;
+      const int row = yp - 1;
+      if (yp >= y0 - 1 && yp < y1 + 1) {
+        const __hip_bfloat16 *rowp = src + ((j + 3) % kNtDepth) * kNtRow * kCh;
+        float p0[kNtT], p1[kNtT], p2[kNtT];
+#pragma unroll
+        for (int t = 0; t < kNtT; ++t) {
+          lfloatx16_t d = lfloatx16_t{};
+#pragma unroll
+          for (int q = 0; q < 6; ++q) {
+            const int ch = q / 3, dx = q - ch * 3;
+            const lbf16x8_t a = *reinterpret_cast<const lbf16x8_t *>(
+                &rowp[chunk_at(t * 32 + px + dx, ch * 2 + (kh >> 3))]);
+            d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wl[q], a, d, 0, 0, 0);
+          }
+          p0[t] = d[0];
+          p1[t] = d[1];
+          p2[t] = d[2];
+        }
+        if (row >= y0 && row < y1 && row >= 0 && row < Hi && lane < 32) {
+          // logits of row `row`, columns x0 .. x0 + kNtOut - 1
+#pragma unroll
+          for (int t = 0; t < kNtT; ++t) {
+            const int c = t * 32 + px, x = x0 + c;
+            if (c < kNtOut && x < Wi)
+              Z[(int64_t)row * W + x] = ((bias0 + q02[t]) + q11[t]) + p2[t];
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < kNtT; ++t) {
+          q02[t] = q01[t];
+          q01[t] = p0[t];
+          q11[t] = p1[t];
+        }
+      }
+      __syncthreads();
+    };
+    for (int s = 0; s < nsteps; s += 4) {
+      step(s, std::integral_constant<int, 0>{});
+      if (s + 1 < nsteps) step(s + 1, std::integral_constant<int, 1>{});
+      if (s + 2 < nsteps) step(s + 2, std::integral_constant<int, 2>{});
+      if (s + 3 < nsteps) step(s + 3, std::integral_constant<int, 3>{});
+    }
   } else {
     // wave w runs layer w + 2: row a - 2(w + 1), rows [y0 - 2 + w, y1 + 2 - w).
     // Steps run four at a time, so every ring slot is a constant offset.
     const __hip_bfloat16 *src = &ring[wave][0][0];
-    __hip_bfloat16 *dst0 = &ring[wave < 2 ? wave + 1 : 0][0][0];
+    __hip_bfloat16 *dst0 = &ring[wave + 1][0][0];
     auto step = [&](const int s, auto J) {
       constexpr int j = decltype(J)::value;
       const int row = y0 - 5 - 2 * wave + s;
@@ -1107,19 +1218,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       const bool inmap = row >= 0 && row < Hi;
       lfloatx16_t acc[kNtT];
       if (run && inmap) layer_row(src, j, acc);
-      if (wave < 2) {
-        __hip_bfloat16 *dst = dst0 + j * kNtRow * kCh;
-        if (run) {
-          if (inmap) store_row(acc, bb, row, x0 - 2 + wave, dst);
-          else zero_row(dst);
-        }
-      } else if (run && inmap && lane < 32) {
-        // logits of row `row`, columns x0 .. x0 + kNtOut - 1
-#pragma unroll
-        for (int t = 0; t < kNtT; ++t) {
-          const int c = t * 32 + px, x = x0 + c;
-          if (c < kNtOut && x < Wi) Z[(int64_t)row * W + x] = acc[t][0] + bb[0][0];
-        }
+      __hip_bfloat16 *dst = dst0 + j * kNtRow * kCh;
+      if (run) {
+        if (inmap) store_row(acc, bb, row, x0 - 2 + wave, dst);
+        else zero_row(dst);
       }
       __syncthreads();
     };
