@@ -58,14 +58,15 @@ METRIC = "DP GCUPS + pairs/s, batched/learned vs plain GPU at >=99.5% path agree
 TRAFFIC_BYTES = {("chroma_batch256_N3000-20000", 500, "materialised"): (83.97 + 12.84) * 1e9,
                  ("chroma_learned128_N20000-60000_L2000", 2000, "materialised"):
                      (46.70 + 6.21) * 1e9,
-                 ("chroma_learned128_N20000-60000_L2000", 2000, "fused"): (1.08 + 6.19) * 1e9}
+                 ("chroma_learned128_N20000-60000_L2000", 2000, "fused"): (0.740 + 6.148) * 1e9}
 # VALU wave instructions per computed cell of the stage-1 kernel, measured
-# with rocprofv3 PMC (SQ_INSTS_VALU / cells; profiles/pmc_valu_*.md): the
-# kernel's real ceiling is VALU issue, 1024 SIMDs x one wave64 VALU
-# instruction per 4 cycles at 2.4 GHz.
+# with rocprofv3 PMC (SQ_INSTS_VALU / cells; profiles/pmc_valu_*.md).  The
+# bench states the kernel's VALU instruction rate per SIMD and cycle (at the
+# 2.4 GHz engine clock, the clock the PMC cycle count shows for this kernel)
+# beside the rate a dense FP32 FMA loop (pf_valu_probe) reaches in the same
+# run.
 VALU_INSTS_PER_CELL = {("chroma_learned128_N20000-60000_L2000", 2000, "fused"):
-                           1.1678e10 / 2.2747e10}
-VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
+                           1.0139e10 / 2.2747e10}
 HBM_PEAK_GBS = 8000.0
 STEPS, WEIGHTS, BETA = [(1, 1), (1, 2), (2, 1)], [2, 3, 3], 0.1
 WORKLOADS = ["chroma_learned128_N20000-60000_L2000", "chroma_batch256_N3000-20000",
@@ -162,6 +163,35 @@ def _cpu_baseline(args, Cs, Cs1, L, what, per_unit, unit, expect=None):
             "sample": f"{len(Cs1)} pair(s), {cells1 / 1e9:.3f} Gcells, one thread, median of "
                       f"{trials} timed passes after 1 warm-up pass"}}
     return out, exact
+
+
+def _feat_kernel(cfg, L, K=12):
+    """Name of the features-form stage-1 kernel a launch shape runs."""
+    S, nw = int(cfg[0]), int(cfg[1])
+    cpl = 2 if S * nw * 128 >= L else 4
+    if S == 1 and cpl == 4 and K <= 16:
+        return "k_sweep4<1024>"
+    return f"k_stage1<{cpl},__hip_bfloat16,float,false,true,true,{1024 if cpl == 2 else 512}>"
+
+
+def _valu_probe_rate(torch):
+    """VALU wave instructions per second of pf_valu_probe: 8 workgroups of
+    4 waves per CU (256 CUs), each wave 16 FMAs x iters, timed with HIP
+    events on the launch stream (best of 3)."""
+    from parflexdtw_amd._lib import check, lib
+    blocks, iters = 2048, 20000
+    out = torch.empty(blocks * 256, dtype=torch.float32, device="cuda")
+    st = torch.cuda.current_stream()
+    best = None
+    for _ in range(4):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        check(lib().pf_valu_probe(out.data_ptr(), blocks, iters, st.cuda_stream), "pf_valu_probe")
+        e1.record(st)
+        st.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    return blocks * 4 * 16 * iters / (best * 1e-3)
 
 
 def _roofline(cells, s1_ms, bytes_per_cell, kernel):
@@ -311,19 +341,27 @@ def bench_learned(args, torch):
     out["evaluation"] = _evaluation(torch, gts, {
         "learned": (res.paths, gts), "plain_fp64": ([wp for _, wp in plain], gts)})
     fused = args.cost == "fused"
-    out["roofline"] = _roofline(
-        st["cells_computed"], s1_ms, 0.25 if fused else 2.25,
-        f"k_stage1<2,__hip_bfloat16,float,false,true,{'true' if fused else 'false'},1024>") if s1_ms else None
+    kname = ("k_sweep4<1024>" if fused else
+             "k_stage1<2,__hip_bfloat16,float,false,true,false,1024>")
+    out["roofline"] = _roofline(st["cells_computed"], s1_ms, 0.25 if fused else 2.25,
+                                kname) if s1_ms else None
     key = (args.workload, L, args.cost)
     if out["roofline"] and P == 128 and key in TRAFFIC_BYTES:
         out["roofline"]["traffic"] = TRAFFIC_BYTES[key]
     if out["roofline"] and P == 128 and key in VALU_INSTS_PER_CELL:
         ipc = VALU_INSTS_PER_CELL[key]
+        rate = ipc * st["cells_computed"] / (s1_ms * 1e-3)
+        probe = _valu_probe_rate(torch)
+        per_cyc = 1024 * 2.4e9
         out["roofline"]["valu_issue"] = {
-            "insts_per_cell": round(ipc, 4),
-            "frac": round(ipc * st["cells_computed"] / (s1_ms * 1e-3) / VALU_ISSUE_PER_S, 4),
-            "note": "fraction of the VALU issue rate (the kernel's ceiling; insts per cell "
-                    "from a PMC run of this workload)"}
+            "insts_per_cell": round(ipc, 4), "insts_per_s": round(rate, -6),
+            "per_simd_cycle": round(rate / per_cyc, 4),
+            "fma_probe_per_simd_cycle": round(probe / per_cyc, 4),
+            "note": "VALU wave instructions of the stage-1 kernel (per cell from a PMC run of "
+                    "this workload) per SIMD and 2.4 GHz cycle, and those of a dense FP32 FMA "
+                    "loop (pf_valu_probe) in the same run; the DP issues more than the FMA loop "
+                    "(which runs at a power-limited clock), so it is bound by its row "
+                    "dependency chain rather than by issue alone"}
     if not args.no_cpu:
         # The three smallest of the first 8 held-out pairs (about 2 Gcells;
         # multi-thread) and the smallest of them (single thread), each checked
@@ -444,7 +482,8 @@ def bench_batch(args, torch):
     fused = args.cost == "fused"
     out["roofline"] = _roofline(
         cells, s1_ms, 0.25 if fused else 2.25,
-        f"k_stage1<2,__hip_bfloat16,float,false,true,{'true' if fused else 'false'},1024>")
+        _feat_kernel(br.ws["s1_config"], L) if fused else
+        "k_stage1<2,__hip_bfloat16,float,false,true,false,1024>")
     if args.pairs == 256 and (args.workload, L, args.cost) in TRAFFIC_BYTES:
         out["roofline"]["traffic"] = TRAFFIC_BYTES[(args.workload, L, args.cost)]
     out["evaluation"] = _evaluation(torch, gts, {
@@ -632,8 +671,7 @@ def bench_precision(args, torch):
                        "frames_within_2": round(agree, 6), "reference": "plain FP64 path"}
     if fused:
         out["stage1_shape"] = list(cfg)
-        out["roofline"] = _roofline(cells, s1_ms, 0.25,
-                                    "k_stage1<CPL,__hip_bfloat16,float,false,true,true,MAXT>")
+        out["roofline"] = _roofline(cells, s1_ms, 0.25, _feat_kernel(cfg, L, K=128))
     else:
         out["roofline"] = _roofline(cells, s1_ms, 2.25,
                                     "k_stage1<2,__hip_bfloat16,float,false,true,false,1024>")

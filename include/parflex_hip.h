@@ -464,6 +464,14 @@ int pf_eval_paths(const int64_t *d_path, int64_t stride, const int64_t *d_offs,
                   const double *d_gt_x, const double *d_gt_y, double hop, const double *d_tol,
                   int64_t ntol, double *d_err, uint64_t *d_hist, void *stream);
 
+/* Measurement helper (no counterpart in the reference): `blocks` workgroups
+ * of 256 threads, each lane running `iters` rounds of 16 independent FP32
+ * fused multiply-adds, 16 * iters VALU wave instructions per wave; d_out
+ * receives blocks * 256 floats.  bench.py times it to state the stage-1
+ * kernel's VALU instruction rate as a fraction of the rate this dense FMA
+ * loop reaches on the same GPU. */
+int pf_valu_probe(float *d_out, int blocks, int iters, void *stream);
+
 /* Fill helpers used instead of hipMemsetAsync (safe under graph capture). */
 int pf_fill_f64(double *d_p, double v, int64_t n, void *stream);
 int pf_fill_i64(int64_t *d_p, int64_t v, int64_t n, void *stream);

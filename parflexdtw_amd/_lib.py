@@ -32,6 +32,7 @@ _flt = ctypes.c_float
 _SIGNATURES = {
     "pf_last_error": (ctypes.c_char_p, []),
     "pf_version": (_int, []),
+    "pf_valu_probe": (_int, [_vp, _int, _int, _vp]),
     "pf_check_steps": (_int, [_vp, _vp, _int]),
     "pf_stage1": (_int, [_int, _vp, _vp, _i64, _i64, _vp, _vp, _int, _int, _int, _int, _vp, _vp,
                          _vp, _vp, _vp, _vp, _vp]),

@@ -46,3 +46,33 @@ extern "C" int pf_check_steps(const int32_t *steps, const double *weights, int K
   pf::StepSet ss;
   return pf::make_stepset(steps, weights, K, &ss);
 }
+
+// VALU issue-rate probe (measurement only): each lane runs `iters` rounds
+// of 16 independent FP32 fused multiply-adds (v_fma_f32 / v_fmac_f32, one
+// VALU wave instruction each), then stores one sum so that nothing is
+// removed.  `blocks` workgroups of 256 threads.
+namespace pf {
+__global__ __launch_bounds__(256) void k_valu_probe(float *__restrict__ out, int iters, float a,
+                                                    float b) {
+  float x[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) x[j] = threadIdx.x * 0.001f + j;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) x[j] = __builtin_fmaf(x[j], a, b);
+    asm volatile("" ::: "memory");
+  }
+  float s = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) s += x[j];
+  out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+}  // namespace pf
+
+extern "C" int pf_valu_probe(float *d_out, int blocks, int iters, void *stream) {
+  PF_REQUIRE(d_out && blocks >= 1 && iters >= 1, "pf_valu_probe: bad arguments");
+  hipLaunchKernelGGL(pf::k_valu_probe, dim3((unsigned)blocks), dim3(256), 0,
+                     pf::as_stream(stream), d_out, iters, 1.0001f, 0.5f);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}

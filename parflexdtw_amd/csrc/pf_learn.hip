@@ -938,7 +938,7 @@ constexpr int kNtT = 2;                  // MFMA tiles per layer row
 constexpr int kNtRow = 32 * kNtT;        // pixels a layer computes per row
 constexpr int kNtOut = kNtRow - 6;       // logits per job row
 constexpr int kNtRing = kNtRow + 2;      // input slots of a row
-constexpr int kNtSeg = 128;              // output rows per job
+constexpr int kNtSeg = 256;              // output rows per job
 constexpr int kNtDepth = 4;              // rows per ring: 3 read, 1 written per step
 constexpr int kNtMaxK = 16;
 
