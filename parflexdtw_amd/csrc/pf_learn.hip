@@ -1015,8 +1015,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const int k = kh + i;
     w1[i] = k < 9 ? (__bf16)bf2f(W1[px * kKdim + k * kCh]) : (__bf16)0.0f;
   }
-  lfloatx4_t bb[4];
-  load_bias4<0>(wave == 0 ? b2 : (wave == 1 ? b3 : (wave == 2 ? b4 : b1)), lane, bb);
+  // biases of layers 2 and 3 in LDS, read where a row is stored (no
+  // registers held across their MFMA chains): bias_s[role][channel]; the
+  // other roles keep theirs in registers (256 bytes keep three workgroups
+  // a CU within the LDS)
+  __shared__ __attribute__((aligned(16))) float bias_s[2][kCh];
+  if (wave < 2 && lane < kCh) bias_s[wave][lane] = (wave == 0 ? b2 : b3)[lane];
   const float inv = st[2 * K];
   // lane k (k < 16) of feat_of(y): centred row feature k of input row y,
   // zero for k >= K and outside the map
@@ -1056,23 +1060,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   // one 3x3 layer row from ring `src`: two tiles, k-steps in the order
   // (channel half, dy, dx) of k_conv3x3
   auto layer_row = [&](const __hip_bfloat16 *src, const int s, lfloatx16_t (&acc)[kNtT]) {
+    // MFMA i = k-step q = i / 2 ((ch * 3 + dy) * 3 + dx) of tile t = i % 2;
+    // its operand is loaded three MFMAs ahead, so the LDS latency overlaps
+    // the chains
+    // Tile 1 reads at tile 0's address plus 32 pixels (same bank swizzle),
+    // so both tiles share their address registers; its pixels 64 and 65 then
+    // lie past the ring row (the next slot's first pixels, or past the last
+    // ring, inside the workgroup's LDS) instead of wrapping to pixels 0 and
+    // 1: they feed only logits at or beyond kNtOut, which are not written.
+    auto opnd = [&](const int i) {
+      const int q = i >> 1, t = i & 1;
+      const int ch = q / 9, dy = (q / 3) % 3, dx = q % 3;
+      const __hip_bfloat16 *rowp = src + ((s + 1 + dy) % kNtDepth) * kNtRow * kCh;
+      return *reinterpret_cast<const lbf16x8_t *>(
+          &rowp[chunk_at(px + dx, ch * 2 + (kh >> 3)) + t * 32 * kCh]);
+    };
 #pragma unroll
     for (int t = 0; t < kNtT; ++t) acc[t] = lfloatx16_t{};
+    lbf16x8_t b0 = opnd(0), b1 = opnd(1), b2 = opnd(2);
 #pragma unroll
-    for (int ch = 0; ch < 2; ++ch)
-#pragma unroll
-      for (int dy = 0; dy < 3; ++dy) {
-        const __hip_bfloat16 *rowp = src + ((s + 1 + dy) % kNtDepth) * kNtRow * kCh;
-#pragma unroll
-        for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-          for (int t = 0; t < kNtT; ++t) {
-            const lbf16x8_t a = *reinterpret_cast<const lbf16x8_t *>(
-                &rowp[chunk_at(t * 32 + px + dx, ch * 2 + (kh >> 3))]);
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wb[(dy * 3 + dx) * 2 + ch], a,
-                                                             acc[t], 0, 0, 0);
-          }
-      }
+    for (int i = 0; i < 36; ++i) {
+      const lbf16x8_t b3 = (i + 3 < 36) ? opnd(i + 3) : b2;
+      // keep the load above this MFMA (the scheduler would sink it)
+      __builtin_amdgcn_sched_barrier(0);
+      const int q = i >> 1, t = i & 1;
+      const int ch = q / 9, dy = (q / 3) % 3, dx = q % 3;
+      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wb[(dy * 3 + dx) * 2 + ch], b0, acc[t], 0,
+                                                       0, 0);
+      b0 = b1;
+      b1 = b2;
+      b2 = b3;
+    }
   };
   // relu(acc + bias) of row y (columns xs0 + slot) into ring row `dst`,
   // zero outside the map
@@ -1110,6 +1128,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   // uniform, the same barrier count), so each holds only its own registers.
   const int nsteps = (y1 - y0) + 9;
   if (wave == 3) {
+    lfloatx4_t bb[4];
+    load_bias4<0>(b1, lane, bb);
     auto step = [&](const int s, auto J) {
       constexpr int j = decltype(J)::value;
       const int a = y0 - 3 + s;          // first-layer row of this step
@@ -1156,7 +1176,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     // yp - 2, yp - 1 and yp (kept per tile in registers)
     lbf16x8_t wl[6];
     logit_frags(W4, lane, wl);
-    const float bias0 = bb[0][0];
+    const float bias0 = b4[0];
     float q02[kNtT], q01[kNtT], q11[kNtT];
 #pragma unroll
     for (int t = 0; t < kNtT; ++t) q02[t] = q01[t] = q11[t] = 0.0f;
@@ -1220,7 +1240,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       if (run && inmap) layer_row(src, j, acc);
       __hip_bfloat16 *dst = dst0 + j * kNtRow * kCh;
       if (run) {
-        if (inmap) store_row(acc, bb, row, x0 - 2 + wave, dst);
+        if (inmap) {
+          lfloatx4_t bl[4];
+#pragma unroll
+          for (int g = 0; g < 4; ++g)
+            bl[g] = *reinterpret_cast<const lfloatx4_t *>(&bias_s[wave][g * 8 + (lane >> 5) * 4]);
+          store_row(acc, bl, row, x0 - 2 + wave, dst);
+        }
         else zero_row(dst);
       }
       __syncthreads();
