@@ -371,17 +371,17 @@ def single_pair_plan(L1, L2, L, beta):
 _TUNED = {}  # (L, dtype code, ntiles bucket) -> stage-1 launch shape
 
 
-def _config_candidates(L, feat=False):
+def _config_candidates(L, feat=False, Kp=16):
     """(strips, waves) shapes with 2 columns per lane and no wave idle (the
     features form keeps its C blocks in LDS: 8 KB per wave, at most 160 KB),
-    and for the features form one strip of 4 columns per lane (k_sweep4,
-    tiles up to 4096 columns)."""
+    and for the features form with K <= 16 one strip of 4 columns per lane
+    (k_sweep4, tiles up to 4096 columns)."""
     out = []
     for nw in (1, 2, 4, 8, 16):
         if (nw - 1) * 128 < L and (not feat or nw * 8 <= 160):
             out.append((-(-L // (nw * 128)), nw))
     nw4 = -(-L // 256)
-    if feat and nw4 <= 16 and nw4 * 128 < L and (1, nw4) not in out:
+    if feat and Kp == 16 and nw4 <= 16 and nw4 * 128 < L and (1, nw4) not in out:
         out.append((1, nw4))
     return out
 
@@ -398,7 +398,8 @@ def autotune_stage1(C_dev, plan, sw, ws, Gs=(32, 64), reps=2, stream=None, feat=
         ws["s1_config"] = _TUNED[key]
         return _TUNED[key], {}
     times = {}
-    for (S, nw) in _config_candidates(plan.L, feat is not None):
+    for (S, nw) in _config_candidates(plan.L, feat is not None,
+                                      int(feat[2]) if feat is not None else 16):
         for G in (Gs if S > 1 else Gs[:1]):
             cfg = (S, nw, G)
             stage1(C_dev, plan, sw, ws, stream, config=cfg, feat=feat)  # warm-up / allocation
