@@ -67,6 +67,46 @@ def test_wait_gives_up_reports_and_recovers(torch_gpu, oracle_mod, dtype, L, cfg
             assert np.array_equal(eP[i, j, 1, :R], s1["edge_P"][i, j, 1, :R])
 
 
+@pytest.mark.parametrize("cfg", [(1, 4, 16), (1, 8, 64)])
+def test_sweep4_wait_gives_up_reports_and_recovers(torch_gpu, cfg):
+    """The features-form sweep with 4 columns per lane (k_sweep4): its ring
+    and progress waits give up at a limit of 0 polls, the launch ends and
+    reports PF_EHIP; with the normal limit its edges and back-pointers
+    equal the materialised BF16 form's again."""
+    torch = torch_gpu
+    from parflexdtw_amd import _lib, engine
+    from parflexdtw_amd.engine import BatchRunner
+    from parflexdtw_amd.synth import make_pair
+    F1, F2, _ = make_pair(700, 760, seed=31)
+    L = 1000 if cfg[1] == 4 else 2000
+    fus = BatchRunner([(F1, F2)], L, STEPS, [2, 3, 3], beta=0.1, dtype=torch.bfloat16,
+                      fused=True)
+    fus.cost()
+    lib = _lib.lib()
+    prev = ctypes.c_int64()
+    assert lib.pf_set_wait_limit(0, ctypes.byref(prev)) == 0
+    try:
+        engine.stage1(fus.C, fus.plan, fus.sw, fus.ws, config=cfg, feat=fus.feat)
+        with pytest.raises(_lib.ParflexError, match="gave up"):
+            engine.device_status()
+        engine.device_status()
+    finally:
+        assert lib.pf_set_wait_limit(prev.value, None) == 0
+    engine.stage1(fus.C, fus.plan, fus.sw, fus.ws, config=cfg, feat=fus.feat)
+    engine.device_status()
+    ref = BatchRunner([(F1, F2)], L, STEPS, [2, 3, 3], beta=0.1, dtype=torch.bfloat16)
+    ref.cost()
+    engine.stage1(ref.C, ref.plan, ref.sw, ref.ws)
+    torch.cuda.synchronize()
+    assert torch.equal(ref.ws["Bp"], fus.ws["Bp"])
+    t = ref.plan.tiles[0]
+    R, W = int(t[2]), int(t[3])
+    for k in ("edge_D", "edge_P"):
+        a = ref.ws[k].cpu().numpy().reshape(2, L)
+        b = fus.ws[k].cpu().numpy().reshape(2, L)
+        assert np.array_equal(a[0, :W], b[0, :W]) and np.array_equal(a[1, :R], b[1, :R]), k
+
+
 def test_wait_limit_rejects_bad_values(torch_gpu):
     from parflexdtw_amd import _lib
     assert _lib.lib().pf_set_wait_limit(-1, None) == _lib.PF_EINVAL
