@@ -366,13 +366,26 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
           s1floatx16_t acc;
 #pragma unroll
           for (int v = 0; v < 16; ++v) acc[v] = 1.0f;
-          for (int kk = 0; kk < Kp; kk += 16) {
-            const uint4 u = *reinterpret_cast<const uint4 *>(ap + kk);
-            const s1bf16x8_t av = __builtin_bit_cast(
-                s1bf16x8_t, uint4{u.x ^ 0x80008000u, u.y ^ 0x80008000u, u.z ^ 0x80008000u,
-                                  u.w ^ 0x80008000u});
-            const s1bf16x8_t bv = *reinterpret_cast<const s1bf16x8_t *>(bp + kk);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+          // k-chunks four at a time: the 8 operand loads of a group are in
+          // flight together (chunks past Kp read chunk 0 and are skipped)
+          for (int kk0 = 0; kk0 < Kp; kk0 += 64) {
+            uint4 u[4];
+            s1bf16x8_t bv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int kk = (kk0 + 16 * q < Kp) ? kk0 + 16 * q : 0;
+              u[q] = *reinterpret_cast<const uint4 *>(ap + kk);
+              bv[q] = *reinterpret_cast<const s1bf16x8_t *>(bp + kk);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              if (kk0 + 16 * q < Kp) {
+                const s1bf16x8_t av = __builtin_bit_cast(
+                    s1bf16x8_t, uint4{u[q].x ^ 0x80008000u, u[q].y ^ 0x80008000u,
+                                      u[q].z ^ 0x80008000u, u[q].w ^ 0x80008000u});
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv[q], acc, 0, 0, 0);
+              }
+            }
           }
 #pragma unroll
           for (int v = 0; v < 16; ++v)
@@ -914,21 +927,23 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
 // ---------------------------------------------------------------------------
 // Features-form sweep with four columns per lane (k_sweep4): the standard
 // step set, FP32 DP, packed outputs, one strip per tile (W <= 256 * waves),
-// K <= 16.  It computes what k_stage1<2 or 4, bf16, float, false, true, true>
+// K <= 16 KC (KC chunks of 16 features, up to 8).  It computes what k_stage1<2 or 4, bf16, float, false, true, true>
 // computes, bit for bit (the same cell arithmetic and the same C values),
 // with less work per row:
 //  * four columns per lane: the four DPP moves and the per-row bookkeeping
 //    serve four cells instead of two;
-//  * C in 16-row blocks: one v_mfma_f32_16x16x16_bf16 per 16 columns (the
-//    same dot products as the 32x32x16 form, bit for bit), accumulator
-//    started at 1 and the column features negated, so the MFMA yields
-//    1 - a.b and the epilogue only rounds to BF16; a wave's block is 8.25 KB
-//    (two 8-wave workgroups per CU);
+//  * C in 16-row blocks: a chain of KC v_mfma_f32_16x16x16_bf16 per 16
+//    columns (the same dot products and chains as the 32x32x16 form, bit
+//    for bit), accumulator started at 1 and the column features negated, so
+//    the MFMA yields 1 - a.b and the epilogue only rounds to BF16; a wave's
+//    block is 8.25 KB (two 8-wave workgroups per CU); with KC = 1 the column
+//    features stay in registers, otherwise each fill loads them (L2);
 //  * every LDS access is a per-lane base plus an immediate offset: rows run
 //    16 to an unrolled step group, r % 16 known in each step;
-//  * the left wave's last two columns of a row come as one 16-byte record
-//    {D, D, X, X} of an 8-slot ring (column 0 of the ring holds the tile's
-//    missing columns -2, -1 for wave 0); progress is published every 2 rows;
+//  * the left wave's last two columns of a row come as two 8-byte records
+//    {D, D} and {X, X} of an 8-slot ring (column 0 of the ring holds the
+//    tile's missing columns -2, -1 for wave 0), stored from the registers
+//    they live in; progress is published every 2 rows;
 //  * the tile's last column (D, X) is staged in LDS and written 16 rows at a
 //    time; the first column of C is written from the block fill.
 // Rows 1 and 2 take the boundary rules of k_stage1 in full; from row 3 on
@@ -944,9 +959,6 @@ constexpr int kS4Ring = 8;                 // rows of wave-to-wave hand-over
 constexpr int kS4MaxWaves = 16;
 constexpr int kS4RingW = kS4MaxWaves + 1;  // ring columns; column w + 1 is written by wave w
 
-struct alignas(16) S4Rec {
-  float da, db, xa, xb;
-};
 
 __device__ __forceinline__ s4bf16x4_t s4_load4(const __hip_bfloat16 *p, uint32_t flip) {
   uint2 v = *reinterpret_cast<const uint2 *>(p);
@@ -955,7 +967,7 @@ __device__ __forceinline__ s4bf16x4_t s4_load4(const __hip_bfloat16 *p, uint32_t
   return __builtin_bit_cast(s4bf16x4_t, v);
 }
 
-template <int MAXT>
+template <int MAXT, int KC>
 __global__ __launch_bounds__(MAXT) void k_sweep4(const TileDesc *__restrict__ tiles, StepSet ss,
                                                  int64_t Lmax, uint32_t *__restrict__ Bp,
                                                  double *__restrict__ eD,
@@ -965,7 +977,7 @@ __global__ __launch_bounds__(MAXT) void k_sweep4(const TileDesc *__restrict__ ti
                                                  const __hip_bfloat16 *__restrict__ Fb,
                                                  int spin_limit) {
   extern __shared__ __attribute__((aligned(16))) __hip_bfloat16 cdyn[];
-  __shared__ S4Rec ring[kS4Ring][kS4RingW];
+  __shared__ float2 ringD[kS4Ring][kS4RingW], ringX[kS4Ring][kS4RingW];
   __shared__ int prog[kS4MaxWaves];
   __shared__ float2 edge_st[kS4Rows];
 
@@ -987,8 +999,9 @@ __global__ __launch_bounds__(MAXT) void k_sweep4(const TileDesc *__restrict__ ti
   int32_t *const ePc = eP + td.e_off + Lmax;
   double *const eCc = eC + td.e_off + Lmax;
   uint32_t *const Bt = Bp + td.b_off;
-  const __hip_bfloat16 *const Fat = Fa + td.c_off * 16;
-  const __hip_bfloat16 *const Fbt = Fb + td.ldc * 16;
+  constexpr int Kp = 16 * KC;
+  const __hip_bfloat16 *const Fat = Fa + td.c_off * Kp;
+  const __hip_bfloat16 *const Fbt = Fb + td.ldc * Kp;
 
   // this wave's block of C and the lane's read / fill positions in it
   __hip_bfloat16 *const cblk = cdyn + vwave * (kS4Rows * kS4Stride);
@@ -997,31 +1010,52 @@ __global__ __launch_bounds__(MAXT) void k_sweep4(const TileDesc *__restrict__ ti
   __hip_bfloat16 *const cwr = cblk + fr * kS4Stride + fk;
 
   const bool active = wcol0 < W;  // wave-uniform
-  // column features (negated) of the wave's 16 column blocks, row features
-  // of the next block
-  s4bf16x4_t fcol[kS4Cols / 16], frow_next;
+  // column features (negated) of the wave's 16 column blocks (KC = 1: in
+  // registers, loaded once), row features of the next block (KC chunks)
+  s4bf16x4_t fcol[KC == 1 ? kS4Cols / 16 : 1], frow_next[KC];
   auto load_frow = [&](const int b) {
     const int ar = (16 * b + fr < R) ? 16 * b + fr : R - 1;
-    return s4_load4(Fat + ar * 16 + fk, 0u);
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) frow_next[kc] = s4_load4(Fat + ar * Kp + kc * 16 + fk, 0u);
+  };
+  auto col_of = [&](const int cb) {
+    return (wcol0 + cb * 16 + fr < W) ? wcol0 + cb * 16 + fr : W - 1;
   };
   if (active) {
+    if constexpr (KC == 1) {
 #pragma unroll
-    for (int cb = 0; cb < kS4Cols / 16; ++cb) {
-      const int gc = (wcol0 + cb * 16 + fr < W) ? wcol0 + cb * 16 + fr : W - 1;
-      fcol[cb] = s4_load4(Fbt + gc * 16 + fk, 0x80008000u);
+      for (int cb = 0; cb < kS4Cols / 16; ++cb)
+        fcol[cb] = s4_load4(Fbt + col_of(cb) * 16 + fk, 0x80008000u);
     }
-    frow_next = load_frow(0);
+    load_frow(0);
   }
   // C of rows 16b .. 16b + 15 into the wave's block
   auto fill = [&](const int b) {
-    const s4bf16x4_t a = frow_next;
-    if (16 * (b + 1) < R) frow_next = load_frow(b + 1);
+    s4bf16x4_t a[KC];
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) a[kc] = frow_next[kc];
+    if (16 * (b + 1) < R) load_frow(b + 1);
     const s4floatx4_t one = {1.0f, 1.0f, 1.0f, 1.0f};
 #pragma unroll
     for (int cb = 0; cb < kS4Cols / 16; cb += 2) {
-      const s4floatx4_t x = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fcol[cb], a, one, 0, 0, 0);
-      const s4floatx4_t y =
-          __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fcol[cb + 1], a, one, 0, 0, 0);
+      s4floatx4_t x = one, y = one;
+      if constexpr (KC == 1) {
+        x = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fcol[cb], a[0], x, 0, 0, 0);
+        y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fcol[cb + 1], a[0], y, 0, 0, 0);
+      } else {
+        s4bf16x4_t cx[KC], cy[KC];
+        const int gx = col_of(cb), gy = col_of(cb + 1);
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) {
+          cx[kc] = s4_load4(Fbt + gx * Kp + kc * 16 + fk, 0x80008000u);
+          cy[kc] = s4_load4(Fbt + gy * Kp + kc * 16 + fk, 0x80008000u);
+        }
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) {
+          x = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(cx[kc], a[kc], x, 0, 0, 0);
+          y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(cy[kc], a[kc], y, 0, 0, 0);
+        }
+      }
       *reinterpret_cast<uint2 *>(cwr + cb * 16) = uint2{bf16_pair(x[0], x[1]), bf16_pair(x[2], x[3])};
       *reinterpret_cast<uint2 *>(cwr + cb * 16 + 16) =
           uint2{bf16_pair(y[0], y[1]), bf16_pair(y[2], y[3])};
@@ -1081,9 +1115,13 @@ __global__ __launch_bounds__(MAXT) void k_sweep4(const TileDesc *__restrict__ ti
     }
     if (own_last) edge_st[0] = float2{D1[jl], X1[jl]};
   }
-  for (int q = tid; q < kS4Ring; q += (int)blockDim.x) ring[q][0] = S4Rec{INF, INF, 0.0f, 0.0f};
+  for (int q = tid; q < kS4Ring; q += (int)blockDim.x) {
+    ringD[q][0] = float2{INF, INF};
+    ringX[q][0] = float2{0.0f, 0.0f};
+  }
   if (lane == 63) {
-    ring[0][vwave + 1] = S4Rec{D1[2], D1[3], X1[2], X1[3]};
+    ringD[0][vwave + 1] = float2{D1[2], D1[3]};
+    ringX[0][vwave + 1] = float2{X1[2], X1[3]};
     prog[wave] = 0;
   }
   __syncthreads();
@@ -1130,9 +1168,10 @@ __global__ __launch_bounds__(MAXT) void k_sweep4(const TileDesc *__restrict__ ti
       }
       if (sync_ok != kGaveUp) sync_ok = seen_left + 1;
     }
-    const S4Rec rec = ring[(U + kS4Ring - 1) % kS4Ring][vwave];
-    const float LD1a = shr1(D1[2], rec.da), LD1b = shr1(D1[3], rec.db);
-    const float LX1a = shr1(X1[2], rec.xa), LX1b = shr1(X1[3], rec.xb);
+    const float2 rd = ringD[(U + kS4Ring - 1) % kS4Ring][vwave];
+    const float2 rx = ringX[(U + kS4Ring - 1) % kS4Ring][vwave];
+    const float LD1a = shr1(D1[2], rd.x), LD1b = shr1(D1[3], rd.y);
+    const float LX1a = shr1(X1[2], rx.x), LX1b = shr1(X1[3], rx.y);
     float cv[4];
     unpack(cq, cv);
     float ND[4], NX[4];
@@ -1242,7 +1281,8 @@ __global__ __launch_bounds__(MAXT) void k_sweep4(const TileDesc *__restrict__ ti
       if (right_ok != kGaveUp) right_ok = seen_right + kS4Ring - 1;
     }
     if (lane == 63) {
-      ring[U % kS4Ring][vwave + 1] = S4Rec{D1[2], D1[3], X1[2], X1[3]};
+      ringD[U % kS4Ring][vwave + 1] = float2{D1[2], D1[3]};
+      ringX[U % kS4Ring][vwave + 1] = float2{X1[2], X1[3]};
       if constexpr (U % 2 == 1)
         __hip_atomic_store(&prog[vwave], r, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
@@ -1252,15 +1292,26 @@ __global__ __launch_bounds__(MAXT) void k_sweep4(const TileDesc *__restrict__ ti
   using F = std::false_type;
   if (R > 1) step(1, std::integral_constant<int, 1>{}, T{});
   if (R > 2) step(2, std::integral_constant<int, 2>{}, T{});
-  for (int rb = 3; rb < R; rb += 16) {
-    // rb = 3 (mod 16): row rb + u has r % 16 = (3 + u) % 16
+  // Full groups of 16 rows with no branch between the steps (the registers
+  // then rotate through the group without copies), then the last rows of
+  // the tile, each step guarded.  rb = 3 (mod 16): row rb + u has
+  // r % 16 = (3 + u) % 16.
+  int rb = 3;
+  for (; rb + 16 <= R; rb += 16) {
     __builtin_assume((rb & 15) == 3);
-    step(rb, std::integral_constant<int, 3>{}, F{});
-#define PF_S4_STEP(u)                                                           \
+#define PF_S4_STEP(u) step(rb + u, std::integral_constant<int, (3 + u) % 16>{}, F{});
+    PF_S4_STEP(0) PF_S4_STEP(1) PF_S4_STEP(2) PF_S4_STEP(3) PF_S4_STEP(4) PF_S4_STEP(5)
+    PF_S4_STEP(6) PF_S4_STEP(7) PF_S4_STEP(8) PF_S4_STEP(9) PF_S4_STEP(10) PF_S4_STEP(11)
+    PF_S4_STEP(12) PF_S4_STEP(13) PF_S4_STEP(14) PF_S4_STEP(15)
+#undef PF_S4_STEP
+  }
+  if (rb < R) {
+    __builtin_assume((rb & 15) == 3);
+#define PF_S4_STEP(u) \
   if (rb + u < R) step(rb + u, std::integral_constant<int, (3 + u) % 16>{}, F{});
-    PF_S4_STEP(1) PF_S4_STEP(2) PF_S4_STEP(3) PF_S4_STEP(4) PF_S4_STEP(5)
-    PF_S4_STEP(6) PF_S4_STEP(7) PF_S4_STEP(8) PF_S4_STEP(9) PF_S4_STEP(10)
-    PF_S4_STEP(11) PF_S4_STEP(12) PF_S4_STEP(13) PF_S4_STEP(14) PF_S4_STEP(15)
+    PF_S4_STEP(0) PF_S4_STEP(1) PF_S4_STEP(2) PF_S4_STEP(3) PF_S4_STEP(4) PF_S4_STEP(5)
+    PF_S4_STEP(6) PF_S4_STEP(7) PF_S4_STEP(8) PF_S4_STEP(9) PF_S4_STEP(10) PF_S4_STEP(11)
+    PF_S4_STEP(12) PF_S4_STEP(13) PF_S4_STEP(14)
 #undef PF_S4_STEP
   }
   // the last row for the wave on the right
@@ -1449,8 +1500,9 @@ extern "C" int pf_stage1_features(const void *d_F1b, const void *d_F2b, int Kp,
   PF_REQUIRE(cpl == 2 || cpl == 4, "pf_stage1_features: strips*waves*64*4 must cover the tile");
   const bool std3 = ss.K == 3 && ss.dr[0] == 1 && ss.dc[0] == 1 && ss.dr[1] == 1 &&
                     ss.dc[1] == 2 && ss.dr[2] == 2 && ss.dc[2] == 1;
-  // one strip of 4 columns per lane, standard steps, K <= 16: k_sweep4
-  const bool sweep4 = std3 && Kp == 16 && strips == 1 && cpl == 4;
+  // one strip of 4 columns per lane, standard steps, K <= 128: k_sweep4
+  const bool sweep4 = std3 && (Kp == 16 || Kp == 32 || Kp == 64 || Kp == 128) &&
+                      strips == 1 && cpl == 4;
   PF_REQUIRE(sweep4 || waves * 64 <= (cpl == 2 ? 1024 : 512), "pf_stage1_features: too many waves");
   const size_t lds = sweep4 ? (size_t)waves * kS4Rows * kS4Stride * sizeof(__hip_bfloat16)
                             : (size_t)waves * 32 * (64 * cpl + 4) * sizeof(__hip_bfloat16);
@@ -1477,7 +1529,9 @@ extern "C" int pf_stage1_features(const void *d_F1b, const void *d_F2b, int Kp,
   const auto *Fb = static_cast<const __hip_bfloat16 *>(d_F2b);
   const dim3 grid((unsigned)n_items), block((unsigned)(waves * 64));
   if (sweep4) {
-    auto kern = k_sweep4<1024>;
+    auto kern = Kp == 16 ? k_sweep4<1024, 1>
+                         : (Kp == 32 ? k_sweep4<1024, 2>
+                                     : (Kp == 64 ? k_sweep4<1024, 4> : k_sweep4<1024, 8>));
     if (lds > 64 * 1024)
       PF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
