@@ -169,7 +169,7 @@ def _feat_kernel(cfg, L, K=12):
     """Name of the features-form stage-1 kernel a launch shape runs."""
     S, nw = int(cfg[0]), int(cfg[1])
     cpl = 2 if S * nw * 128 >= L else 4
-    if S == 1 and cpl == 4 and K <= 16:
+    if S == 1 and cpl == 4 and K <= 128:
         return "k_sweep4<1024>"
     return f"k_stage1<{cpl},__hip_bfloat16,float,false,true,true,{1024 if cpl == 2 else 512}>"
 

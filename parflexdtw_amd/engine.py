@@ -374,14 +374,14 @@ _TUNED = {}  # (L, dtype code, ntiles bucket) -> stage-1 launch shape
 def _config_candidates(L, feat=False, Kp=16):
     """(strips, waves) shapes with 2 columns per lane and no wave idle (the
     features form keeps its C blocks in LDS: 8 KB per wave, at most 160 KB),
-    and for the features form with K <= 16 one strip of 4 columns per lane
+    and for the features form (K <= 128) one strip of 4 columns per lane
     (k_sweep4, tiles up to 4096 columns)."""
     out = []
     for nw in (1, 2, 4, 8, 16):
         if (nw - 1) * 128 < L and (not feat or nw * 8 <= 160):
             out.append((-(-L // (nw * 128)), nw))
     nw4 = -(-L // 256)
-    if feat and Kp == 16 and nw4 <= 16 and nw4 * 128 < L and (1, nw4) not in out:
+    if feat and Kp in (16, 32, 64, 128) and nw4 <= 16 and nw4 * 128 < L and (1, nw4) not in out:
         out.append((1, nw4))
     return out
 

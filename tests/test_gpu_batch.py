@@ -150,7 +150,12 @@ def test_batched_cost_equals_single(torch_gpu, dtype):
                                              (12, "chroma", 1000, (1, 4, 32)),
                                              (12, "chroma", 2000, (1, 8, 64)),
                                              (12, "chroma", 4000, (1, 16, 64)),
-                                             (16, "relu", 700, (1, 4, 16))])
+                                             (16, "relu", 700, (1, 4, 16)),
+                                             # K = 20 and 128: chains of 2 and 8
+                                             # 16x16x16 MFMAs per block
+                                             (20, "relu", 250, (1, 1, 8)),
+                                             (128, "relu", 520, (1, 4, 16)),
+                                             (128, "relu", 2000, (1, 8, 64))])
 def test_fused_stage1_equals_materialised_bf16(torch_gpu, K, kind, L, config):
     """Stage 1 computing C inside the sweep (pf_stage1_features) gives the
     same back-pointers, edges and paths, bit for bit, as stage 1 reading the

@@ -51,14 +51,14 @@ def test_plan_rejects_degenerate():
 
 
 def test_stage1_tuner_candidates():
-    """The features form adds one strip of 4 columns per lane (k_sweep4)
-    only for K <= 16 (padded to 16); every candidate covers the tile."""
+    """The features form adds one strip of 4 columns per lane (k_sweep4,
+    K <= 128); every candidate covers the tile."""
     from parflexdtw_amd.engine import _config_candidates
     for L in (232, 500, 1000, 2000, 4000):
-        for feat, Kp in ((False, 16), (True, 16), (True, 128)):
+        for feat, Kp in ((False, 16), (True, 16), (True, 128), (True, 144)):
             cands = _config_candidates(L, feat, Kp)
             assert cands
             for S, nw in cands:
                 assert S * nw * 64 * 8 >= L
             one4 = [(S, nw) for S, nw in cands if S == 1 and nw * 128 < L]
-            assert bool(one4) == (feat and Kp == 16), (L, feat, Kp, cands)
+            assert bool(one4) == (feat and Kp <= 128), (L, feat, Kp, cands)
