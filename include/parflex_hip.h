@@ -314,6 +314,15 @@ int pf_pool_features(const double *d_F, int64_t K, int64_t n, int pool, float *d
  * are in increasing b_off; blocks of one run are contiguous. */
 int pf_pool_features_batch(const double *d_F, int64_t K, int64_t n, const int64_t *d_seg,
                            int64_t nseg, int pool, float *d_Fp, int64_t nb, void *stream);
+/* pf_pool_features_batch that also writes the BF16 features of every frame
+ * the runs cover to d_Fb (n x Kp bf16, frame-major, Kp = K rounded up to 16,
+ * 16-byte aligned): the values and layout of pf_features_bf16, bit for bit,
+ * from the same reads of d_F.  Frames no run covers are not written.  No
+ * reference counterpart: the learned form's pooling (FlexDTW.py:1207-1210
+ * normalisation) and the fused form's feature preparation in one pass. */
+int pf_pool_features_bf16_batch(const double *d_F, int64_t K, int64_t n, const int64_t *d_seg,
+                                int64_t nseg, int pool, float *d_Fp, int64_t nb, void *d_Fb,
+                                void *stream);
 /* Input statistics of many pairs in one launch (K <= 16): pair p's pooled
  * features are columns [a_off, a_off+H) of d_A (K x lda) and [b_off, b_off+W)
  * of d_B (K x ldb), d_pairs rows int64 {a_off, H, b_off, W}; d_st + p*(2K+1)

@@ -65,6 +65,8 @@ _SIGNATURES = {
     "pf_pool_features": (_int, [_vp, _i64, _i64, _int, _vp, _vp]),
     "pf_pack_paths": (_int, [_vp, _i64, _i64, _vp, _vp, _vp, _vp]),
     "pf_pool_features_batch": (_int, [_vp, _i64, _i64, _vp, _i64, _int, _vp, _i64, _vp]),
+    "pf_pool_features_bf16_batch": (_int, [_vp, _i64, _i64, _vp, _i64, _int, _vp, _i64, _vp,
+                                           _vp]),
     "pf_input_stats_batch": (_int, [_vp, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _vp]),
     "pf_conv3x3_first_pooled": (_int, [_vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp,
                                        _vp, _vp, _vp, _vp]),

@@ -25,6 +25,7 @@ namespace pf {
 
 typedef __bf16 lbf16x8_t __attribute__((ext_vector_type(8)));
 typedef float lfloatx16_t __attribute__((ext_vector_type(16)));
+typedef unsigned int lu32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kCh = 32;           // channels of every layer
 constexpr int kKdim = 9 * kCh;    // im2col width (288)
@@ -46,15 +47,21 @@ struct PoolSeg {
 // 4, 2, 1), so the means equal a shuffle reduction's bit for bit.  The
 // segment of block g is the last one with b_off <= g (binary search over
 // `segs`, or `one` when segs is null), so one launch pools every pair of a
-// batch.
+// batch.  With FB, the thread of frame i also writes the frame's BF16
+// features Fb[i*Kp + k] = bf16(F[k*ldf + i] / (sqrt(sum_k F^2) + 1e-9)),
+// zero padded to Kp (a multiple of 16), from the same loads: the values and
+// layout of k_cost_prep_bf16's frame-major form, bit for bit.
 constexpr int kPoolBlocks = 8;                 // pooled blocks per workgroup
 constexpr int kPoolStride = 36;                // floats per block row in LDS (32 + pad)
+template <bool FB>
 __global__ __launch_bounds__(256) void k_pool_features(const double *__restrict__ F, int64_t K,
                                                        int64_t ldf,
                                                        const PoolSeg *__restrict__ segs,
                                                        int64_t nseg, PoolSeg one,
                                                        float *__restrict__ Fp, int64_t ldp,
-                                                       int64_t nb) {
+                                                       int64_t nb,
+                                                       __hip_bfloat16 *__restrict__ Fb,
+                                                       int64_t Kp) {
   __shared__ __attribute__((aligned(16))) float v[16][kPoolBlocks * kPoolStride];
   const int t = threadIdx.x;
   const int64_t g0 = blockIdx.x * (int64_t)kPoolBlocks;
@@ -78,13 +85,32 @@ __global__ __launch_bounds__(256) void k_pool_features(const double *__restrict_
     const double x = in ? F[k * ldf + i] : 0.0;
     s = s + x * x;
   }
-  const double inv = 1.0 / ((sqrt(s) + 1e-9) * (double)(cnt > 0 ? cnt : 1));
+  const double nrm = sqrt(s) + 1e-9;
+  const double inv = 1.0 / (nrm * (double)(cnt > 0 ? cnt : 1));
   const int slot = (t >> 5) * kPoolStride + (t & 31);
   const int rk = t >> 3, rb = t & 7;   // reducing thread: feature rk of block rb
   for (int64_t k0 = 0; k0 < K; k0 += 16) {
     const int kn = (int)((K - k0) < 16 ? (K - k0) : 16);
-    for (int q = 0; q < kn; ++q)
-      v[q][slot] = in ? (float)(F[(k0 + q) * ldf + i] * inv) : 0.0f;
+    if (FB) {
+      uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        if (q < kn) {
+          const double x = in ? F[(k0 + q) * ldf + i] : 0.0;
+          v[q][slot] = in ? (float)(x * inv) : 0.0f;
+          const uint32_t h = __builtin_bit_cast(uint16_t, __float2bfloat16((float)(x / nrm)));
+          w[q >> 1] |= h << (16 * (q & 1));
+        }
+      }
+      if (in) {
+        lu32x4_t *dst = reinterpret_cast<lu32x4_t *>(Fb + i * Kp + k0);
+        dst[0] = lu32x4_t{w[0], w[1], w[2], w[3]};
+        dst[1] = lu32x4_t{w[4], w[5], w[6], w[7]};
+      }
+    } else {
+      for (int q = 0; q < kn; ++q)
+        v[q][slot] = in ? (float)(F[(k0 + q) * ldf + i] * inv) : 0.0f;
+    }
     __syncthreads();
     if (rk < kn) {
       float a[32];
@@ -295,7 +321,6 @@ __global__ void k_write_input(const float *__restrict__ A, int64_t H, const floa
 constexpr int kRowsPerWave = 4;
 constexpr int kCvRows = 4 * kRowsPerWave, kCvCols = 32;
 constexpr int kPixPad = kCh + 8;      // bf16 per staged pixel (16-byte aligned rows)
-typedef unsigned int lu32x4_t __attribute__((ext_vector_type(4)));
 typedef unsigned int lu32x2_t __attribute__((ext_vector_type(2)));
 typedef float lfloatx4_t __attribute__((ext_vector_type(4)));
 
@@ -1450,8 +1475,9 @@ extern "C" int pf_pool_features(const double *d_F, int64_t K, int64_t n, int poo
   PF_REQUIRE(pool == 32, "pf_pool_features: the pooling factor is 32");
   const int64_t nb = (n + pool - 1) / pool;
   const PoolSeg one{0, n, 0, 0};
-  hipLaunchKernelGGL(k_pool_features, dim3((unsigned)((nb + kPoolBlocks - 1) / kPoolBlocks)), dim3(256), 0, as_stream(stream), d_F,
-                     K, n, (const PoolSeg *)nullptr, (int64_t)1, one, d_Fp, nb, nb);
+  hipLaunchKernelGGL(k_pool_features<false>, dim3((unsigned)((nb + kPoolBlocks - 1) / kPoolBlocks)), dim3(256), 0, as_stream(stream), d_F,
+                     K, n, (const PoolSeg *)nullptr, (int64_t)1, one, d_Fp, nb, nb,
+                     (__hip_bfloat16 *)nullptr, (int64_t)0);
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
 }
@@ -1462,9 +1488,25 @@ extern "C" int pf_pool_features_batch(const double *d_F, int64_t K, int64_t n,
   PF_REQUIRE(d_F && d_seg && d_Fp && K >= 1 && n >= 1 && nseg >= 1 && nb >= 1,
              "pf_pool_features_batch: bad arguments");
   PF_REQUIRE(pool == 32, "pf_pool_features_batch: the pooling factor is 32");
-  hipLaunchKernelGGL(k_pool_features, dim3((unsigned)((nb + kPoolBlocks - 1) / kPoolBlocks)), dim3(256), 0, as_stream(stream), d_F,
+  hipLaunchKernelGGL(k_pool_features<false>, dim3((unsigned)((nb + kPoolBlocks - 1) / kPoolBlocks)), dim3(256), 0, as_stream(stream), d_F,
                      K, n, reinterpret_cast<const PoolSeg *>(d_seg), nseg, PoolSeg{0, 0, 0, 0},
-                     d_Fp, nb, nb);
+                     d_Fp, nb, nb, (__hip_bfloat16 *)nullptr, (int64_t)0);
+  PF_CHECK_HIP(hipGetLastError());
+  return PF_OK;
+}
+
+extern "C" int pf_pool_features_bf16_batch(const double *d_F, int64_t K, int64_t n,
+                                           const int64_t *d_seg, int64_t nseg, int pool,
+                                           float *d_Fp, int64_t nb, void *d_Fb, void *stream) {
+  PF_REQUIRE(d_F && d_seg && d_Fp && d_Fb && K >= 1 && n >= 1 && nseg >= 1 && nb >= 1,
+             "pf_pool_features_bf16_batch: bad arguments");
+  PF_REQUIRE(pool == 32, "pf_pool_features_bf16_batch: the pooling factor is 32");
+  PF_REQUIRE(reinterpret_cast<uintptr_t>(d_Fb) % 16 == 0,
+             "pf_pool_features_bf16_batch: Fb must be 16-byte aligned");
+  const int64_t Kp = (K + 15) / 16 * 16;
+  hipLaunchKernelGGL(k_pool_features<true>, dim3((unsigned)((nb + kPoolBlocks - 1) / kPoolBlocks)), dim3(256), 0, as_stream(stream), d_F,
+                     K, n, reinterpret_cast<const PoolSeg *>(d_seg), nseg, PoolSeg{0, 0, 0, 0},
+                     d_Fp, nb, nb, static_cast<__hip_bfloat16 *>(d_Fb), Kp);
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
 }

@@ -63,9 +63,19 @@ __global__ void k_fill_i64(int64_t *p, int64_t v, int64_t n) {
 // every tile (bit s of map 0: some edge cell starts at (0, s), s > 0; bit a
 // of map 1: some edge cell starts at (a, 0), a >= 0), and D_arr = L_arr =
 // inf (L_arr of tiles with stage 1 only).
+//
+// A tile without stage 1 (column 0 < 0) gets D = inf only in the slots that
+// some later step reads: position 0 and the corner of each edge (corner
+// copies, the overlap sync), its last row when the tile above has stage 1
+// or it is in the top tile row (chaining, the top scan), its last column
+// when the tile to the right has stage 1 or it is in the right tile column.
+// Every other slot of such a tile is never read: a tile without stage 1
+// chains from nowhere, and chaining reads only the edges of the tiles below
+// and to the left of a tile with stage 1.
 constexpr int kMaxBitw = 8192 / 32 + 1;
 
-__global__ __launch_bounds__(256) void k_s2_prep(const TileDesc *__restrict__ tiles, int64_t L,
+__global__ __launch_bounds__(256) void k_s2_prep(const PairDesc *__restrict__ pairs,
+                                                 const TileDesc *__restrict__ tiles, int64_t L,
                                                  const int32_t *__restrict__ eP,
                                                  const double *__restrict__ eC,
                                                  double *__restrict__ Darr,
@@ -82,15 +92,26 @@ __global__ __launch_bounds__(256) void k_s2_prep(const TileDesc *__restrict__ ti
   const TileDesc td = tiles[t];
   // a tile without stage 1 (column 0 < 0): P = 0, C = 0, arrays not read
   const bool live = td.c_off >= 0;
+  bool need0 = true, need1 = true;  // edges whose every slot may be read
+  if (!live) {
+    const PairDesc pd = pairs[td.pair];
+    const int64_t i = td.ij >> 32, j = td.ij & 0xffffffffll;
+    need0 = i + 1 >= pd.n1 || tiles[t + pd.n2].c_off >= 0;
+    need1 = j + 1 >= pd.n2 || tiles[t + 1].c_off >= 0;
+  }
   for (int64_t rem = threadIdx.x; rem < 2 * L; rem += blockDim.x) {
     const int64_t q = t * 2 * L + rem;
     const int e = rem >= L ? 1 : 0;
     const int64_t pos = rem - (int64_t)e * L;
+    if (!live) {
+      // A tile without stage 1: its L slots are read only where a corner
+      // copy or the sync has written them (an unreached slot, D = +inf, is
+      // never read), and its overlap costs not at all (k_s2_diag takes 0).
+      const int64_t last = e == 0 ? td.W - 1 : td.R - 1;
+      if ((e == 0 ? need0 : need1) || pos == 0 || pos == last) Darr[q] = __builtin_huge_val();
+      continue;
+    }
     Darr[q] = __builtin_huge_val();
-    // A tile without stage 1: its L slots are read only where a corner copy
-    // has written them (an unreached slot, D = +inf, is never read), and its
-    // overlap costs not at all (k_s2_diag takes 0).
-    if (!live) continue;
     Larr[q] = __builtin_huge_val();
     const int64_t elen = e == 0 ? td.W : td.R;
     if (pos >= elen) {
@@ -134,10 +155,10 @@ __device__ __forceinline__ void diag_slot(const PairDesc &pd, const TileDesc *__
   if (pos >= elen) return;
   const int64_t t = pd.tile_base + i * n2 + j;
   const int64_t slot = (t * 2 + e) * L + pos;
-  // A tile without stage 1 (column 0 < 0) starts no path: every slot but the
-  // corner copies (pos 0) ends with D = +inf, as k_s2_prep left it, and the
-  // L of a slot with infinite D is never read (sync, scan and chaining test
-  // D first).
+  // A tile without stage 1 (column 0 < 0) starts no path: every slot that
+  // is read, but the corner copies (pos 0), keeps the D = +inf k_s2_prep
+  // wrote, and the L of a slot with infinite D is never read (sync, scan and
+  // chaining test D first).
   const bool live = tiles[t].c_off >= 0;
   if (!live && pos != 0) return;
 
@@ -806,7 +827,7 @@ extern "C" int pf_stage2_phases(int phases, const double *d_edge_C, const int64_
   const TileDesc *tiles = reinterpret_cast<const TileDesc *>(d_tiles);
   const int64_t bitw = (L + 32) / 32;
   if (phases & PF_S2_PROPAGATE) {
-    hipLaunchKernelGGL(k_s2_prep, dim3((unsigned)ntiles), dim3(256), 0, st, tiles, L, d_edge_P,
+    hipLaunchKernelGGL(k_s2_prep, dim3((unsigned)ntiles), dim3(256), 0, st, pairs, tiles, L, d_edge_P,
                        d_edge_C, d_D_arr, d_L_arr, d_olap, d_bits, bitw);
     PF_CHECK_HIP(hipGetLastError());
     for (int64_t d = 0; d < max_diag; ++d) {
@@ -865,7 +886,7 @@ extern "C" int pf_stage2_propagate_listed(const double *d_edge_C, const int64_t 
   const PairDesc *pairs = reinterpret_cast<const PairDesc *>(d_pairs);
   const TileDesc *tiles = reinterpret_cast<const TileDesc *>(d_tiles);
   const int64_t bitw = (L + 32) / 32;
-  hipLaunchKernelGGL(k_s2_prep, dim3((unsigned)ntiles), dim3(256), 0, st, tiles, L, d_edge_P,
+  hipLaunchKernelGGL(k_s2_prep, dim3((unsigned)ntiles), dim3(256), 0, st, pairs, tiles, L, d_edge_P,
                      d_edge_C, d_D_arr, d_L_arr, d_olap, d_bits, bitw);
   PF_CHECK_HIP(hipGetLastError());
   for (int64_t d = 0; d < ndiag; ++d) {

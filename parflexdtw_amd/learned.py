@@ -237,8 +237,10 @@ class LearnedRunner:
     # ------------------------------------------------------------------
     def _score_into(self, out, stream=None, keep_maps=False):
         """Tile logits of every pair into the device buffer `out` (the
-        pairs' grids concatenated in i-major order).  Pooled features and
-        input statistics of all pairs take three launches on `stream`; the
+        pairs' grids concatenated in i-major order).  Pooled features (in
+        the fused form with the BF16 features stage 1 reads, from the same
+        pass) and input statistics of all pairs take three launches on
+        `stream`; the
         network layers of the pairs are spread over side streams (forked
         from and joined back to `stream`) so that one pair's layers overlap
         the tail of another's."""
@@ -247,12 +249,22 @@ class LearnedRunner:
         K, n = self.K, len(self.pairs)
         seg1, seg2, stp = self._seg
         ms = main.cuda_stream
-        check(lib().pf_pool_features_batch(self.F1.data_ptr(), K, int(self.F1.shape[1]),
-                                           seg1.data_ptr(), n, POOL, self.Ap.data_ptr(),
-                                           self.nbA, ms), "pf_pool_features_batch")
-        check(lib().pf_pool_features_batch(self.F2.data_ptr(), K, int(self.F2.shape[1]),
-                                           seg2.data_ptr(), n, POOL, self.Bp.data_ptr(),
-                                           self.nbB, ms), "pf_pool_features_batch")
+        if self.fused:
+            # the fused form's BF16 features come from the same pass over F
+            F1b, F2b, _ = self.feat
+            for F, sg, P, nbp, Fb in ((self.F1, seg1, self.Ap, self.nbA, F1b),
+                                      (self.F2, seg2, self.Bp, self.nbB, F2b)):
+                check(lib().pf_pool_features_bf16_batch(F.data_ptr(), K, int(F.shape[1]),
+                                                        sg.data_ptr(), n, POOL, P.data_ptr(),
+                                                        nbp, Fb.data_ptr(), ms),
+                      "pf_pool_features_bf16_batch")
+        else:
+            check(lib().pf_pool_features_batch(self.F1.data_ptr(), K, int(self.F1.shape[1]),
+                                               seg1.data_ptr(), n, POOL, self.Ap.data_ptr(),
+                                               self.nbA, ms), "pf_pool_features_batch")
+            check(lib().pf_pool_features_batch(self.F2.data_ptr(), K, int(self.F2.shape[1]),
+                                               seg2.data_ptr(), n, POOL, self.Bp.data_ptr(),
+                                               self.nbB, ms), "pf_pool_features_batch")
         check(lib().pf_input_stats_batch(self.Ap.data_ptr(), self.nbA, self.Bp.data_ptr(),
                                          self.nbB, K, stp.data_ptr(), n, self.stats.data_ptr(),
                                          ms), "pf_input_stats_batch")
@@ -499,11 +511,9 @@ class LearnedRunner:
         mark("workspace")
         if nsel:
             if self.fused:
+                # the BF16 features were written by the scoring pass's pooling
                 C = None
                 F1b, F2b, Kp = self.feat
-                for F, Fb in ((self.F1, F1b), (self.F2, F2b)):
-                    check(lib().pf_features_bf16(F.data_ptr(), self.K, int(F.shape[1]),
-                                                 Fb.data_ptr(), st), "pf_features_bf16")
             else:
                 C = torch.empty(c_elems, dtype=torch.bfloat16, device=dev)
                 d_jobs = torch.from_numpy(jobs).to(dev)

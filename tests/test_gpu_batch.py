@@ -246,7 +246,12 @@ def test_stage2_reads_no_edges_of_tiles_marked_without_stage1(torch_gpu):
     makes stage 2 treat the tile's edges as D = +inf, P = 0, C = 0 without
     reading them: the same outputs as filling those edges with +inf, 0, 0,
     whatever the unread slots hold (the marked tiles' overlap costs, never
-    read, are not written either)."""
+    read, are not written either).  D of a marked tile is written only in
+    the slots some step reads (position 0 and the corners; the last row when
+    the tile above is unmarked or the tile is in the top row; the last
+    column when the tile to the right is unmarked or the tile is in the right
+    column); the other slots keep a finite poison value, which a read would
+    carry into the paths."""
     from parflexdtw_amd import engine
     from parflexdtw_amd.engine import BatchRunner
     torch = torch_gpu
@@ -277,18 +282,39 @@ def test_stage2_reads_no_edges_of_tiles_marked_without_stage1(torch_gpu):
             w["edge_D"][idx] = float("inf")
             w["edge_P"][idx] = 0
             w["edge_C"][idx] = 0.0
-        for k in ("D_arr", "L_arr", "olap", "bits", "path", "path_len", "best_f", "best_i"):
+        for k in ("L_arr", "olap", "bits", "path", "path_len", "best_f", "best_i"):
             w[k] = torch.empty_like(ws[k])
+        w["D_arr"] = torch.full_like(ws["D_arr"], -3.0)
         engine.stage2(None, plan, br.sw, w, None)
         torch.cuda.synchronize()
         outs.append(w)
-    a, b = outs
-    for k in ("D_arr", "bits", "best_f", "best_i", "path_len"):
+    a, b = outs  # edges filled, tiles marked
+    # slots of D that are defined (read by some step) after stage 2
+    rd = np.ones((plan.ntiles, 2, L), bool)
+    for p in range(len(pairs)):
+        n1, n2, base = (int(v) for v in plan.pairs[p, 2:5])
+        for i in range(n1):
+            for j in range(n2):
+                t = base + i * n2 + j
+                if not pruned[t]:
+                    continue
+                R, W = int(plan.tiles[t, 2]), int(plan.tiles[t, 3])
+                need0 = i + 1 >= n1 or not pruned[t + n2]
+                need1 = j + 1 >= n2 or not pruned[t + 1]
+                rd[t] = False
+                rd[t, 0, [0, W - 1]] = True
+                rd[t, 1, [0, R - 1]] = True
+                rd[t, 0] |= need0
+                rd[t, 1] |= need1
+    rd = torch.from_numpy(rd.reshape(-1)).cuda()
+    assert torch.equal(a["D_arr"][rd].view(torch.int64), b["D_arr"][rd].view(torch.int64))
+    assert bool((b["D_arr"][~rd] == -3.0).all())
+    for k in ("bits", "best_f", "best_i", "path_len"):
         assert torch.equal(a[k].view(torch.int64) if a[k].dtype == torch.float64 else a[k],
                            b[k].view(torch.int64) if b[k].dtype == torch.float64 else b[k]), k
     # path lengths where a path arrives (finite D): the L of an unreached slot
     # is never read
-    fin = torch.isfinite(a["D_arr"])
+    fin = torch.isfinite(a["D_arr"]) & rd
     assert torch.equal(a["L_arr"][fin].view(torch.int64), b["L_arr"][fin].view(torch.int64))
     # overlap costs of a marked tile are neither written nor read
     live = torch.from_numpy(~slots).cuda()
@@ -321,9 +347,11 @@ def test_listed_stage2_equals_full_stage2(torch_gpu):
         for listed in (False, True):
             w = dict(ws)
             w["tiles"] = torch.from_numpy(tiles).cuda()
-            for k in ("D_arr", "L_arr", "olap", "bits", "path", "path_len", "best_f", "best_i",
+            for k in ("L_arr", "olap", "bits", "path", "path_len", "best_f", "best_i",
                       "cand_D", "cand_L"):
                 w[k] = torch.empty_like(ws[k])
+            # the D slots of marked tiles that no step reads stay unwritten
+            w["D_arr"] = torch.full_like(ws["D_arr"], -3.0)
             if listed:
                 engine.stage2_listed(plan, br.sw, w, ~pruned)
             else:
@@ -336,7 +364,7 @@ def test_listed_stage2_equals_full_stage2(torch_gpu):
             if x.dtype == torch.float64:
                 x, y = x.view(torch.int64), y.view(torch.int64)
             assert torch.equal(x, y), (frac, k)
-        fin = torch.isfinite(a["D_arr"])
+        fin = torch.isfinite(a["D_arr"]) & (a["D_arr"] != -3.0)
         assert torch.equal(a["L_arr"][fin].view(torch.int64), b["L_arr"][fin].view(torch.int64))
         ra, rb = engine.collect(plan, a), engine.collect(plan, b)
         for p in range(len(pairs)):

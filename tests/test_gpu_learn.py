@@ -112,6 +112,37 @@ def test_batched_pooling_and_statistics_equal_per_pair(env):
         assert np.array_equal(sh[p], work.cpu().numpy()[:2 * K + 1]), p
 
 
+@pytest.mark.parametrize("K", [12, 20])
+def test_pooling_with_bf16_features_equals_separate_launches(env, K):
+    """pf_pool_features_bf16_batch gives the pooled features of
+    pf_pool_features_batch and the BF16 features of pf_features_bf16 (two
+    chunks of 16 for K = 20), bit for bit, over ragged concatenated pairs."""
+    torch, tn, tile_net = env
+    L, check = _lib()
+    rng = np.random.default_rng(70 + K)
+    ns = [333, 32, 31, 1, 1500, 65]
+    F = torch.from_numpy(rng.random((K, sum(ns))) ** 3).cuda()
+    nb_ = [-(-v // 32) for v in ns]
+    fo, bo = np.cumsum([0] + ns[:-1]), np.cumsum([0] + nb_[:-1])
+    nb = sum(nb_)
+    seg = torch.from_numpy(np.stack([fo, ns, bo, 0 * bo], 1).astype(np.int64)).cuda()
+    st = torch.cuda.current_stream().cuda_stream
+    Kp = -(-K // 16) * 16
+    n = F.shape[1]
+    P0 = torch.empty(K * nb, dtype=torch.float32, device="cuda")
+    P1 = torch.full((K * nb,), -1.0, dtype=torch.float32, device="cuda")
+    B0 = torch.empty(n * Kp, dtype=torch.bfloat16, device="cuda")
+    B1 = torch.full((n * Kp,), 7.0, dtype=torch.bfloat16, device="cuda")
+    check(L.pf_pool_features_batch(F.data_ptr(), K, n, seg.data_ptr(), len(ns), 32,
+                                   P0.data_ptr(), nb, st), "pool")
+    check(L.pf_features_bf16(F.data_ptr(), K, n, B0.data_ptr(), st), "features")
+    check(L.pf_pool_features_bf16_batch(F.data_ptr(), K, n, seg.data_ptr(), len(ns), 32,
+                                        P1.data_ptr(), nb, B1.data_ptr(), st), "pool + features")
+    torch.cuda.synchronize()
+    assert torch.equal(P0.view(torch.int32), P1.view(torch.int32))
+    assert torch.equal(B0.view(torch.int16), B1.view(torch.int16))
+
+
 def test_first_layer_from_pooled_features_equals_input_map_path(env):
     """pf_conv3x3_first_pooled on a pair inside a batch equals the input map
     of pf_pooled_cost followed by pf_conv3x3 mode 3, bit for bit."""
