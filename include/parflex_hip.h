@@ -124,8 +124,9 @@ int pf_stage1_features(const void *d_F1b, const void *d_F2b, int Kp, const int64
  * (may be null).  No reference counterpart: the reference's kernel has no
  * inter-workgroup waits. */
 int pf_set_wait_limit(int64_t polls, int64_t *previous);
-/* Synchronises `stream`, then returns PF_EHIP (and clears the word) if a
- * stage-1 wait on the current device gave up since the last call, else 0. */
+/* Synchronises `stream`, then returns PF_EHIP (and clears the words) if a
+ * stage-1 wait or a wait of pf_stage2_propagate_dag on the current device
+ * gave up since the last call, else 0. */
 int pf_device_status(void *stream);
 
 /* L2-normalised features as frames x Kp bf16 (Kp = K rounded up to 16, zero
@@ -251,6 +252,26 @@ int pf_stage2_propagate_listed(const double *d_edge_C, const int64_t *d_pairs, i
                                const int32_t *d_list, const int64_t *h_diag, int64_t ndiag,
                                const double *d_edge_D, const int32_t *d_edge_P, double *d_D_arr,
                                double *d_L_arr, double *d_olap, uint32_t *d_bits, void *stream);
+
+/* The same phase as one launch over the tiles with stage 1 only, each tile
+ * after the tiles it reads (a tile DAG): the workgroups take the live tiles
+ * in diagonal order from a ticket counter and wait for the done flags of
+ * their live neighbours below, left and below-left; a finished tile copies
+ * its corners into the position-0 slots of pruned neighbours (the only
+ * slots of a pruned tile the phase changes).  d_list as for
+ * pf_stage2_propagate_listed; d_diag: its per-diagonal table on the device
+ * (as pf_select_tiles writes it at d_counts + 3); d_work: ntiles + 1 int32
+ * (done flags and the counter, cleared here).  The overlap costs are taken
+ * from d_edge_C as needed (no d_olap table).  Results equal those of
+ * PF_S2_PROPAGATE (D everywhere it is read, L where D is finite).  Waits
+ * poll at most pf_set_wait_limit times; one that gives up is reported by
+ * pf_device_status.  No reference counterpart (the reference computes every
+ * tile, one diagonal after another: Parflex.py:763-855). */
+int pf_stage2_propagate_dag(const double *d_edge_C, const int64_t *d_pairs, int64_t npairs,
+                            const int64_t *d_tiles, int64_t ntiles, int64_t L,
+                            const int32_t *d_list, const int64_t *d_diag, int64_t ndiag,
+                            const double *d_edge_D, const int32_t *d_edge_P, double *d_D_arr,
+                            double *d_L_arr, uint32_t *d_bits, int32_t *d_work, void *stream);
 
 /* backtrace_and_stitch (Parflex.py:1032-1103) for n requests, each
  * int64[5] = (pair, ci, cj, et, idx) in d_req; stage-1 outputs as for

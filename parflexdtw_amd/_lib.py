@@ -56,6 +56,8 @@ _SIGNATURES = {
                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "pf_stage2_propagate_listed": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp, _vp, _i64, _vp,
                                           _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pf_stage2_propagate_dag": (_int, [_vp, _vp, _i64, _vp, _i64, _i64, _vp, _vp, _i64, _vp,
+                                       _vp, _vp, _vp, _vp, _vp, _vp]),
     "pf_stitch": (_int, [_vp, _i64, _vp, _vp, _i64, _vp, _int, _vp, _vp, _vp, _vp, _i64, _vp,
                          _vp, _vp]),
     "pf_cost": (_int, [_int, _vp, _vp, _i64, _i64, _i64, _vp, _i64, _vp, _vp]),

@@ -66,6 +66,11 @@ struct PairDesc {
 };
 static_assert(sizeof(PairDesc) == 128, "pair table rows are int64[16]");
 
+// Poll limit of the device waits (pf_set_wait_limit), and the wait error
+// word of the stage-2 tile DAG (read and cleared; pf_device_status).
+int wait_limit();
+int take_s2_wait_error(hipStream_t st, int *word);
+
 // Launch of the stage-2 back-pointer walk (pf_walk.hip): grid (101 segments,
 // requests), one wave each, on stream st.
 void launch_s2_walk(dim3 grid, hipStream_t st, const TileDesc *tiles, int64_t L, StepSet ss,

@@ -1555,12 +1555,23 @@ extern "C" int pf_set_wait_limit(int64_t polls, int64_t *previous) {
   return PF_OK;
 }
 
+namespace pf {
+int wait_limit() { return g_wait_limit; }
+}  // namespace pf
+
 extern "C" int pf_device_status(void *stream) {
   hipStream_t st = as_stream(stream);
-  int word = 0;
+  int word = 0, word2 = 0;
   PF_CHECK_HIP(hipMemcpyFromSymbolAsync(&word, HIP_SYMBOL(g_s1_wait_error), sizeof(int), 0,
                                         hipMemcpyDeviceToHost, st));
   PF_CHECK_HIP(hipStreamSynchronize(st));
+  const int rc = take_s2_wait_error(st, &word2);
+  if (rc != PF_OK) return rc;
+  if (word2 != 0 && word == 0) {
+    set_error("the stage-2 tile sweep gave up waiting for a tile (spin limit reached); its "
+              "outputs are not valid");
+    return PF_EHIP;
+  }
   if (word != 0) {
     const int zero = 0;
     PF_CHECK_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_s1_wait_error), &zero, sizeof(int), 0,

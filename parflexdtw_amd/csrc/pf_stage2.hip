@@ -81,12 +81,18 @@ __global__ __launch_bounds__(256) void k_s2_prep(const PairDesc *__restrict__ pa
                                                  double *__restrict__ Darr,
                                                  double *__restrict__ Larr,
                                                  double *__restrict__ olap,
-                                                 uint32_t *__restrict__ bits, int64_t bitw) {
+                                                 uint32_t *__restrict__ bits, int64_t bitw,
+                                                 int32_t *__restrict__ dag) {
   // One workgroup per tile: its 2L slots, and its two start bitmaps built in
   // LDS and stored whole (the workgroup owns them: no global atomics and no
-  // clearing pass).
+  // clearing pass).  With dag (k_s2_dag follows): the tile's done flag and,
+  // in workgroup 0, the ticket counter after the flags are cleared.
   __shared__ uint32_t lb[2][kMaxBitw];
   const int64_t t = blockIdx.x;
+  if (dag && threadIdx.x == 0) {
+    dag[t] = 0;
+    if (t == 0) dag[gridDim.x] = 0;
+  }
   for (int w = threadIdx.x; w < 2 * bitw; w += blockDim.x) lb[w >= bitw][w - (w >= bitw) * bitw] = 0u;
   __syncthreads();
   const TileDesc td = tiles[t];
@@ -111,14 +117,18 @@ __global__ __launch_bounds__(256) void k_s2_prep(const PairDesc *__restrict__ pa
       if ((e == 0 ? need0 : need1) || pos == 0 || pos == last) Darr[q] = __builtin_huge_val();
       continue;
     }
-    Darr[q] = __builtin_huge_val();
-    Larr[q] = __builtin_huge_val();
+    // (with dag, k_s2_dag writes every slot of the tile and takes the
+    // overlap costs from eC itself)
+    if (!dag) {
+      Darr[q] = __builtin_huge_val();
+      Larr[q] = __builtin_huge_val();
+    }
     const int64_t elen = e == 0 ? td.W : td.R;
     if (pos >= elen) {
-      olap[q] = 0.0;
+      if (!dag) olap[q] = 0.0;
     } else {
       const int64_t s = eP[q];
-      olap[q] = s > 0 ? eC[td.e_off + s] : eC[td.e_off + L - s];
+      if (!dag) olap[q] = s > 0 ? eC[td.e_off + s] : eC[td.e_off + L - s];
       if (s > 0) atomicOr(&lb[0][s >> 5], 1u << (s & 31));
       else atomicOr(&lb[1][(-s) >> 5], 1u << ((-s) & 31));
     }
@@ -138,29 +148,55 @@ __device__ __forceinline__ bool bit_at(const uint32_t *bits, int64_t bitw, int64
   return (bits[(t * 2 + map) * bitw + (pos >> 5)] >> (pos & 31)) & 1u;
 }
 
-// Slot (e, pos) of tile (i, j) of pair pd in the tile anti-diagonal sweep
-// of _nb_initialize_chunks_sparse (Parflex.py:631-760) and
-// _nb_dp_fill_chunks_sparse (Parflex.py:763-855), with the valid mask of
-// _build_valid_mask (Parflex.py:582-624) evaluated per slot.
-__device__ __forceinline__ void diag_slot(const PairDesc &pd, const TileDesc *__restrict__ tiles,
-                                          int64_t L, int64_t i, int64_t j, int e, int64_t pos,
-                                          const double *__restrict__ eD,
-                                          const int32_t *__restrict__ eP,
-                                          const double *__restrict__ olap,
-                                          const uint32_t *__restrict__ bits, int64_t bitw,
-                                          double *__restrict__ Darr, double *__restrict__ Larr) {
+// What slot (e, pos) of tile (i, j) of pair pd takes in the tile
+// anti-diagonal sweep of _nb_initialize_chunks_sparse (Parflex.py:631-760)
+// and _nb_dp_fill_chunks_sparse (Parflex.py:763-855), with the valid mask of
+// _build_valid_mask (Parflex.py:582-624) evaluated per slot.  slot_plan
+// reads only the tile's own edge data, overlap costs and the start bitmaps
+// (all final before the sweep); slot_apply then reads the predecessor slot
+// (pslot, a slot of tile (i-1, j), (i, j-1) or their corners) and writes.
+enum SlotKind : int {
+  kSlotNone = 0,   // the slot keeps its value
+  kSlotSet = 1,    // D = eDv, L = len
+  kSlotChain = 2,  // a path from pslot: if D[pslot] finite, D = (D[pslot] + eDv) - ol,
+                   // L = L[pslot] + len
+  kSlotCopy = 3,   // corner copy: D, L = D[pslot], L[pslot]
+  kSlotCopyFin = 4 // corner copy where D[pslot] is finite
+};
+struct SlotPlan {
+  int64_t slot, pslot;
+  double eDv, ol, len;
+  int kind;
+};
+
+// The overlap cost of a chained slot is olap[slot] (k_s2_prep's table) or,
+// with olap null, taken from the tile's start-cell costs eC at e_off as
+// k_s2_prep takes it.
+__device__ __forceinline__ SlotPlan slot_plan(const PairDesc &pd,
+                                              const TileDesc *__restrict__ tiles, int64_t L,
+                                              int64_t i, int64_t j, int e, int64_t pos,
+                                              const double *__restrict__ eD,
+                                              const int32_t *__restrict__ eP,
+                                              const double *__restrict__ olap,
+                                              const double *__restrict__ eC, int64_t e_off,
+                                              const uint32_t *__restrict__ bits, int64_t bitw) {
+  SlotPlan sp;
+  sp.kind = kSlotNone;
+  sp.pslot = 0;
+  sp.eDv = sp.ol = sp.len = 0.0;
   const int64_t n1 = pd.n1, n2 = pd.n2;
-  const Geo g = tile_geo(pd, L, i, j);
-  const int64_t elen = e == 0 ? g.cols : g.rows;
-  if (pos >= elen) return;
   const int64_t t = pd.tile_base + i * n2 + j;
   const int64_t slot = (t * 2 + e) * L + pos;
+  sp.slot = slot;
+  const Geo g = tile_geo(pd, L, i, j);
+  const int64_t elen = e == 0 ? g.cols : g.rows;
+  if (pos >= elen) return sp;
   // A tile without stage 1 (column 0 < 0) starts no path: every slot that
   // is read, but the corner copies (pos 0), keeps the D = +inf k_s2_prep
   // wrote, and the L of a slot with infinite D is never read (sync, scan and
   // chaining test D first).
   const bool live = tiles[t].c_off >= 0;
-  if (!live && pos != 0) return;
+  if (!live && pos != 0) return sp;
 
   bool valid;
   if (e == 0) {
@@ -177,6 +213,15 @@ __device__ __forceinline__ void diag_slot(const PairDesc &pd, const TileDesc *__
   const double eDv = live ? eD[slot] : __builtin_huge_val();
   const int64_t sr = s < 0 ? -s : 0, sc = s > 0 ? s : 0;
   const double seg_len = (double)(iabs64(lr - sr) + iabs64(lc - sc));
+  auto set = [&]() {
+    sp.kind = kSlotSet;
+    sp.eDv = eDv;
+    sp.len = seg_len;
+  };
+  auto copy = [&](int64_t pslot, bool finite_only) {
+    sp.kind = finite_only ? kSlotCopyFin : kSlotCopy;
+    sp.pslot = pslot;
+  };
 
   // Chain from predecessor tile (pi, pj) as in Parflex.py:683-707/734-758/824-853.
   auto chain = [&](int64_t pi, int64_t pj) {
@@ -196,83 +241,140 @@ __device__ __forceinline__ void diag_slot(const PairDesc &pd, const TileDesc *__
       return;
     }
     if (ppos < 0 || ppos >= pelen) return;
-    const int64_t pslot = ((pd.tile_base + pi * n2 + pj) * 2 + pedge) * L + ppos;
-    const double prev = Darr[pslot];
-    if (!isfinite(prev)) return;
-    Darr[slot] = (prev + eDv) - (live ? olap[slot] : 0.0);
-    Larr[slot] = Larr[pslot] + seg_len;
+    sp.kind = kSlotChain;
+    sp.pslot = ((pd.tile_base + pi * n2 + pj) * 2 + pedge) * L + ppos;
+    sp.eDv = eDv;
+    sp.ol = !live ? 0.0 : (olap ? olap[slot] : (s > 0 ? eC[e_off + s] : eC[e_off + L - s]));
+    sp.len = seg_len;
   };
 
   if (i == 0 && j == 0) {
-    if (valid && (sr == 0 || sc == 0)) {
-      Darr[slot] = eDv;
-      Larr[slot] = seg_len;
-    }
-    return;
+    if (valid && (sr == 0 || sc == 0)) set();
+    return sp;
   }
   if (i == 0) {
     if (pos == 0) {
       if (e == 0) {
         const Geo pg = tile_geo(pd, L, 0, j - 1);
-        const int64_t pslot = ((pd.tile_base + (j - 1)) * 2 + 0) * L + pg.cols - 1;
-        Darr[slot] = Darr[pslot];
-        Larr[slot] = Larr[pslot];
+        copy(((pd.tile_base + (j - 1)) * 2 + 0) * L + pg.cols - 1, false);
       }
-      return;
+      return sp;
     }
-    if (!valid) return;
+    if (!valid) return sp;
     if (sr == 0) {
-      Darr[slot] = eDv;
-      Larr[slot] = seg_len;
+      set();
     } else if (sc == 0) {
       chain(0, j - 1);
     }
-    return;
+    return sp;
   }
   if (j == 0) {
     if (pos == 0) {
       if (e == 1) {
         const Geo pg = tile_geo(pd, L, i - 1, 0);
-        const int64_t pslot = ((pd.tile_base + (i - 1) * n2) * 2 + 1) * L + pg.rows - 1;
-        Darr[slot] = Darr[pslot];
-        Larr[slot] = Larr[pslot];
+        copy(((pd.tile_base + (i - 1) * n2) * 2 + 1) * L + pg.rows - 1, false);
       }
-      return;
+      return sp;
     }
-    if (!valid) return;
+    if (!valid) return sp;
     if (sc == 0) {
-      Darr[slot] = eDv;
-      Larr[slot] = seg_len;
+      set();
     } else if (sr == 0) {
       chain(i - 1, 0);
     }
-    return;
+    return sp;
   }
-  if (!valid) return;
+  if (!valid) return sp;
   if (pos == 0) {
-    int64_t pslot;
     if (e == 0) {
       const Geo pg = tile_geo(pd, L, i, j - 1);
-      pslot = ((t - 1) * 2 + 0) * L + pg.cols - 1;
+      copy(((t - 1) * 2 + 0) * L + pg.cols - 1, true);
     } else {
       const Geo pg = tile_geo(pd, L, i - 1, j);
-      pslot = ((t - n2) * 2 + 1) * L + pg.rows - 1;
+      copy(((t - n2) * 2 + 1) * L + pg.rows - 1, true);
     }
-    const double v = Darr[pslot];
-    if (isfinite(v)) {
-      Darr[slot] = v;
-      Larr[slot] = Larr[pslot];
-    }
-    return;
+    return sp;
   }
   if (sr == 0) {
     chain(i - 1, j);
   } else if (sc == 0) {
     chain(i, j - 1);
   } else {
-    Darr[slot] = eDv;
-    Larr[slot] = seg_len;
+    set();
   }
+  return sp;
+}
+
+// Slot reads and writes: plain, or (AGENT) relaxed atomics at agent scope,
+// which are coherent across the GPU's L2 caches access by access (the tile
+// DAG's slots are read by workgroups on other XCDs within one launch).
+template <bool AGENT>
+__device__ __forceinline__ double slot_ld(const double *p) {
+  if constexpr (AGENT) {
+    return __builtin_bit_cast(double, __hip_atomic_load(reinterpret_cast<const uint64_t *>(p),
+                                                        __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT));
+  } else {
+    return *p;
+  }
+}
+template <bool AGENT>
+__device__ __forceinline__ void slot_st(double *p, double v) {
+  if constexpr (AGENT) {
+    __hip_atomic_store(reinterpret_cast<uint64_t *>(p), __builtin_bit_cast(uint64_t, v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    *p = v;
+  }
+}
+
+// Carries out a plan; a slot it writes gets (*dv, *lv).  Returns whether
+// it wrote.
+template <bool AGENT = false>
+__device__ __forceinline__ bool slot_apply(const SlotPlan &sp, double *__restrict__ Darr,
+                                           double *__restrict__ Larr, double *dv, double *lv) {
+  double d, l;
+  switch (sp.kind) {
+    case kSlotSet:
+      d = sp.eDv;
+      l = sp.len;
+      break;
+    case kSlotChain: {
+      const double prev = slot_ld<AGENT>(Darr + sp.pslot);
+      if (!isfinite(prev)) return false;
+      d = (prev + sp.eDv) - sp.ol;
+      l = slot_ld<AGENT>(Larr + sp.pslot) + sp.len;
+      break;
+    }
+    case kSlotCopy:
+      d = slot_ld<AGENT>(Darr + sp.pslot);
+      l = slot_ld<AGENT>(Larr + sp.pslot);
+      break;
+    case kSlotCopyFin:
+      d = slot_ld<AGENT>(Darr + sp.pslot);
+      if (!isfinite(d)) return false;
+      l = slot_ld<AGENT>(Larr + sp.pslot);
+      break;
+    default:
+      return false;
+  }
+  slot_st<AGENT>(Darr + sp.slot, d);
+  slot_st<AGENT>(Larr + sp.slot, l);
+  *dv = d;
+  *lv = l;
+  return true;
+}
+
+__device__ __forceinline__ void diag_slot(const PairDesc &pd, const TileDesc *__restrict__ tiles,
+                                          int64_t L, int64_t i, int64_t j, int e, int64_t pos,
+                                          const double *__restrict__ eD,
+                                          const int32_t *__restrict__ eP,
+                                          const double *__restrict__ olap,
+                                          const uint32_t *__restrict__ bits, int64_t bitw,
+                                          double *__restrict__ Darr, double *__restrict__ Larr) {
+  const SlotPlan sp = slot_plan(pd, tiles, L, i, j, e, pos, eD, eP, olap, nullptr, 0, bits, bitw);
+  double d, l;
+  slot_apply(sp, Darr, Larr, &d, &l);
 }
 
 // One tile anti-diagonal d of every pair: a thread per slot.
@@ -326,6 +428,200 @@ __global__ void k_s2_diag_list(const int32_t *__restrict__ list, int64_t nlive, 
   const PairDesc pd = pairs[td.pair];
   diag_slot(pd, tiles, L, td.ij >> 32, td.ij & 0xffffffffll, e, pos, eD, eP, olap, bits, bitw,
             Darr, Larr);
+}
+
+// The same sweep as one launch over the tiles with stage 1 only, each
+// after the tiles it reads (a tile DAG instead of a launch per diagonal).
+// The workgroups take tickets k = 0, 1, ... from a counter; ticket k is the
+// k-th live tile of `list` in diagonal order (diag: per diagonal the first
+// list slot and the live count, as pf_select_tiles writes them; the live
+// tiles of a diagonal come first in its slots).  Tile (i, j) reads slots of
+// (i-1, j) and (i, j-1) and, through the corner copies of pruned tiles,
+// (i-1, j-1); it waits for the done flags of those of them that are live.
+// Tickets are taken in diagonal order, so every tile a workgroup waits for
+// holds a smaller ticket, taken by a running workgroup: the smallest
+// unfinished ticket never waits, and every wait ends.
+//
+// A pruned tile's only written slots are its corner copies (position 0):
+// edge 0 copies the edge-0 corner of (i, j-1), edge 1 the edge-1 corner of
+// (i-1, j), always in the top tile row / left tile column, elsewhere where
+// finite.  A pruned corner is +inf (from k_s2_prep), so the copy changes the
+// slot (from +inf) only below or right of a live tile: a live tile pushes
+// its corners into the position-0 slots of pruned tiles to its right and
+// above when it is done, and pruned tiles take no ticket.  (The L of a slot
+// whose D stays +inf is never read.)
+//
+// Each thread plans up to kDagPer slots before the wait (their loads do not
+// depend on other tiles); slots beyond kDagThreads * kDagPer are planned and
+// applied after it.  Waits poll at most spin_limit times; one that gives up
+// sets g_s2_wait_error, and the tile goes on and publishes its flag, so
+// every workgroup still ends.
+//
+// Ordering: every D / L slot access of this kernel and the done flags are
+// relaxed atomics at agent scope (coherent across the XCDs' L2 caches, no
+// cache-wide write-back or invalidation); a tile's slot stores have
+// completed (s_waitcnt 0 in every wave, then the workgroup barrier) before
+// its flag is stored, and a waiting tile reads its neighbours' slots only
+// after its thread 0 has seen their flags (then the barrier).
+constexpr int kDagThreads = 1024;
+constexpr int kDagPer = 4;
+constexpr int kDagMaxDiag = 4096;
+
+__device__ int g_s2_wait_error;
+
+__global__ __launch_bounds__(kDagThreads) void k_s2_dag(
+    const int32_t *__restrict__ list, const int64_t *__restrict__ diag, int ndiag,
+    const PairDesc *__restrict__ pairs, const TileDesc *__restrict__ tiles, int64_t L,
+    const double *__restrict__ eD, const int32_t *__restrict__ eP,
+    const double *__restrict__ eC, const uint32_t *__restrict__ bits, int64_t bitw,
+    double *__restrict__ Darr, double *__restrict__ Larr, int32_t *__restrict__ flags,
+    int32_t *__restrict__ ticket, int spin_limit) {
+  __shared__ int32_t pre[kDagMaxDiag + 1];
+  __shared__ int32_t s_t;
+  __shared__ double cD[2], cL[2];  // this tile's corners (edge 0, edge 1) as they end
+  const int tid = threadIdx.x;
+  // exclusive prefix of the live counts over the diagonals (wave 0)
+  if (tid < 64) {
+    const int per = (ndiag + 63) / 64;
+    const int d0 = tid * per;
+    int sum = 0;
+    for (int q = 0; q < per; ++q)
+      if (d0 + q < ndiag) sum += (int)diag[3 * (d0 + q) + 1];
+    int inc = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(inc, o);
+      if (tid >= o) inc += u;
+    }
+    int run = inc - sum;
+    for (int q = 0; q < per; ++q) {
+      if (d0 + q < ndiag) {
+        pre[d0 + q] = run;
+        run += (int)diag[3 * (d0 + q) + 1];
+      }
+    }
+    if (tid == 63) pre[ndiag] = inc;
+  }
+  __syncthreads();
+  const int total = pre[ndiag];
+  const int64_t slots = 2 * L;
+  for (;;) {
+    if (tid == 0) {
+      const int k = atomicAdd(ticket, 1);
+      int t = -1;
+      if (k < total) {
+        int lo = 0, hi = ndiag - 1;  // the diagonal with pre[d] <= k < pre[d + 1]
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (pre[mid] <= k) lo = mid;
+          else hi = mid - 1;
+        }
+        t = list[diag[3 * lo] + (k - pre[lo])];
+      }
+      s_t = t;
+    }
+    __syncthreads();
+    const int64_t t = __builtin_amdgcn_readfirstlane(s_t);
+    if (t < 0) break;
+    const TileDesc td = wave_uniform(tiles[t]);
+    const PairDesc pd = pairs[td.pair];
+    const int64_t i = td.ij >> 32, j = td.ij & 0xffffffffll;
+    const Geo g = tile_geo(pd, L, i, j);
+    auto plan_at = [&](int64_t idx) {
+      const int e = idx >= L ? 1 : 0;
+      const int64_t pos = idx - (int64_t)e * L;
+      return slot_plan(pd, tiles, L, i, j, e, pos, eD, eP, nullptr, eC, td.e_off, bits, bitw);
+    };
+    // every slot of the tile is written here: its value, or +inf (D and L)
+    // where the sweep leaves k_s2_prep's initial value
+    auto apply = [&](const SlotPlan &sp) {
+      double d, l;
+      if (!slot_apply<true>(sp, Darr, Larr, &d, &l)) {
+        d = l = __builtin_huge_val();
+        slot_st<true>(Darr + sp.slot, d);
+        slot_st<true>(Larr + sp.slot, l);
+      }
+      const int64_t local = sp.slot - t * slots;  // e*L + pos
+      if (local == g.cols - 1) {
+        cD[0] = d;
+        cL[0] = l;
+      }
+      if (local == L + g.rows - 1) {
+        cD[1] = d;
+        cL[1] = l;
+      }
+    };
+    SlotPlan sp[kDagPer];
+#pragma unroll
+    for (int q = 0; q < kDagPer; ++q) {
+      const int64_t idx = (int64_t)q * kDagThreads + tid;
+      sp[q].slot = -1;  // no slot
+      if (idx < slots) {
+        sp[q] = plan_at(idx);
+        // slots that take their own edge values, or keep +inf, need no other
+        // tile
+        if (sp[q].kind == kSlotSet || sp[q].kind == kSlotNone) {
+          apply(sp[q]);
+          sp[q].slot = -1;
+        }
+      }
+    }
+    if (tid == 0) {
+      // tiles (i-1, j), (i, j-1), (i-1, j-1)
+      for (int q = 0; q < 3; ++q) {
+        const bool has = q == 0 ? i > 0 : (q == 1 ? j > 0 : (i > 0 && j > 0));
+        if (!has) continue;
+        const int64_t p = t - (q == 1 ? 0 : pd.n2) - (q == 0 ? 0 : 1);
+        if (tiles[p].c_off < 0) continue;  // pruned: no flag
+        for (int n = spin_limit;
+             __hip_atomic_load(&flags[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+             --n) {
+          if (n <= 0) {
+            g_s2_wait_error = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kDagPer; ++q)
+      if (sp[q].slot >= 0) apply(sp[q]);
+    for (int64_t idx = (int64_t)kDagPer * kDagThreads + tid; idx < slots; idx += kDagThreads)
+      apply(plan_at(idx));
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's slot stores have completed
+    __syncthreads();
+    if (tid == 0) {
+      // corner copies into pruned tiles to the right (edge 0) and above (edge 1)
+      if (j + 1 < pd.n2 && tiles[t + 1].c_off < 0 && (i == 0 || isfinite(cD[0]))) {
+        const int64_t q = ((t + 1) * 2 + 0) * L;
+        slot_st<true>(Darr + q, cD[0]);
+        slot_st<true>(Larr + q, cL[0]);
+      }
+      if (i + 1 < pd.n1 && tiles[t + pd.n2].c_off < 0 && (j == 0 || isfinite(cD[1]))) {
+        const int64_t q = ((t + pd.n2) * 2 + 1) * L;
+        slot_st<true>(Darr + q, cD[1]);
+        slot_st<true>(Larr + q, cL[1]);
+      }
+      __builtin_amdgcn_s_waitcnt(0);
+      __hip_atomic_store(&flags[t], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// Reads and clears the wait error word of k_s2_dag on the current device.
+int take_s2_wait_error(hipStream_t st, int *word) {
+  *word = 0;
+  PF_CHECK_HIP(hipMemcpyFromSymbolAsync(word, HIP_SYMBOL(g_s2_wait_error), sizeof(int), 0,
+                                        hipMemcpyDeviceToHost, st));
+  PF_CHECK_HIP(hipStreamSynchronize(st));
+  if (*word != 0) {
+    const int zero = 0;
+    PF_CHECK_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_s2_wait_error), &zero, sizeof(int), 0,
+                                        hipMemcpyHostToDevice, st));
+    PF_CHECK_HIP(hipStreamSynchronize(st));
+  }
+  return PF_OK;
 }
 
 // sync_tile_overlap_edges (Parflex.py:919-969), pass `pass` (0, 1, 2).  The
@@ -828,7 +1124,7 @@ extern "C" int pf_stage2_phases(int phases, const double *d_edge_C, const int64_
   const int64_t bitw = (L + 32) / 32;
   if (phases & PF_S2_PROPAGATE) {
     hipLaunchKernelGGL(k_s2_prep, dim3((unsigned)ntiles), dim3(256), 0, st, pairs, tiles, L, d_edge_P,
-                       d_edge_C, d_D_arr, d_L_arr, d_olap, d_bits, bitw);
+                       d_edge_C, d_D_arr, d_L_arr, d_olap, d_bits, bitw, (int32_t *)nullptr);
     PF_CHECK_HIP(hipGetLastError());
     for (int64_t d = 0; d < max_diag; ++d) {
       const int64_t ntd = (d + 1) < max_tile_diag ? (d + 1) : max_tile_diag;
@@ -887,7 +1183,7 @@ extern "C" int pf_stage2_propagate_listed(const double *d_edge_C, const int64_t 
   const TileDesc *tiles = reinterpret_cast<const TileDesc *>(d_tiles);
   const int64_t bitw = (L + 32) / 32;
   hipLaunchKernelGGL(k_s2_prep, dim3((unsigned)ntiles), dim3(256), 0, st, pairs, tiles, L, d_edge_P,
-                     d_edge_C, d_D_arr, d_L_arr, d_olap, d_bits, bitw);
+                     d_edge_C, d_D_arr, d_L_arr, d_olap, d_bits, bitw, (int32_t *)nullptr);
   PF_CHECK_HIP(hipGetLastError());
   for (int64_t d = 0; d < ndiag; ++d) {
     const int64_t nl = h_diag[3 * d + 1], np = h_diag[3 * d + 2];
@@ -899,6 +1195,41 @@ extern "C" int pf_stage2_propagate_listed(const double *d_edge_C, const int64_t 
                        d_olap, d_bits, bitw, d_D_arr, d_L_arr);
     PF_CHECK_HIP(hipGetLastError());
   }
+  return PF_OK;
+}
+
+extern "C" int pf_stage2_propagate_dag(const double *d_edge_C, const int64_t *d_pairs,
+                                       int64_t npairs, const int64_t *d_tiles, int64_t ntiles,
+                                       int64_t L, const int32_t *d_list, const int64_t *d_diag,
+                                       int64_t ndiag, const double *d_edge_D,
+                                       const int32_t *d_edge_P, double *d_D_arr,
+                                       double *d_L_arr, uint32_t *d_bits, int32_t *d_work,
+                                       void *stream) {
+  PF_REQUIRE(npairs > 0 && ntiles > 0 && L >= 2 && L <= 8192 && ndiag >= 1,
+             "pf_stage2_propagate_dag: bad arguments");
+  PF_REQUIRE(ndiag <= kDagMaxDiag, "pf_stage2_propagate_dag: more than 4096 tile diagonals");
+  PF_REQUIRE(d_edge_C && d_pairs && d_tiles && d_list && d_diag && d_edge_D && d_edge_P &&
+                 d_D_arr && d_L_arr && d_bits && d_work,
+             "pf_stage2_propagate_dag: null buffer");
+  PF_REQUIRE(ntiles < (1ll << 31) - 1, "pf_stage2_propagate_dag: too many tiles");
+  hipStream_t st = as_stream(stream);
+  const PairDesc *pairs = reinterpret_cast<const PairDesc *>(d_pairs);
+  const TileDesc *tiles = reinterpret_cast<const TileDesc *>(d_tiles);
+  const int64_t bitw = (L + 32) / 32;
+  hipLaunchKernelGGL(k_s2_prep, dim3((unsigned)ntiles), dim3(256), 0, st, pairs, tiles, L,
+                     d_edge_P, d_edge_C, d_D_arr, d_L_arr, (double *)nullptr, d_bits, bitw,
+                     d_work);
+  PF_CHECK_HIP(hipGetLastError());
+  int dev = 0, cus = 0;
+  PF_CHECK_HIP(hipGetDevice(&dev));
+  PF_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  // two resident workgroups per CU; workgroups beyond the live tiles find no
+  // ticket and end
+  const unsigned grid = (unsigned)(2 * (cus > 0 ? cus : 1));
+  hipLaunchKernelGGL(k_s2_dag, dim3(grid), dim3(kDagThreads), 0, st, d_list, d_diag, (int)ndiag,
+                     pairs, tiles, L, d_edge_D, d_edge_P, d_edge_C, d_bits, bitw, d_D_arr,
+                     d_L_arr, d_work, d_work + ntiles, wait_limit());
+  PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
 }
 

@@ -210,15 +210,23 @@ def diagonal_lists(plan, live, cache=None):
     return lst, np.ascontiguousarray(np.stack([off, nl, npr], axis=1).astype(np.int64))
 
 
-def stage2_listed(plan, sw, ws, live, stream=None, cache=None):
+def stage2_listed(plan, sw, ws, live, stream=None, cache=None, dag=False):
     """stage2 for a batch whose tiles with live[t] False had no stage 1
     (tile table column 0 < 0): the propagation launches each tile diagonal
-    over the list of its tiles (pf_stage2_propagate_listed), then the sync,
-    scan and stitch phases.  Returns the device list (keep it alive until
-    the stream has run)."""
+    over the list of its tiles (pf_stage2_propagate_listed), or with
+    dag=True runs the live tiles in one launch, each after the tiles it
+    reads (pf_stage2_propagate_dag); then the sync, scan and stitch phases.
+    Returns the device buffers (keep them alive until the stream has run)."""
     torch = _torch()
     lst, h_diag = diagonal_lists(plan, live, cache)
-    d_list = torch.from_numpy(lst).to(ws["tiles"].device)
+    dev = ws["tiles"].device
+    d_list = torch.from_numpy(lst).to(dev)
+    if dag:
+        d_diag = torch.from_numpy(np.ascontiguousarray(h_diag, np.int64)).to(dev)
+        work = torch.empty(plan.ntiles + 1, dtype=torch.int32, device=dev)
+        _propagate_dag(plan, ws, d_list, d_diag, work, stream)
+        stage2_phases(plan, sw, ws, _lib.PF_S2_SYNC | _lib.PF_S2_SCAN | _lib.PF_S2_STITCH, stream)
+        return d_list, d_diag, work
     check(lib().pf_stage2_propagate_listed(
         ws["edge_C"].data_ptr(), ws["pairs"].data_ptr(), plan.npairs, ws["tiles"].data_ptr(),
         plan.ntiles, plan.L, d_list.data_ptr(), h_diag.ctypes.data, plan.max_diag,
@@ -229,11 +237,26 @@ def stage2_listed(plan, sw, ws, live, stream=None, cache=None):
     return d_list
 
 
-def stage2_listed_device(plan, sw, ws, d_list, h_diag, stream=None):
+def _propagate_dag(plan, ws, d_list, d_diag, work, stream):
+    check(lib().pf_stage2_propagate_dag(
+        ws["edge_C"].data_ptr(), ws["pairs"].data_ptr(), plan.npairs, ws["tiles"].data_ptr(),
+        plan.ntiles, plan.L, d_list.data_ptr(), d_diag.data_ptr(), plan.max_diag,
+        ws["edge_D"].data_ptr(), ws["edge_P"].data_ptr(), ws["D_arr"].data_ptr(),
+        ws["L_arr"].data_ptr(), ws["bits"].data_ptr(), work.data_ptr(), _stream(stream)),
+        "pf_stage2_propagate_dag")
+
+
+def stage2_listed_device(plan, sw, ws, d_list, h_diag, stream=None, d_diag=None, work=None):
     """stage2_listed with the diagonal lists already on the device (d_list,
     int32, one slot per tile) and their layout on the host (h_diag int64
     (max_diag, 3): first slot, live count, other count), as pf_select_tiles
-    writes them."""
+    writes them.  With the layout on the device too (d_diag) and a work
+    buffer of ntiles + 1 int32, the live tiles run as one tile-DAG launch
+    (pf_stage2_propagate_dag)."""
+    if d_diag is not None:
+        _propagate_dag(plan, ws, d_list, d_diag, work, stream)
+        stage2_phases(plan, sw, ws, _lib.PF_S2_SYNC | _lib.PF_S2_SCAN | _lib.PF_S2_STITCH, stream)
+        return
     h_diag = np.ascontiguousarray(np.asarray(h_diag, np.int64).reshape(plan.max_diag, 3))
     check(lib().pf_stage2_propagate_listed(
         ws["edge_C"].data_ptr(), ws["pairs"].data_ptr(), plan.npairs, ws["tiles"].data_ptr(),

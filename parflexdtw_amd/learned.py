@@ -396,6 +396,8 @@ class LearnedRunner:
             "tmp": torch.empty(3 * (-(-n // 1024)) + 2 * plan.max_diag, dtype=torch.int64,
                                device=dev),
             "h_counts": torch.empty(ncount, dtype=torch.int64, pin_memory=True),
+            # done flags and ticket counter of the stage-2 tile DAG
+            "dag": torch.empty(n + 1, dtype=torch.int32, device=dev),
         }
         return self._sel
 
@@ -550,7 +552,9 @@ class LearnedRunner:
                 events[1].record(stream or torch.cuda.current_stream())
             mark("stage1")
         if self.fused:
-            engine.stage2_listed_device(plan, self.sw, ws, self._sel["list"], h_diag, stream)
+            t = self._sel
+            engine.stage2_listed_device(plan, self.sw, ws, t["list"], h_diag, stream,
+                                        d_diag=t["counts"][3:], work=t["dag"])
             d_sel = self._sel["sel"]
         else:
             self._stage2_listed(sel, ws, st)

@@ -326,9 +326,10 @@ def test_stage2_reads_no_edges_of_tiles_marked_without_stage1(torch_gpu):
 
 def test_listed_stage2_equals_full_stage2(torch_gpu):
     """Stage 2 with each tile diagonal launched over a list of its tiles
-    (live tiles first, the others only for their corner copies) gives the
-    outputs of the full stage 2, bit for bit, on a batch with 40% of the
-    tiles marked as having had no stage 1, and with none marked."""
+    (live tiles first, the others only for their corner copies), and as one
+    tile-DAG launch over the live tiles, gives the outputs of the full stage
+    2, bit for bit, on a batch with 40% of the tiles marked as having had no
+    stage 1, and with none marked."""
     from parflexdtw_amd import engine
     from parflexdtw_amd.engine import BatchRunner
     torch = torch_gpu
@@ -344,7 +345,7 @@ def test_listed_stage2_equals_full_stage2(torch_gpu):
         tiles = plan.tiles.copy()
         tiles[pruned, 0] = -1
         outs = []
-        for listed in (False, True):
+        for listed in (False, True, "dag"):
             w = dict(ws)
             w["tiles"] = torch.from_numpy(tiles).cuda()
             for k in ("L_arr", "olap", "bits", "path", "path_len", "best_f", "best_i",
@@ -352,20 +353,25 @@ def test_listed_stage2_equals_full_stage2(torch_gpu):
                 w[k] = torch.empty_like(ws[k])
             # the D slots of marked tiles that no step reads stay unwritten
             w["D_arr"] = torch.full_like(ws["D_arr"], -3.0)
-            if listed:
-                engine.stage2_listed(plan, br.sw, w, ~pruned)
+            if listed == "dag":
+                keep = engine.stage2_listed(plan, br.sw, w, ~pruned, dag=True)
+            elif listed:
+                keep = engine.stage2_listed(plan, br.sw, w, ~pruned)
             else:
                 engine.stage2(None, plan, br.sw, w, None)
             torch.cuda.synchronize()
+            engine.device_status()
             outs.append(w)
-        a, b = outs
-        for k in ("D_arr", "bits", "best_f", "best_i", "path_len", "cand_D", "cand_L"):
-            x, y = a[k], b[k]
-            if x.dtype == torch.float64:
-                x, y = x.view(torch.int64), y.view(torch.int64)
-            assert torch.equal(x, y), (frac, k)
-        fin = torch.isfinite(a["D_arr"]) & (a["D_arr"] != -3.0)
-        assert torch.equal(a["L_arr"][fin].view(torch.int64), b["L_arr"][fin].view(torch.int64))
-        ra, rb = engine.collect(plan, a), engine.collect(plan, b)
-        for p in range(len(pairs)):
-            assert np.array_equal(ra.paths[p], rb.paths[p]), (frac, p)
+        a = outs[0]
+        for b in outs[1:]:
+            for k in ("D_arr", "bits", "best_f", "best_i", "path_len", "cand_D", "cand_L"):
+                x, y = a[k], b[k]
+                if x.dtype == torch.float64:
+                    x, y = x.view(torch.int64), y.view(torch.int64)
+                assert torch.equal(x, y), (frac, k)
+            fin = torch.isfinite(a["D_arr"]) & (a["D_arr"] != -3.0)
+            assert torch.equal(a["L_arr"][fin].view(torch.int64),
+                               b["L_arr"][fin].view(torch.int64))
+            ra, rb = engine.collect(plan, a), engine.collect(plan, b)
+            for p in range(len(pairs)):
+                assert np.array_equal(ra.paths[p], rb.paths[p]), (frac, p)

@@ -111,3 +111,46 @@ def test_wait_limit_rejects_bad_values(torch_gpu):
     from parflexdtw_amd import _lib
     assert _lib.lib().pf_set_wait_limit(-1, None) == _lib.PF_EINVAL
     assert _lib.lib().pf_set_wait_limit(1 << 31, None) == _lib.PF_EINVAL
+
+
+def test_stage2_dag_wait_gives_up_reports_and_recovers(torch_gpu):
+    """The stage-2 tile DAG (pf_stage2_propagate_dag): at a limit of 0 polls
+    a tile whose neighbour is not done gives up at once (a 12 x 14 grid of
+    live tiles: tiles far from the origin take tickets long before their
+    neighbours finish), the launch ends and reports PF_EHIP; with the normal
+    limit its results equal the full stage 2's again."""
+    torch = torch_gpu
+    from parflexdtw_amd import _lib, engine
+    from parflexdtw_amd.engine import BatchRunner
+    from parflexdtw_amd.synth import make_pair
+    F1, F2, _ = make_pair(2200, 2600, seed=33)
+    L = 200
+    br = BatchRunner([(F1, F2)], L, STEPS, [2, 3, 3], beta=0.1, dtype=torch.bfloat16)
+    br.launch()
+    torch.cuda.synchronize()
+    plan, ws = br.plan, br.ws
+    live = np.ones(plan.ntiles, bool)
+    ref = dict(ws)
+    for k in ("D_arr", "L_arr", "best_f", "best_i", "path", "path_len"):
+        ref[k] = ws[k].clone()
+    lib = _lib.lib()
+    prev = ctypes.c_int64()
+    assert lib.pf_set_wait_limit(0, ctypes.byref(prev)) == 0
+    try:
+        keep = engine.stage2_listed(plan, br.sw, ws, live, dag=True)
+        with pytest.raises(_lib.ParflexError, match="gave up"):
+            engine.device_status()
+        engine.device_status()
+    finally:
+        assert lib.pf_set_wait_limit(prev.value, None) == 0
+    keep = engine.stage2_listed(plan, br.sw, ws, live, dag=True)
+    torch.cuda.synchronize()
+    del keep
+    engine.device_status()
+    for k in ("D_arr", "best_f", "best_i", "path_len"):
+        x, y = ref[k], ws[k]
+        if x.dtype == torch.float64:
+            x, y = x.view(torch.int64), y.view(torch.int64)
+        assert torch.equal(x, y), k
+    ra, rb = engine.collect(plan, ref), engine.collect(plan, ws)
+    assert np.array_equal(ra.paths[0], rb.paths[0])
