@@ -32,8 +32,13 @@
 // ring slot, for its right neighbour to have read it), and lane 0 of wave 0
 // from the previous strip through a global exchange buffer: the strip to the
 // left publishes its last two columns row by row and bumps a per-strip row
-// counter (release, agent scope) every G rows; the strip to the right waits
-// on it (acquire) before each block of G rows.
+// counter every G rows; the strip to the right waits on it before each
+// block of G rows.  The exchanged rows and the counter are relaxed atomics
+// at agent scope, each access coherent across the XCDs' L2 caches; the
+// producer stores the counter only after its row stores have completed
+// (s_waitcnt), and the consumer reads the rows only after it has seen the
+// counter.  (A release / acquire pair at agent scope would write back or
+// invalidate the whole L2 of the XCD at every block.)
 // Work items are taken from a ticket counter in strip order, so a strip only
 // ever waits for an item that is already running: no deadlock whatever the
 // grid size or residency.
@@ -106,6 +111,37 @@ struct Xch {
 
 static_assert(sizeof(Xch<double>) == 32 && sizeof(Xch<float>) == 16,
               "pf_stage1_workspace reports these entry sizes");
+
+// Exchange rows are stored and loaded as relaxed 8-byte atomics at agent
+// scope (coherent across the XCDs without cache-wide write-backs).
+template <typename DT>
+__device__ __forceinline__ void xch_put(Xch<DT> *p, const Xch<DT> &v) {
+  constexpr int n = (int)(sizeof(Xch<DT>) / 8);
+  uint64_t w[n];
+  __builtin_memcpy(w, &v, sizeof(w));
+  uint64_t *q = reinterpret_cast<uint64_t *>(p);
+#pragma unroll
+  for (int k = 0; k < n; ++k)
+    __hip_atomic_store(q + k, w[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename DT>
+__device__ __forceinline__ Xch<DT> xch_get(const Xch<DT> *p) {
+  constexpr int n = (int)(sizeof(Xch<DT>) / 8);
+  uint64_t w[n];
+  const uint64_t *q = reinterpret_cast<const uint64_t *>(p);
+#pragma unroll
+  for (int k = 0; k < n; ++k)
+    w[k] = __hip_atomic_load(q + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  Xch<DT> v;
+  __builtin_memcpy(&v, w, sizeof(w));
+  return v;
+}
+// Publishes a strip's row counter once this wave's exchange-row stores have
+// completed.
+__device__ __forceinline__ void xch_publish(int *f, int rows) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __hip_atomic_store(f, rows, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Set by a stage-1 wave whose wait for a neighbour gave up (see k_stage1);
 // read and cleared by pf_device_status.
@@ -451,8 +487,8 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
     for (int j = 0; j < CPL; ++j) Cq[u][j] = c_at(row, j);
   }
   if (producer) {
-    xout[0] = Xch<DT>{D1[CPL - 2], D1[CPL - 1], X1[CPL - 2], X1[CPL - 1]};
-    if (R == 1) __hip_atomic_store(fout, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    xch_put(xout, Xch<DT>{D1[CPL - 2], D1[CPL - 1], X1[CPL - 2], X1[CPL - 1]});
+    if (R == 1) xch_publish(fout, 1);
   }
   if (tid == 0) {
     ctx[0][kCtxNone] = Pair<DT>{INF, INF};
@@ -513,7 +549,7 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
     const int need = (G < R) ? G : R;
     for (int n = spin_limit;
          __builtin_amdgcn_readfirstlane(
-             __hip_atomic_load(fin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) < need;
+             __hip_atomic_load(fin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < need;
          --n) {
       if (n <= 0) {
         give_up();
@@ -523,7 +559,7 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
     }
     for (int q0 = lane; q0 < G; q0 += 64) {
       if (q0 < R) {
-        const Xch<DT> v = xin[q0];
+        const Xch<DT> v = xch_get(xin + q0);
         ctx[0][kCtxLx + q0] = Pair<DT>{v.d0, v.d1};
         ctx[1][kCtxLx + q0] = Pair<DT>{v.x0, v.x1};
       }
@@ -560,7 +596,7 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
           const int need = (last < R) ? last : R;
           for (int n = spin_limit;
                __builtin_amdgcn_readfirstlane(
-                   __hip_atomic_load(fin, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) < need;
+                   __hip_atomic_load(fin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < need;
                --n) {
             if (n <= 0) {
               give_up();
@@ -571,7 +607,7 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
           for (int q0 = lane; q0 < G; q0 += 64) {
             const int q = r - 1 + q0;
             if (q < R) {
-              const Xch<DT> v = xin[q];
+              const Xch<DT> v = xch_get(xin + q);
               const int i = kCtxLx + (q & (kMaxG - 1));
               ctx[0][i] = Pair<DT>{v.d0, v.d1};
               ctx[1][i] = Pair<DT>{v.x0, v.x1};
@@ -831,9 +867,8 @@ __global__ __launch_bounds__(MAXT) void k_stage1(
       // keeps the publish counter in a scalar register
       if (producer_wave) {
         if (lane == 63) {
-          xout[r] = Xch<DT>{D1[CPL - 2], D1[CPL - 1], X1[CPL - 2], X1[CPL - 1]};
-          if (r == next_pub || r == R - 1)
-            __hip_atomic_store(fout, r + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          xch_put(xout + r, Xch<DT>{D1[CPL - 2], D1[CPL - 1], X1[CPL - 2], X1[CPL - 1]});
+          if (r == next_pub || r == R - 1) xch_publish(fout, r + 1);
         }
         if (r == next_pub) next_pub += G;
       }
@@ -1029,37 +1064,48 @@ __global__ __launch_bounds__(MAXT) void k_sweep4(const TileDesc *__restrict__ ti
     }
     load_frow(0);
   }
-  // C of rows 16b .. 16b + 15 into the wave's block
-  auto fill = [&](const int b) {
-    s4bf16x4_t a[KC];
-#pragma unroll
-    for (int kc = 0; kc < KC; ++kc) a[kc] = frow_next[kc];
-    if (16 * (b + 1) < R) load_frow(b + 1);
+  // C of rows 16b .. 16b + 15 into the wave's block (inlined: as a call its
+  // captures would go through scratch memory)
+  // KC = 1: the row features are copied and the next block's load issued
+  // first; KC > 1: column features in groups of up to 4 chunks, the next
+  // block's row features loaded after the MFMAs have read them (registers).
+  auto fill = [&](const int b) __attribute__((always_inline)) {
+    s4bf16x4_t a0 = frow_next[0];
+    if constexpr (KC == 1) {
+      if (16 * (b + 1) < R) load_frow(b + 1);
+    }
     const s4floatx4_t one = {1.0f, 1.0f, 1.0f, 1.0f};
 #pragma unroll
     for (int cb = 0; cb < kS4Cols / 16; cb += 2) {
       s4floatx4_t x = one, y = one;
       if constexpr (KC == 1) {
-        x = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fcol[cb], a[0], x, 0, 0, 0);
-        y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fcol[cb + 1], a[0], y, 0, 0, 0);
+        x = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fcol[cb], a0, x, 0, 0, 0);
+        y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fcol[cb + 1], a0, y, 0, 0, 0);
       } else {
-        s4bf16x4_t cx[KC], cy[KC];
+        constexpr int KH = KC < 4 ? KC : 4;
         const int gx = col_of(cb), gy = col_of(cb + 1);
 #pragma unroll
-        for (int kc = 0; kc < KC; ++kc) {
-          cx[kc] = s4_load4(Fbt + gx * Kp + kc * 16 + fk, 0x80008000u);
-          cy[kc] = s4_load4(Fbt + gy * Kp + kc * 16 + fk, 0x80008000u);
-        }
+        for (int k0 = 0; k0 < KC; k0 += KH) {
+          s4bf16x4_t cx[KH], cy[KH];
 #pragma unroll
-        for (int kc = 0; kc < KC; ++kc) {
-          x = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(cx[kc], a[kc], x, 0, 0, 0);
-          y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(cy[kc], a[kc], y, 0, 0, 0);
+          for (int kc = 0; kc < KH; ++kc) {
+            cx[kc] = s4_load4(Fbt + gx * Kp + (k0 + kc) * 16 + fk, 0x80008000u);
+            cy[kc] = s4_load4(Fbt + gy * Kp + (k0 + kc) * 16 + fk, 0x80008000u);
+          }
+#pragma unroll
+          for (int kc = 0; kc < KH; ++kc) {
+            x = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(cx[kc], frow_next[k0 + kc], x, 0, 0, 0);
+            y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(cy[kc], frow_next[k0 + kc], y, 0, 0, 0);
+          }
         }
       }
       *reinterpret_cast<uint2 *>(cwr + cb * 16) = uint2{bf16_pair(x[0], x[1]), bf16_pair(x[2], x[3])};
       *reinterpret_cast<uint2 *>(cwr + cb * 16 + 16) =
           uint2{bf16_pair(y[0], y[1]), bf16_pair(y[2], y[3])};
       __builtin_amdgcn_sched_barrier(0);
+    }
+    if constexpr (KC > 1) {
+      if (16 * (b + 1) < R) load_frow(b + 1);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();

@@ -93,10 +93,11 @@ def stage1_config(plan, n_cu=256):
         return S, nw, G
     # Workgroups of up to 4 waves, 2 columns per lane (512-column strips):
     # the best measured shape for both one 16000 x 18209 pair at L = 4000
-    # (8 strips per tile) and 1195 tiles at L = 500 (one workgroup per tile).
+    # (8 strips per tile, exchange blocks of 16 rows) and 1195 tiles at
+    # L = 500 (one workgroup per tile).
     nw = max(1, min(4, -(-L // 128)))
     S = -(-L // (nw * 128))
-    return S, nw, int(env.get("PARFLEX_S1_G", "32"))
+    return S, nw, int(env.get("PARFLEX_S1_G", "16"))
 
 
 def stage1(C_dev, plan, sw, ws, stream=None, config=None, feat=None):
