@@ -58,7 +58,7 @@ METRIC = "DP GCUPS + pairs/s, batched/learned vs plain GPU at >=99.5% path agree
 TRAFFIC_BYTES = {("chroma_batch256_N3000-20000", 500, "materialised"): (83.97 + 12.84) * 1e9,
                  ("chroma_learned128_N20000-60000_L2000", 2000, "materialised"):
                      (46.70 + 6.21) * 1e9,
-                 ("chroma_learned128_N20000-60000_L2000", 2000, "fused"): (0.740 + 6.148) * 1e9}
+                 ("chroma_learned128_N20000-60000_L2000", 2000, "fused"): (0.743 + 6.148) * 1e9}
 # VALU wave instructions per computed cell of the stage-1 kernel, measured
 # with rocprofv3 PMC (SQ_INSTS_VALU / cells; profiles/pmc_valu_*.md).  The
 # bench states the kernel's VALU instruction rate per SIMD and cycle (at the
@@ -66,7 +66,7 @@ TRAFFIC_BYTES = {("chroma_batch256_N3000-20000", 500, "materialised"): (83.97 + 
 # beside the rate a dense FP32 FMA loop (pf_valu_probe) reaches in the same
 # run.
 VALU_INSTS_PER_CELL = {("chroma_learned128_N20000-60000_L2000", 2000, "fused"):
-                           1.0139e10 / 2.2747e10}
+                           8.8873e9 / 2.2747e10}
 HBM_PEAK_GBS = 8000.0
 STEPS, WEIGHTS, BETA = [(1, 1), (1, 2), (2, 1)], [2, 3, 3], 0.1
 WORKLOADS = ["chroma_learned128_N20000-60000_L2000", "chroma_batch256_N3000-20000",
@@ -170,7 +170,7 @@ def _feat_kernel(cfg, L, K=12):
     S, nw = int(cfg[0]), int(cfg[1])
     cpl = 2 if S * nw * 128 >= L else 4
     if S == 1 and cpl == 4 and K <= 128:
-        return "k_sweep4<1024>"
+        return f"k_sweep4<1024,{-(-K // 16)}>"
     return f"k_stage1<{cpl},__hip_bfloat16,float,false,true,true,{1024 if cpl == 2 else 512}>"
 
 
@@ -341,7 +341,7 @@ def bench_learned(args, torch):
     out["evaluation"] = _evaluation(torch, gts, {
         "learned": (res.paths, gts), "plain_fp64": ([wp for _, wp in plain], gts)})
     fused = args.cost == "fused"
-    kname = ("k_sweep4<1024>" if fused else
+    kname = ("k_sweep4<1024,1>" if fused else
              "k_stage1<2,__hip_bfloat16,float,false,true,false,1024>")
     out["roofline"] = _roofline(st["cells_computed"], s1_ms, 0.25 if fused else 2.25,
                                 kname) if s1_ms else None
