@@ -50,10 +50,12 @@ struct PoolSeg {
 // batch.  With FB, the thread of frame i also writes the frame's BF16
 // features Fb[i*Kp + k] = bf16(F[k*ldf + i] / (sqrt(sum_k F^2) + 1e-9)),
 // zero padded to Kp (a multiple of 16), from the same loads: the values and
-// layout of k_cost_prep_bf16's frame-major form, bit for bit.
+// layout of k_cost_prep_bf16's frame-major form, bit for bit.  With REG (K
+// <= 16) a frame's features are read once, into registers, for both the
+// norm and the values; otherwise they are read twice.
 constexpr int kPoolBlocks = 8;                 // pooled blocks per workgroup
 constexpr int kPoolStride = 36;                // floats per block row in LDS (32 + pad)
-template <bool FB>
+template <bool FB, bool REG>
 __global__ __launch_bounds__(256) void k_pool_features(const double *__restrict__ F, int64_t K,
                                                        int64_t ldf,
                                                        const PoolSeg *__restrict__ segs,
@@ -81,9 +83,18 @@ __global__ __launch_bounds__(256) void k_pool_features(const double *__restrict_
   const int64_t i = sg.f_off + fi;
   const int64_t cnt = (b * 32 + 32 < sg.n) ? 32 : sg.n - b * 32;
   double s = 0.0;
-  for (int64_t k = 0; k < K; ++k) {
-    const double x = in ? F[k * ldf + i] : 0.0;
-    s = s + x * x;
+  double xr[REG ? 16 : 1];
+  if constexpr (REG) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) xr[k] = (in && k < K) ? F[k * ldf + i] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (k < K) s = s + xr[k] * xr[k];
+  } else {
+    for (int64_t k = 0; k < K; ++k) {
+      const double x = in ? F[k * ldf + i] : 0.0;
+      s = s + x * x;
+    }
   }
   const double nrm = sqrt(s) + 1e-9;
   const double inv = 1.0 / (nrm * (double)(cnt > 0 ? cnt : 1));
@@ -91,12 +102,18 @@ __global__ __launch_bounds__(256) void k_pool_features(const double *__restrict_
   const int rk = t >> 3, rb = t & 7;   // reducing thread: feature rk of block rb
   for (int64_t k0 = 0; k0 < K; k0 += 16) {
     const int kn = (int)((K - k0) < 16 ? (K - k0) : 16);
-    if (FB) {
+    if (REG && !FB) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (q < kn) v[q][slot] = in ? (float)(xr[q] * inv) : 0.0f;
+    } else if (FB) {
       uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         if (q < kn) {
-          const double x = in ? F[(k0 + q) * ldf + i] : 0.0;
+          double x;
+          if constexpr (REG) x = xr[q];
+          else x = in ? F[(k0 + q) * ldf + i] : 0.0;
           v[q][slot] = in ? (float)(x * inv) : 0.0f;
           const uint32_t h = __builtin_bit_cast(uint16_t, __float2bfloat16((float)(x / nrm)));
           w[q >> 1] |= h << (16 * (q & 1));
@@ -1475,7 +1492,8 @@ extern "C" int pf_pool_features(const double *d_F, int64_t K, int64_t n, int poo
   PF_REQUIRE(pool == 32, "pf_pool_features: the pooling factor is 32");
   const int64_t nb = (n + pool - 1) / pool;
   const PoolSeg one{0, n, 0, 0};
-  hipLaunchKernelGGL(k_pool_features<false>, dim3((unsigned)((nb + kPoolBlocks - 1) / kPoolBlocks)), dim3(256), 0, as_stream(stream), d_F,
+  auto kern = K <= 16 ? k_pool_features<false, true> : k_pool_features<false, false>;
+  hipLaunchKernelGGL(kern, dim3((unsigned)((nb + kPoolBlocks - 1) / kPoolBlocks)), dim3(256), 0, as_stream(stream), d_F,
                      K, n, (const PoolSeg *)nullptr, (int64_t)1, one, d_Fp, nb, nb,
                      (__hip_bfloat16 *)nullptr, (int64_t)0);
   PF_CHECK_HIP(hipGetLastError());
@@ -1488,7 +1506,8 @@ extern "C" int pf_pool_features_batch(const double *d_F, int64_t K, int64_t n,
   PF_REQUIRE(d_F && d_seg && d_Fp && K >= 1 && n >= 1 && nseg >= 1 && nb >= 1,
              "pf_pool_features_batch: bad arguments");
   PF_REQUIRE(pool == 32, "pf_pool_features_batch: the pooling factor is 32");
-  hipLaunchKernelGGL(k_pool_features<false>, dim3((unsigned)((nb + kPoolBlocks - 1) / kPoolBlocks)), dim3(256), 0, as_stream(stream), d_F,
+  auto kern = K <= 16 ? k_pool_features<false, true> : k_pool_features<false, false>;
+  hipLaunchKernelGGL(kern, dim3((unsigned)((nb + kPoolBlocks - 1) / kPoolBlocks)), dim3(256), 0, as_stream(stream), d_F,
                      K, n, reinterpret_cast<const PoolSeg *>(d_seg), nseg, PoolSeg{0, 0, 0, 0},
                      d_Fp, nb, nb, (__hip_bfloat16 *)nullptr, (int64_t)0);
   PF_CHECK_HIP(hipGetLastError());
@@ -1504,7 +1523,8 @@ extern "C" int pf_pool_features_bf16_batch(const double *d_F, int64_t K, int64_t
   PF_REQUIRE(reinterpret_cast<uintptr_t>(d_Fb) % 16 == 0,
              "pf_pool_features_bf16_batch: Fb must be 16-byte aligned");
   const int64_t Kp = (K + 15) / 16 * 16;
-  hipLaunchKernelGGL(k_pool_features<true>, dim3((unsigned)((nb + kPoolBlocks - 1) / kPoolBlocks)), dim3(256), 0, as_stream(stream), d_F,
+  auto kern = K <= 16 ? k_pool_features<true, true> : k_pool_features<true, false>;
+  hipLaunchKernelGGL(kern, dim3((unsigned)((nb + kPoolBlocks - 1) / kPoolBlocks)), dim3(256), 0, as_stream(stream), d_F,
                      K, n, reinterpret_cast<const PoolSeg *>(d_seg), nseg, PoolSeg{0, 0, 0, 0},
                      d_Fp, nb, nb, static_cast<__hip_bfloat16 *>(d_Fb), Kp);
   PF_CHECK_HIP(hipGetLastError());

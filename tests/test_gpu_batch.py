@@ -375,3 +375,35 @@ def test_listed_stage2_equals_full_stage2(torch_gpu):
             ra, rb = engine.collect(plan, a), engine.collect(plan, b)
             for p in range(len(pairs)):
                 assert np.array_equal(ra.paths[p], rb.paths[p]), (frac, p)
+
+
+def test_batch_runner_stage2_dag_equals_diagonal_launches(torch_gpu):
+    """BatchRunner(s2_dag=True): stage 2's propagation as one tile-DAG launch
+    over every tile gives the per-diagonal launches' outputs bit for bit,
+    eager and replayed from HIP graphs (three replays)."""
+    from parflexdtw_amd.engine import BatchRunner, collect
+    torch = torch_gpu
+    pairs = _pairs([(1500, 1700), (900, 1250), (2100, 1900), (300, 2500)], 778)
+    outs = []
+    for dag in (False, True):
+        br = BatchRunner(pairs, 300, STEPS, WEIGHTS, beta=0.1, dtype=torch.bfloat16, s2_dag=dag)
+        br.launch()
+        torch.cuda.synchronize()
+        outs.append((br, collect(br.plan, br.ws)))
+    (a, ra), (b, rb) = outs
+    for k in ("D_arr", "best_f", "best_i", "path_len"):
+        x, y = a.ws[k], b.ws[k]
+        if x.dtype == torch.float64:
+            x, y = x.view(torch.int64), y.view(torch.int64)
+        assert torch.equal(x, y), k
+    for p in range(len(pairs)):
+        assert np.array_equal(ra.paths[p], rb.paths[p]), p
+    b.capture()
+    for _ in range(3):
+        b.ws["D_arr"].fill_(-3.0)
+        b.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(a.ws["D_arr"].view(torch.int64), b.ws["D_arr"].view(torch.int64))
+        rr = collect(b.plan, b.ws)
+        for p in range(len(pairs)):
+            assert np.array_equal(ra.paths[p], rr.paths[p]), p
