@@ -82,65 +82,74 @@ __global__ __launch_bounds__(256) void k_s2_prep(const PairDesc *__restrict__ pa
                                                  double *__restrict__ Larr,
                                                  double *__restrict__ olap,
                                                  uint32_t *__restrict__ bits, int64_t bitw,
-                                                 int32_t *__restrict__ dag) {
-  // One workgroup per tile: its 2L slots, and its two start bitmaps built in
-  // LDS and stored whole (the workgroup owns them: no global atomics and no
-  // clearing pass).  With dag (k_s2_dag follows): the tile's done flag and,
-  // in workgroup 0, the ticket counter after the flags are cleared.
+                                                 int32_t *__restrict__ dag, int64_t ntiles) {
+  // A workgroup per tile at a time (tiles blockIdx.x, + gridDim.x, ...): its
+  // 2L slots, and its two start bitmaps built in LDS and stored whole (the
+  // workgroup owns them: no global atomics and no clearing pass).  With dag
+  // (k_s2_dag follows): the tile's done flag and, in workgroup 0, the ticket
+  // counter after the flags are cleared.
   __shared__ uint32_t lb[2][kMaxBitw];
-  const int64_t t = blockIdx.x;
-  if (dag && threadIdx.x == 0) {
-    dag[t] = 0;
-    if (t == 0) dag[gridDim.x] = 0;
-  }
-  for (int w = threadIdx.x; w < 2 * bitw; w += blockDim.x) lb[w >= bitw][w - (w >= bitw) * bitw] = 0u;
-  __syncthreads();
-  const TileDesc td = tiles[t];
-  // a tile without stage 1 (column 0 < 0): P = 0, C = 0, arrays not read
-  const bool live = td.c_off >= 0;
-  bool need0 = true, need1 = true;  // edges whose every slot may be read
-  if (!live) {
-    const PairDesc pd = pairs[td.pair];
-    const int64_t i = td.ij >> 32, j = td.ij & 0xffffffffll;
-    need0 = i + 1 >= pd.n1 || tiles[t + pd.n2].c_off >= 0;
-    need1 = j + 1 >= pd.n2 || tiles[t + 1].c_off >= 0;
-  }
-  for (int64_t rem = threadIdx.x; rem < 2 * L; rem += blockDim.x) {
-    const int64_t q = t * 2 * L + rem;
-    const int e = rem >= L ? 1 : 0;
-    const int64_t pos = rem - (int64_t)e * L;
+  if (dag && blockIdx.x == 0 && threadIdx.x == 0) dag[ntiles] = 0;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    if (dag && threadIdx.x == 0) dag[t] = 0;
+    for (int w = threadIdx.x; w < 2 * bitw; w += blockDim.x) lb[w >= bitw][w - (w >= bitw) * bitw] = 0u;
+    __syncthreads();
+    const TileDesc td = tiles[t];
+    // a tile without stage 1 (column 0 < 0): P = 0, C = 0, arrays not read
+    const bool live = td.c_off >= 0;
+    bool need0 = true, need1 = true;  // edges whose every slot may be read
     if (!live) {
-      // A tile without stage 1: its L slots are read only where a corner
-      // copy or the sync has written them (an unreached slot, D = +inf, is
-      // never read), and its overlap costs not at all (k_s2_diag takes 0).
-      const int64_t last = e == 0 ? td.W - 1 : td.R - 1;
-      if ((e == 0 ? need0 : need1) || pos == 0 || pos == last) Darr[q] = __builtin_huge_val();
-      continue;
+      const PairDesc pd = pairs[td.pair];
+      const int64_t i = td.ij >> 32, j = td.ij & 0xffffffffll;
+      need0 = i + 1 >= pd.n1 || tiles[t + pd.n2].c_off >= 0;
+      need1 = j + 1 >= pd.n2 || tiles[t + 1].c_off >= 0;
     }
-    // (with dag, k_s2_dag writes every slot of the tile and takes the
-    // overlap costs from eC itself)
-    if (!dag) {
-      Darr[q] = __builtin_huge_val();
-      Larr[q] = __builtin_huge_val();
+    for (int64_t rem = threadIdx.x; rem < 2 * L; rem += blockDim.x) {
+      const int64_t q = t * 2 * L + rem;
+      const int e = rem >= L ? 1 : 0;
+      const int64_t pos = rem - (int64_t)e * L;
+      if (!live) {
+        // A tile without stage 1: its L slots are read only where a corner
+        // copy or the sync has written them (an unreached slot, D = +inf, is
+        // never read), and its overlap costs not at all (k_s2_diag takes 0).
+        const int64_t last = e == 0 ? td.W - 1 : td.R - 1;
+        if ((e == 0 ? need0 : need1) || pos == 0 || pos == last) Darr[q] = __builtin_huge_val();
+        continue;
+      }
+      // (with dag, k_s2_dag writes every slot of the tile and takes the
+      // overlap costs from eC itself)
+      if (!dag) {
+        Darr[q] = __builtin_huge_val();
+        Larr[q] = __builtin_huge_val();
+      }
+      const int64_t elen = e == 0 ? td.W : td.R;
+      if (pos >= elen) {
+        if (!dag) olap[q] = 0.0;
+      } else {
+        const int64_t s = eP[q];
+        if (!dag) olap[q] = s > 0 ? eC[td.e_off + s] : eC[td.e_off + L - s];
+        if (s > 0) atomicOr(&lb[0][s >> 5], 1u << (s & 31));
+        else atomicOr(&lb[1][(-s) >> 5], 1u << ((-s) & 31));
+      }
     }
-    const int64_t elen = e == 0 ? td.W : td.R;
-    if (pos >= elen) {
-      if (!dag) olap[q] = 0.0;
-    } else {
-      const int64_t s = eP[q];
-      if (!dag) olap[q] = s > 0 ? eC[td.e_off + s] : eC[td.e_off + L - s];
-      if (s > 0) atomicOr(&lb[0][s >> 5], 1u << (s & 31));
-      else atomicOr(&lb[1][(-s) >> 5], 1u << ((-s) & 31));
+    // a tile without stage 1 has P = 0 on every edge cell: start (0, 0) only
+    if (!live && threadIdx.x == 0) lb[1][0] = 1u;
+    __syncthreads();
+    for (int w = threadIdx.x; w < 2 * bitw; w += blockDim.x) {
+      const int m = w >= bitw ? 1 : 0;
+      const int64_t k = w - m * bitw;
+      bits[(t * 2 + m) * bitw + k] = lb[m][k];
     }
+    // lb is cleared for the next tile only after every thread has stored it
+    __syncthreads();
   }
-  // a tile without stage 1 has P = 0 on every edge cell: start (0, 0) only
-  if (!live && threadIdx.x == 0) lb[1][0] = 1u;
-  __syncthreads();
-  for (int w = threadIdx.x; w < 2 * bitw; w += blockDim.x) {
-    const int m = w >= bitw ? 1 : 0;
-    const int64_t k = w - m * bitw;
-    bits[(t * 2 + m) * bitw + k] = lb[m][k];
-  }
+}
+
+// Workgroups of k_s2_prep: one per tile (fewer workgroups looping over the
+// tiles were slower: 214 against 180 us for the learned pass's 51 831 tiles).
+static unsigned prep_grid(int64_t ntiles) {
+  const int64_t g = 0x7fffffff;
+  return (unsigned)(ntiles < g ? ntiles : g);
 }
 
 __device__ __forceinline__ bool bit_at(const uint32_t *bits, int64_t bitw, int64_t t, int map,
@@ -879,8 +888,10 @@ __global__ void k_s2_chain(int from_best, const int64_t *__restrict__ req, int64
       ec = g.cols - 1;
     }
     if (er < 0 || ec < 0 || er >= g.rows || ec >= g.cols) break;
-    const int64_t S = tiles[t].c_off < 0 ? 0
-                      : (er == g.rows - 1) ? eP[(t * 2 + 0) * L + ec] : eP[(t * 2 + 1) * L + er];
+    // both loads unconditional (a pruned tile's slot is in bounds; its value
+    // is not used), so one memory round trip per tile
+    const int64_t Sl = (er == g.rows - 1) ? eP[(t * 2 + 0) * L + ec] : eP[(t * 2 + 1) * L + er];
+    const int64_t S = tiles[t].c_off < 0 ? 0 : Sl;
     const int64_t sr = S >= 0 ? 0 : -S, sc = S >= 0 ? S : 0;
     int64_t *en = ch + ns * 8;
     en[0] = t;
@@ -929,15 +940,18 @@ __global__ void k_s2_chain(int from_best, const int64_t *__restrict__ req, int64
 }
 
 // Concatenate the segments of request q, drop the repeated overlap cell at
-// each join, reverse (Parflex.py:1060-1062, :1103).
-__global__ __launch_bounds__(256) void k_s2_compact(const int64_t *__restrict__ chain,
-                                                    int64_t *__restrict__ cnt,
-                                                    const int64_t *__restrict__ pts, int64_t L,
-                                                    const PairDesc *__restrict__ pairs,
-                                                    int use_pair_off, int64_t path_cap,
-                                                    int64_t *__restrict__ path) {
+// each join, reverse (Parflex.py:1060-1062, :1103).  One workgroup per
+// request: the segments' lengths and join repeats in parallel, a serial
+// prefix over <= 101 values, then every output point in one pass (its
+// segment found by bisection over the prefix), 16 bytes a point.
+constexpr int kCompactThreads = 1024;
+__global__ __launch_bounds__(kCompactThreads) void k_s2_compact(
+    const int64_t *__restrict__ chain, int64_t *__restrict__ cnt,
+    const int64_t *__restrict__ pts, int64_t L, const PairDesc *__restrict__ pairs,
+    int use_pair_off, int64_t path_cap, int64_t *__restrict__ path) {
   __shared__ int64_t off[102];
   __shared__ int64_t skip[101];
+  __shared__ int64_t len_s[101];
   __shared__ int64_t total;
   const int64_t q = blockIdx.x;
   const int64_t ns = cnt[q];
@@ -953,9 +967,6 @@ __global__ __launch_bounds__(256) void k_s2_compact(const int64_t *__restrict__ 
     cap = path_cap;
     dst = path + q * path_cap * 2;
   }
-  // Per segment (in parallel): its length and whether its first point repeats
-  // the previous segment's last point; then a serial prefix over <= 101 values.
-  __shared__ int64_t len_s[101];
   for (int64_t s = threadIdx.x; s < ns; s += blockDim.x) {
     const int64_t n = ch[s * 8 + 7];
     int64_t sk = 0;
@@ -977,21 +988,24 @@ __global__ __launch_bounds__(256) void k_s2_compact(const int64_t *__restrict__ 
       off[s] = o;
       o += len_s[s] - skip[s];
     }
+    off[ns] = o;
     total = o < cap ? o : cap;
   }
   __syncthreads();
   const int64_t n_out = total;
-  for (int64_t s = 0; s < ns; ++s) {
-    const int64_t n = ch[s * 8 + 7];
-    const int64_t *src = pts + (q * 101 + s) * stride;
-    for (int64_t k = skip[s] + threadIdx.x; k < n; k += blockDim.x) {
-      const int64_t pos = off[s] + (k - skip[s]);
-      if (pos < n_out) {
-        const int64_t o = n_out - 1 - pos;
-        dst[2 * o] = src[2 * k];
-        dst[2 * o + 1] = src[2 * k + 1];
-      }
+  typedef long long ll2_t __attribute__((ext_vector_type(2)));
+  for (int64_t pos = threadIdx.x; pos < n_out; pos += blockDim.x) {
+    // the segment s with off[s] <= pos < off[s + 1] (segments emitting
+    // nothing share their offset with the next one: the last such s)
+    int64_t lo = 0, hi = ns - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= pos) lo = mid;
+      else hi = mid - 1;
     }
+    const int64_t k = skip[lo] + (pos - off[lo]);
+    const ll2_t v = *reinterpret_cast<const ll2_t *>(pts + (q * 101 + lo) * stride + 2 * k);
+    *reinterpret_cast<ll2_t *>(dst + 2 * (n_out - 1 - pos)) = v;
   }
   __syncthreads();
   if (threadIdx.x == 0) cnt[q] = n_out;
@@ -1077,7 +1091,8 @@ static int stitch_common(hipStream_t st, int from_best, const int64_t *d_req, in
   PF_CHECK_HIP(hipGetLastError());
   launch_s2_walk(dim3(101, (unsigned)nreq), st, tiles, L, ss, Bp, chain, cnt, pts);
   PF_CHECK_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_s2_compact, dim3((unsigned)nreq), dim3(256), 0, st, chain, cnt, pts, L,
+  hipLaunchKernelGGL(k_s2_compact, dim3((unsigned)nreq), dim3(kCompactThreads), 0, st, chain, cnt,
+                     pts, L,
                      pairs, use_pair_off, path_cap, path);
   PF_CHECK_HIP(hipGetLastError());
   return PF_OK;
@@ -1123,8 +1138,9 @@ extern "C" int pf_stage2_phases(int phases, const double *d_edge_C, const int64_
   const TileDesc *tiles = reinterpret_cast<const TileDesc *>(d_tiles);
   const int64_t bitw = (L + 32) / 32;
   if (phases & PF_S2_PROPAGATE) {
-    hipLaunchKernelGGL(k_s2_prep, dim3((unsigned)ntiles), dim3(256), 0, st, pairs, tiles, L, d_edge_P,
-                       d_edge_C, d_D_arr, d_L_arr, d_olap, d_bits, bitw, (int32_t *)nullptr);
+    hipLaunchKernelGGL(k_s2_prep, dim3(prep_grid(ntiles)), dim3(256), 0, st, pairs, tiles, L,
+                       d_edge_P, d_edge_C, d_D_arr, d_L_arr, d_olap, d_bits, bitw,
+                       (int32_t *)nullptr, ntiles);
     PF_CHECK_HIP(hipGetLastError());
     for (int64_t d = 0; d < max_diag; ++d) {
       const int64_t ntd = (d + 1) < max_tile_diag ? (d + 1) : max_tile_diag;
@@ -1182,8 +1198,9 @@ extern "C" int pf_stage2_propagate_listed(const double *d_edge_C, const int64_t 
   const PairDesc *pairs = reinterpret_cast<const PairDesc *>(d_pairs);
   const TileDesc *tiles = reinterpret_cast<const TileDesc *>(d_tiles);
   const int64_t bitw = (L + 32) / 32;
-  hipLaunchKernelGGL(k_s2_prep, dim3((unsigned)ntiles), dim3(256), 0, st, pairs, tiles, L, d_edge_P,
-                     d_edge_C, d_D_arr, d_L_arr, d_olap, d_bits, bitw, (int32_t *)nullptr);
+  hipLaunchKernelGGL(k_s2_prep, dim3(prep_grid(ntiles)), dim3(256), 0, st, pairs, tiles, L,
+                     d_edge_P, d_edge_C, d_D_arr, d_L_arr, d_olap, d_bits, bitw,
+                     (int32_t *)nullptr, ntiles);
   PF_CHECK_HIP(hipGetLastError());
   for (int64_t d = 0; d < ndiag; ++d) {
     const int64_t nl = h_diag[3 * d + 1], np = h_diag[3 * d + 2];
@@ -1216,9 +1233,9 @@ extern "C" int pf_stage2_propagate_dag(const double *d_edge_C, const int64_t *d_
   const PairDesc *pairs = reinterpret_cast<const PairDesc *>(d_pairs);
   const TileDesc *tiles = reinterpret_cast<const TileDesc *>(d_tiles);
   const int64_t bitw = (L + 32) / 32;
-  hipLaunchKernelGGL(k_s2_prep, dim3((unsigned)ntiles), dim3(256), 0, st, pairs, tiles, L,
+  hipLaunchKernelGGL(k_s2_prep, dim3(prep_grid(ntiles)), dim3(256), 0, st, pairs, tiles, L,
                      d_edge_P, d_edge_C, d_D_arr, d_L_arr, (double *)nullptr, d_bits, bitw,
-                     d_work);
+                     d_work, ntiles);
   PF_CHECK_HIP(hipGetLastError());
   int dev = 0, cus = 0;
   PF_CHECK_HIP(hipGetDevice(&dev));
