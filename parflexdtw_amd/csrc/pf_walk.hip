@@ -46,18 +46,30 @@ __global__ __launch_bounds__(64) void k_s2_walk(const TileDesc *__restrict__ til
   int steptab = 0;
   for (int z = 0; z < 4; ++z) steptab |= (ss.dr[z] | (ss.dc[z] << 2)) << (4 * z);
   steptab = __builtin_amdgcn_readfirstlane(steptab);
-  int nc = 0;       // step codes appended
-  // codes enter acc from the top (two bits per step), so after 16 steps
-  // code k of the word sits at bits 2k
+  // the same per code as a byte for the fast steps: the row step doubled in
+  // bits 0-5 (63 for a code that is no step) and the column step in bits 6-7
+  uint32_t ftab = 0;
+  for (int z = 0; z < 4; ++z)
+    ftab |= (uint32_t)(z < ss.K ? (2 * ss.dr[z]) | (ss.dc[z] << 6) : 63) << (8 * z);
+  ftab = __builtin_amdgcn_readfirstlane(ftab);
+  // Codes enter acc from the top (two bits per step), so after 16 steps code
+  // k of the word sits at bits 2k.  nw words are complete; rem + 1 codes are
+  // still missing from the current one.
+  int nw = 0, rem = 15;
   uint32_t acc = 0, cw = 0;
+  auto flush = [&]() __attribute__((always_inline)) {
+    cw = (lane == (nw & 63)) ? acc : cw;
+    if ((nw & 63) == 63) codes[(nw >> 6) * 64 + lane] = cw;
+    ++nw;
+    acc = 0;
+    rem = 15;
+  };
   const int sr1 = sr + 1;
   // Window of packed back-pointers: word rows rb0 .. rb0 + kWalkRB - 1,
-  // columns wc0 .. wc0 + 127 (two registers per word row); the block of 16
-  // rows x 64 columns holding (r, c) is word row r >> 4 in lanes
-  // c_lo .. c_lo + 63 of wcur.
-  int rb0, wc0, r_lo, c_lo;
-  uint32_t w[2 * kWalkRB], wcur;
-  auto load_window = [&]() {
+  // columns wc0 .. wc0 + 127 (two registers per word row).
+  int rb0, wc0;
+  uint32_t w[2 * kWalkRB];
+  auto load_window = [&]() __attribute__((always_inline)) {
     rb0 = (r >> 4) - (kWalkRB - 1) > 0 ? (r >> 4) - (kWalkRB - 1) : 0;
     wc0 = c - 127 > 0 ? c - 127 : 0;
     const int rmax = r >> 4;
@@ -70,54 +82,107 @@ __global__ __launch_bounds__(64) void k_s2_walk(const TileDesc *__restrict__ til
         w[b * 2 + h] = base[(int64_t)(rb0 + b <= rmax ? rb0 + b : rmax) * cols];
     }
   };
-  auto select_block = [&]() {
-    const int qi = (r >> 4) - rb0, ci = c - wc0;
-    const int ns = qi * 2 + (ci >> 6);
-    wcur = w[0];
+  // The block the steps read: 64 columns from c_lo (lane = column - c_lo)
+  // and 32 rows from r_lo, word rows r_lo / 16 (wlo) and r_lo / 16 + 1
+  // (whi).  The cell is in the upper word row unless the window's bottom
+  // word row is the tile's first; the walk only goes down and left, so only
+  // the lower bounds are ever crossed.
+  int r_lo, c_lo;
+  uint32_t wlo, whi;
+  auto select_block = [&]() __attribute__((always_inline)) {
+    const int qi = (r >> 4) - rb0, h = (c - wc0) >> 6;
+    const int ql = qi > 0 ? qi - 1 : 0;
+    const int nh = qi * 2 + h, nl = ql * 2 + h;
+    whi = w[0];
+    wlo = w[0];
 #pragma unroll
-    for (int z = 1; z < 2 * kWalkRB; ++z) wcur = (ns == z) ? w[z] : wcur;
-    r_lo = r & ~15;
-    c_lo = wc0 + (ci & ~63);
+    for (int z = 1; z < 2 * kWalkRB; ++z) {
+      whi = (nh == z) ? w[z] : whi;
+      wlo = (nl == z) ? w[z] : wlo;
+    }
+    r_lo = (rb0 + ql) * 16;
+    c_lo = wc0 + h * 64;
+  };
+  // (r, c) into the block; the window is reloaded when the cell is outside
+  // it or in its bottom word row with more rows below
+  auto ensure_block = [&]() __attribute__((always_inline)) {
+    if (__builtin_expect(((r - r_lo) | (c - c_lo)) < 0, 0)) {
+      const int qi = (r >> 4) - rb0;
+      if ((qi | (c - wc0)) < 0 || (qi == 0 && rb0 > 0)) load_window();
+      select_block();
+    }
   };
   load_window();
   select_block();
-  // The loop's only exit is a stop.  Every step lowers the row (1 <= dr <= 2,
+  // The loop's only exits are stops.  Every step lowers the row (1 <= dr <= 2,
   // enforced when the step set is made), so a step off the tile (pr < 0)
   // ends the walk within R steps (R: the tile's rows).
   for (;;) {
-    int dr_ = r - r_lo, dc_ = c - c_lo;
-    if (__builtin_expect((dr_ | dc_) < 0, 0)) {
-      if ((((r >> 4) - rb0) | (c - wc0)) < 0) load_window();
-      select_block();
-      dr_ = r - r_lo;
-      dc_ = c - c_lo;
+    // One step with the exact tests: stop at the start, on an invalid code or
+    // on a step off the tile (dr >= 1 and dc >= 0: only lower bounds).
+    ensure_block();
+    {
+      const int dl = c - c_lo;
+      const uint64_t wv = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)whi, dl) << 32) |
+                          (uint32_t)__builtin_amdgcn_readlane((int)wlo, dl);
+      const int code = (int)((wv >> (2 * (r - r_lo))) & 3u);
+      const int e = steptab >> (4 * code);
+      const int pr = r - (e & 3), pc = c - ((e >> 2) & 3);
+      if ((r == sr && c == sc) || code >= ss.K || (pr | pc) < 0) break;
+      acc = (acc >> 2) | ((uint32_t)code << 30);
+      if (--rem < 0) flush();
+      r = pr;
+      c = pc;
     }
-    const uint32_t word = __builtin_amdgcn_readlane(wcur, dc_);
-    const int code = (int)((word >> (2 * dr_)) & 3u);
-    const int e = steptab >> (4 * code);
-    const int pr = r - (e & 3), pc = c - ((e >> 2) & 3);
-    // Stop at the start, on an invalid code or on a step off the tile
-    // (dr >= 1 and dc >= 0: only lower bounds).  Above the start row
-    // (r > sr) the start cannot be the current cell, so one sign test
-    // covers the common case; the exact test runs from the start row on.
-    if (__builtin_expect(((r - sr1) | (ss.K - 1 - code) | pr | pc) < 0, 0)) {
-      // integer form (no compares): the top bit of (d - 1) & ~d is set
-      // iff d == 0 for d >= 0, here d = 0 iff (r, c) == (sr, sc)
-      const int d = (r ^ sr) | (c ^ sc);
-      const int stop = ((d - 1) & ~d) | (ss.K - 1 - code) | pr | pc;
-      if (stop < 0) break;
+    // Fast steps while the cell is above the start row (there the start
+    // cannot be the current cell).  The block is shifted so that its row
+    // rbase = max(r_lo, sr + 1) is bit 0, so one sign test of the row and
+    // column offsets and of rem covers leaving the block, falling to the
+    // start row, stepping off the tile (offsets below zero), an invalid code
+    // (its row step of 63 takes the row offset below zero) and a full code
+    // word.  A step that was no step (invalid code, off the tile) is taken
+    // back after the loop and ends the walk.
+    while (r >= sr1) {
+      ensure_block();
+      const int rbase = r_lo > sr1 ? r_lo : sr1;
+      const uint64_t f = (((uint64_t)whi << 32) | wlo) >> (2 * (rbase - r_lo));
+      const uint32_t flo = (uint32_t)f, fhi = (uint32_t)(f >> 32);
+      int dr2 = 2 * (r - rbase), dcl = c - c_lo;
+      uint32_t t;  // the step's code in bits 0-1
+      for (;;) {
+        const uint64_t wv = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)fhi, dcl) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane((int)flo, dcl);
+        t = (uint32_t)(wv >> dr2);
+        const uint32_t e = ftab >> ((t << 3) & 31);
+        dr2 -= (int)(e & 63u);
+        dcl -= (int)((e >> 6) & 3u);
+        acc = __builtin_amdgcn_alignbit(t, acc, 2);
+        --rem;
+        if ((dr2 | dcl | rem) < 0) break;
+      }
+      const int last = (int)(t & 3u);
+      c = c_lo + dcl;
+      if (last >= ss.K) {
+        r = rbase + ((dr2 + 63) >> 1);
+        acc <<= 2;
+        ++rem;
+        goto walked;
+      }
+      r = rbase + (dr2 >> 1);
+      if ((r | c) < 0) {
+        const int e = steptab >> (4 * last);
+        r += e & 3;
+        c += (e >> 2) & 3;
+        acc <<= 2;
+        ++rem;
+        goto walked;
+      }
+      if (rem < 0) flush();
     }
-    acc = (acc >> 2) | ((uint32_t)code << 30);
-    if (__builtin_expect((nc & 15) == 15, 0)) {
-      cw = (lane == ((nc >> 4) & 63)) ? acc : cw;
-      acc = 0;
-      if (((nc >> 4) & 63) == 63) codes[(nc >> 10) * 64 + lane] = cw;
-    }
-    ++nc;
-    r = pr;
-    c = pc;
   }
-  const int m = nc + 1;  // cells visited, the last one included
+walked:
+  const int nc = nw * 16 + (15 - rem);  // step codes appended
+  const int m = nc + 1;                 // cells visited, the last one included
   // (sr, sc) is appended after the last visited cell unless the walk ended
   // on it (it stopped on an invalid code or a step off the tile).
   const int tail = (r != sr || c != sc) ? 1 : 0;
