@@ -138,6 +138,14 @@ __global__ __launch_bounds__(256) void k_cost_f64(const double *__restrict__ F1n
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx16_t __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float floatx2_t __attribute__((ext_vector_type(2)));
+// two floats as BF16 (round to nearest even, as __float2bfloat16) in one
+// packed conversion, a in the low half
+__device__ __forceinline__ uint32_t bf16x2_bits(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(floatx2_t{a, b}, bf16x2_t));
+}
 
 // -x of 8 bf16 values (sign bits flipped; exact)
 __device__ __forceinline__ bf16x8_t neg8(bf16x8_t v) {
@@ -156,20 +164,33 @@ __global__ void k_cost_prep_bf16(const double *__restrict__ F, int64_t K, int64_
                                  __hip_bfloat16 *__restrict__ Fb, int blocked) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // the squares summed in feature order; loads eight at a time so that they
+  // are in flight together
   double s = 0.0;
-  for (int64_t k = 0; k < K; ++k) {
-    const double v = F[k * n + i];
+  int64_t k0 = 0;
+  for (; k0 + 8 <= K; k0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = F[(k0 + q) * n + i];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s = s + v[q] * v[q];
+  }
+  for (; k0 < K; ++k0) {
+    const double v = F[k0 * n + i];
     s = s + v * v;
   }
   const double nrm = sqrt(s) + 1e-9;
   for (int64_t kc = 0; kc < Kp; kc += 16) {
+    // the chunk's 16 loads first (features past K read feature K - 1, unused)
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = F[(kc + q < K ? kc + q : K - 1) * n + i];
     uint32_t w[8];
 #pragma unroll
     for (int q = 0; q < 16; q += 2) {
       const int64_t k = kc + q;
-      const __hip_bfloat16 lo = __float2bfloat16(k < K ? (float)(F[k * n + i] / nrm) : 0.0f);
-      const __hip_bfloat16 hi =
-          __float2bfloat16(k + 1 < K ? (float)(F[(k + 1) * n + i] / nrm) : 0.0f);
+      const __hip_bfloat16 lo = __float2bfloat16(k < K ? (float)(v[q] / nrm) : 0.0f);
+      const __hip_bfloat16 hi = __float2bfloat16(k + 1 < K ? (float)(v[q + 1] / nrm) : 0.0f);
       w[q >> 1] = (uint32_t)__builtin_bit_cast(uint16_t, lo) |
                   ((uint32_t)__builtin_bit_cast(uint16_t, hi) << 16);
     }
@@ -180,16 +201,28 @@ __global__ void k_cost_prep_bf16(const double *__restrict__ F, int64_t K, int64_
   }
 }
 
+// four adjacent elements of a staged row (8 bytes as BF16, 16 as FP32)
+__device__ __forceinline__ void store4(float *p, float a, float b, float c, float d) {
+  *reinterpret_cast<floatx4_t *>(p) = floatx4_t{a, b, c, d};
+}
+__device__ __forceinline__ void store4(__hip_bfloat16 *p, float a, float b, float c, float d) {
+  *reinterpret_cast<uint2 *>(p) = uint2{bf16x2_bits(a, b), bf16x2_bits(c, d)};
+}
 __device__ __forceinline__ float cvt_out(float v, float *) { return v; }
 __device__ __forceinline__ __hip_bfloat16 cvt_out(float v, __hip_bfloat16 *) {
   return __float2bfloat16(v);
 }
 
-constexpr int kMfmaRows = 128, kMfmaCols = 256;
+// Workgroup block of C: 128 rows x 256 columns, 4 waves of 64 x 128 (2 x 4
+// sub-blocks of 32 x 32); kWJ sub-block columns per wave.  (128 x 128
+// blocks, 4 waves a SIMD instead of 2, were not faster: the row-major
+// stores of C bound the kernel.)
+constexpr int kWJ = 4;
+constexpr int kMfmaRows = 128, kMfmaCols = 2 * 32 * kWJ;
 constexpr int kTilePad = 8;  // keeps staged rows 16-byte aligned, no bank conflicts
 
-template <typename OT, bool BATCH>
-__global__ __launch_bounds__(256) void k_cost_mfma_bf16(const __hip_bfloat16 *__restrict__ A0,
+template <typename OT, bool BATCH, int WPE = 2>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_cost_mfma_bf16(const __hip_bfloat16 *__restrict__ A0,
                                                         const __hip_bfloat16 *__restrict__ B0,
                                                         int64_t n1, int64_t n2, int64_t Kp,
                                                         CostJob one,
@@ -209,7 +242,7 @@ __global__ __launch_bounds__(256) void k_cost_mfma_bf16(const __hip_bfloat16 *__
   // the last multiple of 8 keep their place).  `lin` runs over strips of
   // kStrip column blocks, each strip row by row: consecutive workgroups of
   // an XCD reuse one A row block kStrip times, and the strip's B columns
-  // (kStrip x 256 frames) stay in that XCD's L2.  A bijection on
+  // (kStrip x kMfmaCols frames) stay in that XCD's L2.  A bijection on
   // [0, nbr*nbc).
   constexpr int64_t kStrip = 16;
   const int64_t nblk = nbr * nbc, q = nblk >> 3;
@@ -228,61 +261,64 @@ __global__ __launch_bounds__(256) void k_cost_mfma_bf16(const __hip_bfloat16 *__
   const int64_t row0 = rb * kMfmaRows, col0 = cbk * kMfmaCols;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int wr = wave >> 1, wc = (wave & 1) * 128;
+  const int wr = wave >> 1, wc = (wave & 1) * 32 * kWJ;
   const int lr = lane & 31, kh = (lane >> 5) * 8;
   // Operands in the blocked layout of k_cost_prep_bf16: frame f, features
   // [kk + kh, kk + kh + 8) at ((kk/16)*n + f)*16 + kh.  Rows/columns past
   // the matrix read the last valid frame (results unused).  Element offsets
   // fit 32 bits: (n1 + n2) * Kp < 2^31 (host).
-  uint32_t ra[2], cb[4];
+  uint32_t ra[2], cb[kWJ];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     const int64_t r = row0 + wr * 64 + i * 32 + lr;
     ra[i] = (uint32_t)((jb.f1_off + (r < N ? r : N - 1)) * 16 + kh);
   }
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < kWJ; ++j) {
     const int64_t c = col0 + wc + j * 32 + lr;
     cb[j] = (uint32_t)((jb.f2_off + (c < M ? c : M - 1)) * 16 + kh);
   }
   const uint32_t sa = (uint32_t)(n1 * 16), sb = (uint32_t)(n2 * 16);
   // C = 1 - a.b as 1 + sum_k (-a_k) b_k: accumulators start at 1 and the row
   // features are negated, so the MFMA chain yields C itself
-  floatx16_t acc[2][4];
+  floatx16_t acc[2][kWJ];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < kWJ; ++j)
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 1.0f;
-  // Operands of step k+16 are loaded before the MFMAs of step k are issued.
-  bf16x8_t a[2], b[4];
-#pragma unroll
-  for (int i = 0; i < 2; ++i) a[i] = neg8(*reinterpret_cast<const bf16x8_t *>(A0 + ra[i]));
-#pragma unroll
-  for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8_t *>(B0 + cb[j]);
+  // Operands of three k-chunks in flight: chunk k + 3 is loaded into the
+  // registers of chunk k once its MFMAs are issued (the accumulation order
+  // over k is unchanged).
   const uint32_t nk = (uint32_t)(Kp >> 4);
-  for (uint32_t kc = 0; kc < nk; ++kc) {
-    bf16x8_t an[2], bn[4];
-    const bool more = kc + 1 < nk;
-    if (more) {
-      const __hip_bfloat16 *An = A0 + (kc + 1) * sa;
-      const __hip_bfloat16 *Bn = B0 + (kc + 1) * sb;
+  bf16x8_t a[3][2], b[3][kWJ];
+  auto load_chunk = [&](const uint32_t kc, bf16x8_t(&ak)[2], bf16x8_t(&bk)[kWJ])
+                        __attribute__((always_inline)) {
+    const __hip_bfloat16 *An = A0 + kc * sa;
+    const __hip_bfloat16 *Bn = B0 + kc * sb;
 #pragma unroll
-      for (int i = 0; i < 2; ++i) an[i] = neg8(*reinterpret_cast<const bf16x8_t *>(An + ra[i]));
+    for (int i = 0; i < 2; ++i) ak[i] = neg8(*reinterpret_cast<const bf16x8_t *>(An + ra[i]));
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bn[j] = *reinterpret_cast<const bf16x8_t *>(Bn + cb[j]);
-    }
+    for (int j = 0; j < kWJ; ++j) bk[j] = *reinterpret_cast<const bf16x8_t *>(Bn + cb[j]);
+  };
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+  for (int d = 0; d < 3; ++d)
+    if (d < (int)nk) load_chunk(d, a[d], b[d]);
+  for (uint32_t k0 = 0; k0 < nk; k0 += 3) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-    if (more) {
+    for (int d = 0; d < 3; ++d) {
+      if (k0 + d < nk) {
+        // column features as the A operand: a lane's accumulator holds 4
+        // adjacent columns of one row (see the epilogue)
 #pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = an[i];
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = bn[j];
+          for (int j = 0; j < kWJ; ++j)
+            acc[i][j] =
+                __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[d][j], a[d][i], acc[i][j], 0, 0, 0);
+        if (k0 + d + 3 < nk) load_chunk(k0 + d + 3, a[d], b[d]);
+      }
     }
   }
   constexpr int E = 16 / (int)sizeof(OT);
@@ -291,42 +327,39 @@ __global__ __launch_bounds__(256) void k_cost_mfma_bf16(const __hip_bfloat16 *__
   for (int h = 0; h < 2; ++h) {
     if (h) __syncthreads();  // the previous half has been written out
     if (wr == h) {
+      // accumulator element v of sub-block (i, j): row i*32 + lr, column
+      // wc + j*32 + 8*(v/4) + 4*(lane/32) + v%4, so each group of four is one
+      // 4-element store
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < kWJ; ++j)
 #pragma unroll
-          for (int v = 0; v < 16; ++v)
-            tile[i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3)][wc + j * 32 + lr] =
-                cvt_out(acc[i][j][v], (OT *)nullptr);
+          for (int g = 0; g < 4; ++g)
+            store4(&tile[i * 32 + lr][wc + j * 32 + g * 8 + (lane >> 5) * 4], acc[i][j][4 * g],
+                   acc[i][j][4 * g + 1], acc[i][j][4 * g + 2], acc[i][j][4 * g + 3]);
     }
     __syncthreads();
-    // One row per wave per step: 16-byte chunks of E elements when the row's
-    // first element is 16-byte aligned in C, else one element per lane.
+    // RPI rows per wave and store instruction (LPR lanes a row): 16-byte
+    // chunks of E elements when the row's first element is 16-byte aligned
+    // in C, else element by element.
+    constexpr int LPR = kMfmaCols / E, RPI = 64 / LPR;
+    const int sub = lane / LPR, c = (lane % LPR) * E;
     const int64_t r0 = row0 + h * kHalf;
-    const int64_t rows = N - r0 < kHalf ? N - r0 : kHalf;
-    for (int rr = wave; rr < rows; rr += 4) {
-      const int64_t e0 = jb.c_off + (r0 + rr) * jb.ldc + col0;
-      OT *dst = C + e0;
-      if (vec_ok && (e0 % E) == 0) {
+    const int rows = (int)(N - r0 < kHalf ? N - r0 : kHalf);
+    for (int rr0 = wave * RPI; rr0 < rows; rr0 += 4 * RPI) {
+      const int rr = rr0 + sub;
+      if (rr < rows) {
+        const int64_t e0 = jb.c_off + (r0 + rr) * jb.ldc + col0;
+        OT *dst = C + e0;
+        if (vec_ok && (e0 % E) == 0 && c + E <= cols) {
+          const u32x4_t v = *reinterpret_cast<const u32x4_t *>(&tile[rr][c]);
+          __builtin_nontemporal_store(v, reinterpret_cast<u32x4_t *>(dst + c));
+        } else {
 #pragma unroll
-        for (int g = 0; g < kMfmaCols; g += 64 * E) {
-          const int c = g + lane * E;
-          if (c < kMfmaCols) {
-            if (c + E <= cols) {
-              const u32x4_t v = *reinterpret_cast<const u32x4_t *>(&tile[rr][c]);
-              __builtin_nontemporal_store(v, reinterpret_cast<u32x4_t *>(dst + c));
-            } else {
-#pragma unroll
-              for (int e = 0; e < E; ++e)
-                if (c + e < cols) st_nt(dst + c + e, tile[rr][c + e]);
-            }
-          }
+          for (int e = 0; e < E; ++e)
+            if (c + e < cols) st_nt(dst + c + e, tile[rr][c + e]);
         }
-      } else {
-#pragma unroll
-        for (int g = 0; g < kMfmaCols; g += 64)
-          if (g + lane < cols) st_nt(dst + g + lane, tile[rr][g + lane]);
       }
     }
   }
