@@ -527,6 +527,9 @@ def bench_plain(args, torch):
     C_dev = cost_matrix_from_features_gpu(F1, F2)
     L1, L2 = int(C_dev.shape[0]), int(C_dev.shape[1])
     plan = BatchPlan([(L1, L2)], L, beta=BETA)
+    sw = engine.StepWeights(STEPS, WEIGHTS)
+    ws = plan.allocate(dev)
+    stream = torch.cuda.current_stream()
     log(f"workload {args.workload}: {L1}x{L2}, L={L}, tiles={plan.ntiles}, cells={plan.cells}")
 
     def step(ev=None):
