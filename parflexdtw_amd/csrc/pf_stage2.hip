@@ -82,12 +82,16 @@ __global__ __launch_bounds__(256) void k_s2_prep(const PairDesc *__restrict__ pa
                                                  double *__restrict__ Larr,
                                                  double *__restrict__ olap,
                                                  uint32_t *__restrict__ bits, int64_t bitw,
-                                                 int32_t *__restrict__ dag, int64_t ntiles) {
+                                                 int32_t *__restrict__ dag, int64_t ntiles,
+                                                 int sweep_all) {
   // A workgroup per tile at a time (tiles blockIdx.x, + gridDim.x, ...): its
   // 2L slots, and its two start bitmaps built in LDS and stored whole (the
   // workgroup owns them: no global atomics and no clearing pass).  With dag
   // (k_s2_dag follows): the tile's done flag and, in workgroup 0, the ticket
-  // counter after the flags are cleared.
+  // counter after the flags are cleared.  With sweep_all (k_s2_diag in its
+  // full form follows: it writes every slot of every tile with stage 1 and
+  // takes the overlap costs from eC) only the slots of tiles without stage
+  // 1 are initialised here.
   __shared__ uint32_t lb[2][kMaxBitw];
   if (dag && blockIdx.x == 0 && threadIdx.x == 0) dag[ntiles] = 0;
   for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -116,18 +120,19 @@ __global__ __launch_bounds__(256) void k_s2_prep(const PairDesc *__restrict__ pa
         if ((e == 0 ? need0 : need1) || pos == 0 || pos == last) Darr[q] = __builtin_huge_val();
         continue;
       }
-      // (with dag, k_s2_dag writes every slot of the tile and takes the
-      // overlap costs from eC itself)
-      if (!dag) {
+      // (with dag or sweep_all, the sweep writes every slot of the tile and
+      // takes the overlap costs from eC itself)
+      const bool init = !dag && !sweep_all;
+      if (init) {
         Darr[q] = __builtin_huge_val();
         Larr[q] = __builtin_huge_val();
       }
       const int64_t elen = e == 0 ? td.W : td.R;
       if (pos >= elen) {
-        if (!dag) olap[q] = 0.0;
+        if (init) olap[q] = 0.0;
       } else {
         const int64_t s = eP[q];
-        if (!dag) olap[q] = s > 0 ? eC[td.e_off + s] : eC[td.e_off + L - s];
+        if (init) olap[q] = s > 0 ? eC[td.e_off + s] : eC[td.e_off + L - s];
         if (s > 0) atomicOr(&lb[0][s >> 5], 1u << (s & 31));
         else atomicOr(&lb[1][(-s) >> 5], 1u << ((-s) & 31));
       }
@@ -374,23 +379,32 @@ __device__ __forceinline__ bool slot_apply(const SlotPlan &sp, double *__restric
   return true;
 }
 
+// With olap null (the full form), the overlap costs come from eC and a slot
+// of a tile with stage 1 that takes no value is written +inf (k_s2_prep
+// initialised only the tiles without stage 1).
 __device__ __forceinline__ void diag_slot(const PairDesc &pd, const TileDesc *__restrict__ tiles,
                                           int64_t L, int64_t i, int64_t j, int e, int64_t pos,
                                           const double *__restrict__ eD,
                                           const int32_t *__restrict__ eP,
                                           const double *__restrict__ olap,
+                                          const double *__restrict__ eC,
                                           const uint32_t *__restrict__ bits, int64_t bitw,
                                           double *__restrict__ Darr, double *__restrict__ Larr) {
-  const SlotPlan sp = slot_plan(pd, tiles, L, i, j, e, pos, eD, eP, olap, nullptr, 0, bits, bitw);
+  const TileDesc *td = tiles + pd.tile_base + i * pd.n2 + j;
+  const int64_t e_off = olap ? 0 : td->e_off;
+  const SlotPlan sp = slot_plan(pd, tiles, L, i, j, e, pos, eD, eP, olap, eC, e_off, bits, bitw);
   double d, l;
-  slot_apply(sp, Darr, Larr, &d, &l);
+  if (!slot_apply(sp, Darr, Larr, &d, &l) && !olap && td->c_off >= 0) {
+    Darr[sp.slot] = __builtin_huge_val();
+    Larr[sp.slot] = __builtin_huge_val();
+  }
 }
 
 // One tile anti-diagonal d of every pair: a thread per slot.
 __global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs,
                           const TileDesc *__restrict__ tiles, int64_t L,
                           const double *__restrict__ eD, const int32_t *__restrict__ eP,
-                          const double *__restrict__ olap, const uint32_t *__restrict__ bits,
+                          const double *__restrict__ eC, const uint32_t *__restrict__ bits,
                           int64_t bitw, double *__restrict__ Darr, double *__restrict__ Larr) {
   const PairDesc pd = pairs[blockIdx.y];
   const int64_t n1 = pd.n1, n2 = pd.n2;
@@ -404,7 +418,7 @@ __global__ void k_s2_diag(int64_t d, const PairDesc *__restrict__ pairs,
   const int e = (int)(rem / L);
   const int64_t pos = rem - (int64_t)e * L;
   const int64_t i = i0 + tq;
-  diag_slot(pd, tiles, L, i, d - i, e, pos, eD, eP, olap, bits, bitw, Darr, Larr);
+  diag_slot(pd, tiles, L, i, d - i, e, pos, eD, eP, nullptr, eC, bits, bitw, Darr, Larr);
 }
 
 // The same diagonal from a list of its tiles: `nlive` tiles with stage 1
@@ -435,8 +449,8 @@ __global__ void k_s2_diag_list(const int32_t *__restrict__ list, int64_t nlive, 
   const int64_t t = list[k];
   const TileDesc td = tiles[t];
   const PairDesc pd = pairs[td.pair];
-  diag_slot(pd, tiles, L, td.ij >> 32, td.ij & 0xffffffffll, e, pos, eD, eP, olap, bits, bitw,
-            Darr, Larr);
+  diag_slot(pd, tiles, L, td.ij >> 32, td.ij & 0xffffffffll, e, pos, eD, eP, olap, nullptr, bits,
+            bitw, Darr, Larr);
 }
 
 // The same sweep as one launch over the tiles with stage 1 only, each
@@ -1138,17 +1152,19 @@ extern "C" int pf_stage2_phases(int phases, const double *d_edge_C, const int64_
   const TileDesc *tiles = reinterpret_cast<const TileDesc *>(d_tiles);
   const int64_t bitw = (L + 32) / 32;
   if (phases & PF_S2_PROPAGATE) {
+    // the full form: the diagonal sweep writes every slot and takes the
+    // overlap costs from edge C, so prep builds only the start bitmaps
     hipLaunchKernelGGL(k_s2_prep, dim3(prep_grid(ntiles)), dim3(256), 0, st, pairs, tiles, L,
                        d_edge_P, d_edge_C, d_D_arr, d_L_arr, d_olap, d_bits, bitw,
-                       (int32_t *)nullptr, ntiles);
+                       (int32_t *)nullptr, ntiles, 1);
     PF_CHECK_HIP(hipGetLastError());
     for (int64_t d = 0; d < max_diag; ++d) {
       const int64_t ntd = (d + 1) < max_tile_diag ? (d + 1) : max_tile_diag;
       const int64_t nth = ntd * 2 * L;
       PF_REQUIRE((nth + 255) / 256 <= (1ll << 31) - 1, "pf_stage2: diagonal too large");
       hipLaunchKernelGGL(k_s2_diag, dim3((unsigned)((nth + 255) / 256), (unsigned)npairs),
-                         dim3(256), 0, st, d, pairs, tiles, L, d_edge_D, d_edge_P, d_olap, d_bits,
-                         bitw, d_D_arr, d_L_arr);
+                         dim3(256), 0, st, d, pairs, tiles, L, d_edge_D, d_edge_P, d_edge_C,
+                         d_bits, bitw, d_D_arr, d_L_arr);
       PF_CHECK_HIP(hipGetLastError());
     }
   }
@@ -1200,7 +1216,7 @@ extern "C" int pf_stage2_propagate_listed(const double *d_edge_C, const int64_t 
   const int64_t bitw = (L + 32) / 32;
   hipLaunchKernelGGL(k_s2_prep, dim3(prep_grid(ntiles)), dim3(256), 0, st, pairs, tiles, L,
                      d_edge_P, d_edge_C, d_D_arr, d_L_arr, d_olap, d_bits, bitw,
-                     (int32_t *)nullptr, ntiles);
+                     (int32_t *)nullptr, ntiles, 0);
   PF_CHECK_HIP(hipGetLastError());
   for (int64_t d = 0; d < ndiag; ++d) {
     const int64_t nl = h_diag[3 * d + 1], np = h_diag[3 * d + 2];
@@ -1235,7 +1251,7 @@ extern "C" int pf_stage2_propagate_dag(const double *d_edge_C, const int64_t *d_
   const int64_t bitw = (L + 32) / 32;
   hipLaunchKernelGGL(k_s2_prep, dim3(prep_grid(ntiles)), dim3(256), 0, st, pairs, tiles, L,
                      d_edge_P, d_edge_C, d_D_arr, d_L_arr, (double *)nullptr, d_bits, bitw,
-                     d_work, ntiles);
+                     d_work, ntiles, 0);
   PF_CHECK_HIP(hipGetLastError());
   int dev = 0, cus = 0;
   PF_CHECK_HIP(hipGetDevice(&dev));
