@@ -96,30 +96,36 @@ __global__ __launch_bounds__(256) void k_s2_prep(const PairDesc *__restrict__ pa
   if (dag && blockIdx.x == 0 && threadIdx.x == 0) dag[ntiles] = 0;
   for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     if (dag && threadIdx.x == 0) dag[t] = 0;
-    for (int w = threadIdx.x; w < 2 * bitw; w += blockDim.x) lb[w >= bitw][w - (w >= bitw) * bitw] = 0u;
-    __syncthreads();
     const TileDesc td = tiles[t];
-    // a tile without stage 1 (column 0 < 0): P = 0, C = 0, arrays not read
-    const bool live = td.c_off >= 0;
-    bool need0 = true, need1 = true;  // edges whose every slot may be read
-    if (!live) {
+    if (td.c_off < 0) {
+      // A tile without stage 1 (column 0 < 0): P = 0, C = 0, arrays not
+      // read.  Its slots are read only where a corner copy or the sync has
+      // written them (an unreached slot, D = +inf, is never read), and its
+      // overlap costs not at all (k_s2_diag takes 0).  Its start maps hold
+      // (0, 0) only, stored directly (no LDS, no barrier: the workgroup
+      // takes this branch as a whole).
       const PairDesc pd = pairs[td.pair];
       const int64_t i = td.ij >> 32, j = td.ij & 0xffffffffll;
-      need0 = i + 1 >= pd.n1 || tiles[t + pd.n2].c_off >= 0;
-      need1 = j + 1 >= pd.n2 || tiles[t + 1].c_off >= 0;
+      // edges whose every slot may be read
+      const bool need0 = i + 1 >= pd.n1 || tiles[t + pd.n2].c_off >= 0;
+      const bool need1 = j + 1 >= pd.n2 || tiles[t + 1].c_off >= 0;
+      for (int64_t rem = threadIdx.x; rem < 2 * L; rem += blockDim.x) {
+        const int e = rem >= L ? 1 : 0;
+        const int64_t pos = rem - (int64_t)e * L;
+        const int64_t last = e == 0 ? td.W - 1 : td.R - 1;
+        if ((e == 0 ? need0 : need1) || pos == 0 || pos == last)
+          Darr[t * 2 * L + rem] = __builtin_huge_val();
+      }
+      for (int w = threadIdx.x; w < 2 * bitw; w += blockDim.x)
+        bits[t * 2 * bitw + w] = (w == bitw) ? 1u : 0u;
+      continue;
     }
+    for (int w = threadIdx.x; w < 2 * bitw; w += blockDim.x) lb[w >= bitw][w - (w >= bitw) * bitw] = 0u;
+    __syncthreads();
     for (int64_t rem = threadIdx.x; rem < 2 * L; rem += blockDim.x) {
       const int64_t q = t * 2 * L + rem;
       const int e = rem >= L ? 1 : 0;
       const int64_t pos = rem - (int64_t)e * L;
-      if (!live) {
-        // A tile without stage 1: its L slots are read only where a corner
-        // copy or the sync has written them (an unreached slot, D = +inf, is
-        // never read), and its overlap costs not at all (k_s2_diag takes 0).
-        const int64_t last = e == 0 ? td.W - 1 : td.R - 1;
-        if ((e == 0 ? need0 : need1) || pos == 0 || pos == last) Darr[q] = __builtin_huge_val();
-        continue;
-      }
       // (with dag or sweep_all, the sweep writes every slot of the tile and
       // takes the overlap costs from eC itself)
       const bool init = !dag && !sweep_all;
@@ -137,8 +143,6 @@ __global__ __launch_bounds__(256) void k_s2_prep(const PairDesc *__restrict__ pa
         else atomicOr(&lb[1][(-s) >> 5], 1u << ((-s) & 31));
       }
     }
-    // a tile without stage 1 has P = 0 on every edge cell: start (0, 0) only
-    if (!live && threadIdx.x == 0) lb[1][0] = 1u;
     __syncthreads();
     for (int w = threadIdx.x; w < 2 * bitw; w += blockDim.x) {
       const int m = w >= bitw ? 1 : 0;
